@@ -1,0 +1,233 @@
+"""Benchmark of the MRPNN radiance predictor on one MI355X.
+
+Default workload (the headline metric): 1920x1080 frames over a 512^3 seeded fBm cloud
+(6 octaves, base frequency 4, density scale alpha = 64, g = 0.857, albedo 0.99), camera at
+distance 2 with a 40 degree field of view, the sun cycling through 16 seeded directions on
+the upper hemisphere.  One step = one frame: TR-mip rebake for the new sun (as the
+reference's Render does on a light change) + march + gather + MLP + composition, inputs
+resident in HBM.  `value` is the BF16 fused path in Mpix/s (frame pixels / wall time);
+the plain per-frame FP32 kernel (the drop-in baseline) and the FP8 path are timed on the
+same frames.  p50/p99 are per-frame latencies of the BF16 path (HIP events on the
+library's stream).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_PIXEL = 97072  # 2 x 48,536 MAC per pixel, one distinct colour channel
+GATHER_BYTES_PER_PIXEL = 18412  # logical fp32 texel bytes of the 192-point gather (+ LUT pairs, output)
+
+
+def sun_dirs(n, seed=11):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        z = rng.uniform(0.15, 0.95)  # elevation sine (upper hemisphere, y up)
+        phi = rng.uniform(0, 2 * math.pi)
+        r = math.sqrt(1 - z * z)
+        d = np.array([r * math.cos(phi), z, r * math.sin(phi)], np.float32)
+        out.append(d / np.linalg.norm(d))
+    return out
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--res", type=int, default=512)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--plain-steps", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import mrpnn_amd as M
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda")
+    ctx = M.Context()
+    ctx.set_timing(True)
+    W, H = args.width, args.height
+    vol = M.Volume(ctx, args.res)
+    t0 = time.time()
+    vol.fill_fbm(6, 4.0, 1)
+    torch.cuda.synchronize()
+    log("fbm %d^3 generated in %.2fs" % (args.res, time.time() - t0))
+    torch.cuda.synchronize()
+    t0 = time.time()
+    vol.update()
+    torch.cuda.synchronize()
+    pyramid_ms = (time.time() - t0) * 1e3
+    log("pyramid build %.2f ms" % pyramid_ms)
+
+    suns = sun_dirs(16)
+
+    def view(i):
+        return M.orbit_view(W, H, dist=2.0, azimuth=0.6, elevation=0.25, fov_deg=40.0, light=suns[i % 16],
+                            alpha=64.0, g=0.857, albedo=0.99, seed=1, frame=0, view_id=0, jitter=0.5)
+
+    out = torch.zeros((H * W, 4), device=dev)
+
+    def frame(i, prec):
+        v = view(i)
+        ctx.prepare_views([vol], [v])
+        if prec == "plain":
+            vol.render_plain_device(v, out)
+        else:
+            ctx.render_batch([vol], [v], [out], M.PRECISIONS[prec])
+
+    results = {}
+    kstats = {}
+    for prec, steps in (("bf16", args.steps), ("fp8", args.steps), ("plain", args.plain_steps)):
+        for i in range(args.warmup if prec != "plain" else 1):
+            frame(i, prec)
+        torch.cuda.synchronize()
+        lat = []
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        kern = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev[0].record()
+        for i in range(steps):
+            frame(args.warmup + i, prec)
+            ev[i + 1].record()
+            if prec != "plain":
+                s = ctx.stats()  # synchronises on the frame's last event
+                kern.append((s.ms_march, s.ms_predict, s.active_pixels, s.march_taps))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+        results[prec] = dict(wall=wall, steps=steps, lat=lat, mpix=W * H * steps / wall / 1e6)
+        kstats[prec] = kern
+        log("%-5s %7.1f Mpix/s  %.2f ms/frame  p50 %.2f p99 %.2f ms" %
+            (prec, results[prec]["mpix"], 1e3 * wall / steps, np.percentile(lat, 50), np.percentile(lat, 99)))
+
+    # accuracy of the timed BF16 / FP8 frames against the FP32 plain kernel (same view, frame 0)
+    def image(prec):
+        frame(0, prec)
+        torch.cuda.synchronize()
+        return out.clone()
+    ref = image("plain")
+    act = ref[:, 3] > 0
+    n_active = int(act.sum().item())
+    errs = {}
+    for prec in ("bf16", "fp8"):
+        img = image(prec)
+        rel = (img[act, :3] - ref[act, :3]).abs() / ref[act, :3].abs().clamp_min(1e-12)
+        errs[prec] = dict(mean=float(rel.mean().item()), p99=float(torch.quantile(rel[:, 0].float(), 0.99).item()))
+    log("active pixels %d (%.1f%%); bf16 mean rel err %.4f, fp8 %.4f" %
+        (n_active, 100.0 * n_active / (W * H), errs["bf16"]["mean"], errs["fp8"]["mean"]))
+
+    # roofline of the dominant kernel (fused gather + BF16 MFMA network), from the library's
+    # HIP events around that launch on its stream
+    ks = kstats["bf16"]
+    pred_ms = float(np.mean([k[1] for k in ks]))
+    march_ms = float(np.mean([k[0] for k in ks]))
+    taps = float(np.mean([k[3] for k in ks]))
+    flops = FLOP_PER_PIXEL * n_active
+    achieved_tf = flops / (pred_ms * 1e-3) / 1e12
+    gather_gbs = GATHER_BYTES_PER_PIXEL * n_active / (pred_ms * 1e-3) / 1e9
+    march_gbs = 32.0 * taps / (march_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_mlp_mfma_active_bf16_1080p_512.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(vol, ctx, view(args.warmup), args.cpu_seconds, W, H)
+
+    bf = results["bf16"]
+    line = {
+        "metric": "Mpix/s predictor @1080p, 512^3 cloud (bf16 vs fp32) + p50/p99 frame latency",
+        "value": round(bf["mpix"], 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * bf["wall"] / bf["steps"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic: seeded 6-octave fBm cloud, 16 seeded sun directions",
+        "config": {"workload": "1920x1080 single view over a 512^3 fBm cloud, sun swept over 16 directions, "
+                               "TR rebake per frame, BF16 MFMA network", "width": W, "height": H,
+                   "volume": args.res, "alpha": 64.0, "g": 0.857, "albedo": 0.99, "fov_deg": 40.0},
+        "p50_ms": round(float(np.percentile(bf["lat"], 50)), 3),
+        "p99_ms": round(float(np.percentile(bf["lat"], 99)), 3),
+        "fp32_plain_mpix_s": round(results["plain"]["mpix"], 2),
+        "fp8_mpix_s": round(results["fp8"]["mpix"], 2),
+        "speedup_bf16_vs_fp32_plain": round(bf["mpix"] / results["plain"]["mpix"], 2),
+        "bf16_mean_rel_err_vs_fp32": round(errs["bf16"]["mean"], 5),
+        "fp8_mean_rel_err_vs_fp32": round(errs["fp8"]["mean"], 5),
+        "active_pixel_fraction": round(n_active / (W * H), 4),
+        "pyramid_build_ms": round(pyramid_ms, 2),
+        "kernel_ms": {"march_compact": round(march_ms, 3), "mlp_mfma_active_bf16": round(pred_ms, 3)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved_tf / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": traffic},
+        "roofline_gather": {"bound": "hbm", "achieved": round(gather_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(gather_gbs / HBM_PEAK_GBS, 4),
+                            "note": "logical fp32 texel bytes of the gather; > 1 means on-chip reuse"},
+        "roofline_march": {"bound": "hbm", "achieved": round(march_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(march_gbs / HBM_PEAK_GBS, 4), "taps_per_frame": int(taps)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(vol, ctx, v, seconds, W, H):
+    """The oracle's restatement of NNPredict<MRPNN> on the host cores (OpenMP over pixels),
+    on a seeded uniform sample of the same frame's pixels, over the same pyramids."""
+    import numpy as np
+    import torch
+
+    import oracle as O
+    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    ctx.prepare_views([vol], [v])
+    torch.cuda.synchronize()
+    sc = O.Scene.from_arrays(vol.get_data(), vol.N, 1.0, vol.download_mips(), vol.download_tr(),
+                             ctx.download_hg_lut())
+    ov = O.View.from_buffer_copy(bytes(v))
+    rng = np.random.default_rng(0)
+    n = 256
+    t0 = time.perf_counter()
+    sc.render_predict_list(ov, rng.integers(0, W * H, n))
+    dt = time.perf_counter() - t0
+    n = int(max(256, min(W * H, n * seconds / max(dt, 1e-3))))
+    pix = rng.integers(0, W * H, n)
+    t0 = time.perf_counter()
+    sc.render_predict_list(ov, pix)
+    dt = time.perf_counter() - t0
+    log("cpu baseline: %d pixels in %.2fs on %d threads" % (n, dt, threads))
+    return {"value": round(n / dt / 1e6, 5), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": "%d pixels drawn uniformly from the benchmark 1080p frame, same pyramids (%.1f s)" % (n, dt)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/* C ABI of the MI355X (gfx950) MRPNN radiance-prediction library, libmrpnn_hip.so.
+ *
+ * This is the boundary the reference's C++ classes VolumeRender / Camera
+ * (core/volume.hpp:45-155, core/camera.hpp:10-43) are rebuilt on (see include/mrpnn/).
+ * Plain C types only: opaque handles, int status codes, caller-owned buffers.
+ *
+ * Conventions
+ *   - Every function returns MRP_OK (0) or an MRP_ERR_* code; mrp_last_error() gives the
+ *     message of the last failure on the calling thread.  Nothing prints and carries on
+ *     (the reference's CheckError does, core/volume.cu:15).
+ *   - Volume grids are float32 N^3 in the reference layout: index (x*N + y)*N + z,
+ *     x slowest (core/volume.cu:1179).  Position p in [-0.5,0.5]^3 maps to x <- p.x.
+ *   - "dev_" pointers are HIP device pointers; all device work is enqueued on the context's
+ *     stream (mrp_ctx_set_stream) and the *_device calls return without synchronising.
+ *     Host-buffer calls synchronise before returning.
+ *   - A context is not thread-safe; use one per thread.  Volumes belong to one context.
+ */
+#ifndef MRP_C_API_H
+#define MRP_C_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRP_OK 0
+#define MRP_ERR_INVALID 1 /* bad argument (null pointer, size out of range, ...) */
+#define MRP_ERR_HIP 2     /* HIP runtime error */
+#define MRP_ERR_NOMEM 3   /* device allocation failed */
+#define MRP_ERR_IO 4      /* weight asset missing or malformed */
+#define MRP_ERR_STATE 5   /* call out of order, e.g. render before mrp_volume_update */
+
+/* MLP arithmetic of the fused predictor. */
+#define MRP_FP32 0 /* scalar fp32, the reference's arithmetic */
+#define MRP_BF16 1 /* bf16 MFMA, fp32 accumulation */
+#define MRP_FP8 2  /* fp8 e4m3 MFMA with per-layer max-abs scales, fp32 accumulation */
+
+/* Number of features per pixel: [X_Val 0:192 | X_Val_Hg 192:384 | X_Val_Sub 384:576 | gamma]
+ * (core/radiancePredict.cu:427-485, row order of PopulateInputData :562-565). */
+#define MRP_NFEAT 577
+
+typedef struct mrp_ctx mrp_ctx;
+typedef struct mrp_volume mrp_volume;
+
+/* One camera view and its lighting; the argument list of VolumeRender::Render
+ * (core/volume.hpp:151) plus the device globals it reads (scatter_rate, core/render.cuh:17).
+ * Ray of pixel (row, col):  u = 1-(col+jx)/width, v = 1-(row+jy)/height,
+ *   dir = normalize(normalize(-ori) + right*((2u-1)*tan_half_fov*width/height)
+ *                                   + up*((2v-1)*tan_half_fov))
+ * tan_half_fov = 1 and width == height reproduce the reference camera
+ * (core/volume.cu:206-216, 90 degree square).  jitter < 0 draws (jx,jy) from the pixel's
+ * Philox stream, otherwise jx = jy = jitter (0.5 = pixel centre).  Output pixel index is
+ * row*width + col. */
+typedef struct mrp_view {
+    float ori[3];
+    float up[3];
+    float right[3];
+    float light_dir[3];
+    float light_color[3];
+    float scatter_rate[3]; /* albedo per channel, SetScatterRate (core/volume.cu:1748-1756) */
+    float alpha;           /* extinction scale sigma_t / max_density */
+    float g;               /* Henyey-Greenstein anisotropy */
+    float tan_half_fov;
+    float jitter;
+    int32_t width;
+    int32_t height;
+    uint32_t seed;         /* Philox key word 0 */
+    uint32_t frame;        /* Philox key word 1 (the reference's frame counter) */
+    uint32_t view_id;      /* Philox counter word; keeps a view's result independent of batching */
+    int32_t volume_index;  /* index into the volume array of mrp_render_batch */
+} mrp_view;
+
+/* Per-call statistics of the last render on a context. */
+typedef struct mrp_stats {
+    int64_t pixels;        /* width*height summed over views */
+    int64_t active_pixels; /* pixels whose ray reached a first-scattering point */
+    int64_t march_taps;    /* full-resolution density taps of the first-hit march (S_march) */
+    float ms_march;        /* device time of the march kernel (when timing is on) */
+    float ms_predict;      /* device time of the gather+MLP kernel(s) */
+    float ms_total;        /* device time of the whole render on the stream */
+} mrp_stats;
+
+const char* mrp_last_error(void);
+const char* mrp_version(void);
+
+/* Context: device weights, HG LUT, stream, scratch.  weights_path NULL = the asset
+ * mrpnn_weights.bin installed in assets/ next to the library. */
+int mrp_ctx_create(const char* weights_path, mrp_ctx** out);
+int mrp_ctx_destroy(mrp_ctx* ctx);
+int mrp_ctx_set_stream(mrp_ctx* ctx, void* hip_stream);
+int mrp_ctx_set_timing(mrp_ctx* ctx, int enabled);
+int mrp_ctx_get_stats(mrp_ctx* ctx, mrp_stats* out);
+/* Device pointer of the MLP weights as loaded (fp32, header order). */
+int mrp_ctx_weights_device(mrp_ctx* ctx, const float** dev_weights, int* count);
+
+/* HG LUT, part of VolumeRender::UpdateHGLut / Fill_Hg (core/volume.cu:1322-1423): 512x512
+ * mean HG phase over a ball, cached on exact g.  seed keys the per-texel jitter. */
+int mrp_ctx_update_hg_lut(mrp_ctx* ctx, float g, uint32_t seed);
+int mrp_ctx_download_hg_lut(mrp_ctx* ctx, float* host_lut /* 512*512 */);
+int mrp_ctx_upload_hg_lut(mrp_ctx* ctx, const float* host_lut, float g); /* testing hook */
+
+/* Volumes: VolumeRender(int resolution) (core/volume.cu:948-957) without its 1024^3
+ * scratch array.  N must be in [2, 2048]. */
+int mrp_volume_create(mrp_ctx* ctx, int resolution, mrp_volume** out);
+int mrp_volume_destroy(mrp_volume* vol);
+/* Device bytes mrp_volume_create(N) allocates (grid + mips + TR mips). */
+int64_t mrp_volume_bytes(int resolution);
+/* SetDatas/SetData + max_density field (core/volume.cu:1178-1184, volume.hpp:104). */
+int mrp_volume_set_data(mrp_volume* vol, const float* host_grid, float max_density);
+int mrp_volume_set_data_device(mrp_volume* vol, const float* dev_grid, float max_density);
+/* Seeded fBm cloud generated on the device (octaves, base frequency, seed); sets
+ * max_density = 1.  Same bits as the CPU generator of the test oracle. */
+int mrp_volume_fill_fbm(mrp_volume* vol, int octaves, float base_freq, uint32_t seed);
+int mrp_volume_get_data(mrp_volume* vol, float* host_grid);
+/* Update(process_mips) (core/volume.cu:1443-1531): mip pyramid 256^3..1^3 on the GPU. */
+int mrp_volume_update(mrp_volume* vol);
+/* Update_TR (core/volume.cu:1589-1667): 8 transmittance mips for (light_dir, alpha),
+ * cached on exact equality like the reference. */
+int mrp_volume_update_tr(mrp_volume* vol, const float light_dir[3], float alpha);
+/* Copies of the device pyramids: mips concatenated 256^3,128^3,...,1^3 (19,173,961 floats);
+ * TR mips 128^3..1^3 (2,396,745 floats). */
+int mrp_volume_download_mips(mrp_volume* vol, float* host_mips);
+int mrp_volume_download_tr(mrp_volume* vol, float* host_tr);
+int mrp_volume_resolution(mrp_volume* vol);
+const float* mrp_volume_grid_device(mrp_volume* vol);
+
+/* Plain per-frame FP32 predictor: VolumeRender::Render(size, ..., MRPNN, sampleNum)
+ * (core/volume.cu:487-548): for each frame, rebuild LUT/TR if (g, light, alpha) changed,
+ * one thread per pixel for ray, march, basis, gather and the scalar fp32 MLP, progressive
+ * average over frames (frame f uses Philox key word 1 = view->frame + f).  Blocking;
+ * host_out4 receives width*height float4 (rgb, distance). */
+int mrp_render_plain(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int n_frames, float* host_out4);
+/* One frame of the same kernel into a device buffer, asynchronous (LUT/TR must be current). */
+int mrp_render_plain_device(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, float* dev_out4);
+
+/* Fused predictor: march + compaction, then gather and MLP (FP32 / BF16 MFMA / FP8 MFMA)
+ * for all views in one pass; views[i].volume_index selects volumes[...].  dev_out4[i] has
+ * width*height float4.  LUT and TR mips must be current for each view (see
+ * mrp_prepare_views).  Asynchronous. */
+int mrp_render_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views,
+                     int precision, float* const* dev_out4);
+/* Brings LUT and TR mips up to date for every view (first view per volume wins if two
+ * views of one volume disagree: call per light then). */
+int mrp_prepare_views(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views);
+
+/* Feature-level entry points (parity and tooling).
+ * mrp_gather_features: per pixel the 577 features (zeros for inactive pixels),
+ *   active flag, hit position, and march taps, exactly as the plain kernel computes them.
+ * mrp_predict_features: RadiancePredictInternal on n feature rows (577 floats each) with
+ *   the given scatterrate (the value the reference passes: scatter_rate/1.001) and g;
+ *   dev_rgb receives n*3 floats max(e^P-1,0)*scatterrate^4. */
+int mrp_gather_features(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, float* dev_feats, int32_t* dev_active,
+                        float* dev_pos3, int32_t* dev_taps);
+int mrp_predict_features(mrp_ctx* ctx, const float* dev_feats, int n, const float scatterrate[3], float g, int precision,
+                         float* dev_rgb);
+
+/* Delta-tracking path tracer (judge): RenderCamera<false> / CalculateRadiance
+ * (core/volume.cu:198-228, core/render.cuh:535-597) with Philox streams keyed
+ * (seed, frame) / (pixel, sample) and FP64 per-pixel Welford accumulators.
+ * nee_baked = 1 uses TR mip 0 for next-event estimation as the reference
+ * (core/render.cuh:589), 0 uses the unbiased stochastic Tr (core/render.cuh:60-103).
+ * dev_mean3 / dev_m2_3: width*height*3 doubles; samples [spp0, spp0+spp) are added to
+ * the running (n = spp0 on entry) accumulators. */
+int mrp_pt_render(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int spp0, int spp, int max_bounce,
+                  int nee_baked, float tr_scale, double* dev_mean3, double* dev_m2_3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MRP_C_API_H */

@@ -1,0 +1,42 @@
+// Camera: the reference's pinhole camera (core/camera.hpp:10-43, core/camera.cu:166-257) on
+// top of mrpnn::VolumeRender.  RenderToFile writes `path + ".png"` (RGBA8, saturate x 255).
+#pragma once
+#include <functional>
+#include <string>
+
+#include "volume.hpp"
+
+namespace mrpnn {
+
+typedef std::function<float3(float3 radiance)> ToneMapFunction;
+
+class Camera {
+protected:
+    float3 ori;
+    float3 forward;
+    float3 right;
+    float3 up;
+    VolumeRender* volume;
+
+public:
+    static float3 None(float3 color) { return color; }
+
+    int resolution;
+    std::string name;
+
+    Camera(VolumeRender& volume, std::string name = "test camera", float3 position = {0.75f, 0.75f, 0.75f},
+           int resolution = 512);
+    void SetPosition(float3 position);
+    float3 GetPosition() const { return ori; }
+    void SetVolume(VolumeRender& volume);
+    // Returns the rendered image (rgb, distance) and writes path.png.
+    std::vector<float4> RenderToFile(std::string path, float3 lightDir, float3 lightColor = {1, 1, 1},
+                                     float alpha = 1, int multiScatter = 1, float g = 0, int sampleNum = 1,
+                                     ToneMapFunction tone = None,
+                                     VolumeRender::RenderType rt = VolumeRender::RenderType::PT, float exp = 1.0f);
+};
+
+// Minimal PNG writer (stored deflate blocks), RGBA8, row-major.
+bool write_png_rgba(const std::string& path, int width, int height, const unsigned char* rgba);
+
+}  // namespace mrpnn

@@ -1,0 +1,83 @@
+// VolumeRender: the reference's public C++ class (core/volume.hpp:45-155) rebuilt on the C ABI
+// of libmrpnn_hip.so (include/mrp_c_api.h).  Same names, argument order, defaults and
+// meaning for the members on the predictor path; errors throw mrpnn::Error instead of
+// printing and carrying on (core/volume.cu:15).
+//
+// Deliberate differences, each a bug or a CUDA-only mechanism of the reference:
+//   * VolumeRender(int) allocates the device grid it renders from (the reference never
+//     assigns datas_dev there, core/volume.cu:948-957) and no 1024^3 scratch array
+//     (core/volume.cu:951-952);
+//   * Update_TR(..., CPU) always bakes on the GPU;
+//   * RenderType::RPNN (a different network, not on this path) throws;
+//   * Render(..., PT, ...) returns the mean radiance with w = 0 (the reference's w is a hit
+//     fraction used only by its GUI).
+#pragma once
+#include <hip/hip_vector_types.h>
+
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../mrp_c_api.h"
+
+namespace mrpnn {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// omp.hpp:8 (x, y, z integer coordinates; u, v, w = (i + 0.5) / resolution)
+typedef std::function<float(int x, int y, int z, float u, float v, float w)> FillFunc;
+
+// Process-wide context shared by all volumes, so they can be batched together.
+mrp_ctx* default_context();
+
+class VolumeRender {
+public:
+    enum RenderType { PT, RPNN, MRPNN };
+
+    float max_density = 0.00001f;  // volume.hpp:104 -- set it to max(grid) before Update()
+    float min_val = 0.0f;
+
+    explicit VolumeRender(int resolution);
+    // .bin file: int32 N, then N^3 float32 in (x*N+y)*N+z order (core/volume.cu:1011-1035)
+    explicit VolumeRender(std::string path, bool cacheTR = false);
+    ~VolumeRender();
+    VolumeRender(const VolumeRender&) = delete;
+    VolumeRender& operator=(const VolumeRender&) = delete;
+
+    void LoadData(std::string path, bool cacheTR);
+    void SetData(int x, int y, int z, float value);
+    void SetDatas(FillFunc func);
+    void Update(bool process_mips = true, bool need_reset_datas_dev = true);
+    void Update_TR(float3 lightDir, float alpha = 64.0f, bool CPU = false);
+    void UpdateHGLut(float g);
+
+    void SetTrScale(float scale);
+    void SetScatterRate(float rate);
+    void SetScatterRate(float3 rate);
+    // Extension: vertical field of view (degrees) of Render/Camera; 90 reproduces the reference.
+    void SetFov(float degrees);
+
+    std::vector<float4> Render(int2 size, float3 ori, float3 up, float3 right, float3 lightDir,
+                               RenderType rt = RenderType::PT, float g = 0.857f, float alpha = 1,
+                               float3 lightColor = {1, 1, 1}, int multiScatter = 512, int sampleNum = 1024);
+
+    int resolution() const { return resolution_; }
+    mrp_volume* handle() const { return vol_; }
+    const std::vector<float>& datas() const { return datas_; }
+
+private:
+    int resolution_ = 0;
+    std::vector<float> datas_;
+    mrp_volume* vol_ = nullptr;
+    float3 scatter_rate_ = {1.001f, 1.001f, 1.001f};  // render.cuh:17
+    float tr_scale_ = 1.0f;
+    float tan_half_fov_ = 1.0f;
+    uint32_t frame_counter_ = 0;
+    void create(int resolution);
+};
+
+}  // namespace mrpnn

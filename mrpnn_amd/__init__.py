@@ -1,0 +1,272 @@
+"""Python binding of libmrpnn_hip.so (the C ABI in include/mrp_c_api.h).
+
+The library is the product: gfx950 HIP kernels behind a C ABI.  This module is a thin
+ctypes layer for tests and benchmarks; device buffers are torch tensors (plumbing only).
+There is no CPU fallback: if the library is missing, loading fails loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmrpnn_hip.so")
+
+FP32, BF16, FP8 = 0, 1, 2
+PRECISIONS = {"fp32": FP32, "bf16": BF16, "fp8": FP8}
+NFEAT = 577
+MIP_TOTAL = sum((256 >> m) ** 3 for m in range(9))
+TR_TOTAL = sum((128 >> m) ** 3 for m in range(8))
+
+
+class MrpError(RuntimeError):
+    pass
+
+
+class View(ctypes.Structure):
+    """mrp_view (include/mrp_c_api.h)."""
+    _fields_ = [("ori", ctypes.c_float * 3), ("up", ctypes.c_float * 3), ("right", ctypes.c_float * 3),
+                ("light_dir", ctypes.c_float * 3), ("light_color", ctypes.c_float * 3),
+                ("scatter_rate", ctypes.c_float * 3), ("alpha", ctypes.c_float), ("g", ctypes.c_float),
+                ("tan_half_fov", ctypes.c_float), ("jitter", ctypes.c_float), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("seed", ctypes.c_uint32), ("frame", ctypes.c_uint32),
+                ("view_id", ctypes.c_uint32), ("volume_index", ctypes.c_int32)]
+
+    def copy(self):
+        v = View()
+        ctypes.pointer(v)[0] = self
+        return v
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("pixels", ctypes.c_int64), ("active_pixels", ctypes.c_int64), ("march_taps", ctypes.c_int64),
+                ("ms_march", ctypes.c_float), ("ms_predict", ctypes.c_float), ("ms_total", ctypes.c_float)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MrpError("libmrpnn_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.mrp_last_error.restype = ctypes.c_char_p
+        L.mrp_version.restype = ctypes.c_char_p
+        L.mrp_volume_bytes.restype = ctypes.c_int64
+        L.mrp_volume_bytes.argtypes = [ctypes.c_int]
+        L.mrp_ctx_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(_vp)]
+        L.mrp_ctx_destroy.argtypes = [_vp]
+        L.mrp_ctx_set_stream.argtypes = [_vp, _vp]
+        L.mrp_ctx_set_timing.argtypes = [_vp, ctypes.c_int]
+        L.mrp_ctx_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
+        L.mrp_ctx_update_hg_lut.argtypes = [_vp, ctypes.c_float, ctypes.c_uint32]
+        L.mrp_ctx_download_hg_lut.argtypes = [_vp, _fp]
+        L.mrp_ctx_upload_hg_lut.argtypes = [_vp, _fp, ctypes.c_float]
+        L.mrp_ctx_set_mfma_waves.argtypes = [_vp, ctypes.c_int]
+        L.mrp_volume_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.mrp_volume_destroy.argtypes = [_vp]
+        L.mrp_volume_set_data.argtypes = [_vp, _fp, ctypes.c_float]
+        L.mrp_volume_set_data_device.argtypes = [_vp, _vp, ctypes.c_float]
+        L.mrp_volume_fill_fbm.argtypes = [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint32]
+        L.mrp_volume_get_data.argtypes = [_vp, _fp]
+        L.mrp_volume_update.argtypes = [_vp]
+        L.mrp_volume_update_tr.argtypes = [_vp, _fp, ctypes.c_float]
+        L.mrp_volume_download_mips.argtypes = [_vp, _fp]
+        L.mrp_volume_download_tr.argtypes = [_vp, _fp]
+        L.mrp_render_plain.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, _fp]
+        L.mrp_render_plain_device.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp]
+        L.mrp_prepare_views.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View), ctypes.c_int]
+        L.mrp_render_batch.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View), ctypes.c_int,
+                                       ctypes.c_int, ctypes.POINTER(_vp)]
+        L.mrp_gather_features.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp, _vp, _vp, _vp]
+        L.mrp_predict_features.argtypes = [_vp, _vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.c_int, _vp]
+        L.mrp_pt_render.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_float, _vp, _vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise MrpError("mrp error %d: %s" % (rc, lib().mrp_last_error().decode()))
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    def __init__(self, weights_path=None):
+        self.h = _vp()
+        check(lib().mrp_ctx_create(weights_path.encode() if weights_path else None, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().mrp_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr):
+        check(lib().mrp_ctx_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
+    def set_timing(self, on):
+        check(lib().mrp_ctx_set_timing(self.h, 1 if on else 0))
+
+    def stats(self):
+        s = Stats()
+        check(lib().mrp_ctx_get_stats(self.h, ctypes.byref(s)))
+        return s
+
+    def set_mfma_waves(self, w):
+        check(lib().mrp_ctx_set_mfma_waves(self.h, int(w)))
+
+    def update_hg_lut(self, g, seed=1):
+        check(lib().mrp_ctx_update_hg_lut(self.h, float(g), int(seed)))
+
+    def download_hg_lut(self):
+        out = np.zeros(512 * 512, np.float32)
+        check(lib().mrp_ctx_download_hg_lut(self.h, out.ctypes.data_as(_fp)))
+        return out
+
+    def upload_hg_lut(self, lut, g):
+        lut = np.ascontiguousarray(lut, np.float32)
+        check(lib().mrp_ctx_upload_hg_lut(self.h, lut.ctypes.data_as(_fp), float(g)))
+
+    def predict_features(self, feats, sr, g, precision=FP32):
+        """feats: torch float32 [n, 577] on the GPU -> torch [n, 3]."""
+        import torch
+        n = feats.shape[0]
+        out = torch.empty((n, 3), dtype=torch.float32, device=feats.device)
+        srv = (ctypes.c_float * 3)(*[float(x) for x in sr])
+        check(lib().mrp_predict_features(self.h, _ptr(feats), n, srv, float(g), int(precision), _ptr(out)))
+        return out
+
+    def render_batch(self, volumes, views, outs, precision=BF16):
+        vh = (_vp * len(volumes))(*[v.h for v in volumes])
+        va = (View * len(views))(*views)
+        oa = (_vp * len(outs))(*[o.data_ptr() for o in outs])
+        check(lib().mrp_render_batch(self.h, vh, len(volumes), va, len(views), int(precision), oa))
+
+    def prepare_views(self, volumes, views):
+        vh = (_vp * len(volumes))(*[v.h for v in volumes])
+        va = (View * len(views))(*views)
+        check(lib().mrp_prepare_views(self.h, vh, len(volumes), va, len(views)))
+
+
+class Volume:
+    def __init__(self, ctx, N):
+        self.ctx = ctx
+        self.N = N
+        self.h = _vp()
+        check(lib().mrp_volume_create(ctx.h, int(N), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().mrp_volume_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_data(self, grid, max_density):
+        grid = np.ascontiguousarray(grid, np.float32)
+        assert grid.size == self.N ** 3
+        check(lib().mrp_volume_set_data(self.h, grid.ctypes.data_as(_fp), float(max_density)))
+
+    def fill_fbm(self, octaves, base_freq, seed):
+        check(lib().mrp_volume_fill_fbm(self.h, int(octaves), float(base_freq), int(seed)))
+
+    def get_data(self):
+        out = np.zeros(self.N ** 3, np.float32)
+        check(lib().mrp_volume_get_data(self.h, out.ctypes.data_as(_fp)))
+        return out
+
+    def update(self):
+        check(lib().mrp_volume_update(self.h))
+
+    def update_tr(self, light, alpha):
+        L = (ctypes.c_float * 3)(*[float(x) for x in light])
+        check(lib().mrp_volume_update_tr(self.h, L, float(alpha)))
+
+    def download_mips(self):
+        out = np.zeros(MIP_TOTAL, np.float32)
+        check(lib().mrp_volume_download_mips(self.h, out.ctypes.data_as(_fp)))
+        return out
+
+    def download_tr(self):
+        out = np.zeros(TR_TOTAL, np.float32)
+        check(lib().mrp_volume_download_tr(self.h, out.ctypes.data_as(_fp)))
+        return out
+
+    def render_plain(self, view, n_frames=1):
+        out = np.zeros((view.height * view.width, 4), np.float32)
+        check(lib().mrp_render_plain(self.ctx.h, self.h, ctypes.byref(view), int(n_frames), out.ctypes.data_as(_fp)))
+        return out
+
+    def render_plain_device(self, view, out):
+        check(lib().mrp_render_plain_device(self.ctx.h, self.h, ctypes.byref(view), _ptr(out)))
+
+    def gather_features(self, view):
+        import torch
+        n = view.width * view.height
+        dev = torch.device("cuda")
+        feats = torch.empty((n, NFEAT), dtype=torch.float32, device=dev)
+        active = torch.empty(n, dtype=torch.int32, device=dev)
+        pos = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        taps = torch.empty(n, dtype=torch.int32, device=dev)
+        check(lib().mrp_gather_features(self.ctx.h, self.h, ctypes.byref(view), _ptr(feats), _ptr(active), _ptr(pos),
+                                        _ptr(taps)))
+        return feats, active, pos, taps
+
+    def pt_render(self, view, spp, max_bounce, nee_baked=0, tr_scale=1.0, chunk=None, mean=None, m2=None, spp0=0):
+        import torch
+        n = view.width * view.height
+        if mean is None:
+            mean = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+            m2 = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+        chunk = chunk or spp
+        done = 0
+        while done < spp:
+            k = min(chunk, spp - done)
+            check(lib().mrp_pt_render(self.ctx.h, self.h, ctypes.byref(view), spp0 + done, k, int(max_bounce),
+                                      int(nee_baked), float(tr_scale), _ptr(mean), _ptr(m2)))
+            done += k
+        return mean, m2
+
+
+def orbit_view(width, height, dist=2.0, azimuth=0.0, elevation=0.0, fov_deg=40.0, light=(0.866, 0.5, 0.0),
+               alpha=64.0, g=0.857, albedo=0.99, seed=1, frame=0, view_id=0, jitter=0.5, volume_index=0):
+    """Camera on a sphere looking at the origin; basis as Camera::SetPosition (camera.cu:176-192)."""
+    ca, sa, ce, se = np.cos(azimuth), np.sin(azimuth), np.cos(elevation), np.sin(elevation)
+    ori = np.array([dist * ce * ca, dist * se, dist * ce * sa], np.float32)
+    fwd = -ori / np.linalg.norm(ori)
+    right = np.cross(np.array([0, 1, 0], np.float32), fwd)
+    right = right / np.linalg.norm(right)
+    up = np.cross(fwd, right)
+    v = View()
+    v.ori[:] = ori.tolist()
+    v.up[:] = up.astype(np.float32).tolist()
+    v.right[:] = right.astype(np.float32).tolist()
+    L = np.asarray(light, np.float64)
+    v.light_dir[:] = (L / np.linalg.norm(L)).astype(np.float32).tolist()
+    v.light_color[:] = [1.0, 1.0, 1.0]
+    v.scatter_rate[:] = [albedo] * 3
+    v.alpha, v.g = alpha, g
+    v.tan_half_fov = float(np.tan(np.radians(fov_deg) / 2))
+    v.jitter = jitter
+    v.width, v.height = width, height
+    v.seed, v.frame, v.view_id, v.volume_index = seed, frame, view_id, volume_index
+    return v

@@ -1,0 +1,275 @@
+// mrpnn::VolumeRender / mrpnn::Camera over the C ABI (see include/mrpnn/*.hpp).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include <fstream>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/mrpnn/camera.hpp"
+#include "../../include/mrpnn/volume.hpp"
+
+namespace mrpnn {
+
+static void check(int rc) {
+    if (rc != MRP_OK) throw Error(rc, mrp_last_error());
+}
+
+mrp_ctx* default_context() {
+    static std::mutex mu;
+    static mrp_ctx* ctx = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ctx) check(mrp_ctx_create(nullptr, &ctx));
+    return ctx;
+}
+
+void VolumeRender::create(int resolution) {
+    resolution_ = resolution;
+    datas_.assign((size_t)resolution * resolution * resolution, 0.0f);
+    check(mrp_volume_create(default_context(), resolution, &vol_));
+}
+
+VolumeRender::VolumeRender(int resolution) { create(resolution); }
+
+VolumeRender::VolumeRender(std::string path, bool cacheTR) { LoadData(path, cacheTR); }
+
+VolumeRender::~VolumeRender() {
+    if (vol_) mrp_volume_destroy(vol_);
+}
+
+// .bin loader (core/volume.cu:1011-1035): int32 N, then N^3 floats; max_density = max(grid).
+void VolumeRender::LoadData(std::string path, bool /*cacheTR*/) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw Error(MRP_ERR_IO, "File not found: " + path);
+    int32_t n = 0;
+    f.read((char*)&n, 4);
+    if (!f || n < 2 || n > 2048) throw Error(MRP_ERR_IO, "bad volume header in " + path);
+    if (vol_) {
+        mrp_volume_destroy(vol_);
+        vol_ = nullptr;
+    }
+    create(n);
+    f.read((char*)datas_.data(), (std::streamsize)(datas_.size() * sizeof(float)));
+    if (!f) throw Error(MRP_ERR_IO, "truncated volume file " + path);
+    float m = 0.00001f;
+    for (float v : datas_) m = v > m ? v : m;
+    max_density = m;
+    Update();
+}
+
+void VolumeRender::SetData(int x, int y, int z, float value) {
+    datas_[((size_t)x * resolution_ + y) * resolution_ + z] = value;  // volume.cu:1178-1180
+}
+
+void VolumeRender::SetDatas(FillFunc func) {  // ParallelFill (omp.cpp:4-18), threads over x slabs
+    const int r = resolution_;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t)
+        pool.emplace_back([&, t]() {
+            for (int i = (int)t; i < r; i += (int)nt)
+                for (int j = 0; j < r; ++j)
+                    for (int k = 0; k < r; ++k) {
+                        float u = (float)((i + 0.5) / r), v = (float)((j + 0.5) / r), w = (float)((k + 0.5) / r);
+                        datas_[((size_t)i * r + j) * r + k] = func(i, j, k, u, v, w);
+                    }
+        });
+    for (auto& th : pool) th.join();
+}
+
+void VolumeRender::Update(bool process_mips, bool need_reset_datas_dev) {
+    if (need_reset_datas_dev) check(mrp_volume_set_data(vol_, datas_.data(), max_density));
+    if (process_mips) check(mrp_volume_update(vol_));
+}
+
+void VolumeRender::Update_TR(float3 lightDir, float alpha, bool /*CPU*/) {
+    const float L[3] = {lightDir.x, lightDir.y, lightDir.z};
+    check(mrp_volume_update_tr(vol_, L, alpha));
+}
+
+void VolumeRender::UpdateHGLut(float g) { check(mrp_ctx_update_hg_lut(default_context(), g, 1u)); }
+
+void VolumeRender::SetTrScale(float scale) { tr_scale_ = scale; }
+void VolumeRender::SetScatterRate(float rate) { scatter_rate_ = float3{rate, rate, rate}; }
+void VolumeRender::SetScatterRate(float3 rate) { scatter_rate_ = rate; }
+void VolumeRender::SetFov(float degrees) {
+    if (!(degrees > 0.0f && degrees < 180.0f)) throw Error(MRP_ERR_INVALID, "fov must be in (0, 180)");
+    tan_half_fov_ = degrees == 90.0f ? 1.0f : (float)tan(degrees * M_PI / 360.0);
+}
+
+std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float3 right, float3 lightDir,
+                                         RenderType rt, float g, float alpha, float3 lightColor, int multiScatter,
+                                         int sampleNum) {
+    if (rt == RenderType::RPNN) throw Error(MRP_ERR_INVALID, "RPNN is not part of this library");
+    mrp_view v;
+    memset(&v, 0, sizeof v);
+    const float3 o[6] = {ori, up, right, lightDir, lightColor, scatter_rate_};
+    float* dst[6] = {v.ori, v.up, v.right, v.light_dir, v.light_color, v.scatter_rate};
+    for (int i = 0; i < 6; ++i) {
+        dst[i][0] = o[i].x;
+        dst[i][1] = o[i].y;
+        dst[i][2] = o[i].z;
+    }
+    v.alpha = alpha;
+    v.g = g;
+    v.tan_half_fov = tan_half_fov_;
+    v.jitter = -1.0f;  // a fresh sub-pixel jitter every frame, as the reference's Rand()
+    v.width = size.x;
+    v.height = size.y;
+    v.seed = 0;
+    v.frame = frame_counter_;
+    frame_counter_ += (uint32_t)(sampleNum > 0 ? sampleNum : 1);
+    std::vector<float4> out((size_t)size.x * size.y);
+    if (rt == RenderType::MRPNN) {
+        check(mrp_render_plain(default_context(), vol_, &v, sampleNum, (float*)out.data()));
+        return out;
+    }
+    // PT: sampleNum frames of 32 paths (core/volume.cu:222), NEE through the TR mips
+    check(mrp_ctx_update_hg_lut(default_context(), g, 1u));
+    check(mrp_volume_update_tr(vol_, v.light_dir, alpha));
+    const size_t n = (size_t)size.x * size.y;
+    double *mean = nullptr, *m2 = nullptr;
+    if (hipMalloc((void**)&mean, n * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&m2, n * 3 * sizeof(double)) != hipSuccess) {
+        hipFree(mean);
+        throw Error(MRP_ERR_NOMEM, "PT accumulator allocation failed");
+    }
+    const int spp = 32 * (sampleNum > 0 ? sampleNum : 1);
+    int rc = MRP_OK;
+    for (int s0 = 0; s0 < spp && rc == MRP_OK; s0 += 32)
+        rc = mrp_pt_render(default_context(), vol_, &v, s0, 32, multiScatter, 1, tr_scale_, mean, m2);
+    std::vector<double> h(n * 3);
+    if (rc == MRP_OK && hipMemcpy(h.data(), mean, n * 3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = MRP_ERR_HIP;
+    hipFree(mean);
+    hipFree(m2);
+    check(rc);
+    for (size_t i = 0; i < n; ++i) out[i] = float4{(float)h[3 * i], (float)h[3 * i + 1], (float)h[3 * i + 2], 0.0f};
+    return out;
+}
+
+// ------------------------------------------------------------------------------------------
+Camera::Camera(VolumeRender& vol, std::string nm, float3 position, int res) {
+    volume = &vol;
+    name = nm;
+    resolution = res;
+    SetPosition(position);
+}
+
+static float3 n3(float3 v) {
+    float l = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    return float3{v.x / l, v.y / l, v.z / l};
+}
+static float3 c3(float3 a, float3 b) {
+    return float3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+void Camera::SetPosition(float3 position) {  // camera.cu:176-192
+    ori = position;
+    forward = n3(float3{-ori.x, -ori.y, -ori.z});
+    right = n3(c3(float3{0, 1, 0}, forward));
+    up = c3(forward, right);
+    if (position.x == 0 && position.z == 0) {
+        right = float3{1, 0, 0};
+        up = position.y > 0 ? float3{0, 0, 1} : float3{0, 0, -1};
+    }
+}
+
+void Camera::SetVolume(VolumeRender& vol) { volume = &vol; }
+
+std::vector<float4> Camera::RenderToFile(std::string path, float3 lightDir, float3 lightColor, float alpha,
+                                         int multiScatter, float g, int sampleNum, ToneMapFunction tone,
+                                         VolumeRender::RenderType rt, float exp) {
+    (void)tone;
+    (void)exp;  // the reference ignores both in RenderToFile (camera.cu:243)
+    std::vector<float4> img = volume->Render(int2{resolution, resolution}, ori, up, right, lightDir, rt, g, alpha,
+                                             lightColor, multiScatter, sampleNum);
+    std::vector<unsigned char> px((size_t)resolution * resolution * 4);
+    auto sat = [](float v) { return (unsigned char)((v < 0 ? 0 : (v > 1 ? 1 : v)) * 255); };
+    for (size_t i = 0; i < img.size(); ++i) {
+        px[4 * i] = sat(img[i].x);
+        px[4 * i + 1] = sat(img[i].y);
+        px[4 * i + 2] = sat(img[i].z);
+        px[4 * i + 3] = 255;
+    }
+    if (!write_png_rgba(path + ".png", resolution, resolution, px.data()))
+        throw Error(MRP_ERR_IO, "cannot write " + path + ".png");
+    return img;
+}
+
+// ------------------------------------------------------------------------------------------
+// PNG: signature, IHDR, one IDAT holding a zlib stream of stored (uncompressed) deflate
+// blocks, IEND.  CRC-32 (ISO 3309 polynomial) and Adler-32 as in RFC 1950/2083.
+static uint32_t crc_table[256];
+static void crc_init() {
+    for (uint32_t n = 0; n < 256; ++n) {
+        uint32_t c = n;
+        for (int k = 0; k < 8; ++k) c = (c & 1) ? 0xedb88320u ^ (c >> 1) : c >> 1;
+        crc_table[n] = c;
+    }
+}
+static uint32_t crc(const unsigned char* b, size_t n, uint32_t c = 0xffffffffu) {
+    for (size_t i = 0; i < n; ++i) c = crc_table[(c ^ b[i]) & 0xff] ^ (c >> 8);
+    return c;
+}
+static void be32(std::vector<unsigned char>& o, uint32_t v) {
+    o.push_back(v >> 24);
+    o.push_back(v >> 16);
+    o.push_back(v >> 8);
+    o.push_back(v);
+}
+static void chunk(std::vector<unsigned char>& out, const char* type, const std::vector<unsigned char>& data) {
+    be32(out, (uint32_t)data.size());
+    std::vector<unsigned char> td(type, type + 4);
+    td.insert(td.end(), data.begin(), data.end());
+    out.insert(out.end(), td.begin(), td.end());
+    be32(out, crc(td.data(), td.size()) ^ 0xffffffffu);
+}
+
+bool write_png_rgba(const std::string& path, int w, int h, const unsigned char* rgba) {
+    static std::once_flag once;
+    std::call_once(once, crc_init);
+    std::vector<unsigned char> raw;
+    raw.reserve((size_t)h * (4 * w + 1));
+    for (int y = 0; y < h; ++y) {
+        raw.push_back(0);  // filter: none
+        raw.insert(raw.end(), rgba + (size_t)y * w * 4, rgba + (size_t)(y + 1) * w * 4);
+    }
+    std::vector<unsigned char> z = {0x78, 0x01};
+    size_t pos = 0;
+    do {
+        size_t len = raw.size() - pos < 65535 ? raw.size() - pos : 65535;
+        z.push_back(pos + len == raw.size() ? 1 : 0);
+        z.push_back(len & 0xff);
+        z.push_back(len >> 8);
+        z.push_back(~len & 0xff);
+        z.push_back((~len >> 8) & 0xff);
+        z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + len);
+        pos += len;
+    } while (pos < raw.size());
+    uint32_t a = 1, b = 0;
+    for (unsigned char c : raw) {
+        a = (a + c) % 65521;
+        b = (b + a) % 65521;
+    }
+    be32(z, (b << 16) | a);
+    std::vector<unsigned char> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    std::vector<unsigned char> ihdr;
+    be32(ihdr, (uint32_t)w);
+    be32(ihdr, (uint32_t)h);
+    ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});
+    chunk(out, "IHDR", ihdr);
+    chunk(out, "IDAT", z);
+    chunk(out, "IEND", {});
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) return false;
+    bool ok = fwrite(out.data(), 1, out.size(), f) == out.size();
+    return fclose(f) == 0 && ok;
+}
+
+}  // namespace mrpnn

@@ -1,0 +1,350 @@
+// Device building blocks of the MRPNN predictor path (gfx950).
+//
+// Compiled with -ffp-contract=off: every float expression rounds as written, and fused
+// multiply-adds appear only where written as fmaf().  The per-pixel functions here follow
+// the reference line by line (citations below) so the plain kernel reproduces the CPU
+// restatement up to transcendental-function ulps.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "philox.h"
+
+namespace mrp {
+
+// ---------------------------------------------------------------------------------------
+// float3 arithmetic with the reference's operator set (core/vector.cu:3-45)
+// ---------------------------------------------------------------------------------------
+struct v3 {
+    float x, y, z;
+};
+MRP_HD v3 mk3(float x, float y, float z) { return v3{x, y, z}; }
+MRP_HD v3 operator*(v3 a, float b) { return {a.x * b, a.y * b, a.z * b}; }
+MRP_HD v3 operator*(v3 a, v3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+MRP_HD v3 operator/(v3 a, float b) { return {a.x / b, a.y / b, a.z / b}; }
+MRP_HD v3 operator+(v3 a, float b) { return {a.x + b, a.y + b, a.z + b}; }
+MRP_HD v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+MRP_HD v3 operator-(v3 a, float b) { return {a.x - b, a.y - b, a.z - b}; }
+MRP_HD v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+MRP_HD v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
+MRP_HD float dot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+MRP_HD v3 cross3(v3 a, v3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+MRP_HD v3 normalize3(v3 v) {
+    float len = sqrtf(dot3(v, v));
+    return {v.x / len, v.y / len, v.z / len};
+}
+MRP_HD float length3(v3 a) { return sqrtf(dot3(a, a)); }
+MRP_HD float lerp_ref(float a, float b, float v) { return a * (1.0f - v) + b * v; }  // vector.cu:33
+MRP_HD v3 vmin3(v3 a, v3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+MRP_HD v3 vmax3(v3 a, v3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+MRP_HD v3 inv3(v3 a) {  // vector.cu:19
+    return {a.x == 0 ? 999999999.0f : 1.0f / a.x, a.y == 0 ? 999999999.0f : 1.0f / a.y,
+            a.z == 0 ? 999999999.0f : 1.0f / a.z};
+}
+
+// RayBoxOffset (vector.cu:46-63): entry distance into [-0.4999,0.4999]^3, -1 on a miss.
+MRP_HD float ray_box_offset(v3 p, v3 dir) {
+    dir = inv3(dir);
+    v3 bmax = {0.4999f, 0.4999f, 0.4999f};
+    v3 to_axil = (-p) * dir;
+    v3 a2f = bmax * dir;
+    v3 d0 = to_axil + a2f, d1 = to_axil - a2f;
+    v3 tmin = vmin3(d0, d1), tmax = vmax3(d0, d1);
+    float tmi = fmaxf(tmin.x, fmaxf(tmin.y, tmin.z));
+    float tma = fminf(tmax.x, fminf(tmax.y, tmax.z));
+    return tma >= tmi ? fmaxf(tmi, 0.0f) : -1.0f;
+}
+// RayBoxDistance (vector.cu:65-80): exit distance from [-0.5,0.5]^3.
+MRP_HD float ray_box_distance(v3 p, v3 dir) {
+    dir = inv3(dir);
+    v3 bmax = {0.5f, 0.5f, 0.5f};
+    v3 to_axil = (-p) * dir;
+    v3 a2f = bmax * dir;
+    v3 d0 = to_axil + a2f, d1 = to_axil - a2f;
+    v3 tmax = vmax3(d0, d1);
+    return fminf(tmax.x, fminf(tmax.y, tmax.z));
+}
+
+// HenyeyGreenstein * pi / (4 pi) convention: no 1/pi, x0.25 (vector.cu:210-214)
+MRP_HD float henyey_greenstein(float cs, float g) {
+    float g2 = g * g;
+    return ((1.0f - g2) / powf((1.0f + g2 - 2.0f * g * cs), 1.5f)) * 0.25f;
+}
+MRP_HD float sample_hg_cos(float s, float g) {  // vector.cu:216-223
+    if (fabsf(g) < 0.0001f) return s * 2.0f - 1.0f;
+    float g2 = g * g;
+    float t0 = (1.0f - g2) / (1.0f - g + 2.0f * g * s);
+    return (1.0f + g2 - t0 * t0) / (2.0f * g);
+}
+MRP_HD v3 sample_hg(float e0, float e1, v3 v, float g) {  // vector.cu:225-242
+    float ct = sample_hg_cos(e0, g);
+    float phi = (float)(2.0 * 3.14159265359 * (double)e1);
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    v3 v2;
+    if (fabsf(v.z) > 0.70710678118654757f) v2 = normalize3(cross3(v, mk3(0, 1, 0)));
+    else v2 = normalize3(cross3(v, mk3(0, 0, 1)));
+    v3 v3_ = cross3(v2, v);
+    return (v3_ * (st * cosf(phi))) + (v2 * (st * sinf(phi))) + (v * ct);
+}
+MRP_HD v3 uniform_sample_sphere(float ex, float ey) {  // vector.cu:195-208
+    float phi = 2.0f * 3.14159265359f * ex;
+    float ct = 1.0f - 2.0f * ey;
+    float st = sqrtf(1.0f - ct * ct);
+    return {st * cosf(phi), st * sinf(phi), ct};
+}
+
+// ---------------------------------------------------------------------------------------
+// Resident volume: full-res grid, density mips, TR mips, HG LUT (all fp32, HBM)
+// ---------------------------------------------------------------------------------------
+struct DevVolume {
+    const float* grid;     // N^3, (x*N+y)*N+z
+    const float* mips[9];  // (256>>m)^3
+    const float* tr[8];    // (128>>m)^3
+    const float* lut;      // 512*512, [j*512+i]
+    int N;
+    float max_density;
+};
+
+// Stencil entry, host-precomputed from GetSamples23_ (vector.cu:304-405) and the
+// direction tables (vector.cu:82-177).
+struct StencilEntry {
+    float dx, dy, dz;  // direction in (XMain,YMain,ZMain) frame (type 5)
+    float offset;      // cube units
+    float lut_v;       // LUT v = atan(0.5*2^mip/256/offset) / (pi/3)  (radiancePredict.cu:458-464)
+    int mip;           // density mip
+    int type;          // 5 sphere/ball, 1 light ray
+    int local;         // 0 = centre point
+};
+constexpr int kPC = 192;
+constexpr int kNFEAT = 577;
+
+// Exact zero-border trilinear as the host Sample() (core/volume.cu:835-863); raw values.
+__device__ inline float fetch_b(const float* __restrict__ d, int res, int x, int y, int z) {
+    if (x < 0 || y < 0 || z < 0 || x >= res || y >= res || z >= res) return 0.0f;
+    return d[((size_t)x * res + y) * res + z];
+}
+__device__ inline float sample_border_exact(const float* __restrict__ d, int res, v3 uv) {
+    v3 pos = uv * (float)res - 0.5f;
+    int x = (int)floorf(pos.x), y = (int)floorf(pos.y), z = (int)floorf(pos.z);
+    v3 w = pos - mk3((float)x, (float)y, (float)z);
+    return lerp_ref(lerp_ref(lerp_ref(fetch_b(d, res, x, y, z), fetch_b(d, res, x, y, z + 1), w.z),
+                             lerp_ref(fetch_b(d, res, x, y + 1, z), fetch_b(d, res, x, y + 1, z + 1), w.z), w.y),
+                    lerp_ref(lerp_ref(fetch_b(d, res, x + 1, y, z), fetch_b(d, res, x + 1, y, z + 1), w.z),
+                             lerp_ref(fetch_b(d, res, x + 1, y + 1, z), fetch_b(d, res, x + 1, y + 1, z + 1), w.z),
+                             w.y),
+                    w.x);
+}
+// Clamp addressing (SampleClamp, volume.cu:839-844,864-882): TR mips.
+__device__ inline float fetch_c(const float* __restrict__ d, int res, int x, int y, int z) {
+    x = x < 0 ? 0 : (x > res - 1 ? res - 1 : x);
+    y = y < 0 ? 0 : (y > res - 1 ? res - 1 : y);
+    z = z < 0 ? 0 : (z > res - 1 ? res - 1 : z);
+    return fmaxf(0.0f, d[((size_t)x * res + y) * res + z]);
+}
+__device__ inline float sample_clamp_exact(const float* __restrict__ d, int res, v3 uv) {
+    v3 pos = uv * (float)res - 0.5f;
+    int x = (int)floorf(pos.x), y = (int)floorf(pos.y), z = (int)floorf(pos.z);
+    v3 w = pos - mk3((float)x, (float)y, (float)z);
+    return lerp_ref(lerp_ref(lerp_ref(fetch_c(d, res, x, y, z), fetch_c(d, res, x, y, z + 1), w.z),
+                             lerp_ref(fetch_c(d, res, x, y + 1, z), fetch_c(d, res, x, y + 1, z + 1), w.z), w.y),
+                    lerp_ref(lerp_ref(fetch_c(d, res, x + 1, y, z), fetch_c(d, res, x + 1, y, z + 1), w.z),
+                             lerp_ref(fetch_c(d, res, x + 1, y + 1, z), fetch_c(d, res, x + 1, y + 1, z + 1), w.z),
+                             w.y),
+                    w.x);
+}
+// HG LUT: 512^2, normalised coordinates, linear filter, clamp (volume.cu:1413-1418).
+__device__ inline float sample_lut_exact(const float* __restrict__ lut, float u, float v) {
+    float px = u * 512.0f - 0.5f, py = v * 512.0f - 0.5f;
+    int x = (int)floorf(px), y = (int)floorf(py);
+    float wx = px - (float)x, wy = py - (float)y;
+    int x0 = min(max(x, 0), 511), x1 = min(max(x + 1, 0), 511);
+    int y0 = min(max(y, 0), 511), y1 = min(max(y + 1, 0), 511);
+    float a = lerp_ref(lut[y0 * 512 + x0], lut[y0 * 512 + x1], wx);
+    float b = lerp_ref(lut[y1 * 512 + x0], lut[y1 * 512 + x1], wx);
+    return lerp_ref(a, b, wy);
+}
+
+// Density() full-res border (render.cuh:34-38); MipDensityDynamic (renderResources.cu:32-57);
+// MipTrDynamic (:59-82); ShadowTerm_TRTex(...).x (:84-102).
+__device__ inline float density_full(const DevVolume& v, v3 p) { return sample_border_exact(v.grid, v.N, p + 0.5f); }
+__device__ inline float mip_density(const DevVolume& v, int mip, v3 p) {
+    return sample_border_exact(v.mips[mip], 256 >> mip, p + 0.5f);
+}
+__device__ inline float mip_tr(const DevVolume& v, int mip, v3 p) {
+    return sample_clamp_exact(v.tr[mip], 128 >> mip, p + 0.5f);
+}
+__device__ inline float shadow_term_tr(const DevVolume& v, v3 ori, v3 light, int mip) {
+    if (ori.x < -0.5f || ori.y < -0.5f || ori.z < -0.5f || ori.x > 0.5f || ori.y > 0.5f || ori.z > 0.5f) {
+        float off = ray_box_offset(ori, light);
+        if (off >= 0) return mip_tr(v, mip, ori + light * off);
+        return 1.0f;
+    }
+    return mip_tr(v, mip, ori);
+}
+
+// ---------------------------------------------------------------------------------------
+// First-hit march: DeterminateEstimatedHit, bool version (render.cuh:105-197), and
+// GetSamplePoint (render.cuh:436-462).  taps counts full-res density taps.
+// ---------------------------------------------------------------------------------------
+__device__ inline bool estimated_hit(const DevVolume& v, float alpha, v3 pos, v3 dir, float dis, v3* next, int* taps) {
+    float smax = v.max_density * alpha;
+    float t = 0.0f, last_non_empty_t = 0.0f, prob = 1.0f;
+    float t_step = 2.0f / (float)v.N;
+    float pass_rate = expf(-t_step * smax);
+    float inv_res = 1.0f / (float)v.N;
+    int loop = 0;
+    float last_dens = 0.0f;
+    while (loop++ < 10000) {
+        if (t > dis) {
+            if (last_non_empty_t != 0.0f) { t = last_non_empty_t; break; }
+            *next = mk3(0, 0, 0);
+            return false;
+        }
+        float d = density_full(v, pos + (dir * t));
+        ++*taps;
+        if (d > 0.0f) {
+            float S = d * alpha;
+            float sp = pass_rate + (1.0f - pass_rate) * (1.0f - S / smax);
+            float np = prob * sp;
+            if (np < 0.5f) {
+                float off = 1.0f - logf(0.5f / prob) / logf(sp);
+                t -= t_step * off;
+                break;
+            }
+            prob = np;
+            t += t_step;
+        } else {
+            if (last_dens > 0.0f) {
+                float S = last_dens * alpha;
+                float ts = t_step + d;
+                float pr = expf(-(ts * smax));
+                float sp = pr + (1.0f - pr) * (1.0f - S / smax);
+                float np = prob * sp;
+                if (np < 0.5f) {
+                    float off = 1.0f - logf(0.5f / prob) / logf(sp);
+                    t -= ts * off;
+                    break;
+                }
+                prob = np;
+                last_non_empty_t = t + d;
+            }
+            t += fmaxf(-d * 1.0f, inv_res);
+        }
+        last_dens = d;
+    }
+    *next = (dir * t) + pos;
+    return true;
+}
+
+// Per-pixel state after ray generation, march and basis (volume.cu:198-216,
+// render.cuh:808-823).  Pixel jitter and basis draws use Philox streams.
+struct ViewDev {
+    v3 ori, up, right, light_dir, light_color, scatter_rate;
+    float alpha, g, tan_half_fov, jitter;
+    int width, height;
+    uint32_t seed, frame, view_id;
+    int volume_index;
+};
+struct PixelState {
+    v3 pos, X, Y, Z, LX, L;
+    float dist;
+    int active;
+    int taps;
+};
+
+__device__ inline v3 pixel_ray(const ViewDev& vw, int row, int col, uint32_t pix) {
+    float xu, xv;
+    if (vw.jitter < 0.0f) {
+        PhiloxStream rs(vw.seed, vw.frame, pix, RNG_JITTER, vw.view_id);
+        xu = rs.next();
+        xv = rs.next();
+    } else {
+        xu = xv = vw.jitter;
+    }
+    float u = 1.0f - ((float)col + xu) / (float)vw.width;
+    float vv = 1.0f - ((float)row + xv) / (float)vw.height;
+    float sy = vw.tan_half_fov;
+    float sx = vw.tan_half_fov * ((float)vw.width / (float)vw.height);
+    v3 fwd = normalize3(-vw.ori);
+    v3 dir = fwd + (vw.right * ((u * 2.0f - 1.0f) * sx)) + (vw.up * ((vv * 2.0f - 1.0f) * sy));
+    return normalize3(dir);  // volume.cu:216
+}
+
+// GetMatrixFromNormal(seed, v1, 0.1) (render.cuh:694-708); at most 64 attempts.
+__device__ inline void basis_from_normal(v3 v1, const ViewDev& vw, uint32_t pix, v3* X, v3* Y, v3* Z) {
+    v1 = normalize3(v1);
+    PhiloxStream rs(vw.seed, vw.frame, pix, RNG_BASIS_MAIN, vw.view_id);
+    v3 v2 = mk3(0, 1, 0), v3_ = mk3(0, 0, 1);
+    for (int att = 0; att < 64; ++att) {
+        float e0 = rs.next() * 0.1f;
+        float e1 = rs.next() * 0.1f;
+        v3 r = uniform_sample_sphere(e0, e1);
+        if ((double)fabsf(dot3(r, v1)) < 0.00001) continue;  // double compare as the reference
+        v2 = normalize3(cross3(v1, r));
+        v3_ = cross3(v2, v1);
+        break;
+    }
+    *X = v1;
+    *Y = v2;
+    *Z = v3_;
+}
+
+__device__ inline PixelState pixel_state(const DevVolume& vol, const ViewDev& vw, int row, int col) {
+    PixelState st;
+    uint32_t pix = (uint32_t)(row * vw.width + col);
+    v3 dir = pixel_ray(vw, row, col, pix);
+    dir = normalize3(dir);                  // render.cuh:811
+    v3 light = normalize3(vw.light_dir);    // render.cuh:812
+    // GetSamplePoint (render.cuh:436-462)
+    v3 mdir = normalize3(dir);
+    st.active = 0;
+    st.taps = 0;
+    st.pos = mk3(0, 0, 0);
+    float dis = ray_box_offset(vw.ori, mdir);
+    if (dis >= 0.0f) {
+        v3 sp = vw.ori + mdir * dis;
+        float maxd = ray_box_distance(sp, mdir);
+        v3 p0;
+        if (estimated_hit(vol, vw.alpha, sp, mdir, maxd, &p0, &st.taps)) {
+            st.active = 1;
+            st.pos = p0;
+        }
+    }
+    basis_from_normal(dir, vw, pix, &st.X, &st.Y, &st.Z);
+    st.LX = normalize3(light);
+    st.L = light;
+    st.dist = length3(vw.ori - st.pos);
+    return st;
+}
+
+// One stencil point of SetupInputData (radiancePredict.cu:433-471): returns
+// (X_Val, X_Val_Hg, X_Val_Sub) of point i.
+__device__ inline v3 gather_point(const DevVolume& vol, const StencilEntry& s, int i, const PixelState& st, float g,
+                                  float alpha) {
+    v3 p;
+    if (s.type >= 4) p = st.pos + ((st.X * s.dx + st.Y * s.dy + st.Z * s.dz) * s.offset);
+    else p = st.pos + st.L * s.offset;
+    float xv = logf(1.0f + alpha * mip_density(vol, s.mip, p) / 64.0f);
+    float xh;
+    if (s.local == 0) {
+        xh = logf(henyey_greenstein(dot3(st.X, st.L), g) + 1.0f);
+    } else {
+        v3 ms = normalize3(p - st.pos);
+        float c1 = dot3(st.X, ms), c2 = dot3(st.LX, ms);
+        float u = c1 * 0.5f + 0.5f, u2 = c2 * 0.5f + 0.5f;
+        float h0 = sample_lut_exact(vol.lut, u, s.lut_v);
+        float h1 = sample_lut_exact(vol.lut, u2, s.lut_v);
+        xh = logf(h0 * h1 + 1.0f);
+    }
+    float xs = shadow_term_tr(vol, p, st.LX, s.mip - 1 > 0 ? s.mip - 1 : 0);
+    return mk3(xv, xh, xs);
+}
+
+__device__ inline float gamma_of(const PixelState& st) {  // radiancePredict.cu:482-484
+    float d = dot3(st.X, st.LX);
+    d = fminf(1.0f, fmaxf(-1.0f, d));
+    return acosf(d);
+}
+
+}  // namespace mrp

@@ -1,0 +1,41 @@
+// Shared types of the batched predictor (march + compaction, active-list MLP kernels).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dev_common.hpp"
+
+namespace mrp {
+
+// One view of a batch as the device sees it.
+struct ViewSlot {
+    ViewDev vw;
+    float4* out;  // width*height (rgb, distance)
+    int vol;      // index into the batch's DevVolume table
+    int tile0;    // first 8x8 tile of this view in the flattened tile grid
+    int tiles_x;  // tiles per row
+    int pad;
+};
+
+// One active pixel after the march: 64 bytes.
+struct ActiveRec {
+    int view;
+    int pixel;
+    float pos[3];
+    float X[3];
+    float Y[3];
+    float Z[3];
+    float dist;
+    int pad;
+};
+static_assert(sizeof(ActiveRec) == 64, "ActiveRec must be 64 bytes");
+
+hipError_t launch_march_compact(const DevVolume* vols, const ViewSlot* slots, int n_views, int n_tiles,
+                                ActiveRec* recs, unsigned* counter, unsigned long long* taps, hipStream_t s);
+hipError_t launch_predict_active_fp32(const DevVolume* vols, const ViewSlot* slots, const StencilEntry* stencil,
+                                      const float* W, const ActiveRec* recs, const unsigned* counter, int capacity,
+                                      hipStream_t s);
+hipError_t launch_reset_counters(unsigned* counter, unsigned long long* taps, hipStream_t s);
+
+// mlp_mfma.hip: packed MFMA weights (see pack_mfma_weights) and the kernels.
+struct MfmaNet;  // device pointers, defined in mlp_mfma.h
+}  // namespace mrp

@@ -1,0 +1,26 @@
+// Host-side launchers of the HIP kernels (one translation unit per kernel family).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dev_common.hpp"
+
+namespace mrp {
+
+// volume_kernels.hip
+hipError_t launch_fbm(float* out, int N, int octaves, float base_freq, const int* perm, uint32_t* minmax,
+                      hipStream_t s);
+hipError_t launch_box2(const float* src, float* dst, int res, hipStream_t s);
+hipError_t launch_resample(const float* src, int sres, float* dst, int res, hipStream_t s);
+hipError_t launch_tr_bake(const float* const mips[9], float* const tr[8], const float alpha_lvl[8], v3 light,
+                          hipStream_t s);
+hipError_t launch_hg_lut(float* lut, float g, uint32_t seed, hipStream_t s);
+
+// predict_plain.hip
+hipError_t launch_predict_plain(const DevVolume& vol, const StencilEntry* stencil, const float* W, const ViewDev& vw,
+                                float* out4, float lerp_rate, int first_frame, hipStream_t s);
+hipError_t launch_gather_features(const DevVolume& vol, const StencilEntry* stencil, const ViewDev& vw, float* feats,
+                                  int* active, float* pos3, int* taps, hipStream_t s);
+hipError_t launch_predict_features_fp32(const float* feats, int n, const float* W, v3 sr, float g, float* rgb,
+                                        hipStream_t s);
+
+}  // namespace mrp

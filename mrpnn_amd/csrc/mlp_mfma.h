@@ -1,0 +1,97 @@
+// MFMA form of the MRPNN network (BF16 and FP8 e4m3, FP32 accumulation) on gfx950.
+//
+// Tile = 16 pixels per wavefront; lane = c + 16*q with c the pixel column and q in 0..3.
+// Every dense layer is v_mfma_f32_16x16x32_{bf16,fp8} with weights as A (16 output rows per
+// M-tile) and activations as B (16 pixels).  Activation vectors live in registers in the
+// D layout of the producing layer: M-tile t contributes rows 16t+4q+i (i = 0..3) to lane
+// (c, q).  The K slots of a 32-wide chunk k are ordered so that slot 8q+i <-> input row
+// 32k+4q+i and slot 8q+4+i <-> row 32k+16+4q+i: exactly the rows a lane already holds, so
+// D feeds the next layer's B without any cross-lane movement.  Layers with K = 16 (or 8)
+// use one chunk whose slots 8q+4..7 are zero.  The host packs each weight fragment (64
+// lanes x 8 values) in that order (pack in mlp_mfma.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fused.h"
+
+namespace mrp {
+
+enum MfmaLayerId {
+    ML_TRUNK = 0,         // + 9*branch + k, k = 0..8 : L01 L11 L21 L31 L41 L51 L61 L71 L81
+    ML_DI = 27,           // + 4*branch + k, k = 0..3 : LD01 LD11 LD21 LD31
+    ML_LD41A_VAL = 39,    // LD41 into rows 0..7 of tile A
+    ML_LD41A_TR = 40,     // LD_Tr41 into rows 8..15 of tile A
+    ML_LD41B_HG = 41,     // LD_Hg41 into rows 0..7 of tile B
+    ML_FIN2 = 42,         // LSEFin2 (6x16), no bias
+    ML_LC0 = 43,
+    ML_LC1 = 44,
+    ML_LC2 = 45,
+    ML_LX = 46,
+    ML_LXK = 47,          // + k, k = 0..11
+    ML_LX12 = 59,
+    ML_COUNT = 60
+};
+
+// Compile-time shape of every layer: rows, inputs, first destination row inside the padded
+// 16-row M-tiles, and whether K comes in 32-wide chunks (else K <= 16, one chunk whose
+// slots 8q+4..7 are zero).
+struct MfmaShape {
+    int M, K, roff, k32;
+};
+constexpr MfmaShape mfma_shape(int id) {
+    if (id < ML_DI) {
+        const int k = (id - ML_TRUNK) % 9;
+        const int n = k < 2 ? 8 : (k < 5 ? 16 : 32);
+        return {n, n, 0, n == 32};
+    }
+    if (id < ML_LD41A_VAL) return {8, 8, 0, 0};
+    if (id == ML_LD41A_VAL) return {8, 8, 0, 0};
+    if (id == ML_LD41A_TR) return {8, 8, 8, 0};
+    if (id == ML_LD41B_HG) return {8, 8, 0, 0};
+    if (id == ML_FIN2) return {6, 16, 0, 0};
+    if (id == ML_LC0) return {128, 128, 0, 1};
+    if (id == ML_LC1) return {64, 128, 0, 1};
+    if (id == ML_LC2) return {32, 64, 0, 1};
+    if (id == ML_LX) return {16, 32, 0, 1};
+    if (id < ML_LX12) return {16, 16, 0, 0};
+    return {1, 16, 0, 0};
+}
+constexpr int mfma_nm(int id) { return (mfma_shape(id).roff + mfma_shape(id).M + 15) / 16; }
+constexpr int mfma_nk(int id) { return mfma_shape(id).k32 ? mfma_shape(id).K / 32 : 1; }
+constexpr bool mfma_has_bias(int id) { return id != ML_FIN2; }
+// first fragment (64 lanes x 8 values) and bias offset (16 floats per M-tile) of a layer
+constexpr int mfma_frag(int id) {
+    int f = 0;
+    for (int i = 0; i < id; ++i) f += mfma_nm(i) * mfma_nk(i);
+    return f;
+}
+constexpr int mfma_bias(int id) {
+    int b = 0;
+    for (int i = 0; i < id; ++i)
+        if (mfma_has_bias(i)) b += 16 * mfma_nm(i);
+    return b;
+}
+constexpr int kMfmaFrags = mfma_frag(ML_COUNT);
+constexpr int kMfmaBias = mfma_bias(ML_COUNT);
+
+struct MfmaNet {
+    const uint4* bf16 = nullptr;  // fragments, 16 bytes per lane
+    const uint2* fp8 = nullptr;   // fragments, 8 bytes per lane
+    const float* bias = nullptr;
+    const int* wexp = nullptr;    // FP8: layer weights stored as fp8(W * 2^wexp[id])
+};
+
+int mfma_net_create(const float* weights_fp32, MfmaNet* out);
+void mfma_net_destroy(MfmaNet* net);
+
+// Host encoders used by the packer (exposed for tests through the C ABI of the library).
+uint16_t host_f32_to_bf16(float f);
+uint8_t host_f32_to_e4m3(float f);
+
+hipError_t launch_mlp_mfma_active(int precision, const MfmaNet& net, const DevVolume* vols, const ViewSlot* slots,
+                                  const StencilEntry* stencil, const float* W, const ActiveRec* recs,
+                                  const unsigned* counter, int grid, hipStream_t s);
+hipError_t launch_mlp_mfma_features(int precision, const MfmaNet& net, const float* W, const float* feats, int n,
+                                    const float sr[3], float g, float* rgb, hipStream_t s);
+
+}  // namespace mrp

@@ -1,0 +1,835 @@
+// MRPNN network on MFMA (gfx950): BF16 and FP8 e4m3, FP32 accumulation.  Layout and
+// K-slot order are described in mlp_mfma.h.  The dataflow is RadiancePredictInternal
+// (core/radiancePredict.cu:301-422):
+//   * 12 SE/SERES blocks (3 branches each) as 16x16x32 MFMAs; gate MLPs (8->8->3), pooling
+//     statistics and residual/replace bookkeeping (ADD3/REP3) in FP32 on the VALU;
+//   * LSEFin1 (75->16) accumulated incrementally in FP32 as each block's pooled statistics
+//     appear, LSEFin2 and the whole tail (128->128->64->32->16, 6 residual pairs, 16->1) on
+//     MFMA.
+// Feature providers: the fused kernel gathers the 192 stencil points of each pixel straight
+// into the B-operand layout (lane (c,q) computes exactly the points it feeds: no LDS
+// transposition); the features kernel reads 577-float rows from HBM.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/mrp_c_api.h"
+#include "capi_internal.h"
+#include "dev_common.hpp"
+#include "mlp_mfma.h"
+#include "net_layout.h"
+
+namespace mrp {
+
+// ------------------------------------------------------------------------------------------
+// Host: fragment packing
+// ------------------------------------------------------------------------------------------
+uint16_t host_f32_to_bf16(float f) {  // round to nearest even (finite inputs)
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    return (uint16_t)u;
+}
+uint8_t host_f32_to_e4m3(float f) {  // OCP e4m3fn, round to nearest even, saturating at 448
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    uint8_t sign = (u >> 24) & 0x80;
+    double a = fabs((double)f);
+    if (a == 0.0) return sign;
+    if (a >= 448.0) return sign | 0x7e;
+    int e;
+    double m = frexp(a, &e);  // a = m * 2^e, m in [0.5,1)
+    int E = e - 1;            // a = (2m) * 2^E, 2m in [1,2)
+    if (E < -6) {             // subnormal: k * 2^-9
+        double k = a * 512.0;
+        double r = nearbyint(k);  // ties to even (default rounding mode)
+        if (r >= 8.0) return sign | 0x08;
+        return sign | (uint8_t)r;
+    }
+    double mant = (2.0 * m - 1.0) * 8.0;
+    double r = nearbyint(mant);
+    if (r >= 8.0) {
+        r = 0.0;
+        ++E;
+    }
+    if (E > 8 || (E == 8 && r > 6.0)) return sign | 0x7e;
+    return sign | (uint8_t)(((E + 7) << 3) | (int)r);
+}
+// largest e with amax * 2^e <= 448 (exact, from the binary exponent)
+__host__ __device__ inline int fp8_exp_for(float amax) {
+    if (!(amax > 0.0f)) return 0;
+    int x;
+    float f = frexpf(amax, &x);  // amax = f * 2^x, f in [0.5,1); 448 = 0.875 * 2^9
+    return f <= 0.875f ? 9 - x : 8 - x;
+}
+
+namespace {
+struct Spec {
+    int w, b;       // offsets in the fp32 weight blob (b < 0: no bias)
+    int M, K;       // rows, inputs of the source matrix
+    int roff;       // first destination row inside the padded M-tiles
+    bool k32;       // true: K in chunks of 32; false: K <= 16 in one chunk, slots 4..7 zero
+};
+void layer_specs(Spec* s) {
+    using namespace net;
+    const int tw[3][9] = {{L01W, L11W, L21W, L31W, L41W, L51W, L61W, L71W, L81W},
+                          {L_Tr01W, L_Tr11W, L_Tr21W, L_Tr31W, L_Tr41W, L_Tr51W, L_Tr61W, L_Tr71W, L_Tr81W},
+                          {L_Hg01W, L_Hg11W, L_Hg21W, L_Hg31W, L_Hg41W, L_Hg51W, L_Hg61W, L_Hg71W, L_Hg81W}};
+    const int tb[3][9] = {{L01B, L11B, L21B, L31B, L41B, L51B, L61B, L71B, L81B},
+                          {L_Tr01B, L_Tr11B, L_Tr21B, L_Tr31B, L_Tr41B, L_Tr51B, L_Tr61B, L_Tr71B, L_Tr81B},
+                          {L_Hg01B, L_Hg11B, L_Hg21B, L_Hg31B, L_Hg41B, L_Hg51B, L_Hg61B, L_Hg71B, L_Hg81B}};
+    const int tn[9] = {8, 8, 16, 16, 16, 32, 32, 32, 32};
+    for (int b = 0; b < 3; ++b)
+        for (int k = 0; k < 9; ++k) s[ML_TRUNK + 9 * b + k] = {tw[b][k], tb[b][k], tn[k], tn[k], 0, tn[k] == 32};
+    const int dw[3][4] = {{LD01W, LD11W, LD21W, LD31W}, {LD_Tr01W, LD_Tr11W, LD_Tr21W, LD_Tr31W},
+                          {LD_Hg01W, LD_Hg11W, LD_Hg21W, LD_Hg31W}};
+    const int db[3][4] = {{LD01B, LD11B, LD21B, LD31B}, {LD_Tr01B, LD_Tr11B, LD_Tr21B, LD_Tr31B},
+                          {LD_Hg01B, LD_Hg11B, LD_Hg21B, LD_Hg31B}};
+    for (int b = 0; b < 3; ++b)
+        for (int k = 0; k < 4; ++k) s[ML_DI + 4 * b + k] = {dw[b][k], db[b][k], 8, 8, 0, false};
+    s[ML_LD41A_VAL] = {LD41W, LD41B, 8, 8, 0, false};
+    s[ML_LD41A_TR] = {LD_Tr41W, LD_Tr41B, 8, 8, 8, false};
+    s[ML_LD41B_HG] = {LD_Hg41W, LD_Hg41B, 8, 8, 0, false};
+    s[ML_FIN2] = {LSEFin2W, -1, 6, 16, 0, false};
+    s[ML_LC0] = {LC0W, LC0B, 128, 128, 0, true};
+    s[ML_LC1] = {LC1W, LC1B, 64, 128, 0, true};
+    s[ML_LC2] = {LC2W, LC2B, 32, 64, 0, true};
+    s[ML_LX] = {LXW, LXB, 16, 32, 0, true};
+    const int xw[12] = {LX0W, LX1W, LX2W, LX3W, LX4W, LX5W, LX6W, LX7W, LX8W, LX9W, LX10W, LX11W};
+    const int xb[12] = {LX0B, LX1B, LX2B, LX3B, LX4B, LX5B, LX6B, LX7B, LX8B, LX9B, LX10B, LX11B};
+    for (int k = 0; k < 12; ++k) s[ML_LXK + k] = {xw[k], xb[k], 16, 16, 0, false};
+    s[ML_LX12] = {LX12W, LX12B, 1, 16, 0, false};
+}
+}  // namespace
+
+int mfma_net_create(const float* W, MfmaNet* out) {
+    Spec spec[ML_COUNT];
+    layer_specs(spec);
+    std::vector<uint16_t> bf;  // 8 per lane per fragment
+    std::vector<uint8_t> f8;
+    std::vector<float> bias;
+    std::vector<int> wexp(ML_COUNT, 0);
+    int frag = 0;
+    for (int id = 0; id < ML_COUNT; ++id) {
+        const Spec& s = spec[id];
+        const MfmaShape sh = mfma_shape(id);
+        if (sh.M != s.M || sh.K != s.K || sh.roff != s.roff || (sh.k32 != 0) != s.k32 || frag != mfma_frag(id) ||
+            (int)bias.size() != mfma_bias(id) || mfma_has_bias(id) != (s.b >= 0))
+            return mrp_fail(MRP_ERR_STATE, "MFMA layer table mismatch at layer %d", id);
+        const int nm = mfma_nm(id), nk = mfma_nk(id);
+        float amax = 0.0f;
+        for (int i = 0; i < s.M * s.K; ++i) amax = fmaxf(amax, fabsf(W[s.w + i]));
+        wexp[id] = fp8_exp_for(amax);
+        const float wsc = ldexpf(1.0f, wexp[id]);
+        for (int mt = 0; mt < nm; ++mt)
+            for (int kc = 0; kc < nk; ++kc) {
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int q = lane >> 4, row = 16 * mt + (lane & 15) - s.roff;
+                    for (int slot = 0; slot < 8; ++slot) {
+                        int k;
+                        if (s.k32) k = 32 * kc + (slot < 4 ? 4 * q + slot : 16 + 4 * q + slot - 4);
+                        else k = slot < 4 ? 4 * q + slot : -1;
+                        float v = (row >= 0 && row < s.M && k >= 0 && k < s.K) ? W[s.w + row * s.K + k] : 0.0f;
+                        bf.push_back(host_f32_to_bf16(v));
+                        f8.push_back(host_f32_to_e4m3(v * wsc));
+                    }
+                }
+                ++frag;
+            }
+        if (s.b >= 0)
+            for (int r = 0; r < 16 * nm; ++r) {
+                int src = r - s.roff;
+                bias.push_back(src >= 0 && src < s.M ? W[s.b + src] : 0.0f);
+            }
+    }
+    void *dbf = nullptr, *df8 = nullptr, *db = nullptr, *dw = nullptr;
+    if (hipMalloc(&dbf, bf.size() * 2) != hipSuccess || hipMalloc(&df8, f8.size()) != hipSuccess ||
+        hipMalloc(&db, bias.size() * 4) != hipSuccess || hipMalloc(&dw, wexp.size() * 4) != hipSuccess)
+        return mrp_fail(MRP_ERR_NOMEM, "MFMA weight allocation failed");
+    if (hipMemcpy(dbf, bf.data(), bf.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(df8, f8.data(), f8.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db, bias.data(), bias.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dw, wexp.data(), wexp.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return mrp_fail(MRP_ERR_HIP, "MFMA weight upload failed");
+    out->bf16 = (const uint4*)dbf;
+    out->fp8 = (const uint2*)df8;
+    out->bias = (const float*)db;
+    out->wexp = (const int*)dw;
+    return MRP_OK;
+}
+
+void mfma_net_destroy(MfmaNet* net) {
+    hipFree((void*)net->bf16);
+    hipFree((void*)net->fp8);
+    hipFree((void*)net->bias);
+    hipFree((void*)net->wexp);
+    net->wexp = nullptr;
+    net->bf16 = nullptr;
+    net->fp8 = nullptr;
+    net->bias = nullptr;
+}
+
+// ------------------------------------------------------------------------------------------
+// Device
+// ------------------------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline uint32_t pack_bf16x2(float a, float b) {
+    uint32_t ua = __float_as_uint(a), ub = __float_as_uint(b);
+    ua = (ua + 0x7fffu + ((ua >> 16) & 1u)) >> 16;
+    ub = (ub + 0x7fffu + ((ub >> 16) & 1u)) >> 16;
+    return ua | (ub << 16);
+}
+__device__ inline uint32_t pack_fp8x4(float a, float b, float c, float d) {
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+    return (uint32_t)w;
+}
+
+// out[NM][4] = act(W_ID * in + b): in[NK][8] in the K-slot order of mlp_mfma.h.  One M-tile
+// at a time behind a scheduling barrier, so at most NK weight fragments are in flight.
+template <int PREC, int ID>
+__device__ inline void dense(const MfmaNet& net, const float (&in)[mfma_nk(ID)][8], float (&out)[mfma_nm(ID)][4],
+                             int lane, bool relu) {
+    constexpr int NM = mfma_nm(ID), NK = mfma_nk(ID), FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
+    const int q = lane >> 4;
+    if constexpr (PREC == MRP_BF16) {
+        u32x4_t b[NK];
+#pragma unroll
+        for (int kc = 0; kc < NK; ++kc)
+            b[kc] = u32x4_t{pack_bf16x2(in[kc][0], in[kc][1]), pack_bf16x2(in[kc][2], in[kc][3]),
+                            pack_bf16x2(in[kc][4], in[kc][5]), pack_bf16x2(in[kc][6], in[kc][7])};
+        const u32x4_t* wf = (const u32x4_t*)net.bf16 + (size_t)FRAG * 64 + lane;
+#pragma unroll
+        for (int mt = 0; mt < NM; ++mt) {
+            f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kc = 0; kc < NK; ++kc) {
+                u32x4_t a = wf[(mt * NK + kc) * 64];
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                              __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
+            }
+            float4 bb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if constexpr (mfma_has_bias(ID)) bb = *(const float4*)(net.bias + BIAS + 16 * mt + 4 * q);
+            const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[i] + bv[i];
+                out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+        float m = 0.0f;
+#pragma unroll
+        for (int kc = 0; kc < NK; ++kc)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) m = fmaxf(m, fabsf(in[kc][s]));
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        const int ea = fp8_exp_for(m);
+        const float sc = ldexpf(1.0f, ea);
+        long b[NK];
+#pragma unroll
+        for (int kc = 0; kc < NK; ++kc) {
+            uint32_t lo = pack_fp8x4(in[kc][0] * sc, in[kc][1] * sc, in[kc][2] * sc, in[kc][3] * sc);
+            uint32_t hi = pack_fp8x4(in[kc][4] * sc, in[kc][5] * sc, in[kc][6] * sc, in[kc][7] * sc);
+            b[kc] = (long)(((uint64_t)hi << 32) | lo);
+        }
+        const float ds = ldexpf(1.0f, -(ea + net.wexp[ID]));
+        const uint64_t* wf = (const uint64_t*)net.fp8 + (size_t)FRAG * 64 + lane;
+#pragma unroll
+        for (int mt = 0; mt < NM; ++mt) {
+            f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kc = 0; kc < NK; ++kc)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)wf[(mt * NK + kc) * 64], b[kc], acc, 0, 0, 0);
+            float4 bb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if constexpr (mfma_has_bias(ID)) bb = *(const float4*)(net.bias + BIAS + 16 * mt + 4 * q);
+            const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = fmaf(acc[i], ds, bv[i]);
+                out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// Makes a pointer opaque to the optimiser once per tile iteration: the network's weights are
+// loop-invariant in the persistent tile loop, and without this the compiler hoists all of
+// their loads (hundreds of registers) above the loop and spills.
+template <class T>
+__device__ inline T* opaque(T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+template <class T>
+__device__ inline T* opaque_v(T* p) {
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
+__device__ inline float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+// Per-pixel lane state shared by the tile routine.
+struct LaneCtx {
+    int lane, c, q;
+    bool ok;        // this column holds a pixel
+    float g, gamma;
+    v3 sr;          // scatterrate passed to the network (scatter_rate / 1.001)
+};
+
+// Pooled statistics and SE gate of one block (radiancePredict.cu:240-266): mean/max of the
+// raw slice of each branch, gate MLP SEW0 (8x8, ReLU) and SEW1 (3x8, sigmoid); the six
+// pooled values are folded into the LSEFin1 accumulators (rows 4q+i of this lane).
+__device__ inline void block_gate(const float* __restrict__ W, const float* __restrict__ sW1, const LaneCtx& lc, int pid,
+                                  int sew0, int sew1, const float (&F)[3][8], int nv, bool lane_has, int n, float (&s)[3],
+                                  float (&hacc)[4]) {
+    float sum[3], mx[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        float a = 0.0f, m = -10000.0f;
+        if (lane_has) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (i < nv) {
+                    a += F[b][i];
+                    m = fmaxf(m, F[b][i]);
+                }
+        }
+        a += __shfl_xor(a, 16);
+        m = fmaxf(m, __shfl_xor(m, 16));
+        a += __shfl_xor(a, 32);
+        m = fmaxf(m, __shfl_xor(m, 32));
+        sum[b] = a / (float)n;
+        mx[b] = m;
+    }
+    const float t[8] = {sum[0], sum[1], sum[2], mx[0], mx[1], mx[2], lc.g, lc.gamma};
+    float h[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc = fmaf(t[j], W[sew0 + j + 8 * i], acc);
+        h[i] = fmaxf(acc, 0.0f);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc = fmaf(h[j], W[sew1 + j + 8 * i], acc);
+        s[i] = sigmoidf_(acc);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float* wr = sW1 + (4 * lc.q + i) * 75;
+        float a = hacc[i];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) a = fmaf(wr[pid + b], sum[b], a);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) a = fmaf(wr[36 + pid + b], mx[b], a);
+        hacc[i] = a;
+    }
+}
+
+// Feature providers fill F[b][slot] for points p0+4q+i (slots 0..3) and, when wide,
+// p0+16+4q+i (slots 4..7); b = 0 X_Val, 1 X_Val_Sub, 2 X_Val_Hg.
+struct GatherSrc {
+    const DevVolume* vol;
+    const StencilEntry* st;  // LDS copy
+    PixelState ps;
+    float g, alpha;
+    __device__ void get(const LaneCtx& lc, int p0, bool wide, float (&F)[3][8]) const {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !wide) break;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = p0 + 16 * h + 4 * lc.q + i;
+                v3 f = lc.ok ? gather_point(*vol, st[k], k, ps, g, alpha) : mk3(0.0f, 0.0f, 0.0f);
+                F[0][4 * h + i] = f.x;
+                F[2][4 * h + i] = f.y;
+                F[1][4 * h + i] = f.z;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if (!wide)
+#pragma unroll
+            for (int i = 4; i < 8; ++i) F[0][i] = F[1][i] = F[2][i] = 0.0f;
+    }
+};
+struct RowSrc {
+    const float* row;  // 577 floats [X_Val | X_Val_Hg | X_Val_Sub | gamma]
+    __device__ void get(const LaneCtx& lc, int p0, bool wide, float (&F)[3][8]) const {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !wide) break;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = p0 + 16 * h + 4 * lc.q + i;
+                F[0][4 * h + i] = lc.ok ? row[k] : 0.0f;
+                F[2][4 * h + i] = lc.ok ? row[kPC + k] : 0.0f;
+                F[1][4 * h + i] = lc.ok ? row[2 * kPC + k] : 0.0f;
+            }
+        }
+        if (!wide)
+#pragma unroll
+            for (int i = 4; i < 8; ++i) F[0][i] = F[1][i] = F[2][i] = 0.0f;
+    }
+};
+
+// One SE/SERES block for all three branches: in = F*s (+ R), out = ReLU(W in + b).
+// Layer ids: ID0 + STRIDE*branch.
+template <int PREC, int ID0, int STRIDE>
+__device__ inline void block_dense(const MfmaNet& net, const LaneCtx& lc, const float (&F)[3][8], const float (&s)[3],
+                                   const float (&R)[3][8], bool has_res, bool lane_has, float (&out)[3][8]) {
+    constexpr int NM = mfma_nm(ID0);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        float in[1][8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            float v = F[b][i] * s[b];
+            if (has_res) v = v + R[b][i];
+            in[0][i] = (lane_has && (NM == 2 || i < 4)) ? v : 0.0f;
+        }
+        float o[NM][4];
+        if (b == 0) dense<PREC, ID0>(net, in, o, lc.lane, true);
+        else if (b == 1) dense<PREC, ID0 + STRIDE>(net, in, o, lc.lane, true);
+        else dense<PREC, ID0 + 2 * STRIDE>(net, in, o, lc.lane, true);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            out[b][i] = o[0][i];
+            out[b][4 + i] = NM == 2 ? o[NM - 1][i] : 0.0f;
+        }
+    }
+}
+
+// Gate statistics + SE/SERES of a 16- or 32-wide trunk block with ADD3 (or REP3 for block 5).
+template <int PREC, int K>  // K = 2..7: blocks 3..8 of the main stack
+__device__ inline void trunk_block(const MfmaNet& net, const float* __restrict__ W, const float* __restrict__ sW1,
+                                   const LaneCtx& lc, const float (&F)[3][8], float (&R)[3][8], float (&hacc)[4]) {
+    using namespace net;
+    constexpr int sew0[8] = {LSE01W, LSE11W, LSE21W, LSE31W, LSE41W, LSE51W, LSE61W, LSE71W};
+    constexpr int sew1[8] = {LSE02W, LSE12W, LSE22W, LSE32W, LSE42W, LSE52W, LSE62W, LSE72W};
+    constexpr bool wide = K >= 5;
+    float s[3], O[3][8];
+    block_gate(W, sW1, lc, 3 * K, sew0[K], sew1[K], F, wide ? 8 : 4, true, wide ? 32 : 16, s, hacc);
+    block_dense<PREC, ML_TRUNK + K, 9>(net, lc, F, s, R, true, true, O);
+    if (K == 4) {  // REP3(32,48,16): A2[32:64] = [out5, X[48:64]]
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                R[b][i] = O[b][i];
+                R[b][4 + i] = F[b][i];
+            }
+    } else {  // ADD3
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) R[b][i] = (wide || i < 4) ? O[b][i] + F[b][i] : 0.0f;
+    }
+}
+
+// One block of the DI stack (K = 0..3), features G already moved to lanes q < 2.
+template <int PREC, int K>
+__device__ inline void di_block(const MfmaNet& net, const float* __restrict__ W, const float* __restrict__ sW1,
+                                const LaneCtx& lc, const float (&G)[3][8], float (&R)[3][8], float (&hacc)[4]) {
+    using namespace net;
+    constexpr int sew0[4] = {LDSE01W, LDSE11W, LDSE21W, LDSE31W};
+    constexpr int sew1[4] = {LDSE02W, LDSE12W, LDSE22W, LDSE32W};
+    const bool lo = lc.q < 2;
+    float s[3], O[3][8];
+    block_gate(W, sW1, lc, 24 + 3 * K, sew0[K], sew1[K], G, 4, lo, 8, s, hacc);
+    block_dense<PREC, ML_DI + K, 4>(net, lc, G, s, R, K > 0, lo, O);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) R[b][i] = (lo && i < 4) ? O[b][i] + G[b][i] : 0.0f;
+}
+
+template <int PREC, int K>  // LX residual pair k: x += ReLU(LX_{2k+1} ReLU(LX_{2k} x))
+__device__ inline void lx_pair(const MfmaNet& net, const LaneCtx& lc, float (&x)[1][4]) {
+    float xi[1][8], y[1][4], yi[1][8], z[1][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        xi[0][i] = x[0][i];
+        xi[0][4 + i] = 0.0f;
+    }
+    dense<PREC, ML_LXK + 2 * K>(net, xi, y, lc.lane, true);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        yi[0][i] = y[0][i];
+        yi[0][4 + i] = 0.0f;
+    }
+    dense<PREC, ML_LXK + 2 * K + 1>(net, yi, z, lc.lane, true);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[0][i] = x[0][i] + z[0][i];
+}
+
+// Tail SC(p) for one channel (radiancePredict.cu:375-409); returns P in lanes q == 0.
+template <int PREC>
+__device__ __attribute__((noinline)) float tail(const MfmaNet& net, const float* __restrict__ W, const float* __restrict__ sW1,
+                             const LaneCtx& lc, float srp, const float (&hacc)[4], const float (&comb)[3][8],
+                             const float (&tA)[4], const float (&tB)[4]) {
+    using namespace net;
+    const int q = lc.q;
+    const bool lo = q < 2;
+    // LSEFin1 rows 4q+i: add the g, gamma, srp columns, ReLU
+    float hh[1][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float* wr = sW1 + (4 * q + i) * 75;
+        float a = fmaf(wr[72], lc.g, hacc[i]);
+        a = fmaf(wr[73], lc.gamma, a);
+        a = fmaf(wr[74], srp, a);
+        hh[0][i] = fmaxf(a, 0.0f);
+        hh[0][4 + i] = 0.0f;
+    }
+    float d6[1][4];
+    dense<PREC, ML_FIN2>(net, hh, d6, lc.lane, false);
+    float w6[6];
+    {
+        float sgm[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sgm[i] = sigmoidf_(d6[0][i]);
+        const int c0 = lc.c, c1 = lc.c + 16;
+        w6[0] = __shfl(sgm[0], c0);
+        w6[1] = __shfl(sgm[1], c0);
+        w6[2] = __shfl(sgm[2], c0);
+        w6[3] = __shfl(sgm[3], c0);
+        w6[4] = __shfl(sgm[0], c1);
+        w6[5] = __shfl(sgm[1], c1);
+    }
+    // LGGS rows 0..7 into tile B rows 8..15 (lanes q >= 2), bias last (:377)
+    float tBc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (!lo) {
+            const int r = 4 * (q - 2) + i;
+            float a = 0.0f;
+            a = fmaf(lc.g, W[LGGSW + 3 * r + 0], a);
+            a = fmaf(lc.gamma, W[LGGSW + 3 * r + 1], a);
+            a = fmaf(srp, W[LGGSW + 3 * r + 2], a);
+            a += W[LGGSB + r];
+            tBc[i] = fmaxf(a, 0.0f);
+        } else {
+            tBc[i] = tB[i];
+        }
+    }
+    float in[4][8];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in[b][i] = comb[b][i] * w6[b];
+    const float sa = lo ? w6[3] : w6[4], sb = lo ? w6[5] : 1.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        in[3][i] = tA[i] * sa;
+        in[3][4 + i] = tBc[i] * sb;
+    }
+    float o0[8][4];
+    dense<PREC, ML_LC0>(net, in, o0, lc.lane, true);
+    float in1[4][8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            in1[k][i] = o0[2 * k][i];
+            in1[k][4 + i] = o0[2 * k + 1][i];
+        }
+    float o1[4][4];
+    dense<PREC, ML_LC1>(net, in1, o1, lc.lane, true);
+    float in2[2][8];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            in2[k][i] = o1[2 * k][i];
+            in2[k][4 + i] = o1[2 * k + 1][i];
+        }
+    float o2[2][4];
+    dense<PREC, ML_LC2>(net, in2, o2, lc.lane, true);
+    float in3[1][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        in3[0][i] = o2[0][i];
+        in3[0][4 + i] = o2[1][i];
+    }
+    float x[1][4];
+    dense<PREC, ML_LX>(net, in3, x, lc.lane, true);
+    lx_pair<PREC, 0>(net, lc, x);
+    lx_pair<PREC, 1>(net, lc, x);
+    lx_pair<PREC, 2>(net, lc, x);
+    lx_pair<PREC, 3>(net, lc, x);
+    lx_pair<PREC, 4>(net, lc, x);
+    lx_pair<PREC, 5>(net, lc, x);
+    float xi[1][8], pout[1][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        xi[0][i] = x[0][i];
+        xi[0][4 + i] = 0.0f;
+    }
+    dense<PREC, ML_LX12>(net, xi, pout, lc.lane, true);
+    return pout[0][0];  // row 0: lanes q == 0
+}
+
+// Full network on one 16-pixel tile; returns, in lanes q == 0, max(e^P - 1, 0) * srp per
+// channel for pixel c.
+template <int PREC, class Src>
+__device__ inline v3 run_tile(const MfmaNet& net, const float* __restrict__ W, const float* __restrict__ sW1,
+                              const LaneCtx& lc, const Src& src) {
+    using namespace net;
+    const bool lo = lc.q < 2;
+    float F[3][8], G[3][8], R[3][8], O[3][8], s[3];
+    float hacc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // ---- block 1: SE(0,0,8), ADD3(0,0,8); points 0..15 gathered, [0,8) in lanes q<2
+    src.get(lc, 0, false, F);
+    block_gate(W, sW1, lc, 0, LSE01W, LSE02W, F, 4, lo, 8, s, hacc);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) R[b][i] = 0.0f;
+    block_dense<PREC, ML_TRUNK + 0, 9>(net, lc, F, s, R, false, lo, O);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) R[b][i] = (lo && i < 4) ? O[b][i] + F[b][i] : 0.0f;
+    // ---- block 2: SERES(0,8,8,8), REP3(0,8,8); [8,16) lives in lanes q>=2: move down
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? __shfl_xor(F[b][i], 32) : 0.0f;
+    block_gate(W, sW1, lc, 3, LSE11W, LSE12W, G, 4, lo, 8, s, hacc);
+    block_dense<PREC, ML_TRUNK + 1, 9>(net, lc, G, s, R, true, lo, O);
+    // A2[0:16] = [out2, X[8:16]]: lanes q<2 hold out2 rows 4q+i, lanes q>=2 their own X[8+4(q-2)+i]
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) R[b][i] = i < 4 ? (lo ? O[b][i] : F[b][i]) : 0.0f;
+    // ---- blocks 3..8
+    src.get(lc, 16, false, F);
+    trunk_block<PREC, 2>(net, W, sW1, lc, F, R, hacc);
+    src.get(lc, 32, false, F);
+    trunk_block<PREC, 3>(net, W, sW1, lc, F, R, hacc);
+    src.get(lc, 48, false, F);
+    trunk_block<PREC, 4>(net, W, sW1, lc, F, R, hacc);
+    src.get(lc, 64, true, F);
+    trunk_block<PREC, 5>(net, W, sW1, lc, F, R, hacc);
+    src.get(lc, 96, true, F);
+    trunk_block<PREC, 6>(net, W, sW1, lc, F, R, hacc);
+    src.get(lc, 128, true, F);
+    trunk_block<PREC, 7>(net, W, sW1, lc, F, R, hacc);
+    // ---- L81 / L_Tr81 / L_Hg81 -> Comb[0:96]
+    float comb[3][8];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        float in[1][8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in[0][i] = R[b][i];
+        float o[2][4];
+        if (b == 0) dense<PREC, ML_TRUNK + 8>(net, in, o, lc.lane, true);
+        else if (b == 1) dense<PREC, ML_TRUNK + 17>(net, in, o, lc.lane, true);
+        else dense<PREC, ML_TRUNK + 26>(net, in, o, lc.lane, true);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            comb[b][i] = o[0][i];
+            comb[b][4 + i] = o[1][i];
+        }
+    }
+    // ---- DI stack: points 160..191 gathered at once; [160,168) lanes q<2 slots 0..3,
+    // [168,176) lanes q>=2 slots 0..3, [176,184) lanes q<2 slots 4..7, [184,192) q>=2 4..7
+    float D[3][8];
+    src.get(lc, 160, true, D);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) R[b][i] = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? D[b][i] : 0.0f;
+    di_block<PREC, 0>(net, W, sW1, lc, G, R, hacc);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? __shfl_xor(D[b][i], 32) : 0.0f;
+    di_block<PREC, 1>(net, W, sW1, lc, G, R, hacc);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? D[b][4 + i] : 0.0f;
+    di_block<PREC, 2>(net, W, sW1, lc, G, R, hacc);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? __shfl_xor(D[b][4 + i], 32) : 0.0f;
+    di_block<PREC, 3>(net, W, sW1, lc, G, R, hacc);
+    // ---- LD41 heads: tile A = [LD41 rows 0..7 | LD_Tr41 rows 8..15], tile B = [LD_Hg41 | LGGS]
+    float tA[4], tB[4];
+    {
+        float in0[1][8], in1[1][8], in2[1][8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            in0[0][i] = R[0][i];
+            in1[0][i] = R[1][i];
+            in2[0][i] = R[2][i];
+        }
+        float o0[1][4], o1[1][4], o2[1][4];
+        dense<PREC, ML_LD41A_VAL>(net, in0, o0, lc.lane, false);
+        dense<PREC, ML_LD41A_TR>(net, in1, o1, lc.lane, false);
+        dense<PREC, ML_LD41B_HG>(net, in2, o2, lc.lane, true);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            tA[i] = fmaxf(o0[0][i] + o1[0][i], 0.0f);
+            tB[i] = o2[0][i];
+        }
+    }
+    // ---- tail per distinct colour channel (radiancePredict.cu:411-421)
+    const bool need_y = __any(lc.ok && lc.sr.y != lc.sr.x);
+    const bool need_z = __any(lc.ok && lc.sr.z != lc.sr.x && lc.sr.z != lc.sr.y);
+    const float sp0 = powf(lc.sr.x, 4.0f), sp1 = powf(lc.sr.y, 4.0f), sp2 = powf(lc.sr.z, 4.0f);
+    float P0 = tail<PREC>(net, W, sW1, lc, sp0, hacc, comb, tA, tB);
+    float P1 = P0, P2 = P0;
+    if (need_y) P1 = tail<PREC>(net, W, sW1, lc, sp1, hacc, comb, tA, tB);
+    if (need_z) P2 = tail<PREC>(net, W, sW1, lc, sp2, hacc, comb, tA, tB);
+    if (lc.sr.y == lc.sr.x) P1 = P0;
+    if (lc.sr.z == lc.sr.x) P2 = P0;
+    else if (lc.sr.z == lc.sr.y) P2 = P1;
+    v3 r;
+    r.x = fmaxf(expf(P0) - 1.0f, 0.0f) * sp0;
+    r.y = fmaxf(expf(P1) - 1.0f, 0.0f) * sp1;
+    r.z = fmaxf(expf(P2) - 1.0f, 0.0f) * sp2;
+    return r;
+}
+
+__device__ inline void load_shared(float* sW1, StencilEntry* sst, const float* __restrict__ W,
+                                   const StencilEntry* __restrict__ stencil) {
+    for (int i = threadIdx.x; i < 1200; i += blockDim.x) sW1[i] = W[net::LSEFin1W + i];
+    if (sst)
+        for (int i = threadIdx.x; i < kPC; i += blockDim.x) sst[i] = stencil[i];
+    __syncthreads();
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevVolume* __restrict__ vols,
+                                                         const ViewSlot* __restrict__ slots,
+                                                         const StencilEntry* __restrict__ stencil,
+                                                         const float* __restrict__ W, const ActiveRec* __restrict__ recs,
+                                                         const unsigned* __restrict__ counter) {
+    __shared__ float sW1[1200];
+    __shared__ StencilEntry sst[kPC];
+    load_shared(sW1, sst, W, stencil);
+    const unsigned n = *counter;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    for (unsigned tile = wave; tile * 16u < n; tile += nwaves) {
+        LaneCtx lc;
+        lc.lane = lane;
+        lc.c = lane & 15;
+        lc.q = lane >> 4;
+        const unsigned idx = tile * 16u + lc.c;
+        // all 16 records of a tile come from one view (padded compaction)
+        const int view = __builtin_amdgcn_readfirstlane(recs[tile * 16u].view);
+        const ActiveRec r = recs[idx];
+        lc.ok = r.pixel >= 0;
+        const ViewSlot& sl = slots[view];
+        const ViewDev vw = sl.vw;
+        GatherSrc src;
+        src.vol = &vols[sl.vol];
+        src.st = sst;
+        src.g = vw.g;
+        src.alpha = vw.alpha;
+        src.ps.pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
+        src.ps.X = mk3(r.X[0], r.X[1], r.X[2]);
+        src.ps.Y = mk3(r.Y[0], r.Y[1], r.Y[2]);
+        src.ps.Z = mk3(r.Z[0], r.Z[1], r.Z[2]);
+        const v3 light = normalize3(vw.light_dir);
+        src.ps.L = light;
+        src.ps.LX = normalize3(light);
+        src.ps.dist = r.dist;
+        src.ps.active = 1;
+        src.ps.taps = 0;
+        lc.g = vw.g;
+        lc.gamma = gamma_of(src.ps);
+        lc.sr = vw.scatter_rate / 1.001f;
+        MfmaNet nt = net;
+        nt.bf16 = opaque(net.bf16);
+        nt.fp8 = opaque(net.fp8);
+        nt.bias = opaque(net.bias);
+        v3 rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
+        if (lc.ok && lc.q == 0) {
+            v3 o = (rgb * vw.light_color) * vw.scatter_rate;
+            sl.out[r.pixel] = make_float4(fmaxf(0.0f, o.x), fmaxf(0.0f, o.y), fmaxf(0.0f, o.z), fmaxf(0.0f, r.dist));
+        }
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const float* __restrict__ W,
+                                                           const float* __restrict__ feats, int n, v3 sr, float g,
+                                                           float* __restrict__ rgb) {
+    __shared__ float sW1[1200];
+    load_shared(sW1, nullptr, W, nullptr);
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    for (int tile = wave; tile * 16 < n; tile += nwaves) {
+        LaneCtx lc;
+        lc.lane = lane;
+        lc.c = lane & 15;
+        lc.q = lane >> 4;
+        const int idx = tile * 16 + lc.c;
+        lc.ok = idx < n;
+        RowSrc src;
+        src.row = feats + (size_t)(lc.ok ? idx : 0) * kNFEAT;
+        lc.g = g;
+        lc.gamma = lc.ok ? src.row[3 * kPC] : 0.0f;
+        lc.sr = sr;
+        MfmaNet nt = net;
+        nt.bf16 = opaque(net.bf16);
+        nt.fp8 = opaque(net.fp8);
+        nt.bias = opaque(net.bias);
+        v3 o = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
+        if (lc.ok && lc.q == 0) {
+            rgb[3 * (size_t)idx] = o.x;
+            rgb[3 * (size_t)idx + 1] = o.y;
+            rgb[3 * (size_t)idx + 2] = o.z;
+        }
+    }
+}
+
+hipError_t launch_mlp_mfma_active(int precision, const MfmaNet& net, const DevVolume* vols, const ViewSlot* slots,
+                                  const StencilEntry* stencil, const float* W, const ActiveRec* recs,
+                                  const unsigned* counter, int grid, hipStream_t s) {
+    if (precision == MRP_BF16)
+        hipLaunchKernelGGL(k_mlp_mfma_active<MRP_BF16>, dim3(grid), dim3(256), 0, s, net, vols, slots, stencil, W, recs,
+                           counter);
+    else
+        hipLaunchKernelGGL(k_mlp_mfma_active<MRP_FP8>, dim3(grid), dim3(256), 0, s, net, vols, slots, stencil, W, recs,
+                           counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_mfma_features(int precision, const MfmaNet& net, const float* W, const float* feats, int n,
+                                    const float sr[3], float g, float* rgb, hipStream_t s) {
+    int tiles = (n + 15) / 16;
+    int grid = (tiles + 3) / 4;
+    if (grid > 4096) grid = 4096;
+    if (precision == MRP_BF16)
+        hipLaunchKernelGGL(k_mlp_mfma_features<MRP_BF16>, dim3(grid), dim3(256), 0, s, net, W, feats, n,
+                           mk3(sr[0], sr[1], sr[2]), g, rgb);
+    else
+        hipLaunchKernelGGL(k_mlp_mfma_features<MRP_FP8>, dim3(grid), dim3(256), 0, s, net, W, feats, n,
+                           mk3(sr[0], sr[1], sr[2]), g, rgb);
+    return hipGetLastError();
+}
+
+}  // namespace mrp

@@ -1,0 +1,291 @@
+// Per-volume and per-light preparation kernels:
+//   * seeded fBm cloud generation (benchmark inputs);
+//   * density mip pyramid, VolumeRender::Update (core/volume.cu:1443-1531), moved from the
+//     reference's OpenMP loop to the GPU: exact 2x levels are 2x2x2 box means with an FP64
+//     sum, other ratios are the zero-border trilinear resample of the host Sample()
+//     (core/volume.cu:845-863) at cell centres;
+//   * transmittance mips, Fill_TR / Update_TR (core/volume.cu:1535-1560, 1589-1667), all
+//     eight levels in one launch;
+//   * HG LUT, Fill_Hg (core/volume.cu:1322-1400).
+// All are HBM-streaming or ALU kernels with coalesced z-fastest access.
+#include "dev_common.hpp"
+#include "fbm.h"
+#include "kernels.h"
+#include "ref_tables.h"
+
+namespace mrp {
+
+// ------------------------------------------------------------------------------------------
+// fBm generator: raw values, grid min/max (order-preserving uint atomics), normalise.
+// ------------------------------------------------------------------------------------------
+__device__ inline uint32_t f2ord(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ inline float ord2f(uint32_t u) {
+    uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+__global__ __launch_bounds__(256) void k_fbm_raw(float* __restrict__ out, int N, int octaves, float base_freq,
+                                                 const int* __restrict__ perm_g, uint32_t* __restrict__ minmax) {
+    __shared__ int perm[512];
+    for (int i = threadIdx.x; i < 512; i += 256) perm[i] = perm_g[i];
+    __syncthreads();
+    const size_t total = (size_t)N * N * N;
+    float lo = 3.0e38f, hi = -3.0e38f;
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        int z = (int)(idx % N);
+        int y = (int)((idx / N) % N);
+        int x = (int)(idx / ((size_t)N * N));
+        float v = fbm_voxel(perm, N, octaves, base_freq, x, y, z);
+        out[idx] = v;
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+    }
+    // wavefront reduction then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&minmax[0], f2ord(lo));
+        atomicMax(&minmax[1], f2ord(hi));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fbm_normalize(float* __restrict__ v, size_t total,
+                                                       const uint32_t* __restrict__ minmax) {
+    const float lo = ord2f(minmax[0]), hi = ord2f(minmax[1]);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256)
+        v[i] = fbm_density(v[i], lo, hi);
+}
+
+// ------------------------------------------------------------------------------------------
+// Pyramid
+// ------------------------------------------------------------------------------------------
+// Exact 2x: dst[x][y][z] = mean of the 2x2x2 children (each clamped >= 0), summed in FP64.
+// The reference's trilinear at (2i+0.5) reduces to this mean with seven fp32 roundings; the
+// FP64 sum rounds once.
+__global__ __launch_bounds__(256) void k_box2(const float* __restrict__ src, float* __restrict__ dst, int res) {
+    const size_t total = (size_t)res * res * res;
+    const int sres = res * 2;
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        int z = (int)(idx % res);
+        int y = (int)((idx / res) % res);
+        int x = (int)(idx / ((size_t)res * res));
+        double s = 0.0;
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy) {
+                const float2 p =
+                    *(const float2*)(src + (((size_t)(2 * x + dx) * sres + (2 * y + dy)) * sres + 2 * z));
+                s += (double)fmaxf(0.0f, p.x);
+                s += (double)fmaxf(0.0f, p.y);
+            }
+        dst[idx] = (float)(s * 0.125);
+    }
+}
+
+// General ratio: Sample(source, sres, uvw) at cell centres, zero border, max(0,.) taps
+// (ParallelFill omp.cpp:4-18 computes u = (i+0.5)/res in double).
+__device__ inline float fetch_b0(const float* __restrict__ d, int res, int x, int y, int z) {
+    if (x < 0 || y < 0 || z < 0 || x >= res || y >= res || z >= res) return 0.0f;
+    return fmaxf(0.0f, d[((size_t)x * res + y) * res + z]);
+}
+__global__ __launch_bounds__(256) void k_resample(const float* __restrict__ src, int sres, float* __restrict__ dst,
+                                                  int res) {
+    const size_t total = (size_t)res * res * res;
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        int k = (int)(idx % res);
+        int j = (int)((idx / res) % res);
+        int i = (int)(idx / ((size_t)res * res));
+        v3 uv = mk3((float)((i + 0.5) / res), (float)((j + 0.5) / res), (float)((k + 0.5) / res));
+        v3 pos = uv * (float)sres - 0.5f;
+        int x = (int)floorf(pos.x), y = (int)floorf(pos.y), z = (int)floorf(pos.z);
+        v3 w = pos - mk3((float)x, (float)y, (float)z);
+        const float* d = src;
+        float r = lerp_ref(lerp_ref(lerp_ref(fetch_b0(d, sres, x, y, z), fetch_b0(d, sres, x, y, z + 1), w.z),
+                                    lerp_ref(fetch_b0(d, sres, x, y + 1, z), fetch_b0(d, sres, x, y + 1, z + 1), w.z),
+                                    w.y),
+                           lerp_ref(lerp_ref(fetch_b0(d, sres, x + 1, y, z), fetch_b0(d, sres, x + 1, y, z + 1), w.z),
+                                    lerp_ref(fetch_b0(d, sres, x + 1, y + 1, z),
+                                             fetch_b0(d, sres, x + 1, y + 1, z + 1), w.z),
+                                    w.y),
+                           w.x);
+        dst[idx] = r;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Transmittance mips: level m (res 128>>m) marches 128 steps towards the light through
+// density mip m+1 with alpha / 1.73^(m+1) (Fill_TR, volume.cu:1535-1560).
+// ------------------------------------------------------------------------------------------
+struct TrArgs {
+    const float* mips[9];
+    float* tr[8];
+    float alpha_lvl[8];
+    long long start[9];  // flattened texel offsets of each level
+    v3 light;
+};
+__global__ __launch_bounds__(256) void k_tr_bake(TrArgs a) {
+    const long long total = a.start[8];
+    for (long long g = (long long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long long)gridDim.x * 256) {
+        int m = 0;
+        while (m < 7 && g >= a.start[m + 1]) ++m;
+        const int res = 128 >> m;
+        const long long l = g - a.start[m];
+        const int z = (int)(l % res);
+        const int y = (int)((l / res) % res);
+        const int x = (int)(l / ((long long)res * res));
+        v3 uv = (mk3((float)x, (float)y, (float)z) + 0.5f) / (float)res;
+        v3 ori = uv - 0.5f;
+        float dis = ray_box_distance(ori, a.light);
+        float st = dis / 128.0f;
+        v3 lp = ori;
+        float sum = 0.0f;
+        const float* src = a.mips[m + 1];
+        const int sres = 256 >> (m + 1);
+        for (int i = 0; i < 128; ++i) {
+            lp = lp + a.light * st;
+            sum = sum + sample_border_exact(src, sres, lp + 0.5f);
+        }
+        a.tr[m][l] = expf(-sum * a.alpha_lvl[m] * st) * 1.0f;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// HG LUT (Fill_Hg, volume.cu:1322-1400): texel (i = cos index, j = cone-angle index).
+// ------------------------------------------------------------------------------------------
+__constant__ unsigned c_halton_primes[5] = MRP_HALTON_PRIMES_INIT;
+__constant__ unsigned c_halton_sums[5] = MRP_HALTON_PRIME_SUMS_INIT;
+__constant__ unsigned c_halton_perm[28] = MRP_HALTON_PERM_INIT;
+
+__device__ inline float halton_scrambled(unsigned dim, unsigned a) {  // volume.cu:1300-1316
+    const unsigned base = c_halton_primes[dim], offset = c_halton_sums[dim];
+    const float inv_base = 1.0f / (float)base;
+    unsigned rev = 0;
+    float inv_base_n = 1.0f;
+    while (a) {
+        unsigned next = a / base;
+        unsigned digit = a - next * base;
+        rev = rev * base + c_halton_perm[offset + digit];
+        inv_base_n *= inv_base;
+        a = next;
+    }
+    float r = inv_base_n * ((float)rev + inv_base * (float)c_halton_perm[offset] / (1.0f - inv_base));
+    return fminf(r, 0.99999994f);
+}
+__device__ inline float frac_ref(float f) { return f - (float)(long)f; }  // vector.cu:20
+
+__global__ __launch_bounds__(64) void k_hg_lut(float* __restrict__ lut, float g, uint32_t seed, float mis_a) {
+    const int i = blockIdx.x * 8 + (threadIdx.x & 7);
+    const int j = blockIdx.y * 8 + (threadIdx.x >> 3);
+    if (i >= 512 || j >= 512) return;
+    float costheta = (float)((((double)(float)i + 0.5) / 512) * 2.0 - 1.0);
+    float aa = (float)(((double)(float)j + 0.5) / 512);
+    float angle = (float)((double)aa * 3.1415926535 * 60.0 / 180.0);
+    PhiloxStream rs(seed, 0u, (uint32_t)(i + j * 512), RNG_LUT, 0u);
+    float bn = rs.next();
+    float hg = 0.0f;
+    float R = tanf(angle);
+    const v3 view = mk3(1, 0, 0);
+    v3 D = mk3(costheta, sqrtf(1.0f - costheta * costheta), 0.0f);
+    float mis = 0.0f;
+    {
+        float p = R * R - 1.0f + D.x * D.x;
+        if (p > 0.0f) {
+            float sdotl = g > 0.0f ? D.x : -D.x;
+            float w = sqrtf(p);
+            float d0 = fmaxf(0.0f, sdotl - w) / R;
+            float d1 = (sdotl + w) / R;
+            float d = d1 * d1 * d1 - d0 * d0 * d0;
+            mis = (float)(d1 > d0) * d;
+        }
+        mis = fminf(fmaxf(mis, 1e-8f), 1.0f);
+        mis = 1.0f - 1.0f / (1.0f + mis_a * mis / (1.0f - mis_a));
+    }
+    for (int k = 0; k < 1024; ++k) {
+        if (frac_ref(halton_scrambled(4, k) + bn) < mis) {
+            float rx = frac_ref(halton_scrambled(0, k) + bn), ry = frac_ref(halton_scrambled(1, k) + bn);
+            v3 l = sample_hg(rx, ry, view, g);
+            float sdotl = dot3(D, l);
+            float p = R * R - 1.0f + sdotl * sdotl;
+            if (p <= 0.0f) continue;
+            float w = sqrtf(p);
+            float d0 = sdotl - w;
+            float d1 = (sdotl + w) / R;
+            d0 = fmaxf(0.0f, d0) / R;
+            float d = d1 * d1 * d1 - d0 * d0 * d0;
+            hg += fmaxf(0.0f, d / 4.0f);
+        } else {
+            float rx = frac_ref(halton_scrambled(0, k) + bn), ry = frac_ref(halton_scrambled(1, k) + bn),
+                  rz = frac_ref(halton_scrambled(2, k) + bn);
+            float theta = (float)((double)rx * 2.0 * 3.1415926535);
+            float phi = (float)acos(2.0 * (double)ry - 1.0);
+            float r = powf(rz, 1.0f / 3.0f);
+            float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);
+            float x = R * r * sp * ct, y = R * r * sp * st, z = R * r * cp;
+            v3 rp = mk3(x, y, z) + D;
+            float c = dot3(normalize3(rp), view);
+            hg += henyey_greenstein(c, g);
+        }
+    }
+    hg /= 1024.0f;
+    lut[j * 512 + i] = hg;
+}
+
+// ------------------------------------------------------------------------------------------
+static inline int grid_for(size_t total, int block) {
+    size_t b = (total + block - 1) / block;
+    return (int)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+hipError_t launch_fbm(float* out, int N, int octaves, float base_freq, const int* perm, uint32_t* minmax,
+                      hipStream_t s) {
+    const size_t total = (size_t)N * N * N;
+    hipLaunchKernelGGL(k_fbm_raw, dim3(grid_for(total, 256)), dim3(256), 0, s, out, N, octaves, base_freq, perm,
+                       minmax);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fbm_normalize, dim3(grid_for(total, 256)), dim3(256), 0, s, out, total,
+                       (const uint32_t*)minmax);
+    return hipGetLastError();
+}
+hipError_t launch_box2(const float* src, float* dst, int res, hipStream_t s) {
+    hipLaunchKernelGGL(k_box2, dim3(grid_for((size_t)res * res * res, 256)), dim3(256), 0, s, src, dst, res);
+    return hipGetLastError();
+}
+hipError_t launch_resample(const float* src, int sres, float* dst, int res, hipStream_t s) {
+    hipLaunchKernelGGL(k_resample, dim3(grid_for((size_t)res * res * res, 256)), dim3(256), 0, s, src, sres, dst,
+                       res);
+    return hipGetLastError();
+}
+hipError_t launch_tr_bake(const float* const mips[9], float* const tr[8], const float alpha_lvl[8], v3 light,
+                          hipStream_t s) {
+    TrArgs a;
+    long long acc = 0;
+    for (int m = 0; m < 9; ++m) a.mips[m] = mips[m];
+    for (int m = 0; m < 8; ++m) {
+        a.tr[m] = tr[m];
+        a.alpha_lvl[m] = alpha_lvl[m];
+        a.start[m] = acc;
+        acc += (long long)(128 >> m) * (128 >> m) * (128 >> m);
+    }
+    a.start[8] = acc;
+    a.light = light;
+    hipLaunchKernelGGL(k_tr_bake, dim3(grid_for((size_t)acc, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_hg_lut(float* lut, float g, uint32_t seed, hipStream_t s) {
+    // anisotropy weight of the mixture (volume.cu:1357-1358): |g| in float, the rest in double
+    const double a = (double)fabsf(g);
+    const float mis_a = a < 0.6 ? 0.0f : (a < 0.7 ? (float)((a - 0.6) * 9) : (float)(pow((a - 0.7) / 0.3, 0.05) * 0.1 + 0.9));
+    hipLaunchKernelGGL(k_hg_lut, dim3(64, 64), dim3(64), 0, s, lut, g, seed, mis_a);
+    return hipGetLastError();
+}
+
+}  // namespace mrp

@@ -1,0 +1,310 @@
+"""ctypes binding of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker.  The product library (mrpnn_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+WEIGHTS = os.path.join(HERE, "..", "mrpnn_amd", "assets", "mrpnn_weights.bin")
+
+NFEAT = 577
+PC = 192
+MIP_TOTAL = sum((256 >> m) ** 3 for m in range(9))
+TR_TOTAL = sum((128 >> m) ** 3 for m in range(8))
+LUT = 512
+
+_f = ctypes.POINTER(ctypes.c_float)
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int32)
+_u = ctypes.POINTER(ctypes.c_uint32)
+
+
+class View(ctypes.Structure):
+    """Same layout as mrp_view (include/mrp_c_api.h)."""
+    _fields_ = [("ori", ctypes.c_float * 3), ("up", ctypes.c_float * 3), ("right", ctypes.c_float * 3),
+                ("light_dir", ctypes.c_float * 3), ("light_color", ctypes.c_float * 3),
+                ("scatter_rate", ctypes.c_float * 3), ("alpha", ctypes.c_float), ("g", ctypes.c_float),
+                ("tan_half_fov", ctypes.c_float), ("jitter", ctypes.c_float), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("seed", ctypes.c_uint32), ("frame", ctypes.c_uint32),
+                ("view_id", ctypes.c_uint32), ("volume_index", ctypes.c_int32)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_load_weights.argtypes = [_f, ctypes.c_int]
+        L.orc_philox.argtypes = [_u, ctypes.c_uint32, ctypes.c_uint32, _u]
+        L.orc_u01.argtypes = [ctypes.c_uint32]
+        L.orc_u01.restype = ctypes.c_float
+        L.orc_halton.argtypes = [ctypes.c_uint, ctypes.c_uint]
+        L.orc_halton.restype = ctypes.c_float
+        L.orc_stencil.argtypes = [ctypes.c_int, _f, _i, _i, _i]
+        L.orc_predict_f32.argtypes = [_f, ctypes.c_int, _f, ctypes.c_float, _f]
+        L.orc_predict_f64.argtypes = [_d, ctypes.c_int, _d, ctypes.c_double, _d]
+        L.orc_build_pyramid.argtypes = [_f, ctypes.c_int, _f]
+        L.orc_bake_tr.argtypes = [_f, _f, ctypes.c_float, _f]
+        L.orc_hg_lut_qmc.argtypes = [ctypes.c_float, ctypes.c_uint32, _f]
+        L.orc_hg_ball_mean.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_hg_ball_mean.restype = ctypes.c_double
+        L.orc_sample_lut.argtypes = [_f, ctypes.c_float, ctypes.c_float]
+        L.orc_sample_lut.restype = ctypes.c_float
+        vol = [_f, ctypes.c_int, ctypes.c_float, _f, _f, _f]
+        L.orc_render_predict.argtypes = vol + [ctypes.POINTER(View), _f, _f, _i, _i, _f]
+        L.orc_render_predict_rows.argtypes = vol + [ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _f]
+        L.orc_render_predict_list.argtypes = vol + [ctypes.POINTER(View), _i, ctypes.c_int, _f]
+        L.orc_march.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, _f, ctypes.c_float, _f, _i, _i]
+        L.orc_gather_state.argtypes = vol + [ctypes.c_int, _f, _f, _f, _f, _f, ctypes.c_float, ctypes.c_float, _f]
+        L.orc_basis.argtypes = [_f, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _f]
+        L.orc_pt_render.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, ctypes.POINTER(View), ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_float, _d, _d,
+                                    ctypes.POINTER(ctypes.c_int64)]
+        L.orc_tr_stochastic.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, _f, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_uint32, ctypes.c_int, _f]
+        L.orc_fbm_volume.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, _f]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        w = np.fromfile(WEIGHTS, dtype="<f4")
+        if L.orc_load_weights(w.ctypes.data_as(_f), int(w.size)) != 0:
+            raise RuntimeError("oracle: weight asset does not match the network layout")
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_f):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def set_threads(n):
+    return lib().orc_set_threads(int(n))
+
+
+def weights():
+    return np.fromfile(WEIGHTS, dtype="<f4")
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox(_p(c, _u), int(key[0]), int(key[1]), _p(out, _u))
+    return out
+
+
+def u01(x):
+    return lib().orc_u01(int(x))
+
+
+def halton(dim, index):
+    return lib().orc_halton(dim, index)
+
+
+def stencil(index):
+    off = ctypes.c_float()
+    mip, typ, loc = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    lib().orc_stencil(index, ctypes.byref(off), ctypes.byref(mip), ctypes.byref(typ), ctypes.byref(loc))
+    return off.value, mip.value, typ.value, loc.value
+
+
+def predict_f32(feats, sr, g):
+    feats = np.ascontiguousarray(feats, np.float32).reshape(-1, NFEAT)
+    sr = np.asarray(sr, np.float32)
+    out = np.zeros((feats.shape[0], 3), np.float32)
+    lib().orc_predict_f32(_p(feats), feats.shape[0], _p(sr), float(g), _p(out))
+    return out
+
+
+def predict_f64(feats, sr, g):
+    feats = np.ascontiguousarray(feats, np.float64).reshape(-1, NFEAT)
+    sr = np.asarray(sr, np.float64)
+    out = np.zeros((feats.shape[0], 3), np.float64)
+    lib().orc_predict_f64(_p(feats, _d), feats.shape[0], _p(sr, _d), float(g), _p(out, _d))
+    return out
+
+
+def fbm_volume(N, octaves, base_freq, seed):
+    out = np.zeros(N ** 3, np.float32)
+    lib().orc_fbm_volume(N, octaves, base_freq, seed, _p(out))
+    return out
+
+
+def build_pyramid(grid, N):
+    grid = np.ascontiguousarray(grid, np.float32)
+    out = np.zeros(MIP_TOTAL, np.float32)
+    lib().orc_build_pyramid(_p(grid), N, _p(out))
+    return out
+
+
+def split_mips(mips):
+    res, o = [], 0
+    for m in range(9):
+        r = 256 >> m
+        res.append(mips[o:o + r ** 3].reshape(r, r, r))
+        o += r ** 3
+    return res
+
+
+def split_tr(tr):
+    res, o = [], 0
+    for m in range(8):
+        r = 128 >> m
+        res.append(tr[o:o + r ** 3].reshape(r, r, r))
+        o += r ** 3
+    return res
+
+
+def bake_tr(mips, light, alpha):
+    mips = np.ascontiguousarray(mips, np.float32)
+    light = np.asarray(light, np.float32)
+    out = np.zeros(TR_TOTAL, np.float32)
+    lib().orc_bake_tr(_p(mips), _p(light), float(alpha), _p(out))
+    return out
+
+
+def hg_lut(g, seed):
+    out = np.zeros(LUT * LUT, np.float32)
+    lib().orc_hg_lut_qmc(float(g), int(seed), _p(out))
+    return out
+
+
+def hg_ball_mean(costheta, angle, g, n_psi=96, n_phi=96):
+    return lib().orc_hg_ball_mean(costheta, angle, g, n_psi, n_phi)
+
+
+def sample_lut(lut, u, v):
+    lut = np.ascontiguousarray(lut, np.float32)
+    return lib().orc_sample_lut(_p(lut), u, v)
+
+
+class Scene:
+    """A volume with its pyramids on the host (the oracle's view of VolumeRender)."""
+
+    def __init__(self, grid, N, max_density=1.0):
+        self.grid = np.ascontiguousarray(grid, np.float32)
+        self.N = N
+        self.max_density = float(max_density)
+        self.mips = build_pyramid(self.grid, N)
+        self.tr = None
+        self.lut = None
+
+    def update_tr(self, light, alpha):
+        self.tr = bake_tr(self.mips, light, alpha)
+
+    def set_lut(self, lut):
+        self.lut = np.ascontiguousarray(lut, np.float32)
+
+    def _vol(self):
+        return [_p(self.grid), self.N, self.max_density, _p(self.mips), _p(self.tr), _p(self.lut)]
+
+    def render_predict(self, view, want_feats=False):
+        W, H = view.width, view.height
+        out = np.zeros((H * W, 4), np.float32)
+        feats = np.zeros((H * W, NFEAT), np.float32) if want_feats else None
+        active = np.zeros(H * W, np.int32)
+        steps = np.zeros(H * W, np.int32)
+        pos = np.zeros((H * W, 3), np.float32)
+        lib().orc_render_predict(*self._vol(), ctypes.byref(view), _p(out), _p(feats), _p(active, _i),
+                                 _p(steps, _i), _p(pos))
+        return dict(out=out, feats=feats, active=active, steps=steps, pos=pos)
+
+    def render_predict_rows(self, view, p0, n):
+        out = np.zeros((n, 4), np.float32)
+        lib().orc_render_predict_rows(*self._vol(), ctypes.byref(view), int(p0), int(n), _p(out))
+        return out
+
+    def render_predict_list(self, view, pix):
+        pix = np.ascontiguousarray(pix, np.int32)
+        out = np.zeros((pix.size, 4), np.float32)
+        lib().orc_render_predict_list(*self._vol(), ctypes.byref(view), _p(pix, _i), int(pix.size), _p(out))
+        return out
+
+    @classmethod
+    def from_arrays(cls, grid, N, max_density, mips, tr, lut):
+        """Scene over externally built pyramids (no CPU pyramid build)."""
+        sc = cls.__new__(cls)
+        sc.grid = np.ascontiguousarray(grid, np.float32)
+        sc.N = N
+        sc.max_density = float(max_density)
+        sc.mips = np.ascontiguousarray(mips, np.float32)
+        sc.tr = np.ascontiguousarray(tr, np.float32)
+        sc.lut = np.ascontiguousarray(lut, np.float32)
+        return sc
+
+    def gather_state(self, active, pos, X, Y, Z, light, g, alpha):
+        out = np.zeros(NFEAT, np.float32)
+        a = [np.asarray(v, np.float32) for v in (pos, X, Y, Z, light)]
+        lib().orc_gather_state(*self._vol(), int(active), *[_p(x) for x in a], float(g), float(alpha), _p(out))
+        return out
+
+    def pt_render(self, view, spp, max_bounce, nee_baked=0, tr_scale=1.0):
+        W, H = view.width, view.height
+        mean = np.zeros((H * W, 3), np.float64)
+        m2 = np.zeros((H * W, 3), np.float64)
+        taps = ctypes.c_int64()
+        lib().orc_pt_render(_p(self.grid), self.N, self.max_density, _p(self.tr), ctypes.byref(view), spp,
+                            max_bounce, nee_baked, tr_scale, _p(mean, _d), _p(m2, _d), ctypes.byref(taps))
+        return mean, m2, taps.value
+
+
+def march(grid, N, max_density, ori, d, alpha):
+    grid = np.ascontiguousarray(grid, np.float32)
+    pos = np.zeros(3, np.float32)
+    act, taps = ctypes.c_int32(), ctypes.c_int32()
+    o, dd = np.asarray(ori, np.float32), np.asarray(d, np.float32)
+    lib().orc_march(_p(grid), N, max_density, _p(o), _p(dd), float(alpha), _p(pos), ctypes.byref(act),
+                    ctypes.byref(taps))
+    return bool(act.value), pos, taps.value
+
+
+def basis(d, seed, frame, pixel, view_id):
+    out = np.zeros(9, np.float32)
+    dd = np.asarray(d, np.float32)
+    lib().orc_basis(_p(dd), seed, frame, pixel, view_id, _p(out))
+    return out.reshape(3, 3)
+
+
+def tr_stochastic(grid, N, max_density, pos, d, dis, alpha, n, seed=7, tr_scale=1.0):
+    grid = np.ascontiguousarray(grid, np.float32)
+    out = np.zeros(n, np.float32)
+    p, dd = np.asarray(pos, np.float32), np.asarray(d, np.float32)
+    lib().orc_tr_stochastic(_p(grid), N, max_density, _p(p), _p(dd), float(dis), float(alpha), float(tr_scale),
+                            seed, n, _p(out))
+    return out
+
+
+def orbit_view(width, height, dist=2.0, azimuth=0.0, elevation=0.0, fov_deg=40.0, light=(0.866, 0.5, 0.0),
+               alpha=64.0, g=0.857, albedo=0.99, seed=1, frame=0, view_id=0, jitter=0.5, volume_index=0):
+    """Camera on a sphere looking at the origin, basis as Camera::SetPosition (camera.cu:176-192)."""
+    ca, sa, ce, se = np.cos(azimuth), np.sin(azimuth), np.cos(elevation), np.sin(elevation)
+    ori = np.array([dist * ce * ca, dist * se, dist * ce * sa], np.float32)
+    fwd = -ori / np.linalg.norm(ori)
+    right = np.cross(np.array([0, 1, 0], np.float32), fwd)
+    right = right / np.linalg.norm(right)
+    up = np.cross(fwd, right)
+    v = View()
+    v.ori[:] = ori.tolist()
+    v.up[:] = up.astype(np.float32).tolist()
+    v.right[:] = right.astype(np.float32).tolist()
+    L = np.asarray(light, np.float64)
+    v.light_dir[:] = (L / np.linalg.norm(L)).astype(np.float32).tolist()
+    v.light_color[:] = [1.0, 1.0, 1.0]
+    v.scatter_rate[:] = [albedo] * 3
+    v.alpha, v.g = alpha, g
+    v.tan_half_fov = float(np.tan(np.radians(fov_deg) / 2))
+    v.jitter = jitter
+    v.width, v.height = width, height
+    v.seed, v.frame, v.view_id, v.volume_index = seed, frame, view_id, volume_index
+    return v
