@@ -33,9 +33,18 @@ extern "C" {
 #define MRP_ERR_STATE 5   /* call out of order, e.g. render before mrp_volume_update */
 
 /* MLP arithmetic of the fused predictor. */
-#define MRP_FP32 0 /* scalar fp32, the reference's arithmetic */
-#define MRP_BF16 1 /* bf16 MFMA, fp32 accumulation */
-#define MRP_FP8 2  /* fp8 e4m3 MFMA with per-layer max-abs scales, fp32 accumulation */
+#define MRP_FP32 0    /* scalar fp32, the reference's arithmetic */
+#define MRP_BF16 1    /* bf16 MFMA, fp32 accumulation; weights as two bf16 terms (hi + lo) */
+#define MRP_FP8 2     /* fp8 e4m3 MFMA, per-layer max-abs weight scales, per-pixel activation
+                         scales, fp32 accumulation */
+#define MRP_BF16_1T 3 /* bf16 MFMA with single-term bf16 weights (fastest, least accurate) */
+
+/* Context options (mrp_ctx_set_option).  No counterpart in the reference, whose launch
+ * shapes are fixed at compile time. */
+#define MRP_OPT_MFMA_WAVES_PER_CU 1 /* persistent-grid waves per CU of the fused MFMA kernel (1..32, default 8) */
+#define MRP_OPT_EXACT_GATHER 2      /* 1: the fused MFMA kernels gather features with the FP32 path's exact
+                                       arithmetic (bit-identical features); 0 (default): fast gather from
+                                       zero-padded pyramid copies (features within a few ulps) */
 
 /* Number of features per pixel: [X_Val 0:192 | X_Val_Hg 192:384 | X_Val_Sub 384:576 | gamma]
  * (core/radiancePredict.cu:427-485, row order of PopulateInputData :562-565). */
@@ -72,7 +81,9 @@ typedef struct mrp_view {
     int32_t volume_index;  /* index into the volume array of mrp_render_batch */
 } mrp_view;
 
-/* Per-call statistics of the last render on a context. */
+/* Statistics of the last render on a context (counts, device times) and device-time sums over
+ * the renders timed since timing was switched on.  Timing records HIP events on the
+ * context's stream and never blocks a render; mrp_ctx_get_stats waits for them. */
 typedef struct mrp_stats {
     int64_t pixels;        /* width*height summed over views */
     int64_t active_pixels; /* pixels whose ray reached a first-scattering point */
@@ -80,6 +91,9 @@ typedef struct mrp_stats {
     float ms_march;        /* device time of the march kernel (when timing is on) */
     float ms_predict;      /* device time of the gather+MLP kernel(s) */
     float ms_total;        /* device time of the whole render on the stream */
+    int64_t timed_renders; /* renders timed since timing was last switched (mrp_ctx_set_timing) */
+    double ms_march_sum;   /* their summed march time */
+    double ms_predict_sum; /* their summed gather+MLP time */
 } mrp_stats;
 
 const char* mrp_last_error(void);
@@ -92,6 +106,8 @@ int mrp_ctx_destroy(mrp_ctx* ctx);
 int mrp_ctx_set_stream(mrp_ctx* ctx, void* hip_stream);
 int mrp_ctx_set_timing(mrp_ctx* ctx, int enabled);
 int mrp_ctx_get_stats(mrp_ctx* ctx, mrp_stats* out);
+int mrp_ctx_set_option(mrp_ctx* ctx, int option, int value);
+int mrp_ctx_get_option(mrp_ctx* ctx, int option, int* value);
 /* Device pointer of the MLP weights as loaded (fp32, header order). */
 int mrp_ctx_weights_device(mrp_ctx* ctx, const float** dev_weights, int* count);
 

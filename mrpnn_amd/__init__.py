@@ -12,8 +12,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmrpnn_hip.so")
 
-FP32, BF16, FP8 = 0, 1, 2
-PRECISIONS = {"fp32": FP32, "bf16": BF16, "fp8": FP8}
+FP32, BF16, FP8, BF16_1T = 0, 1, 2, 3
+PRECISIONS = {"fp32": FP32, "bf16": BF16, "fp8": FP8, "bf16_1t": BF16_1T}
+OPT_MFMA_WAVES_PER_CU, OPT_EXACT_GATHER = 1, 2
 NFEAT = 577
 MIP_TOTAL = sum((256 >> m) ** 3 for m in range(9))
 TR_TOTAL = sum((128 >> m) ** 3 for m in range(8))
@@ -40,7 +41,9 @@ class View(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("pixels", ctypes.c_int64), ("active_pixels", ctypes.c_int64), ("march_taps", ctypes.c_int64),
-                ("ms_march", ctypes.c_float), ("ms_predict", ctypes.c_float), ("ms_total", ctypes.c_float)]
+                ("ms_march", ctypes.c_float), ("ms_predict", ctypes.c_float), ("ms_total", ctypes.c_float),
+                ("timed_renders", ctypes.c_int64), ("ms_march_sum", ctypes.c_double),
+                ("ms_predict_sum", ctypes.c_double)]
 
 
 _lib = None
@@ -66,7 +69,8 @@ def lib():
         L.mrp_ctx_update_hg_lut.argtypes = [_vp, ctypes.c_float, ctypes.c_uint32]
         L.mrp_ctx_download_hg_lut.argtypes = [_vp, _fp]
         L.mrp_ctx_upload_hg_lut.argtypes = [_vp, _fp, ctypes.c_float]
-        L.mrp_ctx_set_mfma_waves.argtypes = [_vp, ctypes.c_int]
+        L.mrp_ctx_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.mrp_ctx_get_option.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.mrp_volume_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
         L.mrp_volume_destroy.argtypes = [_vp]
         L.mrp_volume_set_data.argtypes = [_vp, _fp, ctypes.c_float]
@@ -127,8 +131,19 @@ class Context:
         check(lib().mrp_ctx_get_stats(self.h, ctypes.byref(s)))
         return s
 
+    def set_option(self, option, value):
+        check(lib().mrp_ctx_set_option(self.h, int(option), int(value)))
+
+    def get_option(self, option):
+        v = ctypes.c_int()
+        check(lib().mrp_ctx_get_option(self.h, int(option), ctypes.byref(v)))
+        return v.value
+
     def set_mfma_waves(self, w):
-        check(lib().mrp_ctx_set_mfma_waves(self.h, int(w)))
+        self.set_option(OPT_MFMA_WAVES_PER_CU, w)
+
+    def set_exact_gather(self, on):
+        self.set_option(OPT_EXACT_GATHER, 1 if on else 0)
 
     def update_hg_lut(self, g, seed=1):
         check(lib().mrp_ctx_update_hg_lut(self.h, float(g), int(seed)))

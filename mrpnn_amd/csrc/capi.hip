@@ -148,8 +148,11 @@ extern "C" int mrp_ctx_create(const char* weights_path, mrp_ctx** out) {
     HIP_TRY(hipMalloc(&c->d_stencil, sizeof st));
     HIP_TRY(hipMemcpy(c->d_stencil, st, sizeof st, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->d_lut, 512 * 512 * sizeof(float)));
+    HIP_TRY(hipMalloc(&c->d_fast, kPC * sizeof(FastStencil)));
     HIP_TRY(hipMemset(c->d_lut, 0, 512 * 512 * sizeof(float)));
     for (int i = 0; i < 4; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
+    for (int i = 0; i < TimingRing::kSlots; ++i)
+        for (int j = 0; j < 3; ++j) HIP_TRY(hipEventCreate(&c->ring.ev[i][j]));
     int prc = mrp_fused_init(c);
     if (prc) return prc;
     *out = c;
@@ -163,8 +166,11 @@ extern "C" int mrp_ctx_destroy(mrp_ctx* c) {
     hipFree(c->d_weights);
     hipFree(c->d_stencil);
     hipFree(c->d_lut);
+    hipFree(c->d_fast);
     hipFree(c->d_plain_out);
     for (int i = 0; i < 4; ++i) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < TimingRing::kSlots; ++i)
+        for (int j = 0; j < 3; ++j) hipEventDestroy(c->ring.ev[i][j]);
     delete c;
     return MRP_OK;
 }
@@ -174,9 +180,51 @@ extern "C" int mrp_ctx_set_stream(mrp_ctx* c, void* s) {
     c->stream = (hipStream_t)s;
     return MRP_OK;
 }
+static int timing_read(mrp_ctx* c, int slot) {
+    TimingRing& r = c->ring;
+    if (!r.pending[slot]) return MRP_OK;
+    HIP_TRY(hipEventSynchronize(r.ev[slot][2]));
+    float a = 0.0f, b = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&a, r.ev[slot][0], r.ev[slot][1]));
+    HIP_TRY(hipEventElapsedTime(&b, r.ev[slot][1], r.ev[slot][2]));
+    r.pending[slot] = false;
+    r.count += 1;
+    r.sum_march += a;
+    r.sum_predict += b;
+    r.last_march = a;
+    r.last_predict = b;
+    return MRP_OK;
+}
+int mrp_timing_begin(mrp_ctx* c, hipEvent_t** ev) {
+    TimingRing& r = c->ring;
+    int rc = timing_read(c, r.head);
+    if (rc) return rc;
+    r.pending[r.head] = true;
+    *ev = r.ev[r.head];
+    r.head = (r.head + 1) % TimingRing::kSlots;
+    return MRP_OK;
+}
+int mrp_timing_collect(mrp_ctx* c) {
+    for (int k = 0; k < TimingRing::kSlots; ++k) {  // oldest first
+        int rc = timing_read(c, (c->ring.head + k) % TimingRing::kSlots);
+        if (rc) return rc;
+    }
+    c->stats.ms_march = c->ring.last_march;
+    c->stats.ms_predict = c->ring.last_predict;
+    c->stats.ms_total = c->ring.last_march + c->ring.last_predict;
+    c->stats.timed_renders = c->ring.count;
+    c->stats.ms_march_sum = c->ring.sum_march;
+    c->stats.ms_predict_sum = c->ring.sum_predict;
+    return MRP_OK;
+}
+
 extern "C" int mrp_ctx_set_timing(mrp_ctx* c, int enabled) {
     CHECK_ARG(c, "ctx is null");
+    int rc = mrp_timing_collect(c);
+    if (rc) return rc;
     c->timing = enabled != 0;
+    c->ring.count = 0;
+    c->ring.sum_march = c->ring.sum_predict = 0.0;
     return MRP_OK;
 }
 extern "C" int mrp_ctx_get_stats(mrp_ctx* c, mrp_stats* out) {
@@ -198,6 +246,7 @@ extern "C" int mrp_ctx_update_hg_lut(mrp_ctx* c, float g, uint32_t seed) {
     CHECK_ARG(g > -1.0f && g < 1.0f, "g must be in (-1, 1)");
     if (c->lut_valid && c->lut_g == g && c->lut_seed == seed) return MRP_OK;  // volume.cu:1403
     HIP_TRY(launch_hg_lut(c->d_lut, g, seed, c->stream));
+    HIP_TRY(launch_fast_stencil(c->d_stencil, c->d_lut, c->d_fast, c->stream));
     c->lut_valid = true;
     c->lut_g = g;
     c->lut_seed = seed;
@@ -212,6 +261,7 @@ extern "C" int mrp_ctx_download_hg_lut(mrp_ctx* c, float* host) {
 extern "C" int mrp_ctx_upload_hg_lut(mrp_ctx* c, const float* host, float g) {
     CHECK_ARG(c && host, "null argument");
     HIP_TRY(hipMemcpyAsync(c->d_lut, host, 512 * 512 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_fast_stencil(c->d_stencil, c->d_lut, c->d_fast, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->lut_valid = true;
     c->lut_g = g;
@@ -226,7 +276,7 @@ static const size_t kMipFloats = 19173961;  // sum (256>>m)^3, m = 0..8
 static const size_t kTrFloats = 2396745;    // sum (128>>m)^3, m = 0..7
 
 extern "C" int64_t mrp_volume_bytes(int N) {
-    return (int64_t)((size_t)N * N * N + kMipFloats + kTrFloats) * 4;
+    return (int64_t)((size_t)N * N * N + kMipFloats + kTrFloats + kPadMipFloats + kPadTrFloats) * 4;
 }
 
 extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
@@ -243,12 +293,18 @@ extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
         return mrp_fail(MRP_ERR_NOMEM, "grid allocation (%zu bytes) failed", n * 4);
     }
     if (hipMalloc(&v->d_mips, kMipFloats * sizeof(float)) != hipSuccess ||
-        hipMalloc(&v->d_tr, kTrFloats * sizeof(float)) != hipSuccess) {
+        hipMalloc(&v->d_tr, kTrFloats * sizeof(float)) != hipSuccess ||
+        hipMalloc(&v->d_pmips, kPadMipFloats * sizeof(float)) != hipSuccess ||
+        hipMalloc(&v->d_ptr, kPadTrFloats * sizeof(float)) != hipSuccess) {
         hipFree(v->d_grid);
         hipFree(v->d_mips);
+        hipFree(v->d_tr);
+        hipFree(v->d_pmips);
         delete v;
         return mrp_fail(MRP_ERR_NOMEM, "pyramid allocation failed");
     }
+    HIP_TRY(hipMemsetAsync(v->d_pmips, 0, kPadMipFloats * sizeof(float), c->stream));
+    HIP_TRY(hipMemsetAsync(v->d_ptr, 0, kPadTrFloats * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_grid, 0, n * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_mips, 0, kMipFloats * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_tr, 0, kTrFloats * sizeof(float), c->stream));
@@ -272,6 +328,8 @@ extern "C" int mrp_volume_destroy(mrp_volume* v) {
     hipFree(v->d_grid);
     hipFree(v->d_mips);
     hipFree(v->d_tr);
+    hipFree(v->d_pmips);
+    hipFree(v->d_ptr);
     delete v;
     return MRP_OK;
 }
@@ -357,6 +415,11 @@ extern "C" int mrp_volume_update(mrp_volume* v) {
         src = v->mip[m];
         sres = r;
     }
+    {
+        const float* mips[9];
+        for (int m = 0; m < 9; ++m) mips[m] = v->mip[m];
+        HIP_TRY(launch_pad_mips(mips, v->d_pmips, s));
+    }
     if (tmp[0] || tmp[1]) {
         HIP_TRY(hipStreamSynchronize(s));
         hipFree(tmp[0]);
@@ -379,7 +442,7 @@ int mrp_volume_update_tr_impl(mrp_volume* v, const float L[3], float alpha) {
     float* tr[8];
     for (int m = 0; m < 9; ++m) mips[m] = v->mip[m];
     for (int m = 0; m < 8; ++m) tr[m] = v->trm[m];
-    HIP_TRY(launch_tr_bake(mips, tr, alpha_lvl, mk3(L[0], L[1], L[2]), v->ctx->stream));
+    HIP_TRY(launch_tr_bake(mips, tr, v->d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), v->ctx->stream));
     v->tr_valid = true;
     memcpy(v->tr_light, L, sizeof v->tr_light);
     v->tr_alpha = alpha;
@@ -444,6 +507,8 @@ DevVolume mrp_dev_volume(const mrp_volume* v) {
     for (int m = 0; m < 9; ++m) d.mips[m] = v->mip[m];
     for (int m = 0; m < 8; ++m) d.tr[m] = v->trm[m];
     d.lut = v->ctx->d_lut;
+    d.pmips = v->d_pmips;
+    d.ptr = v->d_ptr;
     d.N = v->N;
     d.max_density = v->max_density;
     return d;
@@ -468,12 +533,15 @@ extern "C" int mrp_render_plain_device(mrp_ctx* c, mrp_volume* v, const mrp_view
         return mrp_fail(MRP_ERR_STATE, "LUT / TR mips not prepared for this view");
     DevVolume dv = mrp_dev_volume(v);
     ViewDev d = mrp_to_dev(vw);
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    HIP_TRY(launch_predict_plain(dv, c->d_stencil, c->d_weights, d, dev_out4, 1.0f, 1, c->stream));
+    hipEvent_t* ev = nullptr;
     if (c->timing) {
-        HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-        c->pending_plain_timing = true;
+        rc = mrp_timing_begin(c, &ev);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(ev[0], c->stream));
+        HIP_TRY(hipEventRecord(ev[1], c->stream));
     }
+    HIP_TRY(launch_predict_plain(dv, c->d_stencil, c->d_weights, d, dev_out4, 1.0f, 1, c->stream));
+    if (c->timing) HIP_TRY(hipEventRecord(ev[2], c->stream));
     c->stats.pixels = (int64_t)vw->width * vw->height;
     c->stats.active_pixels = -1;
     c->stats.march_taps = -1;

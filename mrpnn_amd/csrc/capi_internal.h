@@ -9,6 +9,21 @@
 
 struct FusedState;  // fused.hip
 
+// Device timing of renders without a host sync per render: each timed render records three
+// events (start, after the march, end) into a ring slot; a slot is read back when it is
+// reused or when the statistics are requested.
+struct TimingRing {
+    static constexpr int kSlots = 32;
+    hipEvent_t ev[kSlots][3]{};
+    bool pending[kSlots]{};
+    int head = 0;
+    int64_t count = 0;
+    double sum_march = 0.0, sum_predict = 0.0;
+    float last_march = 0.0f, last_predict = 0.0f;
+};
+int mrp_timing_begin(struct mrp_ctx* c, hipEvent_t** ev);  // ev = the three events of a fresh slot
+int mrp_timing_collect(struct mrp_ctx* c);                  // reads back every pending slot
+
 struct mrp_ctx {
     int device = 0;
     hipDeviceProp_t props{};
@@ -17,14 +32,16 @@ struct mrp_ctx {
     float* d_weights = nullptr;  // fp32, header order
     mrp::StencilEntry* d_stencil = nullptr;
     float* d_lut = nullptr;  // 512x512
+    void* d_fast = nullptr;  // FastStencil[192], rebuilt with the LUT
     bool lut_valid = false;
     float lut_g = 0.0f;
     uint32_t lut_seed = 0;
     float* d_plain_out = nullptr;
     size_t plain_out_bytes = 0;
     bool timing = false;
-    bool pending_plain_timing = false;
+    bool exact_gather = false;  // fused MFMA kernels gather with the exact (plain) arithmetic
     hipEvent_t ev[4]{};
+    TimingRing ring;
     mrp_stats stats{};
     FusedState* fused = nullptr;
 };
@@ -36,6 +53,8 @@ struct mrp_volume {
     float* d_grid = nullptr;
     float* d_mips = nullptr;
     float* d_tr = nullptr;
+    float* d_pmips = nullptr;  // zero-padded density mips 0..7
+    float* d_ptr = nullptr;    // zero-padded TR mips 0..6
     float* mip[9]{};
     float* trm[8]{};
     bool mips_valid = false;

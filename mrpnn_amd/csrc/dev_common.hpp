@@ -102,6 +102,8 @@ struct DevVolume {
     const float* mips[9];  // (256>>m)^3
     const float* tr[8];    // (128>>m)^3
     const float* lut;      // 512*512, [j*512+i]
+    const float* pmips;    // zero-padded density mips 0..7 (fast gather)
+    const float* ptr;      // zero-padded TR mips 0..6 (fast gather)
     int N;
     float max_density;
 };

@@ -4,7 +4,7 @@
 //   * untemplated LinearReLU / *_Unbias / LinearSm_Unbias: zero start, bias last (:41-86);
 //   * every multiply-accumulate is one fmaf (nvcc contracts `Out += In * W`);
 //   * AVGMAX: sequential fp32 sum divided by n, max from -10000 (:21-33);
-//   * sigmoid computed as 1.0/(1.0+exp(-x)) in double from an fp32 exp (:84).
+//   * sigmoid as 1.0/(1.0+exp(-x)) in double (:84); exp and pow rounded once from FP64.
 // Weights are the fp32 asset in header order, addressed by the offsets in net_layout.h;
 // their addresses are wavefront-uniform, so loads go through the scalar cache.
 // This is the drop-in baseline and the FP32 parity target; the fast path lives in the
@@ -24,7 +24,10 @@ struct Fp32Mlp {
     float pool[75], aw[16], awp[6], tap[8], comb[128];
 
     __device__ static float relu(float x) { return x >= 0.0f ? x : 0.0f; }
-    __device__ static float sigm(float x) { return (float)(1.0 / (1.0 + (double)expf(-x))); }
+    // exp / pow are evaluated in FP64 and rounded once (correctly rounded): device and host libm
+    // differ by an ulp in fp32, which exp(P)-1 amplifies past 1e-5 for small outputs.
+    __device__ static float sigm(float x) { return (float)(1.0 / (1.0 + exp(-(double)x))); }
+    __device__ static float expr(float x) { return (float)exp((double)x); }
 
     __device__ void lin_relu_bf(const float* in, float* out, int from, int to, int w, int b) {
         for (int i = 0; i < to; ++i) {
@@ -130,7 +133,7 @@ struct Fp32Mlp {
         const int DI = 160, P_DI = 24;
         tap[6] = g;
         tap[7] = gamma;
-        float srp[3] = {powf(sr.x, 4.0f), powf(sr.y, 4.0f), powf(sr.z, 4.0f)};
+        float srp[3] = {(float)pow((double)sr.x, 4.0), (float)pow((double)sr.y, 4.0), (float)pow((double)sr.z, 4.0)};
         pool[72] = g;
         pool[73] = gamma;
         block(-1, 0, 0, 8, 0, LSE01W, LSE02W, L01W, L01B, L_Tr01W, L_Tr01B, L_Hg01W, L_Hg01B);
@@ -166,8 +169,8 @@ struct Fp32Mlp {
         float P0 = tail(srp[0]);
         float P1 = (sr.y == sr.x) ? P0 : tail(srp[1]);
         float P2 = (sr.z == sr.x) ? P0 : ((sr.z == sr.y) ? P1 : tail(srp[2]));
-        return mk3(fmaxf(expf(P0) - 1.0f, 0.0f) * srp[0], fmaxf(expf(P1) - 1.0f, 0.0f) * srp[1],
-                   fmaxf(expf(P2) - 1.0f, 0.0f) * srp[2]);
+        return mk3(fmaxf(expr(P0) - 1.0f, 0.0f) * srp[0], fmaxf(expr(P1) - 1.0f, 0.0f) * srp[1],
+                   fmaxf(expr(P2) - 1.0f, 0.0f) * srp[2]);
     }
 };
 

@@ -22,7 +22,6 @@ struct FusedState {
     DevVolume* h_vols = nullptr;  // pinned staging
     ViewSlot* h_slots = nullptr;
     hipEvent_t staged = nullptr;  // staging buffers free once this completes
-    bool timed = false;
     bool have_counts = false;
     MfmaNet net;  // packed BF16 / FP8 weights
     int mfma_waves_per_cu = 8;
@@ -65,24 +64,8 @@ void mrp_fused_free(mrp_ctx* c) {
 
 int mrp_fused_collect_stats(mrp_ctx* c) {
     FusedState* f = c->fused;
-    if (c->timing && f->timed) {
-        float a = 0, b = 0;
-        if (hipEventSynchronize(c->ev[2]) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "event sync failed");
-        hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-        hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
-        c->stats.ms_march = a;
-        c->stats.ms_predict = b;
-        c->stats.ms_total = a + b;
-        f->timed = false;
-    } else if (c->timing && c->pending_plain_timing) {
-        float a = 0;
-        if (hipEventSynchronize(c->ev[1]) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "event sync failed");
-        hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-        c->stats.ms_march = 0;
-        c->stats.ms_predict = a;
-        c->stats.ms_total = a;
-        c->pending_plain_timing = false;
-    }
+    int rc = mrp_timing_collect(c);
+    if (rc) return rc;
     if (f->have_counts) {
         unsigned cnt = 0;
         unsigned long long taps = 0;
@@ -124,7 +107,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
                                 int precision, float* const* dev_out4) {
     if (!c || !vols || !views || !dev_out4 || n_vols < 1 || n_views < 1)
         return mrp_fail(MRP_ERR_INVALID, "bad batch arguments");
-    if (precision != MRP_FP32 && precision != MRP_BF16 && precision != MRP_FP8)
+    if (precision != MRP_FP32 && precision != MRP_BF16 && precision != MRP_FP8 && precision != MRP_BF16_1T)
         return mrp_fail(MRP_ERR_INVALID, "unknown precision %d", precision);
     FusedState* f = c->fused;
     if (!c->lut_valid || c->lut_g != views[0].g) return mrp_fail(MRP_ERR_STATE, "HG LUT not prepared for g");
@@ -194,22 +177,24 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
     HT(hipMemcpyAsync(f->d_vols, f->h_vols, n_vols * sizeof(DevVolume), hipMemcpyHostToDevice, st));
     HT(hipMemcpyAsync(f->d_slots, f->h_slots, n_views * sizeof(ViewSlot), hipMemcpyHostToDevice, st));
     HT(hipEventRecord(f->staged, st));
-    if (c->timing) HT(hipEventRecord(c->ev[0], st));
+    hipEvent_t* ev = nullptr;
+    if (c->timing) {
+        int rc = mrp_timing_begin(c, &ev);
+        if (rc) return rc;
+        HT(hipEventRecord(ev[0], st));
+    }
     HT(launch_reset_counters(f->d_counter, f->d_taps, st));
     HT(launch_march_compact(f->d_vols, f->d_slots, n_views, n_tiles, f->d_recs, f->d_counter, f->d_taps, st));
-    if (c->timing) HT(hipEventRecord(c->ev[1], st));
+    if (c->timing) HT(hipEventRecord(ev[1], st));
     if (precision == MRP_FP32) {
         HT(launch_predict_active_fp32(f->d_vols, f->d_slots, c->d_stencil, c->d_weights, f->d_recs, f->d_counter,
                                       (int)need, st));
     } else {
         int grid = c->props.multiProcessorCount * f->mfma_waves_per_cu / 4;
-        HT(launch_mlp_mfma_active(precision, f->net, f->d_vols, f->d_slots, c->d_stencil, c->d_weights, f->d_recs,
-                                  f->d_counter, grid, st));
+        HT(launch_mlp_mfma_active(precision, !c->exact_gather, f->net, f->d_vols, f->d_slots, c->d_stencil, c->d_fast,
+                                  c->d_weights, f->d_recs, f->d_counter, grid, st));
     }
-    if (c->timing) {
-        HT(hipEventRecord(c->ev[2], st));
-        f->timed = true;
-    }
+    if (c->timing) HT(hipEventRecord(ev[2], st));
     f->have_counts = true;
     c->stats.pixels = (int64_t)pixels;
     c->stats.active_pixels = -1;
@@ -220,16 +205,34 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
 
 int mrp_fused_predict_features(mrp_ctx* c, const float* dev_feats, int n, const float sr[3], float g, int precision,
                                float* dev_rgb) {
-    if (precision != MRP_BF16 && precision != MRP_FP8) return mrp_fail(MRP_ERR_INVALID, "unknown precision %d", precision);
+    if (precision != MRP_BF16 && precision != MRP_FP8 && precision != MRP_BF16_1T)
+        return mrp_fail(MRP_ERR_INVALID, "unknown precision %d", precision);
     hipError_t e = launch_mlp_mfma_features(precision, c->fused->net, c->d_weights, dev_feats, n, sr, g, dev_rgb,
                                             c->stream);
     if (e != hipSuccess) return mrp_fail(MRP_ERR_HIP, "mfma predictor: %s", hipGetErrorString(e));
     return MRP_OK;
 }
 
-// Tuning hook used by the autotuner: persistent-grid waves per CU of the MFMA kernel.
-extern "C" int mrp_ctx_set_mfma_waves(mrp_ctx* c, int waves_per_cu) {
-    if (!c || waves_per_cu < 1 || waves_per_cu > 32) return mrp_fail(MRP_ERR_INVALID, "waves_per_cu in [1, 32]");
-    c->fused->mfma_waves_per_cu = waves_per_cu;
-    return MRP_OK;
+extern "C" int mrp_ctx_set_option(mrp_ctx* c, int option, int value) {
+    if (!c) return mrp_fail(MRP_ERR_INVALID, "ctx is null");
+    switch (option) {
+        case MRP_OPT_MFMA_WAVES_PER_CU:
+            if (value < 1 || value > 32) return mrp_fail(MRP_ERR_INVALID, "MRP_OPT_MFMA_WAVES_PER_CU in [1, 32]");
+            c->fused->mfma_waves_per_cu = value;
+            return MRP_OK;
+        case MRP_OPT_EXACT_GATHER:
+            c->exact_gather = value != 0;
+            return MRP_OK;
+        default:
+            return mrp_fail(MRP_ERR_INVALID, "unknown option %d", option);
+    }
+}
+
+extern "C" int mrp_ctx_get_option(mrp_ctx* c, int option, int* value) {
+    if (!c || !value) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    switch (option) {
+        case MRP_OPT_MFMA_WAVES_PER_CU: *value = c->fused->mfma_waves_per_cu; return MRP_OK;
+        case MRP_OPT_EXACT_GATHER: *value = c->exact_gather ? 1 : 0; return MRP_OK;
+        default: return mrp_fail(MRP_ERR_INVALID, "unknown option %d", option);
+    }
 }

@@ -75,7 +75,8 @@ constexpr int kMfmaFrags = mfma_frag(ML_COUNT);
 constexpr int kMfmaBias = mfma_bias(ML_COUNT);
 
 struct MfmaNet {
-    const uint4* bf16 = nullptr;  // fragments, 16 bytes per lane
+    const uint4* bf16 = nullptr;     // fragments, 16 bytes per lane: bf16(W)
+    const uint4* bf16_lo = nullptr;  // bf16(W - bf16(W)): second term of the two-term weights
     const uint2* fp8 = nullptr;   // fragments, 8 bytes per lane
     const float* bias = nullptr;
     const int* wexp = nullptr;    // FP8: layer weights stored as fp8(W * 2^wexp[id])
@@ -88,9 +89,10 @@ void mfma_net_destroy(MfmaNet* net);
 uint16_t host_f32_to_bf16(float f);
 uint8_t host_f32_to_e4m3(float f);
 
-hipError_t launch_mlp_mfma_active(int precision, const MfmaNet& net, const DevVolume* vols, const ViewSlot* slots,
-                                  const StencilEntry* stencil, const float* W, const ActiveRec* recs,
-                                  const unsigned* counter, int grid, hipStream_t s);
+hipError_t launch_mlp_mfma_active(int precision, bool fast, const MfmaNet& net, const DevVolume* vols,
+                                  const ViewSlot* slots, const StencilEntry* stencil, const void* fast_stencil,
+                                  const float* W, const ActiveRec* recs, const unsigned* counter, int grid,
+                                  hipStream_t s);
 hipError_t launch_mlp_mfma_features(int precision, const MfmaNet& net, const float* W, const float* feats, int n,
                                     const float sr[3], float g, float* rgb, hipStream_t s);
 

@@ -1,4 +1,5 @@
-// MRPNN network on MFMA (gfx950): BF16 and FP8 e4m3, FP32 accumulation.  Layout and
+// MRPNN network on MFMA (gfx950): BF16 (weights as one or two bf16 terms) and FP8 e4m3, FP32
+// accumulation.  Layout and
 // K-slot order are described in mlp_mfma.h.  The dataflow is RadiancePredictInternal
 // (core/radiancePredict.cu:301-422):
 //   * 12 SE/SERES blocks (3 branches each) as 16x16x32 MFMAs; gate MLPs (8->8->3), pooling
@@ -18,6 +19,7 @@
 #include "capi_internal.h"
 #include "dev_common.hpp"
 #include "mlp_mfma.h"
+#include "dev_gather_fast.hpp"
 #include "net_layout.h"
 
 namespace mrp {
@@ -106,7 +108,7 @@ void layer_specs(Spec* s) {
 int mfma_net_create(const float* W, MfmaNet* out) {
     Spec spec[ML_COUNT];
     layer_specs(spec);
-    std::vector<uint16_t> bf;  // 8 per lane per fragment
+    std::vector<uint16_t> bf, bflo;  // 8 per lane per fragment
     std::vector<uint8_t> f8;
     std::vector<float> bias;
     std::vector<int> wexp(ML_COUNT, 0);
@@ -131,7 +133,12 @@ int mfma_net_create(const float* W, MfmaNet* out) {
                         if (s.k32) k = 32 * kc + (slot < 4 ? 4 * q + slot : 16 + 4 * q + slot - 4);
                         else k = slot < 4 ? 4 * q + slot : -1;
                         float v = (row >= 0 && row < s.M && k >= 0 && k < s.K) ? W[s.w + row * s.K + k] : 0.0f;
-                        bf.push_back(host_f32_to_bf16(v));
+                        const uint16_t hi = host_f32_to_bf16(v);
+                        const uint32_t hb = (uint32_t)hi << 16;
+                        float hf;
+                        memcpy(&hf, &hb, 4);
+                        bf.push_back(hi);
+                        bflo.push_back(host_f32_to_bf16(v - hf));
                         f8.push_back(host_f32_to_e4m3(v * wsc));
                     }
                 }
@@ -143,16 +150,19 @@ int mfma_net_create(const float* W, MfmaNet* out) {
                 bias.push_back(src >= 0 && src < s.M ? W[s.b + src] : 0.0f);
             }
     }
-    void *dbf = nullptr, *df8 = nullptr, *db = nullptr, *dw = nullptr;
-    if (hipMalloc(&dbf, bf.size() * 2) != hipSuccess || hipMalloc(&df8, f8.size()) != hipSuccess ||
+    void *dbf = nullptr, *dbl = nullptr, *df8 = nullptr, *db = nullptr, *dw = nullptr;
+    if (hipMalloc(&dbf, bf.size() * 2) != hipSuccess || hipMalloc(&dbl, bflo.size() * 2) != hipSuccess ||
+        hipMalloc(&df8, f8.size()) != hipSuccess ||
         hipMalloc(&db, bias.size() * 4) != hipSuccess || hipMalloc(&dw, wexp.size() * 4) != hipSuccess)
         return mrp_fail(MRP_ERR_NOMEM, "MFMA weight allocation failed");
     if (hipMemcpy(dbf, bf.data(), bf.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dbl, bflo.data(), bflo.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(df8, f8.data(), f8.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(db, bias.data(), bias.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dw, wexp.data(), wexp.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
         return mrp_fail(MRP_ERR_HIP, "MFMA weight upload failed");
     out->bf16 = (const uint4*)dbf;
+    out->bf16_lo = (const uint4*)dbl;
     out->fp8 = (const uint2*)df8;
     out->bias = (const float*)db;
     out->wexp = (const int*)dw;
@@ -161,6 +171,7 @@ int mfma_net_create(const float* W, MfmaNet* out) {
 
 void mfma_net_destroy(MfmaNet* net) {
     hipFree((void*)net->bf16);
+    hipFree((void*)net->bf16_lo);
     hipFree((void*)net->fp8);
     hipFree((void*)net->bias);
     hipFree((void*)net->wexp);
@@ -196,13 +207,14 @@ __device__ inline void dense(const MfmaNet& net, const float (&in)[mfma_nk(ID)][
                              int lane, bool relu) {
     constexpr int NM = mfma_nm(ID), NK = mfma_nk(ID), FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
     const int q = lane >> 4;
-    if constexpr (PREC == MRP_BF16) {
+    if constexpr (PREC == MRP_BF16 || PREC == MRP_BF16_1T) {
         u32x4_t b[NK];
 #pragma unroll
         for (int kc = 0; kc < NK; ++kc)
             b[kc] = u32x4_t{pack_bf16x2(in[kc][0], in[kc][1]), pack_bf16x2(in[kc][2], in[kc][3]),
                             pack_bf16x2(in[kc][4], in[kc][5]), pack_bf16x2(in[kc][6], in[kc][7])};
         const u32x4_t* wf = (const u32x4_t*)net.bf16 + (size_t)FRAG * 64 + lane;
+        const u32x4_t* wl = (const u32x4_t*)net.bf16_lo + (size_t)FRAG * 64 + lane;
 #pragma unroll
         for (int mt = 0; mt < NM; ++mt) {
             f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
@@ -211,6 +223,11 @@ __device__ inline void dense(const MfmaNet& net, const float (&in)[mfma_nk(ID)][
                 u32x4_t a = wf[(mt * NK + kc) * 64];
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
                                                               __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
+                if constexpr (PREC == MRP_BF16) {  // second weight term
+                    u32x4_t al = wl[(mt * NK + kc) * 64];
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, al),
+                                                                  __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
+                }
             }
             float4 bb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if constexpr (mfma_has_bias(ID)) bb = *(const float4*)(net.bias + BIAS + 16 * mt + 4 * q);
@@ -352,6 +369,34 @@ struct GatherSrc {
             for (int i = 0; i < 4; ++i) {
                 const int k = p0 + 16 * h + 4 * lc.q + i;
                 v3 f = lc.ok ? gather_point(*vol, st[k], k, ps, g, alpha) : mk3(0.0f, 0.0f, 0.0f);
+                F[0][4 * h + i] = f.x;
+                F[2][4 * h + i] = f.y;
+                F[1][4 * h + i] = f.z;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if (!wide)
+#pragma unroll
+            for (int i = 4; i < 8; ++i) F[0][i] = F[1][i] = F[2][i] = 0.0f;
+    }
+};
+// Fast gather (dev_gather_fast.hpp) from the zero-padded pyramid copies.
+struct FastGatherSrc {
+    const FastStencil* st;  // LDS copy
+    FastView fv;
+    FastPixel px;
+    __device__ void get(const LaneCtx& lc, int p0, bool wide, float (&F)[3][8]) const {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !wide) break;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = p0 + 16 * h + 4 * lc.q + i;
+                v3 f = mk3(0.0f, 0.0f, 0.0f);
+                if (lc.ok) {
+                    if (p0 >= 160) f = gather_fast<true>(fv, st[k], px, false);
+                    else f = gather_fast<false>(fv, st[k], px, p0 == 0 && h == 0 && i == 0 && lc.q == 0);
+                }
                 F[0][4 * h + i] = f.x;
                 F[2][4 * h + i] = f.y;
                 F[1][4 * h + i] = f.z;
@@ -717,15 +762,22 @@ __device__ inline void load_shared(float* sW1, StencilEntry* sst, const float* _
     __syncthreads();
 }
 
-template <int PREC>
+template <int PREC, bool FAST>
 __global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevVolume* __restrict__ vols,
                                                          const ViewSlot* __restrict__ slots,
                                                          const StencilEntry* __restrict__ stencil,
+                                                         const FastStencil* __restrict__ fast,
                                                          const float* __restrict__ W, const ActiveRec* __restrict__ recs,
                                                          const unsigned* __restrict__ counter) {
     __shared__ float sW1[1200];
-    __shared__ StencilEntry sst[kPC];
-    load_shared(sW1, sst, W, stencil);
+    __shared__ StencilEntry sst[FAST ? 1 : kPC];
+    __shared__ FastStencil sfs[FAST ? kPC : 1];
+    if (FAST) {
+        for (int i = threadIdx.x; i < kPC; i += blockDim.x) sfs[i] = fast[i];
+        load_shared(sW1, nullptr, W, nullptr);
+    } else {
+        load_shared(sW1, sst, W, stencil);
+    }
     const unsigned n = *counter;
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -742,29 +794,57 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevV
         lc.ok = r.pixel >= 0;
         const ViewSlot& sl = slots[view];
         const ViewDev vw = sl.vw;
-        GatherSrc src;
-        src.vol = &vols[sl.vol];
-        src.st = sst;
-        src.g = vw.g;
-        src.alpha = vw.alpha;
-        src.ps.pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
-        src.ps.X = mk3(r.X[0], r.X[1], r.X[2]);
-        src.ps.Y = mk3(r.Y[0], r.Y[1], r.Y[2]);
-        src.ps.Z = mk3(r.Z[0], r.Z[1], r.Z[2]);
+        PixelState ps;
+        ps.pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
+        ps.X = mk3(r.X[0], r.X[1], r.X[2]);
+        ps.Y = mk3(r.Y[0], r.Y[1], r.Y[2]);
+        ps.Z = mk3(r.Z[0], r.Z[1], r.Z[2]);
         const v3 light = normalize3(vw.light_dir);
-        src.ps.L = light;
-        src.ps.LX = normalize3(light);
-        src.ps.dist = r.dist;
-        src.ps.active = 1;
-        src.ps.taps = 0;
+        ps.L = light;
+        ps.LX = normalize3(light);
+        ps.dist = r.dist;
+        ps.active = 1;
+        ps.taps = 0;
         lc.g = vw.g;
-        lc.gamma = gamma_of(src.ps);
+        lc.gamma = gamma_of(ps);
         lc.sr = vw.scatter_rate / 1.001f;
         MfmaNet nt = net;
         nt.bf16 = opaque(net.bf16);
+        nt.bf16_lo = opaque(net.bf16_lo);
         nt.fp8 = opaque(net.fp8);
         nt.bias = opaque(net.bias);
-        v3 rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
+        v3 rgb;
+        if (FAST) {
+            const DevVolume& vol = vols[sl.vol];
+            FastGatherSrc src;
+            src.st = opaque_v(sfs);
+            src.fv.pmips = vol.pmips;
+            src.fv.ptr = vol.ptr;
+            src.fv.lut = vol.lut;
+            src.fv.L = ps.L;
+            src.fv.LX = ps.LX;
+            src.fv.invLX = inv3(ps.LX);
+            src.fv.alpha64 = vw.alpha / 64.0f;
+            src.px.pos = ps.pos;
+            src.px.X = ps.X;
+            src.px.Y = ps.Y;
+            src.px.Z = ps.Z;
+            src.px.ax = dot3(ps.LX, ps.X);
+            src.px.ay = dot3(ps.LX, ps.Y);
+            src.px.az = dot3(ps.LX, ps.Z);
+            src.px.uc1 = fmaf(dot3(ps.X, ps.LX), 0.5f, 0.5f);
+            src.px.uc2 = fmaf(dot3(ps.LX, ps.LX), 0.5f, 0.5f);
+            src.px.xh_centre = logf(henyey_greenstein(dot3(ps.X, ps.L), vw.g) + 1.0f);
+            rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
+        } else {
+            GatherSrc src;
+            src.vol = &vols[sl.vol];
+            src.st = sst;
+            src.g = vw.g;
+            src.alpha = vw.alpha;
+            src.ps = ps;
+            rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
+        }
         if (lc.ok && lc.q == 0) {
             v3 o = (rgb * vw.light_color) * vw.scatter_rate;
             sl.out[r.pixel] = make_float4(fmaxf(0.0f, o.x), fmaxf(0.0f, o.y), fmaxf(0.0f, o.z), fmaxf(0.0f, r.dist));
@@ -795,6 +875,7 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
         lc.sr = sr;
         MfmaNet nt = net;
         nt.bf16 = opaque(net.bf16);
+        nt.bf16_lo = opaque(net.bf16_lo);
         nt.fp8 = opaque(net.fp8);
         nt.bias = opaque(net.bias);
         v3 o = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
@@ -806,15 +887,29 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
     }
 }
 
-hipError_t launch_mlp_mfma_active(int precision, const MfmaNet& net, const DevVolume* vols, const ViewSlot* slots,
-                                  const StencilEntry* stencil, const float* W, const ActiveRec* recs,
-                                  const unsigned* counter, int grid, hipStream_t s) {
-    if (precision == MRP_BF16)
-        hipLaunchKernelGGL(k_mlp_mfma_active<MRP_BF16>, dim3(grid), dim3(256), 0, s, net, vols, slots, stencil, W, recs,
-                           counter);
+template <int PREC>
+static void launch_active_prec(bool fast, const MfmaNet& net, const DevVolume* vols, const ViewSlot* slots,
+                               const StencilEntry* stencil, const FastStencil* fst, const float* W,
+                               const ActiveRec* recs, const unsigned* counter, int grid, hipStream_t s) {
+    if (fast)
+        hipLaunchKernelGGL((k_mlp_mfma_active<PREC, true>), dim3(grid), dim3(256), 0, s, net, vols, slots, stencil, fst,
+                           W, recs, counter);
     else
-        hipLaunchKernelGGL(k_mlp_mfma_active<MRP_FP8>, dim3(grid), dim3(256), 0, s, net, vols, slots, stencil, W, recs,
-                           counter);
+        hipLaunchKernelGGL((k_mlp_mfma_active<PREC, false>), dim3(grid), dim3(256), 0, s, net, vols, slots, stencil,
+                           fst, W, recs, counter);
+}
+
+hipError_t launch_mlp_mfma_active(int precision, bool fast, const MfmaNet& net, const DevVolume* vols,
+                                  const ViewSlot* slots, const StencilEntry* stencil, const void* fast_stencil,
+                                  const float* W, const ActiveRec* recs, const unsigned* counter, int grid,
+                                  hipStream_t s) {
+    const FastStencil* fst = (const FastStencil*)fast_stencil;
+    if (precision == MRP_BF16)
+        launch_active_prec<MRP_BF16>(fast, net, vols, slots, stencil, fst, W, recs, counter, grid, s);
+    else if (precision == MRP_BF16_1T)
+        launch_active_prec<MRP_BF16_1T>(fast, net, vols, slots, stencil, fst, W, recs, counter, grid, s);
+    else
+        launch_active_prec<MRP_FP8>(fast, net, vols, slots, stencil, fst, W, recs, counter, grid, s);
     return hipGetLastError();
 }
 
@@ -825,6 +920,9 @@ hipError_t launch_mlp_mfma_features(int precision, const MfmaNet& net, const flo
     if (grid > 4096) grid = 4096;
     if (precision == MRP_BF16)
         hipLaunchKernelGGL(k_mlp_mfma_features<MRP_BF16>, dim3(grid), dim3(256), 0, s, net, W, feats, n,
+                           mk3(sr[0], sr[1], sr[2]), g, rgb);
+    else if (precision == MRP_BF16_1T)
+        hipLaunchKernelGGL(k_mlp_mfma_features<MRP_BF16_1T>, dim3(grid), dim3(256), 0, s, net, W, feats, n,
                            mk3(sr[0], sr[1], sr[2]), g, rgb);
     else
         hipLaunchKernelGGL(k_mlp_mfma_features<MRP_FP8>, dim3(grid), dim3(256), 0, s, net, W, feats, n,

@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) void k_resample(const float* __restrict__ src,
 struct TrArgs {
     const float* mips[9];
     float* tr[8];
+    float* tr_pad;       // zero-padded copy of levels 0..6 (fast gather), may be null
     float alpha_lvl[8];
     long long start[9];  // flattened texel offsets of each level
     v3 light;
@@ -153,7 +154,12 @@ __global__ __launch_bounds__(256) void k_tr_bake(TrArgs a) {
             lp = lp + a.light * st;
             sum = sum + sample_border_exact(src, sres, lp + 0.5f);
         }
-        a.tr[m][l] = expf(-sum * a.alpha_lvl[m] * st) * 1.0f;
+        const float T = expf(-sum * a.alpha_lvl[m] * st) * 1.0f;
+        a.tr[m][l] = T;
+        if (a.tr_pad && m < 7) {
+            const int R = pad_dim(res);
+            a.tr_pad[ptr_offset(m) + ((long long)(x + 1) * R + (y + 1)) * R + (z + 1)] = T;
+        }
     }
 }
 
@@ -239,6 +245,52 @@ __global__ __launch_bounds__(64) void k_hg_lut(float* __restrict__ lut, float g,
 }
 
 // ------------------------------------------------------------------------------------------
+// Zero-padded copies of density mips 0..7 for the fast gather (borders stay zero from the
+// allocation-time clear).
+struct PadArgs {
+    const float* mips[8];
+    long long start[9];
+};
+__global__ __launch_bounds__(256) void k_pad_mips(PadArgs a, float* __restrict__ pm) {
+    const long long total = a.start[8];
+    for (long long g = (long long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long long)gridDim.x * 256) {
+        int m = 0;
+        while (m < 7 && g >= a.start[m + 1]) ++m;
+        const int res = 256 >> m, R = pad_dim(res);
+        const long long l = g - a.start[m];
+        const int z = (int)(l % res), y = (int)((l / res) % res), x = (int)(l / ((long long)res * res));
+        pm[pmip_offset(m) + ((long long)(x + 1) * R + (y + 1)) * R + (z + 1)] = a.mips[m][l];
+    }
+}
+
+// Fast-gather stencil table: direction * offset, LUT v, the per-point LUT lookup of sphere
+// points (dot(XMain, direction) = dx/|d| for an orthonormal basis), padded level geometry.
+__global__ void k_fast_stencil(const StencilEntry* __restrict__ st, const float* __restrict__ lut,
+                               FastStencil* __restrict__ out) {
+    const int i = threadIdx.x;
+    if (i >= kPC) return;
+    const StencilEntry e = st[i];
+    FastStencil f;
+    f.kind = i == 0 ? 0 : (e.type >= 4 ? 1 : 2);
+    f.ddx = e.dx * e.offset;
+    f.ddy = e.dy * e.offset;
+    f.ddz = e.dz * e.offset;
+    const float len = sqrtf(e.dx * e.dx + e.dy * e.dy + e.dz * e.dz);
+    f.cscale = f.kind == 1 ? 1.0f / (len * e.offset) : 0.0f;
+    f.off = e.offset;
+    f.lut_v = e.lut_v;
+    f.hg0 = f.kind == 1 ? sample_lut_exact(lut, (e.dx / len) * 0.5f + 0.5f, e.lut_v) : 0.0f;
+    const int tm = e.mip - 1 > 0 ? e.mip - 1 : 0;
+    f.dres = (float)(256 >> e.mip);
+    f.tres = (float)(128 >> tm);
+    f.dbase = (unsigned)pmip_offset(e.mip);
+    f.tbase = (unsigned)ptr_offset(tm);
+    f.dR = pad_dim(256 >> e.mip);
+    f.tR = pad_dim(128 >> tm);
+    f.pad0 = f.pad1 = 0;
+    out[i] = f;
+}
+
 static inline int grid_for(size_t total, int block) {
     size_t b = (total + block - 1) / block;
     return (int)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
@@ -264,9 +316,10 @@ hipError_t launch_resample(const float* src, int sres, float* dst, int res, hipS
                        res);
     return hipGetLastError();
 }
-hipError_t launch_tr_bake(const float* const mips[9], float* const tr[8], const float alpha_lvl[8], v3 light,
-                          hipStream_t s) {
+hipError_t launch_tr_bake(const float* const mips[9], float* const tr[8], float* tr_pad, const float alpha_lvl[8],
+                          v3 light, hipStream_t s) {
     TrArgs a;
+    a.tr_pad = tr_pad;
     long long acc = 0;
     for (int m = 0; m < 9; ++m) a.mips[m] = mips[m];
     for (int m = 0; m < 8; ++m) {
@@ -280,6 +333,23 @@ hipError_t launch_tr_bake(const float* const mips[9], float* const tr[8], const 
     hipLaunchKernelGGL(k_tr_bake, dim3(grid_for((size_t)acc, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
+hipError_t launch_pad_mips(const float* const mips[9], float* pmips, hipStream_t s) {
+    PadArgs a;
+    long long acc = 0;
+    for (int m = 0; m < 8; ++m) {
+        a.mips[m] = mips[m];
+        a.start[m] = acc;
+        acc += (long long)(256 >> m) * (256 >> m) * (256 >> m);
+    }
+    a.start[8] = acc;
+    hipLaunchKernelGGL(k_pad_mips, dim3(grid_for((size_t)acc, 256)), dim3(256), 0, s, a, pmips);
+    return hipGetLastError();
+}
+hipError_t launch_fast_stencil(const StencilEntry* st, const float* lut, void* fast, hipStream_t s) {
+    hipLaunchKernelGGL(k_fast_stencil, dim3(1), dim3(kPC), 0, s, st, lut, (FastStencil*)fast);
+    return hipGetLastError();
+}
+
 hipError_t launch_hg_lut(float* lut, float g, uint32_t seed, hipStream_t s) {
     // anisotropy weight of the mixture (volume.cu:1357-1358): |g| in float, the rest in double
     const double a = (double)fabsf(g);

@@ -76,10 +76,13 @@ const float* Net::get(const char* name) const {
 // T = double is the FP64 evaluation of the same network.
 // ---------------------------------------------------------------------------------------
 template <class T> static inline T sigm(T x);
-template <> inline float sigm<float>(float x) { return (float)(1.0 / (1.0 + (double)expf(-x))); }  // :84
+// 1.0/(1.0+exp(-x)) in double (radiancePredict.cu:84).  exp and pow of the fp32 network are
+// evaluated in FP64 and rounded once, i.e. correctly rounded, so host libm and device math
+// agree exactly (the reference's -use_fast_math versions are not reproducible anyway).
+template <> inline float sigm<float>(float x) { return (float)(1.0 / (1.0 + exp(-(double)x))); }
 template <> inline double sigm<double>(double x) { return 1.0 / (1.0 + exp(-x)); }
 template <class T> static inline T expT(T x);
-template <> inline float expT<float>(float x) { return expf(x); }
+template <> inline float expT<float>(float x) { return (float)exp((double)x); }
 template <> inline double expT<double>(double x) { return exp(x); }
 template <class T> static inline T relu(T x) { return x >= (T)0 ? x : (T)0; }
 
@@ -206,7 +209,7 @@ struct Mlp {
 };
 
 template <class T> static inline T powT(T a, T b);
-template <> inline float powT<float>(float a, float b) { return powf(a, b); }
+template <> inline float powT<float>(float a, float b) { return (float)pow((double)a, (double)b); }
 template <> inline double powT<double>(double a, double b) { return pow(a, b); }
 
 template <class T>

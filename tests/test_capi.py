@@ -48,7 +48,11 @@ def test_argument_errors_are_status_codes(lib):
     assert lib.mrp_render_batch(None, None, 0, None, 0, M.BF16, None) == 1
     assert lib.mrp_pt_render(None, None, None, 0, 1, 1, 0, 1.0, None, None) == 1
     assert lib.mrp_predict_features(None, None, 1, None, 0.5, 0, None) == 1
-    assert lib.mrp_volume_bytes(512) == (512 ** 3 + 19173961 + 2396745) * 4
+    # grid + density mips 0..8 + TR mips 0..7 + zero-padded copies ((res+3)^3) of density
+    # mips 0..7 and TR mips 0..6
+    pad_d = sum(((256 >> m) + 3) ** 3 for m in range(8))
+    pad_t = sum(((128 >> m) + 3) ** 3 for m in range(7))
+    assert lib.mrp_volume_bytes(512) == (512 ** 3 + 19173961 + 2396745 + pad_d + pad_t) * 4
 
 
 def test_version_string(lib):

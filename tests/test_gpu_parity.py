@@ -113,10 +113,11 @@ def test_batch_fp32_equals_plain(ctx, orc, scene):
     assert s.pixels == v.width * v.height and s.active_pixels % 16 == 0
 
 
-@pytest.mark.parametrize("prec,mean_tol", [("bf16", 0.02), ("fp8", 0.25)])
+@pytest.mark.parametrize("prec,mean_tol", [("bf16", 0.01), ("bf16_1t", 0.03), ("fp8", 0.25)])
 def test_mfma_predictor_error(ctx, orc, scene, prec, mean_tol):
-    """BF16 / FP8 MFMA network: error vs the FP32 predictor and the FP64 network on identical
-    features (mean relative radiance error; target <= 1% for BF16 on the benchmark cloud)."""
+    """BF16 / FP8 MFMA network vs the FP32 predictor on identical features: mean relative
+    radiance error over pixels brighter than 1e-3 of the image maximum (the target for the
+    two-term BF16 form is <= 1%)."""
     import mrpnn_amd as M
     import torch
     v, _ = _view(orc, M, 48, 40)
@@ -125,23 +126,60 @@ def test_mfma_predictor_error(ctx, orc, scene, prec, mean_tol):
     sr = np.array([0.99 / 1.001] * 3, np.float32)
     want = orc.predict_f32(f, sr, G)
     got = ctx.predict_features(torch.tensor(f, device="cuda"), sr, G, M.PRECISIONS[prec]).cpu().numpy()
-    rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-12)
+    m = want[:, 0] > 1e-3 * want[:, 0].max()
+    rel = np.abs(got[m] - want[m]) / want[m]
+    print("%s mean rel %.5f p99 %.4f" % (prec, rel.mean(), np.quantile(rel, 0.99)))
     assert rel.mean() < mean_tol
 
 
 def test_batch_mfma_matches_features_path(ctx, orc, scene):
-    """The fused gather + MFMA kernel equals the features-from-HBM MFMA kernel bit for bit
-    (same network, same features)."""
+    """With the exact gather selected, the fused gather + MFMA kernel equals the
+    features-from-HBM MFMA kernel bit for bit (same network, same features)."""
     import mrpnn_amd as M
     import torch
     v, vm = _view(orc, M)
     feats, active, pos, taps = scene["vol"].gather_features(vm)
     act = active.cpu().numpy() > 0
     sr = np.array([0.99 / 1.001] * 3, np.float32)
-    for prec in (M.BF16, M.FP8):
+    ctx.set_exact_gather(True)
+    try:
+        for prec in (M.BF16, M.BF16_1T, M.FP8):
+            out = torch.zeros((v.height * v.width, 4), device="cuda")
+            ctx.prepare_views([scene["vol"]], [vm])
+            ctx.render_batch([scene["vol"]], [vm], [out], prec)
+            rgb = ctx.predict_features(feats[torch.tensor(act, device="cuda")], sr, G, prec).cpu().numpy()
+            got = out.cpu().numpy()[act, :3]
+            np.testing.assert_allclose(got, rgb * 0.99, rtol=1e-6, atol=0)
+    finally:
+        ctx.set_exact_gather(False)
+
+
+def test_fast_gather_close_to_exact(ctx, orc, scene):
+    """The default fast gather (zero-padded pyramid copies, FMA lerps, hardware log) against
+    the exact gather, both through the BF16 network: the radiance moves by at most a few
+    BF16 roundings (mean relative difference <= 2e-3 over pixels brighter than 1e-3 of the
+    maximum), and against the FP32 predictor the fast path keeps the 1% mean bar."""
+    import mrpnn_amd as M
+    import torch
+    v, vm = _view(orc, M, 48, 40)
+    res = {}
+    for exact in (True, False):
+        ctx.set_exact_gather(exact)
         out = torch.zeros((v.height * v.width, 4), device="cuda")
         ctx.prepare_views([scene["vol"]], [vm])
-        ctx.render_batch([scene["vol"]], [vm], [out], prec)
-        rgb = ctx.predict_features(feats[torch.tensor(act, device="cuda")], sr, G, prec).cpu().numpy()
-        got = out.cpu().numpy()[act, :3]
-        np.testing.assert_allclose(got, rgb * 0.99, rtol=1e-6, atol=0)
+        ctx.render_batch([scene["vol"]], [vm], [out], M.BF16)
+        res[exact] = out.cpu().numpy()
+    ctx.set_exact_gather(False)
+    a, b = res[True][:, 0], res[False][:, 0]
+    assert np.array_equal(a > 0, b > 0) or np.abs(a - b).max() < 1e-6
+    m = a > 1e-3 * a.max()
+    d = np.abs(a[m] - b[m]) / a[m]
+    print("fast vs exact gather: mean %.2e max %.2e" % (d.mean(), d.max()))
+    assert d.mean() <= 2e-3
+    np.testing.assert_array_equal(res[True][:, 3], res[False][:, 3])
+    ref = scene["sc"].render_predict(v)
+    want = ref["out"][:, 0]
+    mw = want > 1e-3 * want.max()
+    rel = np.abs(b[mw] - want[mw]) / want[mw]
+    print("fast gather bf16 vs fp32 oracle: mean %.4f" % rel.mean())
+    assert rel.mean() < 0.01
