@@ -442,7 +442,7 @@ int mrp_volume_update_tr_impl(mrp_volume* v, const float L[3], float alpha) {
     float* tr[8];
     for (int m = 0; m < 9; ++m) mips[m] = v->mip[m];
     for (int m = 0; m < 8; ++m) tr[m] = v->trm[m];
-    HIP_TRY(launch_tr_bake(mips, tr, v->d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), v->ctx->stream));
+    HIP_TRY(launch_tr_bake(mips, v->d_pmips, tr, v->d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), v->ctx->stream));
     v->tr_valid = true;
     memcpy(v->tr_light, L, sizeof v->tr_light);
     v->tr_alpha = alpha;

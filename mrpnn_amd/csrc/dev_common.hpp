@@ -239,6 +239,16 @@ __device__ inline bool estimated_hit(const DevVolume& v, float alpha, v3 pos, v3
     return true;
 }
 
+// XCD-aware block numbering.  Workgroups are dispatched round-robin over the 8 XCDs of an
+// MI355X (block b runs on XCD b % 8) and each XCD has its own L2: renumbering blocks so that
+// every XCD owns one contiguous range of work items keeps neighbouring pixels (which read
+// the same texels) on the same L2.  Grids not divisible by 8 keep the plain order.
+constexpr int kXcds = 8;
+__device__ inline int xcd_swizzle(int b, int nb) {
+    if (nb % kXcds) return b;
+    return (b % kXcds) * (nb / kXcds) + b / kXcds;
+}
+
 // Per-pixel state after ray generation, march and basis (volume.cu:198-216,
 // render.cuh:808-823).  Pixel jitter and basis draws use Philox streams.
 struct ViewDev {

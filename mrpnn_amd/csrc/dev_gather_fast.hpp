@@ -7,12 +7,14 @@
 //     level, data at [1,res] per axis): the continuous coordinate is clamped into the padded
 //     range, so every tap is in bounds and no per-tap border test is needed; the clamp also
 //     reproduces the border (density) and clamp (TR) addressing exactly;
-//   * every tap is one uniform base pointer plus a 32-bit offset;
+//   * every tap pair (z, z+1) is one 64-bit buffer load at a 32-bit offset from a uniform
+//     buffer resource (hardware bounds-checked);
 //   * the per-point constants (direction * offset, 1/|direction|, LUT v, mip geometry) come
 //     from a 192-entry table; for sphere points the first LUT lookup depends only on the
 //     point (the basis is orthonormal, dot(XMain, direction) = dx/|d|), so it is tabulated
 //     once per LUT;
-//   * lerps are a + w*(b - a) with one FMA.
+//   * lerps are a + w*(b - a) with one FMA; ln(1 + x) of the non-negative feature terms uses
+//     the hardware log2.
 // Results differ from the exact path by a few ulps of the features.
 #pragma once
 #include "dev_common.hpp"
@@ -42,18 +44,28 @@ struct FastStencil {
     float lut_v;          // LUT v coordinate
     float hg0;            // LUT(dx/|d| * 0.5 + 0.5, v) (sphere points)
     float dres, tres;     // density / TR mip resolution
-    unsigned dbase, tbase;  // padded level offsets (floats)
-    int dR, tR;             // padded dims
+    unsigned dbase, tbase;  // padded level offsets (bytes)
+    unsigned dR, tR;        // padded dims
+    unsigned dR2, tR2;      // padded dims squared
     int kind;               // 0 centre, 1 sphere, 2 light
-    int pad0, pad1;
 };
 static_assert(sizeof(FastStencil) == 64, "FastStencil must be 64 bytes");
 
+// Buffer resource over a device array: loads are bounds-checked by the hardware (reads
+// past num_records return 0) and take a 32-bit byte offset, so a trilinear tap pair costs one
+// 64-bit load and no 64-bit address arithmetic.
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ inline BufRsrc make_rsrc(const float* p, long long n_floats) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n_floats * 4), 0x00020000);
+}
+__device__ inline f32x2_t ld_pair(BufRsrc r, int byte_off) {
+    return __builtin_bit_cast(f32x2_t, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+
 // Per-tile (uniform) inputs.
 struct FastView {
-    const float* __restrict__ pmips;
-    const float* __restrict__ ptr;
-    const float* __restrict__ lut;
+    BufRsrc pmips, ptr, lut;
     v3 L, LX, invLX;  // light direction as the gather uses it; LX = normalize(L); inv3(LX)
     float alpha64;    // alpha / 64
 };
@@ -66,51 +78,50 @@ struct FastPixel {
 };
 
 __device__ inline float flerp(float a, float b, float w) { return fmaf(w, b - a, a); }
+__device__ inline float clampf(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
+// ln(x) for x >= 1 (no denormal inputs): hardware log2 times ln 2.
+__device__ inline float log_ge1(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
 
+// Trilinear blend of a padded level (byte offset base, dims R, R*R) at the non-negative
+// continuous coordinate (cx, cy, cz): corner = trunc (= floor), weights = fract.  Indices stay
+// below 2^24, so the products use the 24-bit multiplier.
+__device__ inline float tri_taps(BufRsrc r, unsigned base, unsigned R, unsigned R2, float cx, float cy, float cz) {
+    const unsigned ix = (unsigned)cx, iy = (unsigned)cy, iz = (unsigned)cz;
+    const float wx = __builtin_amdgcn_fractf(cx), wy = __builtin_amdgcn_fractf(cy), wz = __builtin_amdgcn_fractf(cz);
+    const unsigned t = __umul24(__umul24(ix, R) + iy, R) + iz;
+    const int o = (int)(t * 4u + base);
+    const f32x2_t c00 = ld_pair(r, o), c01 = ld_pair(r, o + (int)(R * 4u));
+    const f32x2_t c10 = ld_pair(r, o + (int)(R2 * 4u)), c11 = ld_pair(r, o + (int)((R2 + R) * 4u));
+    return flerp(flerp(flerp(c00.x, c00.y, wz), flerp(c01.x, c01.y, wz), wy),
+                 flerp(flerp(c10.x, c10.y, wz), flerp(c11.x, c11.y, wz), wy), wx);
+}
 // Zero-border trilinear on a padded level: uvw = p + 0.5, data coordinate uvw*res - 0.5,
 // padded coordinate +1, clamped to [0, res+1] (both taps outside -> exact 0).
-__device__ inline float tri_border(const float* __restrict__ base, unsigned lvl, int R, float res, v3 p) {
+__device__ inline float tri_border(BufRsrc r, unsigned base, unsigned R, unsigned R2, float res, v3 p) {
     const float k = fmaf(0.5f, res, 0.5f), hi = res + 1.0f;
-    const float cx = fminf(fmaxf(fmaf(p.x, res, k), 0.0f), hi);
-    const float cy = fminf(fmaxf(fmaf(p.y, res, k), 0.0f), hi);
-    const float cz = fminf(fmaxf(fmaf(p.z, res, k), 0.0f), hi);
-    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-    const float wx = cx - fx, wy = cy - fy, wz = cz - fz;
-    const unsigned R2 = (unsigned)(R * R);
-    const unsigned i = lvl + ((unsigned)fx * (unsigned)R + (unsigned)fy) * (unsigned)R + (unsigned)fz;
-    const float* q = base;
-    const float c000 = q[i], c001 = q[i + 1], c010 = q[i + R], c011 = q[i + R + 1];
-    const float c100 = q[i + R2], c101 = q[i + R2 + 1], c110 = q[i + R2 + R], c111 = q[i + R2 + R + 1];
-    return flerp(flerp(flerp(c000, c001, wz), flerp(c010, c011, wz), wy),
-                 flerp(flerp(c100, c101, wz), flerp(c110, c111, wz), wy), wx);
+    const float cx = clampf(fmaf(p.x, res, k), 0.0f, hi);
+    const float cy = clampf(fmaf(p.y, res, k), 0.0f, hi);
+    const float cz = clampf(fmaf(p.z, res, k), 0.0f, hi);
+    return tri_taps(r, base, R, R2, cx, cy, cz);
 }
 // Clamp-addressing trilinear (TR mips): data coordinate clamped to [0, res-1].
-__device__ inline float tri_clamp(const float* __restrict__ base, unsigned lvl, int R, float res, v3 p) {
+__device__ inline float tri_clamp(BufRsrc r, unsigned base, unsigned R, unsigned R2, float res, v3 p) {
     const float k = fmaf(0.5f, res, -0.5f), hi = res - 1.0f;
-    const float cx = fminf(fmaxf(fmaf(p.x, res, k), 0.0f), hi) + 1.0f;
-    const float cy = fminf(fmaxf(fmaf(p.y, res, k), 0.0f), hi) + 1.0f;
-    const float cz = fminf(fmaxf(fmaf(p.z, res, k), 0.0f), hi) + 1.0f;
-    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-    const float wx = cx - fx, wy = cy - fy, wz = cz - fz;
-    const unsigned R2 = (unsigned)(R * R);
-    const unsigned i = lvl + ((unsigned)fx * (unsigned)R + (unsigned)fy) * (unsigned)R + (unsigned)fz;
-    const float* q = base;
-    const float c000 = q[i], c001 = q[i + 1], c010 = q[i + R], c011 = q[i + R + 1];
-    const float c100 = q[i + R2], c101 = q[i + R2 + 1], c110 = q[i + R2 + R], c111 = q[i + R2 + R + 1];
-    return flerp(flerp(flerp(c000, c001, wz), flerp(c010, c011, wz), wy),
-                 flerp(flerp(c100, c101, wz), flerp(c110, c111, wz), wy), wx);
+    const float cx = clampf(fmaf(p.x, res, k), 0.0f, hi) + 1.0f;
+    const float cy = clampf(fmaf(p.y, res, k), 0.0f, hi) + 1.0f;
+    const float cz = clampf(fmaf(p.z, res, k), 0.0f, hi) + 1.0f;
+    return tri_taps(r, base, R, R2, cx, cy, cz);
 }
-// 512x512 LUT, bilinear, clamp.
-__device__ inline float lut_fast(const float* __restrict__ lut, float u, float v) {
-    const float px = fminf(fmaxf(fmaf(u, 512.0f, -0.5f), 0.0f), 511.0f);
-    const float py = fminf(fmaxf(fmaf(v, 512.0f, -0.5f), 0.0f), 511.0f);
-    const float fx = floorf(px), fy = floorf(py);
-    const float wx = px - fx, wy = py - fy;
-    const unsigned x0 = (unsigned)fx, y0 = (unsigned)fy;
-    const unsigned x1 = min(x0 + 1u, 511u), y1 = min(y0 + 1u, 511u);
-    const float a = flerp(lut[y0 * 512u + x0], lut[y0 * 512u + x1], wx);
-    const float b = flerp(lut[y1 * 512u + x0], lut[y1 * 512u + x1], wx);
-    return flerp(a, b, wy);
+// 512x512 LUT, bilinear, clamp.  At the clamped edge x0 = 511 the pair's second value belongs
+// to the next row (or past the end: 0) and has weight 0.
+__device__ inline float lut_fast(BufRsrc lut, float u, float v) {
+    const float px = clampf(fmaf(u, 512.0f, -0.5f), 0.0f, 511.0f);
+    const float py = clampf(fmaf(v, 512.0f, -0.5f), 0.0f, 511.0f);
+    const float wx = __builtin_amdgcn_fractf(px), wy = __builtin_amdgcn_fractf(py);
+    const unsigned x0 = (unsigned)px, y0 = (unsigned)py;
+    const unsigned y1 = min(y0 + 1u, 511u);
+    const f32x2_t a = ld_pair(lut, (int)((y0 * 512u + x0) * 4u)), b = ld_pair(lut, (int)((y1 * 512u + x0) * 4u));
+    return flerp(flerp(a.x, a.y, wx), flerp(b.x, b.y, wx), wy);
 }
 
 // ShadowTerm_TRTex(...).x (renderResources.cu:84-102) on the padded TR level.
@@ -127,7 +138,7 @@ __device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3
         const float off = fmaxf(tmi, 0.0f);
         p = mk3(fmaf(fv.LX.x, off, p.x), fmaf(fv.LX.y, off, p.y), fmaf(fv.LX.z, off, p.z));
     }
-    return tri_clamp(fv.ptr, s.tbase, s.tR, s.tres, p);
+    return tri_clamp(fv.ptr, s.tbase, s.tR, s.tR2, s.tres, p);
 }
 
 // (X_Val, X_Val_Hg, X_Val_Sub) of one stencil point.  LIGHT: points 160..191 (along the
@@ -139,7 +150,7 @@ __device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const
     if (LIGHT) {  // MsDir = normalize(L * off) = L
         p = mk3(fmaf(fv.L.x, s.off, px.pos.x), fmaf(fv.L.y, s.off, px.pos.y), fmaf(fv.L.z, s.off, px.pos.z));
         const float h0 = lut_fast(fv.lut, px.uc1, s.lut_v), h1 = lut_fast(fv.lut, px.uc2, s.lut_v);
-        xh = __logf(fmaf(h0, h1, 1.0f));
+        xh = log_ge1(fmaf(h0, h1, 1.0f));
     } else {
         p.x = fmaf(px.X.x, s.ddx, fmaf(px.Y.x, s.ddy, fmaf(px.Z.x, s.ddz, px.pos.x)));
         p.y = fmaf(px.X.y, s.ddx, fmaf(px.Y.y, s.ddy, fmaf(px.Z.y, s.ddz, px.pos.y)));
@@ -147,10 +158,10 @@ __device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const
         // cos between LX and the point direction: (ax*dx + ay*dy + az*dz)/|d|, ddx = dx*off
         const float c2 = fmaf(px.ax, s.ddx, fmaf(px.ay, s.ddy, px.az * s.ddz)) * s.cscale;
         const float h1 = lut_fast(fv.lut, fmaf(c2, 0.5f, 0.5f), s.lut_v);
-        xh = centre ? px.xh_centre : __logf(fmaf(s.hg0, h1, 1.0f));
+        xh = centre ? px.xh_centre : log_ge1(fmaf(s.hg0, h1, 1.0f));
     }
-    const float rho = tri_border(fv.pmips, s.dbase, s.dR, s.dres, p);
-    const float xv = __logf(fmaf(fv.alpha64, rho, 1.0f));
+    const float rho = tri_border(fv.pmips, s.dbase, s.dR, s.dR2, s.dres, p);
+    const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
     const float xs = shadow_fast(fv, s, p);
     return mk3(xv, xh, xs);
 }

@@ -188,11 +188,10 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
 
-__device__ inline uint32_t pack_bf16x2(float a, float b) {
-    uint32_t ua = __float_as_uint(a), ub = __float_as_uint(b);
-    ua = (ua + 0x7fffu + ((ua >> 16) & 1u)) >> 16;
-    ub = (ub + 0x7fffu + ((ub >> 16) & 1u)) >> 16;
-    return ua | (ub << 16);
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t pack_bf16x2(float a, float b) {  // v_cvt_pk_bf16_f32, round to nearest even
+    const bf16x2_t v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(uint32_t, v);
 }
 __device__ inline uint32_t pack_fp8x4(float a, float b, float c, float d) {
     int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
@@ -200,10 +199,57 @@ __device__ inline uint32_t pack_fp8x4(float a, float b, float c, float d) {
     return (uint32_t)w;
 }
 
+// Cross-row lane exchanges of the tile layout (lane = c + 16q) with the gfx950 permlane
+// swaps (plain VALU, no LDS round trip).  permlane16_swap(x, x) returns rows (0,0,2,2) and
+// (1,1,3,3) of x; permlane32_swap(x, x) returns halves (lo,lo) and (hi,hi).
+__device__ inline float q_sum(float x) {  // sum over the 4 lanes of column c, same order as xor 16 then 32
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    const float s = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(s), __float_as_uint(s), false, false);
+    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+__device__ inline float q_max(float x) {
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    const float s = fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
+    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(s), __float_as_uint(s), false, false);
+    return fmaxf(__uint_as_float(b[0]), __uint_as_float(b[1]));
+}
+__device__ inline float xor32(float x, int lane) {  // value of lane ^ 32
+    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(lane < 32 ? b[1] : b[0]);
+}
+
+// Weight and bias fragments through buffer resources: the per-layer offset is a compile-time
+// scalar offset and the lane offset a fixed VGPR, so a fragment load needs no address
+// arithmetic.
+struct NetRs {
+    BufRsrc w;    // bf16 hi (16 bytes per lane per fragment) or fp8 (8 bytes) fragments
+    BufRsrc wlo;  // bf16 lo fragments
+    BufRsrc bias;
+    const int* wexp;
+};
+// The pointers are uniform (kernel arguments); reading them through readfirstlane keeps the
+// descriptors in scalar registers also where they arrive in vector registers (call arguments).
+__device__ inline void* uniform_ptr(const void* p) {
+    const uint64_t a = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return (void*)(((uint64_t)hi << 32) | lo);
+}
+__device__ inline NetRs make_net_rs(const MfmaNet& n, bool fp8) {
+    NetRs r;
+    r.w = fp8 ? __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.fp8), (short)0, kMfmaFrags * 512, 0x00020000)
+              : __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.bf16), (short)0, kMfmaFrags * 1024, 0x00020000);
+    r.wlo = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.bf16_lo), (short)0, kMfmaFrags * 1024, 0x00020000);
+    r.bias = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.bias), (short)0, kMfmaBias * 4, 0x00020000);
+    r.wexp = (const int*)uniform_ptr(n.wexp);
+    return r;
+}
+
 // out[NM][4] = act(W_ID * in + b): in[NK][8] in the K-slot order of mlp_mfma.h.  One M-tile
 // at a time behind a scheduling barrier, so at most NK weight fragments are in flight.
 template <int PREC, int ID>
-__device__ inline void dense(const MfmaNet& net, const float (&in)[mfma_nk(ID)][8], float (&out)[mfma_nm(ID)][4],
+__device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8], float (&out)[mfma_nm(ID)][4],
                              int lane, bool relu) {
     constexpr int NM = mfma_nm(ID), NK = mfma_nk(ID), FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
     const int q = lane >> 4;
@@ -213,28 +259,27 @@ __device__ inline void dense(const MfmaNet& net, const float (&in)[mfma_nk(ID)][
         for (int kc = 0; kc < NK; ++kc)
             b[kc] = u32x4_t{pack_bf16x2(in[kc][0], in[kc][1]), pack_bf16x2(in[kc][2], in[kc][3]),
                             pack_bf16x2(in[kc][4], in[kc][5]), pack_bf16x2(in[kc][6], in[kc][7])};
-        const u32x4_t* wf = (const u32x4_t*)net.bf16 + (size_t)FRAG * 64 + lane;
-        const u32x4_t* wl = (const u32x4_t*)net.bf16_lo + (size_t)FRAG * 64 + lane;
 #pragma unroll
         for (int mt = 0; mt < NM; ++mt) {
             f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int kc = 0; kc < NK; ++kc) {
-                u32x4_t a = wf[(mt * NK + kc) * 64];
+                const int fo = (FRAG + mt * NK + kc) * 1024;
+                u32x4_t a = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, fo, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
                                                               __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
                 if constexpr (PREC == MRP_BF16) {  // second weight term
-                    u32x4_t al = wl[(mt * NK + kc) * 64];
+                    u32x4_t al = __builtin_amdgcn_raw_buffer_load_b128(net.wlo, lane * 16, fo, 0);
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, al),
                                                                   __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
                 }
             }
-            float4 bb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if constexpr (mfma_has_bias(ID)) bb = *(const float4*)(net.bias + BIAS + 16 * mt + 4 * q);
-            const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+            u32x4_t bb = u32x4_t{0u, 0u, 0u, 0u};
+            if constexpr (mfma_has_bias(ID))
+                bb = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float v = acc[i] + bv[i];
+                float v = acc[i] + __uint_as_float(bb[i]);
                 out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -245,8 +290,7 @@ __device__ inline void dense(const MfmaNet& net, const float (&in)[mfma_nk(ID)][
         for (int kc = 0; kc < NK; ++kc)
 #pragma unroll
             for (int s = 0; s < 8; ++s) m = fmaxf(m, fabsf(in[kc][s]));
-        m = fmaxf(m, __shfl_xor(m, 16));
-        m = fmaxf(m, __shfl_xor(m, 32));
+        m = q_max(m);
         const int ea = fp8_exp_for(m);
         const float sc = ldexpf(1.0f, ea);
         long b[NK];
@@ -257,19 +301,21 @@ __device__ inline void dense(const MfmaNet& net, const float (&in)[mfma_nk(ID)][
             b[kc] = (long)(((uint64_t)hi << 32) | lo);
         }
         const float ds = ldexpf(1.0f, -(ea + net.wexp[ID]));
-        const uint64_t* wf = (const uint64_t*)net.fp8 + (size_t)FRAG * 64 + lane;
 #pragma unroll
         for (int mt = 0; mt < NM; ++mt) {
             f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int kc = 0; kc < NK; ++kc)
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)wf[(mt * NK + kc) * 64], b[kc], acc, 0, 0, 0);
-            float4 bb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if constexpr (mfma_has_bias(ID)) bb = *(const float4*)(net.bias + BIAS + 16 * mt + 4 * q);
-            const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+            for (int kc = 0; kc < NK; ++kc) {
+                const long a = __builtin_bit_cast(
+                    long, __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, (FRAG + mt * NK + kc) * 512, 0));
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b[kc], acc, 0, 0, 0);
+            }
+            u32x4_t bb = u32x4_t{0u, 0u, 0u, 0u};
+            if constexpr (mfma_has_bias(ID))
+                bb = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float v = fmaf(acc[i], ds, bv[i]);
+                float v = fmaf(acc[i], ds, __uint_as_float(bb[i]));
                 out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -279,19 +325,26 @@ __device__ inline void dense(const MfmaNet& net, const float (&in)[mfma_nk(ID)][
 
 // Makes a pointer opaque to the optimiser once per tile iteration: the network's weights are
 // loop-invariant in the persistent tile loop, and without this the compiler hoists all of
-// their loads (hundreds of registers) above the loop and spills.
+// their loads (hundreds of registers) above the loop and spills.  The pointer is offset by a
+// zero the compiler cannot see through, so it keeps its address space (global loads for
+// kernel arguments, LDS loads for shared arrays).
+__device__ inline int opaque_zero() {
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    return z;
+}
 template <class T>
 __device__ inline T* opaque(T* p) {
-    asm volatile("" : "+s"(p));
-    return p;
+    return p + opaque_zero();
 }
 template <class T>
 __device__ inline T* opaque_v(T* p) {
-    asm volatile("" : "+v"(p));
-    return p;
+    return p + opaque_zero();
 }
 
-__device__ inline float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
+// Gate sigmoid with the hardware exp2 and reciprocal (about 2 ulp): the gates feed BF16 / FP8
+// products whose rounding is far coarser.
+__device__ inline float sigmoidf_(float x) { return __frcp_rn(1.0f + __expf(-x)); }
 
 // Per-pixel lane state shared by the tile routine.
 struct LaneCtx {
@@ -319,11 +372,9 @@ __device__ inline void block_gate(const float* __restrict__ W, const float* __re
                     m = fmaxf(m, F[b][i]);
                 }
         }
-        a += __shfl_xor(a, 16);
-        m = fmaxf(m, __shfl_xor(m, 16));
-        a += __shfl_xor(a, 32);
-        m = fmaxf(m, __shfl_xor(m, 32));
-        sum[b] = a / (float)n;
+        a = q_sum(a);
+        m = q_max(m);
+        sum[b] = a * (1.0f / (float)n);  // n is a power of two: exact
         mx[b] = m;
     }
     const float t[8] = {sum[0], sum[1], sum[2], mx[0], mx[1], mx[2], lc.g, lc.gamma};
@@ -400,7 +451,9 @@ struct FastGatherSrc {
                 F[0][4 * h + i] = f.x;
                 F[2][4 * h + i] = f.y;
                 F[1][4 * h + i] = f.z;
-                __builtin_amdgcn_sched_barrier(0);
+                // two points' loads in flight at a time: more hides memory latency, fewer
+                // registers keep the kernel at two waves per SIMD
+                if (i == 3) __builtin_amdgcn_sched_barrier(0);
             }
         }
         if (!wide)
@@ -431,7 +484,7 @@ struct RowSrc {
 // One SE/SERES block for all three branches: in = F*s (+ R), out = ReLU(W in + b).
 // Layer ids: ID0 + STRIDE*branch.
 template <int PREC, int ID0, int STRIDE>
-__device__ inline void block_dense(const MfmaNet& net, const LaneCtx& lc, const float (&F)[3][8], const float (&s)[3],
+__device__ inline void block_dense(const NetRs& net, const LaneCtx& lc, const float (&F)[3][8], const float (&s)[3],
                                    const float (&R)[3][8], bool has_res, bool lane_has, float (&out)[3][8]) {
     constexpr int NM = mfma_nm(ID0);
 #pragma unroll
@@ -457,7 +510,7 @@ __device__ inline void block_dense(const MfmaNet& net, const LaneCtx& lc, const 
 
 // Gate statistics + SE/SERES of a 16- or 32-wide trunk block with ADD3 (or REP3 for block 5).
 template <int PREC, int K>  // K = 2..7: blocks 3..8 of the main stack
-__device__ inline void trunk_block(const MfmaNet& net, const float* __restrict__ W, const float* __restrict__ sW1,
+__device__ inline void trunk_block(const NetRs& net, const float* __restrict__ W, const float* __restrict__ sW1,
                                    const LaneCtx& lc, const float (&F)[3][8], float (&R)[3][8], float (&hacc)[4]) {
     using namespace net;
     constexpr int sew0[8] = {LSE01W, LSE11W, LSE21W, LSE31W, LSE41W, LSE51W, LSE61W, LSE71W};
@@ -484,7 +537,7 @@ __device__ inline void trunk_block(const MfmaNet& net, const float* __restrict__
 
 // One block of the DI stack (K = 0..3), features G already moved to lanes q < 2.
 template <int PREC, int K>
-__device__ inline void di_block(const MfmaNet& net, const float* __restrict__ W, const float* __restrict__ sW1,
+__device__ inline void di_block(const NetRs& net, const float* __restrict__ W, const float* __restrict__ sW1,
                                 const LaneCtx& lc, const float (&G)[3][8], float (&R)[3][8], float (&hacc)[4]) {
     using namespace net;
     constexpr int sew0[4] = {LDSE01W, LDSE11W, LDSE21W, LDSE31W};
@@ -500,7 +553,7 @@ __device__ inline void di_block(const MfmaNet& net, const float* __restrict__ W,
 }
 
 template <int PREC, int K>  // LX residual pair k: x += ReLU(LX_{2k+1} ReLU(LX_{2k} x))
-__device__ inline void lx_pair(const MfmaNet& net, const LaneCtx& lc, float (&x)[1][4]) {
+__device__ inline void lx_pair(const NetRs& net, const LaneCtx& lc, float (&x)[1][4]) {
     float xi[1][8], y[1][4], yi[1][8], z[1][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -520,10 +573,11 @@ __device__ inline void lx_pair(const MfmaNet& net, const LaneCtx& lc, float (&x)
 
 // Tail SC(p) for one channel (radiancePredict.cu:375-409); returns P in lanes q == 0.
 template <int PREC>
-__device__ __attribute__((noinline)) float tail(const MfmaNet& net, const float* __restrict__ W, const float* __restrict__ sW1,
+__device__ __attribute__((noinline)) float tail(MfmaNet nt, const float* __restrict__ W, const float* __restrict__ sW1,
                              const LaneCtx& lc, float srp, const float (&hacc)[4], const float (&comb)[3][8],
                              const float (&tA)[4], const float (&tB)[4]) {
     using namespace net;
+    const NetRs net = make_net_rs(nt, PREC == MRP_FP8);
     const int q = lc.q;
     const bool lo = q < 2;
     // LSEFin1 rows 4q+i: add the g, gamma, srp columns, ReLU
@@ -628,9 +682,10 @@ __device__ __attribute__((noinline)) float tail(const MfmaNet& net, const float*
 // Full network on one 16-pixel tile; returns, in lanes q == 0, max(e^P - 1, 0) * srp per
 // channel for pixel c.
 template <int PREC, class Src>
-__device__ inline v3 run_tile(const MfmaNet& net, const float* __restrict__ W, const float* __restrict__ sW1,
+__device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, const float* __restrict__ sW1,
                               const LaneCtx& lc, const Src& src) {
     using namespace net;
+    const NetRs net = make_net_rs(nt, PREC == MRP_FP8);
     const bool lo = lc.q < 2;
     float F[3][8], G[3][8], R[3][8], O[3][8], s[3];
     float hacc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -650,7 +705,7 @@ __device__ inline v3 run_tile(const MfmaNet& net, const float* __restrict__ W, c
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? __shfl_xor(F[b][i], 32) : 0.0f;
+        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? xor32(F[b][i], lc.lane) : 0.0f;
     block_gate(W, sW1, lc, 3, LSE11W, LSE12W, G, 4, lo, 8, s, hacc);
     block_dense<PREC, ML_TRUNK + 1, 9>(net, lc, G, s, R, true, lo, O);
     // A2[0:16] = [out2, X[8:16]]: lanes q<2 hold out2 rows 4q+i, lanes q>=2 their own X[8+4(q-2)+i]
@@ -704,7 +759,7 @@ __device__ inline v3 run_tile(const MfmaNet& net, const float* __restrict__ W, c
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? __shfl_xor(D[b][i], 32) : 0.0f;
+        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? xor32(D[b][i], lc.lane) : 0.0f;
     di_block<PREC, 1>(net, W, sW1, lc, G, R, hacc);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
@@ -714,7 +769,7 @@ __device__ inline v3 run_tile(const MfmaNet& net, const float* __restrict__ W, c
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? __shfl_xor(D[b][4 + i], 32) : 0.0f;
+        for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? xor32(D[b][4 + i], lc.lane) : 0.0f;
     di_block<PREC, 3>(net, W, sW1, lc, G, R, hacc);
     // ---- LD41 heads: tile A = [LD41 rows 0..7 | LD_Tr41 rows 8..15], tile B = [LD_Hg41 | LGGS]
     float tA[4], tB[4];
@@ -740,10 +795,10 @@ __device__ inline v3 run_tile(const MfmaNet& net, const float* __restrict__ W, c
     const bool need_y = __any(lc.ok && lc.sr.y != lc.sr.x);
     const bool need_z = __any(lc.ok && lc.sr.z != lc.sr.x && lc.sr.z != lc.sr.y);
     const float sp0 = powf(lc.sr.x, 4.0f), sp1 = powf(lc.sr.y, 4.0f), sp2 = powf(lc.sr.z, 4.0f);
-    float P0 = tail<PREC>(net, W, sW1, lc, sp0, hacc, comb, tA, tB);
+    float P0 = tail<PREC>(nt, W, sW1, lc, sp0, hacc, comb, tA, tB);
     float P1 = P0, P2 = P0;
-    if (need_y) P1 = tail<PREC>(net, W, sW1, lc, sp1, hacc, comb, tA, tB);
-    if (need_z) P2 = tail<PREC>(net, W, sW1, lc, sp2, hacc, comb, tA, tB);
+    if (need_y) P1 = tail<PREC>(nt, W, sW1, lc, sp1, hacc, comb, tA, tB);
+    if (need_z) P2 = tail<PREC>(nt, W, sW1, lc, sp2, hacc, comb, tA, tB);
     if (lc.sr.y == lc.sr.x) P1 = P0;
     if (lc.sr.z == lc.sr.x) P2 = P0;
     else if (lc.sr.z == lc.sr.y) P2 = P1;
@@ -780,7 +835,7 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevV
     }
     const unsigned n = *counter;
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int wave = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     for (unsigned tile = wave; tile * 16u < n; tile += nwaves) {
         LaneCtx lc;
@@ -818,9 +873,9 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevV
             const DevVolume& vol = vols[sl.vol];
             FastGatherSrc src;
             src.st = opaque_v(sfs);
-            src.fv.pmips = vol.pmips;
-            src.fv.ptr = vol.ptr;
-            src.fv.lut = vol.lut;
+            src.fv.pmips = make_rsrc(vol.pmips, kPadMipFloats);
+            src.fv.ptr = make_rsrc(vol.ptr, kPadTrFloats);
+            src.fv.lut = make_rsrc(vol.lut, 512 * 512);
             src.fv.L = ps.L;
             src.fv.LX = ps.LX;
             src.fv.invLX = inv3(ps.LX);
@@ -859,7 +914,7 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
     __shared__ float sW1[1200];
     load_shared(sW1, nullptr, W, nullptr);
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int wave = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     for (int tile = wave; tile * 16 < n; tile += nwaves) {
         LaneCtx lc;

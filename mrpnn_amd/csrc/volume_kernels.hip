@@ -128,6 +128,7 @@ struct TrArgs {
     const float* mips[9];
     float* tr[8];
     float* tr_pad;       // zero-padded copy of levels 0..6 (fast gather), may be null
+    const float* pmips;  // zero-padded density mips 0..7, may be null (exact border sampling)
     float alpha_lvl[8];
     long long start[9];  // flattened texel offsets of each level
     v3 light;
@@ -148,11 +149,24 @@ __global__ __launch_bounds__(256) void k_tr_bake(TrArgs a) {
         float st = dis / 128.0f;
         v3 lp = ori;
         float sum = 0.0f;
-        const float* src = a.mips[m + 1];
         const int sres = 256 >> (m + 1);
-        for (int i = 0; i < 128; ++i) {
-            lp = lp + a.light * st;
-            sum = sum + sample_border_exact(src, sres, lp + 0.5f);
+        if (a.pmips && m + 1 < 8) {
+            // zero-padded copy of the density mip: no per-tap border tests
+            const unsigned base = (unsigned)pmip_offset(m + 1) * 4u;
+            const unsigned R = (unsigned)pad_dim(sres);
+            const float fres = (float)sres;
+            const BufRsrc rs = make_rsrc(a.pmips, kPadMipFloats);
+#pragma unroll 4
+            for (int i = 0; i < 128; ++i) {
+                lp = lp + a.light * st;
+                sum = sum + tri_border(rs, base, R, R * R, fres, lp);
+            }
+        } else {
+            const float* src = a.mips[m + 1];
+            for (int i = 0; i < 128; ++i) {
+                lp = lp + a.light * st;
+                sum = sum + sample_border_exact(src, sres, lp + 0.5f);
+            }
         }
         const float T = expf(-sum * a.alpha_lvl[m] * st) * 1.0f;
         a.tr[m][l] = T;
@@ -283,11 +297,12 @@ __global__ void k_fast_stencil(const StencilEntry* __restrict__ st, const float*
     const int tm = e.mip - 1 > 0 ? e.mip - 1 : 0;
     f.dres = (float)(256 >> e.mip);
     f.tres = (float)(128 >> tm);
-    f.dbase = (unsigned)pmip_offset(e.mip);
-    f.tbase = (unsigned)ptr_offset(tm);
-    f.dR = pad_dim(256 >> e.mip);
-    f.tR = pad_dim(128 >> tm);
-    f.pad0 = f.pad1 = 0;
+    f.dbase = (unsigned)pmip_offset(e.mip) * 4u;
+    f.tbase = (unsigned)ptr_offset(tm) * 4u;
+    f.dR = (unsigned)pad_dim(256 >> e.mip);
+    f.tR = (unsigned)pad_dim(128 >> tm);
+    f.dR2 = f.dR * f.dR;
+    f.tR2 = f.tR * f.tR;
     out[i] = f;
 }
 
@@ -316,10 +331,11 @@ hipError_t launch_resample(const float* src, int sres, float* dst, int res, hipS
                        res);
     return hipGetLastError();
 }
-hipError_t launch_tr_bake(const float* const mips[9], float* const tr[8], float* tr_pad, const float alpha_lvl[8],
-                          v3 light, hipStream_t s) {
+hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, float* const tr[8], float* tr_pad,
+                          const float alpha_lvl[8], v3 light, hipStream_t s) {
     TrArgs a;
     a.tr_pad = tr_pad;
+    a.pmips = pmips;
     long long acc = 0;
     for (int m = 0; m < 9; ++m) a.mips[m] = mips[m];
     for (int m = 0; m < 8; ++m) {
