@@ -248,7 +248,7 @@ __device__ inline NetRs make_net_rs(const MfmaNet& n, bool fp8) {
 
 // out[NM][4] = act(W_ID * in + b): in[NK][8] in the K-slot order of mlp_mfma.h.  One M-tile
 // at a time behind a scheduling barrier, so at most NK weight fragments are in flight.
-template <int PREC, int ID>
+template <int PREC, int ID, bool SB = true>
 __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8], float (&out)[mfma_nm(ID)][4],
                              int lane, bool relu) {
     constexpr int NM = mfma_nm(ID), NK = mfma_nk(ID), FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
@@ -282,7 +282,7 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
                 float v = acc[i] + __uint_as_float(bb[i]);
                 out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
             }
-            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
         }
     } else {
         float m = 0.0f;
@@ -318,7 +318,7 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
                 float v = fmaf(acc[i], ds, __uint_as_float(bb[i]));
                 out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
             }
-            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -354,11 +354,41 @@ struct LaneCtx {
     v3 sr;          // scatterrate passed to the network (scatter_rate / 1.001)
 };
 
+// SE gate weights of the 12 gated blocks (trunk blocks 0..7, DI blocks 0..3) staged in LDS:
+// per gate 64 floats SEW0 (8x8, row-major) then 24 floats SEW1 (3x8).
+constexpr int kGates = 12, kGateFloats = 88;
+__device__ inline int gate_w0(int gi) {
+    using namespace net;
+    constexpr int w0[kGates] = {LSE01W, LSE11W, LSE21W, LSE31W, LSE41W, LSE51W, LSE61W, LSE71W,
+                                LDSE01W, LDSE11W, LDSE21W, LDSE31W};
+    return w0[gi];
+}
+__device__ inline int gate_w1(int gi) {
+    using namespace net;
+    constexpr int w1[kGates] = {LSE02W, LSE12W, LSE22W, LSE32W, LSE42W, LSE52W, LSE62W, LSE72W,
+                                LDSE02W, LDSE12W, LDSE22W, LDSE32W};
+    return w1[gi];
+}
+
+// x0..x3 = value of x in rows 0..3 of the lane's column (lane = c + 16 row), in every lane.
+__device__ inline void q_gather(float x, float& x0, float& x1, float& x2, float& x3) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    const auto a = __builtin_amdgcn_permlane32_swap(p[0], p[0], false, false);  // rows (0,0,2,2)
+    const auto b = __builtin_amdgcn_permlane32_swap(p[1], p[1], false, false);  // rows (1,1,3,3)
+    x0 = __uint_as_float(a[0]);
+    x2 = __uint_as_float(a[1]);
+    x1 = __uint_as_float(b[0]);
+    x3 = __uint_as_float(b[1]);
+}
+
 // Pooled statistics and SE gate of one block (radiancePredict.cu:240-266): mean/max of the
 // raw slice of each branch, gate MLP SEW0 (8x8, ReLU) and SEW1 (3x8, sigmoid); the six
-// pooled values are folded into the LSEFin1 accumulators (rows 4q+i of this lane).
-__device__ inline void block_gate(const float* __restrict__ W, const float* __restrict__ sW1, const LaneCtx& lc, int pid,
-                                  int sew0, int sew1, const float (&F)[3][8], int nv, bool lane_has, int n, float (&s)[3],
+// pooled values are folded into the LSEFin1 accumulators (rows 4q+i of this lane).  The
+// four lanes of a pixel split the gate: lane q computes hidden units 2q, 2q+1 and gate
+// output min(q, 2); the values are exchanged with permlane swaps.  Every unit is the same
+// FMA chain as the scalar form, so the gate values are unchanged.
+__device__ inline void block_gate(const float* __restrict__ sG, const float* __restrict__ sW1, const LaneCtx& lc,
+                                  int pid, int gi, const float (&F)[3][8], int nv, bool lane_has, int n, float (&s)[3],
                                   float (&hacc)[4]) {
     float sum[3], mx[3];
 #pragma unroll
@@ -378,21 +408,23 @@ __device__ inline void block_gate(const float* __restrict__ W, const float* __re
         mx[b] = m;
     }
     const float t[8] = {sum[0], sum[1], sum[2], mx[0], mx[1], mx[2], lc.g, lc.gamma};
+    const float* g = sG + gi * kGateFloats;
+    const float* w0 = g + 16 * lc.q;  // rows 2q, 2q+1 of SEW0
+    float ha = 0.0f, hb = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        ha = fmaf(t[j], w0[j], ha);
+        hb = fmaf(t[j], w0[8 + j], hb);
+    }
     float h[8];
+    q_gather(fmaxf(ha, 0.0f), h[0], h[2], h[4], h[6]);
+    q_gather(fmaxf(hb, 0.0f), h[1], h[3], h[5], h[7]);
+    const float* w1 = g + 64 + 8 * min(lc.q, 2);
+    float acc = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc = fmaf(t[j], W[sew0 + j + 8 * i], acc);
-        h[i] = fmaxf(acc, 0.0f);
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc = fmaf(h[j], W[sew1 + j + 8 * i], acc);
-        s[i] = sigmoidf_(acc);
-    }
+    for (int j = 0; j < 8; ++j) acc = fmaf(h[j], w1[j], acc);
+    float s3;
+    q_gather(sigmoidf_(acc), s[0], s[1], s[2], s3);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float* wr = sW1 + (4 * lc.q + i) * 75;
@@ -497,27 +529,28 @@ __device__ inline void block_dense(const NetRs& net, const LaneCtx& lc, const fl
             in[0][i] = (lane_has && (NM == 2 || i < 4)) ? v : 0.0f;
         }
         float o[NM][4];
-        if (b == 0) dense<PREC, ID0>(net, in, o, lc.lane, true);
-        else if (b == 1) dense<PREC, ID0 + STRIDE>(net, in, o, lc.lane, true);
-        else dense<PREC, ID0 + 2 * STRIDE>(net, in, o, lc.lane, true);
+        // the three branches are independent: no scheduling barrier between them, so their
+        // weight loads and MFMAs overlap
+        if (b == 0) dense<PREC, ID0, false>(net, in, o, lc.lane, true);
+        else if (b == 1) dense<PREC, ID0 + STRIDE, false>(net, in, o, lc.lane, true);
+        else dense<PREC, ID0 + 2 * STRIDE, false>(net, in, o, lc.lane, true);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             out[b][i] = o[0][i];
             out[b][4 + i] = NM == 2 ? o[NM - 1][i] : 0.0f;
         }
     }
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 // Gate statistics + SE/SERES of a 16- or 32-wide trunk block with ADD3 (or REP3 for block 5).
 template <int PREC, int K>  // K = 2..7: blocks 3..8 of the main stack
-__device__ inline void trunk_block(const NetRs& net, const float* __restrict__ W, const float* __restrict__ sW1,
+__device__ inline void trunk_block(const NetRs& net, const float* __restrict__ sG, const float* __restrict__ sW1,
                                    const LaneCtx& lc, const float (&F)[3][8], float (&R)[3][8], float (&hacc)[4]) {
     using namespace net;
-    constexpr int sew0[8] = {LSE01W, LSE11W, LSE21W, LSE31W, LSE41W, LSE51W, LSE61W, LSE71W};
-    constexpr int sew1[8] = {LSE02W, LSE12W, LSE22W, LSE32W, LSE42W, LSE52W, LSE62W, LSE72W};
     constexpr bool wide = K >= 5;
     float s[3], O[3][8];
-    block_gate(W, sW1, lc, 3 * K, sew0[K], sew1[K], F, wide ? 8 : 4, true, wide ? 32 : 16, s, hacc);
+    block_gate(sG, sW1, lc, 3 * K, K, F, wide ? 8 : 4, true, wide ? 32 : 16, s, hacc);
     block_dense<PREC, ML_TRUNK + K, 9>(net, lc, F, s, R, true, true, O);
     if (K == 4) {  // REP3(32,48,16): A2[32:64] = [out5, X[48:64]]
 #pragma unroll
@@ -537,14 +570,12 @@ __device__ inline void trunk_block(const NetRs& net, const float* __restrict__ W
 
 // One block of the DI stack (K = 0..3), features G already moved to lanes q < 2.
 template <int PREC, int K>
-__device__ inline void di_block(const NetRs& net, const float* __restrict__ W, const float* __restrict__ sW1,
+__device__ inline void di_block(const NetRs& net, const float* __restrict__ sG, const float* __restrict__ sW1,
                                 const LaneCtx& lc, const float (&G)[3][8], float (&R)[3][8], float (&hacc)[4]) {
     using namespace net;
-    constexpr int sew0[4] = {LDSE01W, LDSE11W, LDSE21W, LDSE31W};
-    constexpr int sew1[4] = {LDSE02W, LDSE12W, LDSE22W, LDSE32W};
     const bool lo = lc.q < 2;
     float s[3], O[3][8];
-    block_gate(W, sW1, lc, 24 + 3 * K, sew0[K], sew1[K], G, 4, lo, 8, s, hacc);
+    block_gate(sG, sW1, lc, 24 + 3 * K, 8 + K, G, 4, lo, 8, s, hacc);
     block_dense<PREC, ML_DI + K, 4>(net, lc, G, s, R, K > 0, lo, O);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
@@ -682,7 +713,8 @@ __device__ __attribute__((noinline)) float tail(MfmaNet nt, const float* __restr
 // Full network on one 16-pixel tile; returns, in lanes q == 0, max(e^P - 1, 0) * srp per
 // channel for pixel c.
 template <int PREC, class Src>
-__device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, const float* __restrict__ sW1,
+__device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, const float* __restrict__ sG,
+                              const float* __restrict__ sW1,
                               const LaneCtx& lc, const Src& src) {
     using namespace net;
     const NetRs net = make_net_rs(nt, PREC == MRP_FP8);
@@ -691,7 +723,7 @@ __device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, co
     float hacc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     // ---- block 1: SE(0,0,8), ADD3(0,0,8); points 0..15 gathered, [0,8) in lanes q<2
     src.get(lc, 0, false, F);
-    block_gate(W, sW1, lc, 0, LSE01W, LSE02W, F, 4, lo, 8, s, hacc);
+    block_gate(sG, sW1, lc, 0, 0, F, 4, lo, 8, s, hacc);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -706,7 +738,7 @@ __device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, co
     for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? xor32(F[b][i], lc.lane) : 0.0f;
-    block_gate(W, sW1, lc, 3, LSE11W, LSE12W, G, 4, lo, 8, s, hacc);
+    block_gate(sG, sW1, lc, 3, 1, G, 4, lo, 8, s, hacc);
     block_dense<PREC, ML_TRUNK + 1, 9>(net, lc, G, s, R, true, lo, O);
     // A2[0:16] = [out2, X[8:16]]: lanes q<2 hold out2 rows 4q+i, lanes q>=2 their own X[8+4(q-2)+i]
 #pragma unroll
@@ -715,17 +747,17 @@ __device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, co
         for (int i = 0; i < 8; ++i) R[b][i] = i < 4 ? (lo ? O[b][i] : F[b][i]) : 0.0f;
     // ---- blocks 3..8
     src.get(lc, 16, false, F);
-    trunk_block<PREC, 2>(net, W, sW1, lc, F, R, hacc);
+    trunk_block<PREC, 2>(net, sG, sW1, lc, F, R, hacc);
     src.get(lc, 32, false, F);
-    trunk_block<PREC, 3>(net, W, sW1, lc, F, R, hacc);
+    trunk_block<PREC, 3>(net, sG, sW1, lc, F, R, hacc);
     src.get(lc, 48, false, F);
-    trunk_block<PREC, 4>(net, W, sW1, lc, F, R, hacc);
+    trunk_block<PREC, 4>(net, sG, sW1, lc, F, R, hacc);
     src.get(lc, 64, true, F);
-    trunk_block<PREC, 5>(net, W, sW1, lc, F, R, hacc);
+    trunk_block<PREC, 5>(net, sG, sW1, lc, F, R, hacc);
     src.get(lc, 96, true, F);
-    trunk_block<PREC, 6>(net, W, sW1, lc, F, R, hacc);
+    trunk_block<PREC, 6>(net, sG, sW1, lc, F, R, hacc);
     src.get(lc, 128, true, F);
-    trunk_block<PREC, 7>(net, W, sW1, lc, F, R, hacc);
+    trunk_block<PREC, 7>(net, sG, sW1, lc, F, R, hacc);
     // ---- L81 / L_Tr81 / L_Hg81 -> Comb[0:96]
     float comb[3][8];
 #pragma unroll
@@ -755,22 +787,22 @@ __device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, co
     for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? D[b][i] : 0.0f;
-    di_block<PREC, 0>(net, W, sW1, lc, G, R, hacc);
+    di_block<PREC, 0>(net, sG, sW1, lc, G, R, hacc);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? xor32(D[b][i], lc.lane) : 0.0f;
-    di_block<PREC, 1>(net, W, sW1, lc, G, R, hacc);
+    di_block<PREC, 1>(net, sG, sW1, lc, G, R, hacc);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? D[b][4 + i] : 0.0f;
-    di_block<PREC, 2>(net, W, sW1, lc, G, R, hacc);
+    di_block<PREC, 2>(net, sG, sW1, lc, G, R, hacc);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int i = 0; i < 8; ++i) G[b][i] = i < 4 ? xor32(D[b][4 + i], lc.lane) : 0.0f;
-    di_block<PREC, 3>(net, W, sW1, lc, G, R, hacc);
+    di_block<PREC, 3>(net, sG, sW1, lc, G, R, hacc);
     // ---- LD41 heads: tile A = [LD41 rows 0..7 | LD_Tr41 rows 8..15], tile B = [LD_Hg41 | LGGS]
     float tA[4], tB[4];
     {
@@ -809,9 +841,13 @@ __device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, co
     return r;
 }
 
-__device__ inline void load_shared(float* sW1, StencilEntry* sst, const float* __restrict__ W,
+__device__ inline void load_shared(float* sW1, float* sG, StencilEntry* sst, const float* __restrict__ W,
                                    const StencilEntry* __restrict__ stencil) {
     for (int i = threadIdx.x; i < 1200; i += blockDim.x) sW1[i] = W[net::LSEFin1W + i];
+    for (int i = threadIdx.x; i < kGates * kGateFloats; i += blockDim.x) {
+        const int gi = i / kGateFloats, k = i % kGateFloats;
+        sG[i] = k < 64 ? W[gate_w0(gi) + k] : W[gate_w1(gi) + k - 64];
+    }
     if (sst)
         for (int i = threadIdx.x; i < kPC; i += blockDim.x) sst[i] = stencil[i];
     __syncthreads();
@@ -825,13 +861,14 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevV
                                                          const float* __restrict__ W, const ActiveRec* __restrict__ recs,
                                                          const unsigned* __restrict__ counter) {
     __shared__ float sW1[1200];
+    __shared__ float sG[kGates * kGateFloats];
     __shared__ StencilEntry sst[FAST ? 1 : kPC];
     __shared__ FastStencil sfs[FAST ? kPC : 1];
     if (FAST) {
         for (int i = threadIdx.x; i < kPC; i += blockDim.x) sfs[i] = fast[i];
-        load_shared(sW1, nullptr, W, nullptr);
+        load_shared(sW1, sG, nullptr, W, nullptr);
     } else {
-        load_shared(sW1, sst, W, stencil);
+        load_shared(sW1, sG, sst, W, stencil);
     }
     const unsigned n = *counter;
     const int lane = threadIdx.x & 63;
@@ -890,7 +927,7 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevV
             src.px.uc1 = fmaf(dot3(ps.X, ps.LX), 0.5f, 0.5f);
             src.px.uc2 = fmaf(dot3(ps.LX, ps.LX), 0.5f, 0.5f);
             src.px.xh_centre = logf(henyey_greenstein(dot3(ps.X, ps.L), vw.g) + 1.0f);
-            rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
+            rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sG), opaque_v(sW1), lc, src);
         } else {
             GatherSrc src;
             src.vol = &vols[sl.vol];
@@ -898,7 +935,7 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevV
             src.g = vw.g;
             src.alpha = vw.alpha;
             src.ps = ps;
-            rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
+            rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sG), opaque_v(sW1), lc, src);
         }
         if (lc.ok && lc.q == 0) {
             v3 o = (rgb * vw.light_color) * vw.scatter_rate;
@@ -912,7 +949,8 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
                                                            const float* __restrict__ feats, int n, v3 sr, float g,
                                                            float* __restrict__ rgb) {
     __shared__ float sW1[1200];
-    load_shared(sW1, nullptr, W, nullptr);
+    __shared__ float sG[kGates * kGateFloats];
+    load_shared(sW1, sG, nullptr, W, nullptr);
     const int lane = threadIdx.x & 63;
     const int wave = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * (blockDim.x >> 6);
@@ -933,7 +971,7 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
         nt.bf16_lo = opaque(net.bf16_lo);
         nt.fp8 = opaque(net.fp8);
         nt.bias = opaque(net.bias);
-        v3 o = run_tile<PREC>(nt, opaque(W), opaque_v(sW1), lc, src);
+        v3 o = run_tile<PREC>(nt, opaque(W), opaque_v(sG), opaque_v(sW1), lc, src);
         if (lc.ok && lc.q == 0) {
             rgb[3 * (size_t)idx] = o.x;
             rgb[3 * (size_t)idx + 1] = o.y;
