@@ -276,7 +276,8 @@ static const size_t kMipFloats = 19173961;  // sum (256>>m)^3, m = 0..8
 static const size_t kTrFloats = 2396745;    // sum (128>>m)^3, m = 0..7
 
 extern "C" int64_t mrp_volume_bytes(int N) {
-    return (int64_t)((size_t)N * N * N + kMipFloats + kTrFloats + kPadMipFloats + kPadTrFloats) * 4;
+    const int64_t on = (N + 8) >> 3;
+    return (int64_t)((size_t)N * N * N + kMipFloats + kTrFloats + kPadMipFloats + kPadTrFloats) * 4 + on * on * on;
 }
 
 extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
@@ -303,6 +304,18 @@ extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
         delete v;
         return mrp_fail(MRP_ERR_NOMEM, "pyramid allocation failed");
     }
+    v->occ_n = (N + 8) >> 3;
+    const size_t occ_bytes = (size_t)v->occ_n * v->occ_n * v->occ_n;
+    if (hipMalloc(&v->d_occ, occ_bytes) != hipSuccess) {
+        hipFree(v->d_grid);
+        hipFree(v->d_mips);
+        hipFree(v->d_tr);
+        hipFree(v->d_pmips);
+        hipFree(v->d_ptr);
+        delete v;
+        return mrp_fail(MRP_ERR_NOMEM, "occupancy allocation failed");
+    }
+    HIP_TRY(hipMemsetAsync(v->d_occ, 0, occ_bytes, c->stream));
     HIP_TRY(hipMemsetAsync(v->d_pmips, 0, kPadMipFloats * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_ptr, 0, kPadTrFloats * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_grid, 0, n * sizeof(float), c->stream));
@@ -330,6 +343,7 @@ extern "C" int mrp_volume_destroy(mrp_volume* v) {
     hipFree(v->d_tr);
     hipFree(v->d_pmips);
     hipFree(v->d_ptr);
+    hipFree(v->d_occ);
     delete v;
     return MRP_OK;
 }
@@ -420,6 +434,7 @@ extern "C" int mrp_volume_update(mrp_volume* v) {
         for (int m = 0; m < 9; ++m) mips[m] = v->mip[m];
         HIP_TRY(launch_pad_mips(mips, v->d_pmips, s));
     }
+    HIP_TRY(launch_occupancy(v->d_grid, v->N, v->d_occ, v->occ_n, s));
     if (tmp[0] || tmp[1]) {
         HIP_TRY(hipStreamSynchronize(s));
         hipFree(tmp[0]);
@@ -509,6 +524,8 @@ DevVolume mrp_dev_volume(const mrp_volume* v) {
     d.lut = v->ctx->d_lut;
     d.pmips = v->d_pmips;
     d.ptr = v->d_ptr;
+    d.occ = v->d_occ;
+    d.occ_n = v->occ_n;
     d.N = v->N;
     d.max_density = v->max_density;
     return d;

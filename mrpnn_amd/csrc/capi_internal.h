@@ -55,6 +55,8 @@ struct mrp_volume {
     float* d_tr = nullptr;
     float* d_pmips = nullptr;  // zero-padded density mips 0..7
     float* d_ptr = nullptr;    // zero-padded TR mips 0..6
+    uint8_t* d_occ = nullptr;  // brick emptiness for the first-hit march, occ_n^3
+    int occ_n = 0;
     float* mip[9]{};
     float* trm[8]{};
     bool mips_valid = false;

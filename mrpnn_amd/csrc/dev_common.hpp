@@ -104,6 +104,8 @@ struct DevVolume {
     const float* lut;      // 512*512, [j*512+i]
     const float* pmips;    // zero-padded density mips 0..7 (fast gather)
     const float* ptr;      // zero-padded TR mips 0..6 (fast gather)
+    const uint8_t* occ;    // brick emptiness of the grid, occ_n^3 (see brick_of)
+    int occ_n;
     int N;
     float max_density;
 };
@@ -189,6 +191,19 @@ __device__ inline float shadow_term_tr(const DevVolume& v, v3 ori, v3 light, int
 // First-hit march: DeterminateEstimatedHit, bool version (render.cuh:105-197), and
 // GetSamplePoint (render.cuh:436-462).  taps counts full-res density taps.
 // ---------------------------------------------------------------------------------------
+// Empty-brick test of the first-hit march.  The grid is covered by bricks of 8 texel
+// corners: a sample whose lower trilinear corner is x0 (per axis) lies in brick (x0+1)>>3 and
+// reads texels x0, x0+1 in [8b-1, 8b+7].  occ[b] = 1 when every texel of that range (inside
+// the grid) is exactly zero, so such a sample is exactly 0 and its fetch can be skipped
+// without changing any result.
+__device__ inline bool sample_in_empty_brick(const DevVolume& v, v3 p) {
+    const v3 c = (p + 0.5f) * (float)v.N - 0.5f;  // as sample_border_exact
+    const int bx = ((int)floorf(c.x) + 1) >> 3, by = ((int)floorf(c.y) + 1) >> 3, bz = ((int)floorf(c.z) + 1) >> 3;
+    const int n = v.occ_n;
+    if ((unsigned)bx >= (unsigned)n || (unsigned)by >= (unsigned)n || (unsigned)bz >= (unsigned)n) return false;
+    return v.occ[((size_t)bx * n + by) * n + bz] != 0;
+}
+
 __device__ inline bool estimated_hit(const DevVolume& v, float alpha, v3 pos, v3 dir, float dis, v3* next, int* taps) {
     float smax = v.max_density * alpha;
     float t = 0.0f, last_non_empty_t = 0.0f, prob = 1.0f;
@@ -203,7 +218,8 @@ __device__ inline bool estimated_hit(const DevVolume& v, float alpha, v3 pos, v3
             *next = mk3(0, 0, 0);
             return false;
         }
-        float d = density_full(v, pos + (dir * t));
+        const v3 sp = pos + (dir * t);
+        float d = (v.occ && sample_in_empty_brick(v, sp)) ? 0.0f : density_full(v, sp);
         ++*taps;
         if (d > 0.0f) {
             float S = d * alpha;

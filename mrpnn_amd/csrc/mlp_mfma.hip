@@ -252,34 +252,45 @@ template <int PREC, int ID, bool SB = true>
 __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8], float (&out)[mfma_nm(ID)][4],
                              int lane, bool relu) {
     constexpr int NM = mfma_nm(ID), NK = mfma_nk(ID), FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
+    constexpr bool TWO = PREC == MRP_BF16;  // second weight term
     const int q = lane >> 4;
+    // Fragments of M-tile mt+1 are requested before the MFMAs of M-tile mt (two buffers), so
+    // the weight latency of a multi-tile layer is paid once.
     if constexpr (PREC == MRP_BF16 || PREC == MRP_BF16_1T) {
         u32x4_t b[NK];
 #pragma unroll
         for (int kc = 0; kc < NK; ++kc)
             b[kc] = u32x4_t{pack_bf16x2(in[kc][0], in[kc][1]), pack_bf16x2(in[kc][2], in[kc][3]),
                             pack_bf16x2(in[kc][4], in[kc][5]), pack_bf16x2(in[kc][6], in[kc][7])};
-#pragma unroll
-        for (int mt = 0; mt < NM; ++mt) {
-            f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        u32x4_t wa[2][NK], wl[2][NK], bb[2];
+        auto fetch = [&](int mt, int buf) {
 #pragma unroll
             for (int kc = 0; kc < NK; ++kc) {
                 const int fo = (FRAG + mt * NK + kc) * 1024;
-                u32x4_t a = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, fo, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
-                                                              __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
-                if constexpr (PREC == MRP_BF16) {  // second weight term
-                    u32x4_t al = __builtin_amdgcn_raw_buffer_load_b128(net.wlo, lane * 16, fo, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, al),
-                                                                  __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
-                }
+                wa[buf][kc] = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, fo, 0);
+                if constexpr (TWO) wl[buf][kc] = __builtin_amdgcn_raw_buffer_load_b128(net.wlo, lane * 16, fo, 0);
             }
-            u32x4_t bb = u32x4_t{0u, 0u, 0u, 0u};
+            bb[buf] = u32x4_t{0u, 0u, 0u, 0u};
             if constexpr (mfma_has_bias(ID))
-                bb = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
+                bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
+        };
+        fetch(0, 0);
+#pragma unroll
+        for (int mt = 0; mt < NM; ++mt) {
+            const int cur = mt & 1;
+            if (mt + 1 < NM) fetch(mt + 1, cur ^ 1);
+            f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kc = 0; kc < NK; ++kc) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, wa[cur][kc]),
+                                                              __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
+                if constexpr (TWO)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, wl[cur][kc]),
+                                                                  __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float v = acc[i] + __uint_as_float(bb[i]);
+                float v = acc[i] + __uint_as_float(bb[cur][i]);
                 out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
             }
             if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
@@ -301,21 +312,28 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
             b[kc] = (long)(((uint64_t)hi << 32) | lo);
         }
         const float ds = ldexpf(1.0f, -(ea + net.wexp[ID]));
+        long wa[2][NK];
+        u32x4_t bb[2];
+        auto fetch = [&](int mt, int buf) {
+#pragma unroll
+            for (int kc = 0; kc < NK; ++kc)
+                wa[buf][kc] = __builtin_bit_cast(
+                    long, __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, (FRAG + mt * NK + kc) * 512, 0));
+            bb[buf] = u32x4_t{0u, 0u, 0u, 0u};
+            if constexpr (mfma_has_bias(ID))
+                bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
+        };
+        fetch(0, 0);
 #pragma unroll
         for (int mt = 0; mt < NM; ++mt) {
+            const int cur = mt & 1;
+            if (mt + 1 < NM) fetch(mt + 1, cur ^ 1);
             f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int kc = 0; kc < NK; ++kc) {
-                const long a = __builtin_bit_cast(
-                    long, __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, (FRAG + mt * NK + kc) * 512, 0));
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b[kc], acc, 0, 0, 0);
-            }
-            u32x4_t bb = u32x4_t{0u, 0u, 0u, 0u};
-            if constexpr (mfma_has_bias(ID))
-                bb = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
+            for (int kc = 0; kc < NK; ++kc) acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(wa[cur][kc], b[kc], acc, 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float v = fmaf(acc[i], ds, __uint_as_float(bb[i]));
+                float v = fmaf(acc[i], ds, __uint_as_float(bb[cur][i]));
                 out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
             }
             if constexpr (SB) __builtin_amdgcn_sched_barrier(0);

@@ -139,10 +139,11 @@ __global__ __launch_bounds__(256) void k_tr_bake(TrArgs a) {
         int m = 0;
         while (m < 7 && g >= a.start[m + 1]) ++m;
         const int res = 128 >> m;
-        const long long l = g - a.start[m];
-        const int z = (int)(l % res);
-        const int y = (int)((l / res) % res);
-        const int x = (int)(l / ((long long)res * res));
+        const long long i_lvl = g - a.start[m];
+        const int z = (int)(i_lvl % res);
+        const int y = (int)((i_lvl / res) % res);
+        const int x = (int)(i_lvl / ((long long)res * res));
+        const long long l = i_lvl;
         v3 uv = (mk3((float)x, (float)y, (float)z) + 0.5f) / (float)res;
         v3 ori = uv - 0.5f;
         float dis = ray_box_distance(ori, a.light);
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(256) void k_tr_bake(TrArgs a) {
             const unsigned R = (unsigned)pad_dim(sres);
             const float fres = (float)sres;
             const BufRsrc rs = make_rsrc(a.pmips, kPadMipFloats);
-#pragma unroll 4
+#pragma unroll 8
             for (int i = 0; i < 128; ++i) {
                 lp = lp + a.light * st;
                 sum = sum + tri_border(rs, base, R, R * R, fres, lp);
@@ -277,6 +278,28 @@ __global__ __launch_bounds__(256) void k_pad_mips(PadArgs a, float* __restrict__
     }
 }
 
+// Brick emptiness of the full-resolution grid for the first-hit march: brick b covers
+// texels [8b-1, 8b+7] per axis (the corners a sample in that brick can read); 1 = all zero.
+__global__ __launch_bounds__(256) void k_occupancy(const float* __restrict__ grid, int N, uint8_t* __restrict__ occ,
+                                                   int n) {
+    const long long total = (long long)n * n * n;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+        const int bz = (int)(i % n), by = (int)((i / n) % n), bx = (int)(i / ((long long)n * n));
+        const int x0 = max(8 * bx - 1, 0), x1 = min(8 * bx + 7, N - 1);
+        const int y0 = max(8 * by - 1, 0), y1 = min(8 * by + 7, N - 1);
+        const int z0 = max(8 * bz - 1, 0), z1 = min(8 * bz + 7, N - 1);
+        bool empty = true;
+        for (int x = x0; x <= x1 && empty; ++x)
+            for (int y = y0; y <= y1 && empty; ++y)
+                for (int z = z0; z <= z1; ++z)
+                    if (grid[((size_t)x * N + y) * N + z] != 0.0f) {
+                        empty = false;
+                        break;
+                    }
+        occ[i] = empty ? 1 : 0;
+    }
+}
+
 // Fast-gather stencil table: direction * offset, LUT v, the per-point LUT lookup of sphere
 // points (dot(XMain, direction) = dx/|d| for an orthonormal basis), padded level geometry.
 __global__ void k_fast_stencil(const StencilEntry* __restrict__ st, const float* __restrict__ lut,
@@ -359,6 +382,11 @@ hipError_t launch_pad_mips(const float* const mips[9], float* pmips, hipStream_t
     }
     a.start[8] = acc;
     hipLaunchKernelGGL(k_pad_mips, dim3(grid_for((size_t)acc, 256)), dim3(256), 0, s, a, pmips);
+    return hipGetLastError();
+}
+hipError_t launch_occupancy(const float* grid, int N, uint8_t* occ, int occ_n, hipStream_t s) {
+    hipLaunchKernelGGL(k_occupancy, dim3(grid_for((size_t)occ_n * occ_n * occ_n, 256)), dim3(256), 0, s, grid, N, occ,
+                       occ_n);
     return hipGetLastError();
 }
 hipError_t launch_fast_stencil(const StencilEntry* st, const float* lut, void* fast, hipStream_t s) {
