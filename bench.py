@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--plain-steps", type=int, default=4)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 1 GiB device-to-device (scale reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -69,6 +71,13 @@ def main():
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda")
+    if args.copy_probe:
+        src = torch.ones(256 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy probe: 1 GiB read + 1 GiB written")
     ctx = M.Context()
     ctx.set_timing(True)
     W, H = args.width, args.height
