@@ -45,6 +45,12 @@ extern "C" {
 #define MRP_OPT_EXACT_GATHER 2      /* 1: the fused MFMA kernels gather features with the FP32 path's exact
                                        arithmetic (bit-identical features); 0 (default): fast gather from
                                        zero-padded pyramid copies (features within a few ulps) */
+#define MRP_OPT_GRAPH 4             /* 1: mrp_render_batch captures its work (staging copies, counter reset,
+                                       march, network) into a HIP graph once per launch shape and
+                                       replays it; timing then reports the whole replay as ms_predict */
+#define MRP_OPT_TR_SLOTS 3          /* capacity of the batched path's TR cache: one set of transmittance
+                                       mips (about 20 MB) per distinct (volume, light_dir, alpha) of a
+                                       batch (1..4096, default 64) */
 
 /* Number of features per pixel: [X_Val 0:192 | X_Val_Hg 192:384 | X_Val_Sub 384:576 | gamma]
  * (core/radiancePredict.cu:427-485, row order of PopulateInputData :562-565). */
@@ -148,7 +154,8 @@ const float* mrp_volume_grid_device(mrp_volume* vol);
  * average over frames (frame f uses Philox key word 1 = view->frame + f).  Blocking;
  * host_out4 receives width*height float4 (rgb, distance). */
 int mrp_render_plain(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int n_frames, float* host_out4);
-/* One frame of the same kernel into a device buffer, asynchronous (LUT/TR must be current). */
+/* One frame of the same kernel into a device buffer, asynchronous; rebuilds the LUT and the
+ * volume's TR mips first when (g, light, alpha) changed, as mrp_render_plain does. */
 int mrp_render_plain_device(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, float* dev_out4);
 
 /* Fused predictor: march + compaction, then gather and MLP (FP32 / BF16 MFMA / FP8 MFMA)
@@ -157,8 +164,17 @@ int mrp_render_plain_device(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view,
  * mrp_prepare_views).  Asynchronous. */
 int mrp_render_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views,
                      int precision, float* const* dev_out4);
-/* Brings LUT and TR mips up to date for every view (first view per volume wins if two
- * views of one volume disagree: call per light then). */
+/* Launch-shape autotuner (no counterpart in the reference, whose launch shapes are fixed):
+ * renders the prepared batch with 4, 8, 12, 16, 24 and 32 persistent waves per CU for the
+ * network kernel, keeps the fastest as MRP_OPT_MFMA_WAVES_PER_CU and reports it with its
+ * time per batch.  Blocking. */
+int mrp_autotune_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views,
+                       int precision, float* const* dev_out4, int* best_waves_per_cu, float* best_ms);
+/* Brings the HG LUT (all views share g) and the batch TR cache up to date: one set of TR mips
+ * per distinct (volume, light_dir, alpha) of the views, baked only when not cached
+ * (Update_TR, core/volume.cu:1589-1667).  Views of one volume may use different suns.  Sets
+ * least recently used by earlier calls are reused when the cache (MRP_OPT_TR_SLOTS) is full;
+ * MRP_ERR_NOMEM when one call needs more sets than the cache holds. */
 int mrp_prepare_views(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views);
 
 /* Feature-level entry points (parity and tooling).

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmrpnn_hip.so")
 
 FP32, BF16, FP8, BF16_1T = 0, 1, 2, 3
 PRECISIONS = {"fp32": FP32, "bf16": BF16, "fp8": FP8, "bf16_1t": BF16_1T}
-OPT_MFMA_WAVES_PER_CU, OPT_EXACT_GATHER = 1, 2
+OPT_MFMA_WAVES_PER_CU, OPT_EXACT_GATHER, OPT_TR_SLOTS, OPT_GRAPH = 1, 2, 3, 4
 NFEAT = 577
 MIP_TOTAL = sum((256 >> m) ** 3 for m in range(9))
 TR_TOTAL = sum((128 >> m) ** 3 for m in range(8))
@@ -70,6 +70,9 @@ def lib():
         L.mrp_ctx_download_hg_lut.argtypes = [_vp, _fp]
         L.mrp_ctx_upload_hg_lut.argtypes = [_vp, _fp, ctypes.c_float]
         L.mrp_ctx_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.mrp_autotune_batch.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View), ctypes.c_int,
+                                         ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_float)]
         L.mrp_ctx_get_option.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.mrp_volume_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
         L.mrp_volume_destroy.argtypes = [_vp]
@@ -171,6 +174,17 @@ class Context:
         va = (View * len(views))(*views)
         oa = (_vp * len(outs))(*[o.data_ptr() for o in outs])
         check(lib().mrp_render_batch(self.h, vh, len(volumes), va, len(views), int(precision), oa))
+
+    def autotune(self, volumes, views, outs, precision=BF16):
+        """Times the prepared batch per candidate waves-per-CU; keeps and returns the best
+        (waves_per_cu, ms per batch)."""
+        vh = (_vp * len(volumes))(*[v.h for v in volumes])
+        va = (View * len(views))(*views)
+        oa = (_vp * len(outs))(*[o.data_ptr() for o in outs])
+        w, ms = ctypes.c_int(), ctypes.c_float()
+        check(lib().mrp_autotune_batch(self.h, vh, len(volumes), va, len(views), int(precision), oa,
+                                       ctypes.byref(w), ctypes.byref(ms)))
+        return w.value, ms.value
 
     def prepare_views(self, volumes, views):
         vh = (_vp * len(volumes))(*[v.h for v in volumes])

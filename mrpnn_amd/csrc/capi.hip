@@ -168,6 +168,10 @@ extern "C" int mrp_ctx_destroy(mrp_ctx* c) {
     hipFree(c->d_lut);
     hipFree(c->d_fast);
     hipFree(c->d_plain_out);
+    for (TrSlot& t : c->tr_slots) {
+        hipFree(t.d_tr);
+        hipFree(t.d_ptr);
+    }
     for (int i = 0; i < 4; ++i) hipEventDestroy(c->ev[i]);
     for (int i = 0; i < TimingRing::kSlots; ++i)
         for (int j = 0; j < 3; ++j) hipEventDestroy(c->ring.ev[i][j]);
@@ -272,8 +276,6 @@ extern "C" int mrp_ctx_upload_hg_lut(mrp_ctx* c, const float* host, float g) {
 // ------------------------------------------------------------------------------------------
 // Volumes
 // ------------------------------------------------------------------------------------------
-static const size_t kMipFloats = 19173961;  // sum (256>>m)^3, m = 0..8
-static const size_t kTrFloats = 2396745;    // sum (128>>m)^3, m = 0..7
 
 extern "C" int64_t mrp_volume_bytes(int N) {
     const int64_t on = (N + 8) >> 3;
@@ -338,6 +340,7 @@ extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
 extern "C" int mrp_volume_destroy(mrp_volume* v) {
     if (!v) return MRP_OK;
     hipStreamSynchronize(v->ctx->stream);
+    mrp_tr_slots_drop(v->ctx, v);
     hipFree(v->d_grid);
     hipFree(v->d_mips);
     hipFree(v->d_tr);
@@ -442,22 +445,35 @@ extern "C" int mrp_volume_update(mrp_volume* v) {
     }
     v->mips_valid = true;
     v->tr_valid = false;
+    mrp_tr_slots_drop(v->ctx, v);
     return MRP_OK;
 }
 
 // Update_TR (core/volume.cu:1589-1667), cached on exact (light, alpha)
-int mrp_volume_update_tr_impl(mrp_volume* v, const float L[3], float alpha) {
+// Fill_TR for all 8 levels into the given buffers (levels packed as in mrp_volume::d_tr).
+int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, float* d_ptr) {
     if (!v->mips_valid) return mrp_fail(MRP_ERR_STATE, "mrp_volume_update must run before the TR bake");
-    if (v->tr_valid && v->tr_light[0] == L[0] && v->tr_light[1] == L[1] && v->tr_light[2] == L[2] &&
-        v->tr_alpha == alpha)
-        return MRP_OK;
     float alpha_lvl[8];
     for (int m = 0; m < 8; ++m) alpha_lvl[m] = alpha / powf(1.73f, (float)m + 1.0f);
     const float* mips[9];
     float* tr[8];
     for (int m = 0; m < 9; ++m) mips[m] = v->mip[m];
-    for (int m = 0; m < 8; ++m) tr[m] = v->trm[m];
-    HIP_TRY(launch_tr_bake(mips, v->d_pmips, tr, v->d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), v->ctx->stream));
+    size_t o = 0;
+    for (int m = 0; m < 8; ++m) {
+        tr[m] = d_tr + o;
+        o += (size_t)(128 >> m) * (128 >> m) * (128 >> m);
+    }
+    HIP_TRY(launch_tr_bake(mips, v->d_pmips, tr, d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), v->ctx->stream));
+    return MRP_OK;
+}
+
+int mrp_volume_update_tr_impl(mrp_volume* v, const float L[3], float alpha) {
+    if (!v->mips_valid) return mrp_fail(MRP_ERR_STATE, "mrp_volume_update must run before the TR bake");
+    if (v->tr_valid && v->tr_light[0] == L[0] && v->tr_light[1] == L[1] && v->tr_light[2] == L[2] &&
+        v->tr_alpha == alpha)
+        return MRP_OK;
+    int rc = mrp_tr_bake_into(v, L, alpha, v->d_tr, v->d_ptr);
+    if (rc) return rc;
     v->tr_valid = true;
     memcpy(v->tr_light, L, sizeof v->tr_light);
     v->tr_alpha = alpha;
@@ -546,8 +562,9 @@ extern "C" int mrp_render_plain_device(mrp_ctx* c, mrp_volume* v, const mrp_view
     CHECK_ARG(c && v && dev_out4, "null argument");
     int rc = mrp_check_view(vw);
     if (rc) return rc;
-    if (!v->mips_valid || !c->lut_valid || !v->tr_valid)
-        return mrp_fail(MRP_ERR_STATE, "LUT / TR mips not prepared for this view");
+    // LUT for g and TR mips for (light, alpha), rebuilt only when they change (volume.cu:511-514)
+    rc = mrp_prepare_view(c, v, vw);
+    if (rc) return rc;
     DevVolume dv = mrp_dev_volume(v);
     ViewDev d = mrp_to_dev(vw);
     hipEvent_t* ev = nullptr;

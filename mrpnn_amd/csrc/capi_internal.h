@@ -9,6 +9,10 @@
 
 struct FusedState;  // fused.hip
 
+// Packed pyramid sizes (floats): density mips 0..8 ((256>>m)^3), TR mips 0..7 ((128>>m)^3).
+constexpr size_t kMipFloats = 19173961;
+constexpr size_t kTrFloats = 2396745;
+
 // Device timing of renders without a host sync per render: each timed render records three
 // events (start, after the march, end) into a ring slot; a slot is read back when it is
 // reused or when the statistics are requested.
@@ -23,6 +27,18 @@ struct TimingRing {
 };
 int mrp_timing_begin(struct mrp_ctx* c, hipEvent_t** ev);  // ev = the three events of a fresh slot
 int mrp_timing_collect(struct mrp_ctx* c);                  // reads back every pending slot
+
+// One transmittance-mip set of the batched path, keyed by (volume, light direction, alpha):
+// views of one volume under different suns each get their own set.
+struct TrSlot {
+    const struct mrp_volume* vol = nullptr;
+    float light[3]{};
+    float alpha = 0.0f;
+    float* d_tr = nullptr;   // TR mips 0..7 (kTrFloats)
+    float* d_ptr = nullptr;  // zero-padded TR mips 0..6 (kPadTrFloats)
+    uint64_t stamp = 0;      // last prepare call that used it
+    bool valid = false;
+};
 
 struct mrp_ctx {
     int device = 0;
@@ -44,6 +60,9 @@ struct mrp_ctx {
     TimingRing ring;
     mrp_stats stats{};
     FusedState* fused = nullptr;
+    std::vector<TrSlot> tr_slots;  // TR cache of the batched path
+    int tr_slot_cap = 64;
+    uint64_t tr_stamp = 0;
 };
 
 struct mrp_volume {
@@ -71,6 +90,9 @@ mrp::ViewDev mrp_to_dev(const mrp_view* v);
 mrp::DevVolume mrp_dev_volume(const mrp_volume* v);
 int mrp_prepare_view(mrp_ctx* c, mrp_volume* v, const mrp_view* vw);
 int mrp_volume_update_tr_impl(mrp_volume* v, const float L[3], float alpha);
+int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, float* d_ptr);
+void mrp_tr_slots_drop(mrp_ctx* c, const mrp_volume* v);  // invalidate a volume's cached sets
+int mrp_tr_slot_find(const mrp_ctx* c, const mrp_volume* v, const float L[3], float alpha);
 
 // fused.hip
 int mrp_fused_init(mrp_ctx* c);

@@ -8,6 +8,7 @@
 #include "capi_internal.h"
 #include "fused.h"
 #include "mlp_mfma.h"
+#include "kernels.h"
 
 using namespace mrp;
 
@@ -25,10 +26,24 @@ struct FusedState {
     bool have_counts = false;
     MfmaNet net;  // packed BF16 / FP8 weights
     int mfma_waves_per_cu = 8;
+    // graph mode: the batch (staging copies, counter reset, march, network) is captured once
+    // per launch shape on a private stream and replayed on the context's stream
+    bool use_graph = false;
+    hipStream_t cap_stream = nullptr;
+    struct Graph {
+        int n_views, n_entries, n_tiles, precision, fast, grid;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
 };
 
+static void clear_graphs(FusedState* f) {
+    for (auto& g : f->graphs) hipGraphExecDestroy(g.exec);
+    f->graphs.clear();
+}
+
 static int grow(void** p, size_t bytes) {
-    hipFree(*p);
+    hipFree(*p);  // callers drop captured graphs first: they hold the old address
     *p = nullptr;
     if (hipMalloc(p, bytes) != hipSuccess) return mrp_fail(MRP_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
     return MRP_OK;
@@ -57,6 +72,8 @@ void mrp_fused_free(mrp_ctx* c) {
     hipHostFree(f->h_vols);
     hipHostFree(f->h_slots);
     if (f->staged) hipEventDestroy(f->staged);
+    clear_graphs(f);
+    if (f->cap_stream) hipStreamDestroy(f->cap_stream);
     mfma_net_destroy(&f->net);
     delete f;
     c->fused = nullptr;
@@ -80,25 +97,81 @@ int mrp_fused_collect_stats(mrp_ctx* c) {
     return MRP_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// TR cache of the batched path
+// ------------------------------------------------------------------------------------------
+static bool tr_key_eq(const TrSlot& t, const mrp_volume* v, const float L[3], float alpha) {
+    return t.valid && t.vol == v && t.light[0] == L[0] && t.light[1] == L[1] && t.light[2] == L[2] &&
+           t.alpha == alpha;
+}
+int mrp_tr_slot_find(const mrp_ctx* c, const mrp_volume* v, const float L[3], float alpha) {
+    for (size_t i = 0; i < c->tr_slots.size(); ++i)
+        if (tr_key_eq(c->tr_slots[i], v, L, alpha)) return (int)i;
+    return -1;
+}
+void mrp_tr_slots_drop(mrp_ctx* c, const mrp_volume* v) {
+    for (TrSlot& t : c->tr_slots)
+        if (t.vol == v) t.valid = false;
+}
+// A slot for a new key: an unused one, a new one below the cap, else the least recently used
+// slot that the current prepare call has not claimed.
+static int tr_slot_claim(mrp_ctx* c) {
+    int best = -1;
+    for (size_t i = 0; i < c->tr_slots.size(); ++i) {
+        const TrSlot& t = c->tr_slots[i];
+        if (!t.valid) return (int)i;
+        if (t.stamp != c->tr_stamp && (best < 0 || t.stamp < c->tr_slots[best].stamp)) best = (int)i;
+    }
+    if ((int)c->tr_slots.size() < c->tr_slot_cap) {
+        TrSlot t;
+        if (hipMalloc(&t.d_tr, kTrFloats * sizeof(float)) != hipSuccess ||
+            hipMalloc(&t.d_ptr, kPadTrFloats * sizeof(float)) != hipSuccess) {
+            hipFree(t.d_tr);
+            return -1;
+        }
+        // the padded copy's border must read as zero; the bake writes the interior only
+        if (hipMemsetAsync(t.d_ptr, 0, kPadTrFloats * sizeof(float), c->stream) != hipSuccess) return -1;
+        c->tr_slots.push_back(t);
+        return (int)c->tr_slots.size() - 1;
+    }
+    return best;
+}
+
+// LUT for the batch's g, and a TR set for every distinct (volume, light, alpha) of the views
+// (Update_TR, volume.cu:1589-1667, once per key).
 extern "C" int mrp_prepare_views(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views) {
     if (!c || !vols || !views || n_vols < 1 || n_views < 1) return mrp_fail(MRP_ERR_INVALID, "bad batch arguments");
-    std::vector<int> light_of(n_vols, -1);
     for (int i = 0; i < n_views; ++i) {
         int rc = mrp_check_view(&views[i]);
         if (rc) return rc;
         int vi = views[i].volume_index;
         if (vi < 0 || vi >= n_vols || !vols[vi]) return mrp_fail(MRP_ERR_INVALID, "view %d: bad volume_index", i);
+        if (vols[vi]->ctx != c) return mrp_fail(MRP_ERR_INVALID, "view %d: volume belongs to another context", i);
+        if (!vols[vi]->mips_valid) return mrp_fail(MRP_ERR_STATE, "view %d: mrp_volume_update must run first", i);
         if (views[i].g != views[0].g) return mrp_fail(MRP_ERR_INVALID, "all views of a batch must share g");
-        if (light_of[vi] >= 0) {
-            const mrp_view& a = views[light_of[vi]];
-            if (memcmp(a.light_dir, views[i].light_dir, sizeof a.light_dir) != 0 || a.alpha != views[i].alpha)
-                return mrp_fail(MRP_ERR_INVALID, "views %d and %d share a volume but not (light, alpha)",
-                                light_of[vi], i);
-            continue;
+    }
+    int rc = mrp_ctx_update_hg_lut(c, views[0].g, c->lut_valid ? c->lut_seed : 1u);
+    if (rc) return rc;
+    ++c->tr_stamp;
+    for (int i = 0; i < n_views; ++i) {
+        mrp_volume* v = vols[views[i].volume_index];
+        const float* L = views[i].light_dir;
+        int s = mrp_tr_slot_find(c, v, L, views[i].alpha);
+        if (s < 0) {
+            s = tr_slot_claim(c);
+            if (s < 0)
+                return mrp_fail(MRP_ERR_NOMEM, "TR cache: no free set for view %d (raise MRP_OPT_TR_SLOTS, now %d)", i,
+                                c->tr_slot_cap);
+            TrSlot& t = c->tr_slots[s];
+            t.valid = false;
+            rc = mrp_tr_bake_into(v, L, views[i].alpha, t.d_tr, t.d_ptr);
+            if (rc) return rc;
+            t.vol = v;
+            memcpy(t.light, L, sizeof t.light);
+            t.alpha = views[i].alpha;
+            t.valid = true;
         }
-        light_of[vi] = i;
-        rc = mrp_prepare_view(c, vols[vi], &views[i]);
-        if (rc) return rc;
+        c->tr_slots[s].stamp = c->tr_stamp;
     }
     return MRP_OK;
 }
@@ -111,6 +184,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         return mrp_fail(MRP_ERR_INVALID, "unknown precision %d", precision);
     FusedState* f = c->fused;
     if (!c->lut_valid || c->lut_g != views[0].g) return mrp_fail(MRP_ERR_STATE, "HG LUT not prepared for g");
+    std::vector<int> view_slot(n_views, -1);
     int n_tiles = 0;
     size_t pixels = 0;
     for (int i = 0; i < n_views; ++i) {
@@ -123,28 +197,33 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         if (v.g != views[0].g) return mrp_fail(MRP_ERR_INVALID, "all views of a batch must share g");
         const mrp_volume* vol = vols[v.volume_index];
         if (vol->ctx != c) return mrp_fail(MRP_ERR_INVALID, "volume belongs to another context");
-        if (!vol->mips_valid || !vol->tr_valid || vol->tr_alpha != v.alpha ||
-            memcmp(vol->tr_light, v.light_dir, sizeof vol->tr_light) != 0)
-            return mrp_fail(MRP_ERR_STATE, "view %d: TR mips not prepared (call mrp_prepare_views)", i);
+        const int slot = vol->mips_valid ? mrp_tr_slot_find(c, vol, v.light_dir, v.alpha) : -1;
+        if (slot < 0) return mrp_fail(MRP_ERR_STATE, "view %d: TR mips not prepared (call mrp_prepare_views)", i);
+        view_slot[i] = slot;
         n_tiles += ((v.width + 7) / 8) * ((v.height + 7) / 8);
         pixels += (size_t)v.width * v.height;
     }
     // staging buffers are reused: wait for the previous batch's uploads
     if (hipEventSynchronize(f->staged) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "event sync failed");
-    if (n_vols > f->vols_cap) {
+    // one DevVolume entry per distinct (volume, TR set) of the batch
+    if (n_views > f->vols_cap) {
+        clear_graphs(f);
         hipHostFree(f->h_vols);
         f->h_vols = nullptr;
-        if (hipHostMalloc(&f->h_vols, n_vols * sizeof(DevVolume)) != hipSuccess)
+        if (hipHostMalloc(&f->h_vols, n_views * sizeof(DevVolume)) != hipSuccess)
             return mrp_fail(MRP_ERR_NOMEM, "pinned allocation failed");
-        int rc = grow((void**)&f->d_vols, n_vols * sizeof(DevVolume));
+        clear_graphs(f);
+        int rc = grow((void**)&f->d_vols, n_views * sizeof(DevVolume));
         if (rc) return rc;
-        f->vols_cap = n_vols;
+        f->vols_cap = n_views;
     }
     if (n_views > f->slots_cap) {
+        clear_graphs(f);
         hipHostFree(f->h_slots);
         f->h_slots = nullptr;
         if (hipHostMalloc(&f->h_slots, n_views * sizeof(ViewSlot)) != hipSuccess)
             return mrp_fail(MRP_ERR_NOMEM, "pinned allocation failed");
+        clear_graphs(f);
         int rc = grow((void**)&f->d_slots, n_views * sizeof(ViewSlot));
         if (rc) return rc;
         f->slots_cap = n_views;
@@ -152,17 +231,33 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
     // each 8x8 tile takes its active pixels rounded up to 16 records: at most 64 per tile
     const size_t need = (size_t)n_tiles * 64;
     if (need > f->cap) {
+        clear_graphs(f);
         int rc = grow((void**)&f->d_recs, need * sizeof(ActiveRec));
         if (rc) return rc;
         f->cap = need;
     }
-    for (int i = 0; i < n_vols; ++i) f->h_vols[i] = mrp_dev_volume(vols[i]);
+    std::vector<int> entry_of_slot(c->tr_slots.size(), -1);
+    int n_entries = 0;
+    for (int i = 0; i < n_views; ++i) {
+        const int sl = view_slot[i];
+        if (entry_of_slot[sl] >= 0) continue;
+        const TrSlot& t = c->tr_slots[sl];
+        DevVolume d = mrp_dev_volume(vols[views[i].volume_index]);
+        size_t o = 0;
+        for (int m = 0; m < 8; ++m) {
+            d.tr[m] = t.d_tr + o;
+            o += (size_t)(128 >> m) * (128 >> m) * (128 >> m);
+        }
+        d.ptr = t.d_ptr;
+        entry_of_slot[sl] = n_entries;
+        f->h_vols[n_entries++] = d;
+    }
     int t0 = 0;
     for (int i = 0; i < n_views; ++i) {
         ViewSlot& s = f->h_slots[i];
         s.vw = mrp_to_dev(&views[i]);
         s.out = (float4*)dev_out4[i];
-        s.vol = views[i].volume_index;
+        s.vol = entry_of_slot[view_slot[i]];
         s.tile0 = t0;
         s.tiles_x = (views[i].width + 7) / 8;
         s.pad = 0;
@@ -174,27 +269,69 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         hipError_t e_ = (x);                                                                \
         if (e_ != hipSuccess) return mrp_fail(MRP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
     } while (0)
-    HT(hipMemcpyAsync(f->d_vols, f->h_vols, n_vols * sizeof(DevVolume), hipMemcpyHostToDevice, st));
-    HT(hipMemcpyAsync(f->d_slots, f->h_slots, n_views * sizeof(ViewSlot), hipMemcpyHostToDevice, st));
-    HT(hipEventRecord(f->staged, st));
+    const int grid = c->props.multiProcessorCount * f->mfma_waves_per_cu / 4;
+    const bool fast = !c->exact_gather;
+    // the work of one batch on stream q; ev (optional) brackets march and network
+    auto enqueue = [&](hipStream_t q, hipEvent_t* ev, hipEvent_t staged) -> hipError_t {
+        hipError_t e;
+        if ((e = hipMemcpyAsync(f->d_vols, f->h_vols, n_entries * sizeof(DevVolume), hipMemcpyHostToDevice, q)))
+            return e;
+        if ((e = hipMemcpyAsync(f->d_slots, f->h_slots, n_views * sizeof(ViewSlot), hipMemcpyHostToDevice, q)))
+            return e;
+        if (staged && (e = hipEventRecord(staged, q))) return e;
+        if (ev && (e = hipEventRecord(ev[0], q))) return e;
+        if ((e = launch_reset_counters(f->d_counter, f->d_taps, q))) return e;
+        if ((e = launch_march_compact(f->d_vols, f->d_slots, n_views, n_tiles, f->d_recs, f->d_counter, f->d_taps, q)))
+            return e;
+        if (ev && (e = hipEventRecord(ev[1], q))) return e;
+        if (precision == MRP_FP32)
+            e = launch_predict_active_fp32(f->d_vols, f->d_slots, c->d_stencil, c->d_weights, f->d_recs, f->d_counter,
+                                           (int)need, q);
+        else
+            e = launch_mlp_mfma_active(precision, fast, f->net, f->d_vols, f->d_slots, c->d_stencil, c->d_fast,
+                                       c->d_weights, f->d_recs, f->d_counter, grid, q);
+        if (e) return e;
+        if (ev && (e = hipEventRecord(ev[2], q))) return e;
+        return hipSuccess;
+    };
     hipEvent_t* ev = nullptr;
     if (c->timing) {
         int rc = mrp_timing_begin(c, &ev);
         if (rc) return rc;
-        HT(hipEventRecord(ev[0], st));
     }
-    HT(launch_reset_counters(f->d_counter, f->d_taps, st));
-    HT(launch_march_compact(f->d_vols, f->d_slots, n_views, n_tiles, f->d_recs, f->d_counter, f->d_taps, st));
-    if (c->timing) HT(hipEventRecord(ev[1], st));
-    if (precision == MRP_FP32) {
-        HT(launch_predict_active_fp32(f->d_vols, f->d_slots, c->d_stencil, c->d_weights, f->d_recs, f->d_counter,
-                                      (int)need, st));
+    if (!f->use_graph) {
+        HT(enqueue(st, ev, f->staged));
     } else {
-        int grid = c->props.multiProcessorCount * f->mfma_waves_per_cu / 4;
-        HT(launch_mlp_mfma_active(precision, !c->exact_gather, f->net, f->d_vols, f->d_slots, c->d_stencil, c->d_fast,
-                                  c->d_weights, f->d_recs, f->d_counter, grid, st));
+        FusedState::Graph* g = nullptr;
+        for (auto& x : f->graphs)
+            if (x.n_views == n_views && x.n_entries == n_entries && x.n_tiles == n_tiles && x.precision == precision &&
+                x.fast == (int)fast && x.grid == grid)
+                g = &x;
+        if (!g) {
+            if (!f->cap_stream) HT(hipStreamCreateWithFlags(&f->cap_stream, hipStreamNonBlocking));
+            if (f->graphs.size() >= 16) clear_graphs(f);
+            hipGraph_t graph = nullptr;
+            HT(hipStreamBeginCapture(f->cap_stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = enqueue(f->cap_stream, nullptr, nullptr);
+            hipError_t ee = hipStreamEndCapture(f->cap_stream, &graph);
+            HT(ce);
+            HT(ee);
+            hipGraphExec_t exec = nullptr;
+            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            HT(ie);
+            f->graphs.push_back({n_views, n_entries, n_tiles, precision, (int)fast, grid, exec});
+            g = &f->graphs.back();
+        }
+        // graph mode times the whole replay as the network time (march time reads 0)
+        if (ev) {
+            HT(hipEventRecord(ev[0], st));
+            HT(hipEventRecord(ev[1], st));
+        }
+        HT(hipGraphLaunch(g->exec, st));
+        if (ev) HT(hipEventRecord(ev[2], st));
+        HT(hipEventRecord(f->staged, st));
     }
-    if (c->timing) HT(hipEventRecord(ev[2], st));
     f->have_counts = true;
     c->stats.pixels = (int64_t)pixels;
     c->stats.active_pixels = -1;
@@ -223,6 +360,21 @@ extern "C" int mrp_ctx_set_option(mrp_ctx* c, int option, int value) {
         case MRP_OPT_EXACT_GATHER:
             c->exact_gather = value != 0;
             return MRP_OK;
+        case MRP_OPT_GRAPH:
+            c->fused->use_graph = value != 0;
+            return MRP_OK;
+        case MRP_OPT_TR_SLOTS:
+            if (value < 1 || value > 4096) return mrp_fail(MRP_ERR_INVALID, "MRP_OPT_TR_SLOTS in [1, 4096]");
+            if (value < (int)c->tr_slots.size()) {  // shrink: free the tail sets
+                if (hipStreamSynchronize(c->stream) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "sync failed");
+                for (size_t i = value; i < c->tr_slots.size(); ++i) {
+                    hipFree(c->tr_slots[i].d_tr);
+                    hipFree(c->tr_slots[i].d_ptr);
+                }
+                c->tr_slots.resize(value);
+            }
+            c->tr_slot_cap = value;
+            return MRP_OK;
         default:
             return mrp_fail(MRP_ERR_INVALID, "unknown option %d", option);
     }
@@ -233,6 +385,47 @@ extern "C" int mrp_ctx_get_option(mrp_ctx* c, int option, int* value) {
     switch (option) {
         case MRP_OPT_MFMA_WAVES_PER_CU: *value = c->fused->mfma_waves_per_cu; return MRP_OK;
         case MRP_OPT_EXACT_GATHER: *value = c->exact_gather ? 1 : 0; return MRP_OK;
+        case MRP_OPT_TR_SLOTS: *value = c->tr_slot_cap; return MRP_OK;
+        case MRP_OPT_GRAPH: *value = c->fused->use_graph ? 1 : 0; return MRP_OK;
         default: return mrp_fail(MRP_ERR_INVALID, "unknown option %d", option);
     }
+}
+
+// Launch-shape autotuner: times the batch (already prepared) with each candidate number of
+// persistent waves per CU of the network kernel and keeps the fastest.
+extern "C" int mrp_autotune_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
+                                  int precision, float* const* dev_out4, int* best_waves, float* best_ms) {
+    if (!c) return mrp_fail(MRP_ERR_INVALID, "ctx is null");
+    static const int cand[] = {4, 8, 12, 16, 24, 32};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return mrp_fail(MRP_ERR_HIP, "event creation failed");
+    const int keep = c->fused->mfma_waves_per_cu;
+    int best = keep;
+    float best_t = 1e30f;
+    int rc = MRP_OK;
+    for (int w : cand) {
+        c->fused->mfma_waves_per_cu = w;
+        if ((rc = mrp_render_batch(c, vols, n_vols, views, n_views, precision, dev_out4))) break;  // warm-up
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = mrp_fail(MRP_ERR_HIP, "event record failed"); break; }
+        for (int r = 0; r < 3 && !rc; ++r) rc = mrp_render_batch(c, vols, n_vols, views, n_views, precision, dev_out4);
+        if (rc) break;
+        float ms = 0.0f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = mrp_fail(MRP_ERR_HIP, "event timing failed");
+            break;
+        }
+        if (ms / 3.0f < best_t) {
+            best_t = ms / 3.0f;
+            best = w;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->fused->mfma_waves_per_cu = rc ? keep : best;
+    if (rc) return rc;
+    if (best_waves) *best_waves = best;
+    if (best_ms) *best_ms = best_t;
+    return MRP_OK;
 }
