@@ -152,7 +152,7 @@ extern "C" int mrp_ctx_create(const char* weights_path, mrp_ctx** out) {
     HIP_TRY(hipMemset(c->d_lut, 0, 512 * 512 * sizeof(float)));
     for (int i = 0; i < 4; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
     for (int i = 0; i < TimingRing::kSlots; ++i)
-        for (int j = 0; j < 3; ++j) HIP_TRY(hipEventCreate(&c->ring.ev[i][j]));
+        for (int j = 0; j < 4; ++j) HIP_TRY(hipEventCreate(&c->ring.ev[i][j]));
     int prc = mrp_fused_init(c);
     if (prc) return prc;
     *out = c;
@@ -174,7 +174,7 @@ extern "C" int mrp_ctx_destroy(mrp_ctx* c) {
     }
     for (int i = 0; i < 4; ++i) hipEventDestroy(c->ev[i]);
     for (int i = 0; i < TimingRing::kSlots; ++i)
-        for (int j = 0; j < 3; ++j) hipEventDestroy(c->ring.ev[i][j]);
+        for (int j = 0; j < 4; ++j) hipEventDestroy(c->ring.ev[i][j]);
     delete c;
     return MRP_OK;
 }
@@ -190,7 +190,7 @@ static int timing_read(mrp_ctx* c, int slot) {
     HIP_TRY(hipEventSynchronize(r.ev[slot][2]));
     float a = 0.0f, b = 0.0f;
     HIP_TRY(hipEventElapsedTime(&a, r.ev[slot][0], r.ev[slot][1]));
-    HIP_TRY(hipEventElapsedTime(&b, r.ev[slot][1], r.ev[slot][2]));
+    HIP_TRY(hipEventElapsedTime(&b, r.ev[slot][3], r.ev[slot][2]));
     r.pending[slot] = false;
     r.count += 1;
     r.sum_march += a;
@@ -339,6 +339,7 @@ extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
 
 extern "C" int mrp_volume_destroy(mrp_volume* v) {
     if (!v) return MRP_OK;
+    mrp_fused_join_aux(v->ctx);
     hipStreamSynchronize(v->ctx->stream);
     mrp_tr_slots_drop(v->ctx, v);
     hipFree(v->d_grid);
@@ -410,6 +411,8 @@ extern "C" const float* mrp_volume_grid_device(mrp_volume* v) { return v ? v->d_
 extern "C" int mrp_volume_update(mrp_volume* v) {
     CHECK_ARG(v, "volume is null");
     hipStream_t s = v->ctx->stream;
+    int jrc = mrp_fused_join_aux(v->ctx);  // TR bakes in flight read the old pyramid
+    if (jrc) return jrc;
     int res = 256;
     while (res * 2 < v->N) res <<= 1;
     const float* src = v->d_grid;
@@ -451,7 +454,7 @@ extern "C" int mrp_volume_update(mrp_volume* v) {
 
 // Update_TR (core/volume.cu:1589-1667), cached on exact (light, alpha)
 // Fill_TR for all 8 levels into the given buffers (levels packed as in mrp_volume::d_tr).
-int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, float* d_ptr) {
+int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, float* d_ptr, hipStream_t s) {
     if (!v->mips_valid) return mrp_fail(MRP_ERR_STATE, "mrp_volume_update must run before the TR bake");
     float alpha_lvl[8];
     for (int m = 0; m < 8; ++m) alpha_lvl[m] = alpha / powf(1.73f, (float)m + 1.0f);
@@ -463,7 +466,7 @@ int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, 
         tr[m] = d_tr + o;
         o += (size_t)(128 >> m) * (128 >> m) * (128 >> m);
     }
-    HIP_TRY(launch_tr_bake(mips, v->d_pmips, tr, d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), v->ctx->stream));
+    HIP_TRY(launch_tr_bake(mips, v->d_pmips, tr, d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), s));
     return MRP_OK;
 }
 
@@ -472,7 +475,7 @@ int mrp_volume_update_tr_impl(mrp_volume* v, const float L[3], float alpha) {
     if (v->tr_valid && v->tr_light[0] == L[0] && v->tr_light[1] == L[1] && v->tr_light[2] == L[2] &&
         v->tr_alpha == alpha)
         return MRP_OK;
-    int rc = mrp_tr_bake_into(v, L, alpha, v->d_tr, v->d_ptr);
+    int rc = mrp_tr_bake_into(v, L, alpha, v->d_tr, v->d_ptr, v->ctx->stream);
     if (rc) return rc;
     v->tr_valid = true;
     memcpy(v->tr_light, L, sizeof v->tr_light);
@@ -573,6 +576,7 @@ extern "C" int mrp_render_plain_device(mrp_ctx* c, mrp_volume* v, const mrp_view
         if (rc) return rc;
         HIP_TRY(hipEventRecord(ev[0], c->stream));
         HIP_TRY(hipEventRecord(ev[1], c->stream));
+        HIP_TRY(hipEventRecord(ev[3], c->stream));
     }
     HIP_TRY(launch_predict_plain(dv, c->d_stencil, c->d_weights, d, dev_out4, 1.0f, 1, c->stream));
     if (c->timing) HIP_TRY(hipEventRecord(ev[2], c->stream));

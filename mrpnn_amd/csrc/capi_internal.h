@@ -18,14 +18,14 @@ constexpr size_t kTrFloats = 2396745;
 // reused or when the statistics are requested.
 struct TimingRing {
     static constexpr int kSlots = 32;
-    hipEvent_t ev[kSlots][3]{};
+    hipEvent_t ev[kSlots][4]{};  // start, march done, network start, end
     bool pending[kSlots]{};
     int head = 0;
     int64_t count = 0;
     double sum_march = 0.0, sum_predict = 0.0;
     float last_march = 0.0f, last_predict = 0.0f;
 };
-int mrp_timing_begin(struct mrp_ctx* c, hipEvent_t** ev);  // ev = the three events of a fresh slot
+int mrp_timing_begin(struct mrp_ctx* c, hipEvent_t** ev);  // ev = the four events of a fresh slot
 int mrp_timing_collect(struct mrp_ctx* c);                  // reads back every pending slot
 
 // One transmittance-mip set of the batched path, keyed by (volume, light direction, alpha):
@@ -90,7 +90,7 @@ mrp::ViewDev mrp_to_dev(const mrp_view* v);
 mrp::DevVolume mrp_dev_volume(const mrp_volume* v);
 int mrp_prepare_view(mrp_ctx* c, mrp_volume* v, const mrp_view* vw);
 int mrp_volume_update_tr_impl(mrp_volume* v, const float L[3], float alpha);
-int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, float* d_ptr);
+int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, float* d_ptr, hipStream_t s);
 void mrp_tr_slots_drop(mrp_ctx* c, const mrp_volume* v);  // invalidate a volume's cached sets
 int mrp_tr_slot_find(const mrp_ctx* c, const mrp_volume* v, const float L[3], float alpha);
 
@@ -98,5 +98,6 @@ int mrp_tr_slot_find(const mrp_ctx* c, const mrp_volume* v, const float L[3], fl
 int mrp_fused_init(mrp_ctx* c);
 void mrp_fused_free(mrp_ctx* c);
 int mrp_fused_collect_stats(mrp_ctx* c);
+int mrp_fused_join_aux(mrp_ctx* c);
 int mrp_fused_predict_features(mrp_ctx* c, const float* dev_feats, int n, const float sr[3], float g, int precision,
                                float* dev_rgb);

@@ -35,6 +35,11 @@ struct FusedState {
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
+    // TR sets of the batched path are baked on a second stream: the march of a batch (which
+    // does not read TR mips) overlaps the bakes of its suns; the network waits for tr_done
+    hipStream_t aux = nullptr;
+    hipEvent_t pre = nullptr, tr_done = nullptr;
+    bool tr_recorded = false;
 };
 
 static void clear_graphs(FusedState* f) {
@@ -54,8 +59,12 @@ int mrp_fused_init(mrp_ctx* c) {
     c->fused = f;
     if (hipMalloc(&f->d_counter, 16) != hipSuccess || hipMalloc(&f->d_taps, 16) != hipSuccess)
         return mrp_fail(MRP_ERR_NOMEM, "counter allocation failed");
-    if (hipEventCreateWithFlags(&f->staged, hipEventDisableTiming) != hipSuccess)
+    if (hipEventCreateWithFlags(&f->staged, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&f->pre, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&f->tr_done, hipEventDisableTiming) != hipSuccess)
         return mrp_fail(MRP_ERR_HIP, "event creation failed");
+    if (hipStreamCreateWithFlags(&f->aux, hipStreamNonBlocking) != hipSuccess)
+        return mrp_fail(MRP_ERR_HIP, "stream creation failed");
     int rc = mfma_net_create(c->weights_host.data(), &f->net);
     if (rc) return rc;
     return MRP_OK;
@@ -72,6 +81,12 @@ void mrp_fused_free(mrp_ctx* c) {
     hipHostFree(f->h_vols);
     hipHostFree(f->h_slots);
     if (f->staged) hipEventDestroy(f->staged);
+    if (f->pre) hipEventDestroy(f->pre);
+    if (f->tr_done) hipEventDestroy(f->tr_done);
+    if (f->aux) {
+        hipStreamSynchronize(f->aux);
+        hipStreamDestroy(f->aux);
+    }
     clear_graphs(f);
     if (f->cap_stream) hipStreamDestroy(f->cap_stream);
     mfma_net_destroy(&f->net);
@@ -94,6 +109,15 @@ int mrp_fused_collect_stats(mrp_ctx* c) {
         c->stats.march_taps = (int64_t)taps;
         f->have_counts = false;
     }
+    return MRP_OK;
+}
+
+// Orders the context's stream after the TR bakes issued so far on the second stream (before
+// a volume's pyramid is rewritten or freed).
+int mrp_fused_join_aux(mrp_ctx* c) {
+    FusedState* f = c->fused;
+    if (f && f->tr_recorded && hipStreamWaitEvent(c->stream, f->tr_done, 0) != hipSuccess)
+        return mrp_fail(MRP_ERR_HIP, "stream ordering failed");
     return MRP_OK;
 }
 
@@ -130,7 +154,7 @@ static int tr_slot_claim(mrp_ctx* c) {
             return -1;
         }
         // the padded copy's border must read as zero; the bake writes the interior only
-        if (hipMemsetAsync(t.d_ptr, 0, kPadTrFloats * sizeof(float), c->stream) != hipSuccess) return -1;
+        if (hipMemsetAsync(t.d_ptr, 0, kPadTrFloats * sizeof(float), c->fused->aux) != hipSuccess) return -1;
         c->tr_slots.push_back(t);
         return (int)c->tr_slots.size() - 1;
     }
@@ -153,6 +177,11 @@ extern "C" int mrp_prepare_views(mrp_ctx* c, mrp_volume* const* vols, int n_vols
     int rc = mrp_ctx_update_hg_lut(c, views[0].g, c->lut_valid ? c->lut_seed : 1u);
     if (rc) return rc;
     ++c->tr_stamp;
+    // bakes on the second stream start after all work enqueued so far on the context's stream
+    // (pyramids they read; earlier batches still reading a set that gets re-baked)
+    FusedState* f = c->fused;
+    if (hipEventRecord(f->pre, c->stream) != hipSuccess || hipStreamWaitEvent(f->aux, f->pre, 0) != hipSuccess)
+        return mrp_fail(MRP_ERR_HIP, "stream ordering failed");
     for (int i = 0; i < n_views; ++i) {
         mrp_volume* v = vols[views[i].volume_index];
         const float* L = views[i].light_dir;
@@ -164,7 +193,7 @@ extern "C" int mrp_prepare_views(mrp_ctx* c, mrp_volume* const* vols, int n_vols
                                 c->tr_slot_cap);
             TrSlot& t = c->tr_slots[s];
             t.valid = false;
-            rc = mrp_tr_bake_into(v, L, views[i].alpha, t.d_tr, t.d_ptr);
+            rc = mrp_tr_bake_into(v, L, views[i].alpha, t.d_tr, t.d_ptr, f->aux);
             if (rc) return rc;
             t.vol = v;
             memcpy(t.light, L, sizeof t.light);
@@ -173,6 +202,8 @@ extern "C" int mrp_prepare_views(mrp_ctx* c, mrp_volume* const* vols, int n_vols
         }
         c->tr_slots[s].stamp = c->tr_stamp;
     }
+    if (hipEventRecord(f->tr_done, f->aux) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "event record failed");
+    f->tr_recorded = true;
     return MRP_OK;
 }
 
@@ -272,7 +303,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
     const int grid = c->props.multiProcessorCount * f->mfma_waves_per_cu / 4;
     const bool fast = !c->exact_gather;
     // the work of one batch on stream q; ev (optional) brackets march and network
-    auto enqueue = [&](hipStream_t q, hipEvent_t* ev, hipEvent_t staged) -> hipError_t {
+    auto enqueue = [&](hipStream_t q, hipEvent_t* ev, hipEvent_t staged, bool wait_tr) -> hipError_t {
         hipError_t e;
         if ((e = hipMemcpyAsync(f->d_vols, f->h_vols, n_entries * sizeof(DevVolume), hipMemcpyHostToDevice, q)))
             return e;
@@ -284,6 +315,8 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         if ((e = launch_march_compact(f->d_vols, f->d_slots, n_views, n_tiles, f->d_recs, f->d_counter, f->d_taps, q)))
             return e;
         if (ev && (e = hipEventRecord(ev[1], q))) return e;
+        if (wait_tr && (e = hipStreamWaitEvent(q, f->tr_done, 0))) return e;
+        if (ev && (e = hipEventRecord(ev[3], q))) return e;
         if (precision == MRP_FP32)
             e = launch_predict_active_fp32(f->d_vols, f->d_slots, c->d_stencil, c->d_weights, f->d_recs, f->d_counter,
                                            (int)need, q);
@@ -300,7 +333,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         if (rc) return rc;
     }
     if (!f->use_graph) {
-        HT(enqueue(st, ev, f->staged));
+        HT(enqueue(st, ev, f->staged, f->tr_recorded));
     } else {
         FusedState::Graph* g = nullptr;
         for (auto& x : f->graphs)
@@ -312,7 +345,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
             if (f->graphs.size() >= 16) clear_graphs(f);
             hipGraph_t graph = nullptr;
             HT(hipStreamBeginCapture(f->cap_stream, hipStreamCaptureModeThreadLocal));
-            hipError_t ce = enqueue(f->cap_stream, nullptr, nullptr);
+            hipError_t ce = enqueue(f->cap_stream, nullptr, nullptr, false);
             hipError_t ee = hipStreamEndCapture(f->cap_stream, &graph);
             HT(ce);
             HT(ee);
@@ -324,9 +357,11 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
             g = &f->graphs.back();
         }
         // graph mode times the whole replay as the network time (march time reads 0)
+        if (f->tr_recorded) HT(hipStreamWaitEvent(st, f->tr_done, 0));
         if (ev) {
             HT(hipEventRecord(ev[0], st));
             HT(hipEventRecord(ev[1], st));
+            HT(hipEventRecord(ev[3], st));
         }
         HT(hipGraphLaunch(g->exec, st));
         if (ev) HT(hipEventRecord(ev[2], st));
