@@ -34,7 +34,7 @@ extern "C" {
 
 /* MLP arithmetic of the fused predictor. */
 #define MRP_FP32 0    /* scalar fp32, the reference's arithmetic */
-#define MRP_BF16 1    /* bf16 MFMA, fp32 accumulation; weights as two bf16 terms (hi + lo) */
+#define MRP_BF16 1    /* bf16 MFMA, fp32 accumulation; tail-layer weights as two bf16 terms (hi + lo) */
 #define MRP_FP8 2     /* fp8 e4m3 MFMA, per-layer max-abs weight scales, per-pixel activation
                          scales, fp32 accumulation */
 #define MRP_BF16_1T 3 /* bf16 MFMA with single-term bf16 weights (fastest, least accurate) */
@@ -108,6 +108,7 @@ const char* mrp_version(void);
 /* Context: device weights, HG LUT, stream, scratch.  weights_path NULL = the asset
  * mrpnn_weights.bin installed in assets/ next to the library. */
 int mrp_ctx_create(const char* weights_path, mrp_ctx** out);
+/* Volumes and servers of a context must be destroyed before the context. */
 int mrp_ctx_destroy(mrp_ctx* ctx);
 int mrp_ctx_set_stream(mrp_ctx* ctx, void* hip_stream);
 int mrp_ctx_set_timing(mrp_ctx* ctx, int enabled);
@@ -197,6 +198,76 @@ int mrp_predict_features(mrp_ctx* ctx, const float* dev_feats, int n, const floa
  * the running (n = spp0 on entry) accumulators. */
 int mrp_pt_render(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int spp0, int spp, int max_bounce,
                   int nee_baked, float tr_scale, double* dev_mean3, double* dev_m2_3);
+
+/* ---------------------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, which renders one blocking view per
+ * Render call, core/volume.cu:487-548).  Volumes are registered with the server and made
+ * device-resident on demand when their grid + pyramids + padded copies (mrp_volume_bytes)
+ * fit in free HBM; idle resident volumes are evicted least recently used first to make
+ * room.  Requests are batched as they arrive: while up to max_inflight batches run on the
+ * GPU, the next one is formed from the requests that arrived meanwhile.  Each request's
+ * pixels are computed exactly as when it is rendered alone.
+ * --------------------------------------------------------------------------------------- */
+typedef struct mrp_server mrp_server;
+typedef struct mrp_server_config {
+    int32_t precision;        /* MRP_BF16 / MRP_FP8 / MRP_BF16_1T / MRP_FP32 (batched path) */
+    int32_t plain;            /* 1: one plain per-frame FP32 launch per request (the baseline) */
+    int32_t max_batch_views;  /* requests per batch */
+    int64_t max_batch_pixels; /* a batch closes once it holds this many pixels */
+    int32_t max_inflight;     /* batches on the GPU at once */
+    float g;                  /* Henyey-Greenstein g of every request (one LUT) */
+    int64_t hbm_limit_bytes;  /* > 0: admit against this budget instead of hipMemGetInfo */
+    int64_t hbm_reserve_bytes;/* kept free beside every admitted volume */
+} mrp_server_config;
+typedef struct mrp_request {
+    int32_t volume_id;        /* from mrp_server_add_volume / mrp_server_add_fbm_volume */
+    mrp_view view;            /* volume_index is ignored */
+    float* dev_out4;          /* width*height float4, caller-owned, valid until completion */
+    uint64_t user_tag;
+} mrp_request;
+typedef struct mrp_completion {
+    uint64_t user_tag;
+    double t_submit, t_launch, t_done; /* host steady-clock seconds */
+    int32_t status;
+} mrp_completion;
+typedef struct mrp_server_stats {
+    int64_t submitted, completed, pixels, batches, batched_views, admissions, evictions;
+    int64_t resident_volumes, resident_bytes, pending;
+} mrp_server_stats;
+typedef struct mrp_load_spec {
+    double rate_rps;          /* Poisson arrival rate */
+    double duration_s;        /* arrivals over [0, duration_s) */
+    uint64_t seed;
+    int32_t n_sizes;          /* 1..8 resolutions, drawn uniformly */
+    int32_t sizes[16];        /* (width, height) pairs */
+    int32_t buffers_per_size; /* output buffers per resolution (default 16) */
+} mrp_load_spec;
+typedef struct mrp_load_report {
+    int64_t requests, completed, pixels;
+    double wall_s, offered_s, mpix_s;
+    double p50_ms, p99_ms, max_ms; /* arrival to completion */
+} mrp_load_report;
+
+int mrp_server_create(mrp_ctx* ctx, const mrp_server_config* cfg, mrp_server** out);
+int mrp_server_destroy(mrp_server* srv);
+/* Registers a volume (the server keeps a host copy for re-admission after eviction). */
+int mrp_server_add_volume(mrp_server* srv, int N, const float* host_grid, float max_density, int* volume_id);
+/* Registers a seeded fBm volume (regenerated on the device at admission). */
+int mrp_server_add_fbm_volume(mrp_server* srv, int N, int octaves, float base_freq, uint32_t seed, int* volume_id);
+/* Queues a request (thread-safe with respect to mrp_server_step). */
+int mrp_server_submit(mrp_server* srv, const mrp_request* req);
+/* One scheduling round: reap finished batches, admit volumes, launch batches. Non-blocking. */
+int mrp_server_step(mrp_server* srv);
+/* Steps until every submitted request has completed. */
+int mrp_server_drain(mrp_server* srv);
+/* Moves up to max finished requests into out. */
+int mrp_server_poll(mrp_server* srv, mrp_completion* out, int max, int* n);
+int mrp_server_get_stats(mrp_server* srv, mrp_server_stats* out);
+int mrp_server_is_resident(mrp_server* srv, int volume_id);
+/* Open-loop load over the registered volumes: Poisson arrivals, random orbit (radius 2,
+ * 40 degree field of view), sun on the upper hemisphere, alpha in [16, 128] and resolution
+ * from spec->sizes; latency is measured from arrival to completion.  Blocking. */
+int mrp_server_run_poisson(mrp_server* srv, const mrp_load_spec* spec, mrp_load_report* report);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,39 @@ class Stats(ctypes.Structure):
                 ("ms_predict_sum", ctypes.c_double)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("plain", ctypes.c_int32), ("max_batch_views", ctypes.c_int32),
+                ("max_batch_pixels", ctypes.c_int64), ("max_inflight", ctypes.c_int32), ("g", ctypes.c_float),
+                ("hbm_limit_bytes", ctypes.c_int64), ("hbm_reserve_bytes", ctypes.c_int64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("volume_id", ctypes.c_int32), ("view", View), ("dev_out4", ctypes.c_void_p),
+                ("user_tag", ctypes.c_uint64)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("user_tag", ctypes.c_uint64), ("t_submit", ctypes.c_double), ("t_launch", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("status", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "pixels", "batches", "batched_views",
+                                              "admissions", "evictions", "resident_volumes", "resident_bytes",
+                                              "pending")]
+
+
+class LoadSpec(ctypes.Structure):
+    _fields_ = [("rate_rps", ctypes.c_double), ("duration_s", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("n_sizes", ctypes.c_int32), ("sizes", ctypes.c_int32 * 16), ("buffers_per_size", ctypes.c_int32)]
+
+
+class LoadReport(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_int64), ("completed", ctypes.c_int64), ("pixels", ctypes.c_int64),
+                ("wall_s", ctypes.c_double), ("offered_s", ctypes.c_double), ("mpix_s", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double)]
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -70,6 +103,18 @@ def lib():
         L.mrp_ctx_download_hg_lut.argtypes = [_vp, _fp]
         L.mrp_ctx_upload_hg_lut.argtypes = [_vp, _fp, ctypes.c_float]
         L.mrp_ctx_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.mrp_server_create.argtypes = [_vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
+        L.mrp_server_destroy.argtypes = [_vp]
+        L.mrp_server_add_volume.argtypes = [_vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]
+        L.mrp_server_add_fbm_volume.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32,
+                                                ctypes.POINTER(ctypes.c_int)]
+        L.mrp_server_submit.argtypes = [_vp, ctypes.POINTER(Request)]
+        L.mrp_server_step.argtypes = [_vp]
+        L.mrp_server_drain.argtypes = [_vp]
+        L.mrp_server_poll.argtypes = [_vp, ctypes.POINTER(Completion), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.mrp_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+        L.mrp_server_is_resident.argtypes = [_vp, ctypes.c_int]
+        L.mrp_server_run_poisson.argtypes = [_vp, ctypes.POINTER(LoadSpec), ctypes.POINTER(LoadReport)]
         L.mrp_autotune_batch.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View), ctypes.c_int,
                                          ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_float)]
@@ -109,11 +154,18 @@ def _ptr(t):
 
 class Context:
     def __init__(self, weights_path=None):
+        import weakref
         self.h = _vp()
+        self._children = weakref.WeakSet()  # volumes and servers: closed before the context
         check(lib().mrp_ctx_create(weights_path.encode() if weights_path else None, ctypes.byref(self.h)))
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                if isinstance(ch, Server):
+                    ch.close()
+            for ch in list(self._children):
+                ch.close()
             lib().mrp_ctx_destroy(self.h)
             self.h = _vp()
 
@@ -198,6 +250,7 @@ class Volume:
         self.N = N
         self.h = _vp()
         check(lib().mrp_volume_create(ctx.h, int(N), ctypes.byref(self.h)))
+        ctx._children.add(self)
 
     def close(self):
         if self.h:
@@ -299,3 +352,75 @@ def orbit_view(width, height, dist=2.0, azimuth=0.0, elevation=0.0, fov_deg=40.0
     v.width, v.height = width, height
     v.seed, v.frame, v.view_id, v.volume_index = seed, frame, view_id, volume_index
     return v
+
+
+class Server:
+    """Serving loop over resident volumes (mrp_server_* in include/mrp_c_api.h)."""
+
+    def __init__(self, ctx, precision=BF16, plain=False, max_batch_views=64, max_batch_pixels=16 << 20,
+                 max_inflight=2, g=0.857, hbm_limit_bytes=0, hbm_reserve_bytes=0):
+        self.ctx = ctx
+        cfg = ServerConfig(int(precision), 1 if plain else 0, int(max_batch_views), int(max_batch_pixels),
+                           int(max_inflight), float(g), int(hbm_limit_bytes), int(hbm_reserve_bytes))
+        self.h = _vp()
+        check(lib().mrp_server_create(ctx.h, ctypes.byref(cfg), ctypes.byref(self.h)))
+        ctx._children.add(self)
+
+    def close(self):
+        if self.h:
+            lib().mrp_server_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_volume(self, grid, N, max_density=1.0):
+        grid = np.ascontiguousarray(grid, np.float32)
+        vid = ctypes.c_int()
+        check(lib().mrp_server_add_volume(self.h, int(N), grid.ctypes.data_as(_fp), float(max_density),
+                                          ctypes.byref(vid)))
+        return vid.value
+
+    def add_fbm_volume(self, N, octaves=5, base_freq=4.0, seed=1):
+        vid = ctypes.c_int()
+        check(lib().mrp_server_add_fbm_volume(self.h, int(N), int(octaves), float(base_freq), int(seed),
+                                              ctypes.byref(vid)))
+        return vid.value
+
+    def submit(self, volume_id, view, out, tag=0):
+        r = Request(int(volume_id), view, out.data_ptr(), int(tag))
+        check(lib().mrp_server_submit(self.h, ctypes.byref(r)))
+
+    def step(self):
+        check(lib().mrp_server_step(self.h))
+
+    def drain(self):
+        check(lib().mrp_server_drain(self.h))
+
+    def poll(self, max_n=1024):
+        buf = (Completion * max_n)()
+        n = ctypes.c_int()
+        check(lib().mrp_server_poll(self.h, buf, max_n, ctypes.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().mrp_server_get_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def is_resident(self, volume_id):
+        return bool(lib().mrp_server_is_resident(self.h, int(volume_id)))
+
+    def run_poisson(self, rate_rps, duration_s, sizes, seed=1, buffers_per_size=16):
+        spec = LoadSpec()
+        spec.rate_rps, spec.duration_s, spec.seed = float(rate_rps), float(duration_s), int(seed)
+        spec.n_sizes = len(sizes)
+        for i, (w, h) in enumerate(sizes):
+            spec.sizes[2 * i], spec.sizes[2 * i + 1] = int(w), int(h)
+        spec.buffers_per_size = int(buffers_per_size)
+        rep = LoadReport()
+        check(lib().mrp_server_run_poisson(self.h, ctypes.byref(spec), ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in LoadReport._fields_}

@@ -24,7 +24,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contr
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 
 SOURCES = ["capi.hip", "volume_kernels.hip", "predict_plain.hip", "fused.hip", "fused_host.hip", "mlp_mfma.hip",
-           "pathtrace.hip", "serve.hip", "cxx_api.cpp"]
+           "pathtrace.hip", "serve.cpp", "cxx_api.cpp"]
 
 
 def _compile(src):

@@ -252,7 +252,9 @@ template <int PREC, int ID, bool SB = true>
 __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8], float (&out)[mfma_nm(ID)][4],
                              int lane, bool relu) {
     constexpr int NM = mfma_nm(ID), NK = mfma_nk(ID), FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
-    constexpr bool TWO = PREC == MRP_BF16;  // second weight term
+    // second weight term: MRP_BF16 applies it to the tail (LSEFin2, LC0..LX12), where the
+    // single-term rounding error concentrates; the trunk's 8..32-wide layers use one term
+    constexpr bool TWO = PREC == MRP_BF16 && ID >= ML_FIN2;
     const int q = lane >> 4;
     // Fragments of M-tile mt+1 are requested before the MFMAs of M-tile mt (two buffers), so
     // the weight latency of a multi-tile layer is paid once.
@@ -279,7 +281,9 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
         for (int mt = 0; mt < NM; ++mt) {
             const int cur = mt & 1;
             if (mt + 1 < NM) fetch(mt + 1, cur ^ 1);
-            f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            // bias first, then the products (the reference's LinearReLU_SFL order)
+            f32x4_t acc = f32x4_t{__uint_as_float(bb[cur][0]), __uint_as_float(bb[cur][1]),
+                                  __uint_as_float(bb[cur][2]), __uint_as_float(bb[cur][3])};
 #pragma unroll
             for (int kc = 0; kc < NK; ++kc) {
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, wa[cur][kc]),
@@ -289,10 +293,7 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
                                                                   __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float v = acc[i] + __uint_as_float(bb[cur][i]);
-                out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
-            }
+            for (int i = 0; i < 4; ++i) out[mt][i] = relu ? fmaxf(acc[i], 0.0f) : acc[i];
             if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
         }
     } else {
