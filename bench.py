@@ -58,6 +58,13 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=4)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device-to-device (scale reference for HBM counters)")
+    ap.add_argument("--config", default="headline", choices=["headline", "c1", "c2", "c3", "c4", "c5"],
+                    help="headline = the metric line; c1..c5 = the other workloads (one JSON line each)")
+    ap.add_argument("--pt-res", type=int, default=1024, help="c3: image side")
+    ap.add_argument("--pt-spp", type=int, default=1024, help="c3: samples per pixel")
+    ap.add_argument("--pt-bounces", type=int, default=512, help="c3: maximum bounces")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="c5: arrival window")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="c5: offered requests per second")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -71,6 +78,10 @@ def main():
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda")
+    if args.config != "headline":
+        import bench_configs
+        print(json.dumps(getattr(bench_configs, args.config)(args)))
+        return
     if args.copy_probe:
         src = torch.ones(256 * 1024 * 1024, device=dev)
         dst = torch.empty_like(src)

@@ -1,0 +1,298 @@
+"""The workloads beside the headline metric, one JSON line each (`python bench.py --config cN`).
+
+c1  plain FP32 and BF16 predictor at 256x256 over a 128^3 cloud, the oracle's CPU predictor
+    on the host cores, and the delta-tracking judge (256 spp, <= 64 bounces);
+c2  precision sweep at 1920x1080 over 512^3 clouds (seeds 1..4, 16 suns): FP32 / BF16 / FP8
+    against the FP64 network on identical features (sampled pixels) and against FP32;
+c3  delta-tracking judge at 1024^2, 1024 spp, <= 512 bounces, albedo 0.999: samples/s and
+    the fraction of predictor pixels inside the 3-sigma band;
+c4  64 views over 8 resident volumes (two each of 128^3, 256^3, 384^3, 512^3), resolutions
+    from {256^2, 512^2, 1280x720, 1920x1080}, one graph-replayed BF16 batch per step with
+    the autotuned launch shape;
+c5  serving loop: Poisson arrivals over 32 resident volumes, BF16 / FP8 batched against
+    the plain per-frame FP32 launch.
+
+All volumes are the seeded fBm clouds of the library (mrp_volume_fill_fbm); the oracle is the
+CPU restatement under oracle/ and is used here only as the timed CPU baseline and as the FP64
+reference network.
+"""
+import math
+import os
+import time
+
+import numpy as np
+
+G = 0.857
+
+
+def _log(*a):
+    import sys
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def _sun(elev_deg, az_deg):
+    e, a = math.radians(elev_deg), math.radians(az_deg)
+    return (math.cos(e) * math.cos(a), math.sin(e), math.cos(e) * math.sin(a))
+
+
+def _band(pred, mean, m2, n):
+    """Fraction of pixels (channel 0) with |pred - mean| <= 3 * standard error."""
+    se = np.sqrt(np.maximum(m2[:, 0], 0.0) / (n * (n - 1)))
+    return float(np.mean(np.abs(pred[:, 0] - mean[:, 0]) <= 3.0 * se))
+
+
+def _rel(a, b, floor_frac=1e-3):
+    """Mean and p99 relative error of a against b over pixels brighter than floor_frac * max."""
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    m = b > floor_frac * b.max() if b.size and b.max() > 0 else np.zeros_like(b, bool)
+    if not m.any():
+        return dict(mean=None, p99=None)
+    r = np.abs(a[m] - b[m]) / b[m]
+    return dict(mean=round(float(r.mean()), 6), p99=round(float(np.quantile(r, 0.99)), 5))
+
+
+def c1(args):
+    import torch
+
+    import mrpnn_amd as M
+    import oracle as O
+    ctx = M.Context()
+    N = 128
+    vol = M.Volume(ctx, N)
+    vol.fill_fbm(5, 4.0, 1)
+    vol.update()
+    v = M.orbit_view(256, 256, dist=2.0, azimuth=0.0, elevation=0.0, fov_deg=40.0, light=_sun(30, 0), alpha=64.0,
+                     g=G, albedo=0.99, seed=1, jitter=0.5)
+    out = torch.zeros((256 * 256, 4), device="cuda")
+    res = {}
+    for name, fn in (("fp32_plain", lambda: vol.render_plain_device(v, out)),
+                     ("bf16", lambda: (ctx.prepare_views([vol], [v]), ctx.render_batch([vol], [v], [out], M.BF16)))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 20
+        fn()
+        torch.cuda.synchronize()
+        res[name] = dict(img=out.cpu().numpy().copy(), mpix=256 * 256 / dt / 1e6)
+    # CPU baseline: the oracle's NNPredict<MRPNN> restatement on all host cores, full frame
+    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    vol.update_tr(np.asarray(v.light_dir[:], np.float32), v.alpha)
+    torch.cuda.synchronize()
+    sc = O.Scene.from_arrays(vol.get_data(), N, 1.0, vol.download_mips(), vol.download_tr(), ctx.download_hg_lut())
+    ov = O.View.from_buffer_copy(bytes(v))
+    t0 = time.perf_counter()
+    cpu = sc.render_predict(ov)["out"]
+    cpu_s = time.perf_counter() - t0
+    # judge: delta tracking, 256 spp, <= 64 bounces
+    t0 = time.perf_counter()
+    mean, m2 = vol.pt_render(v, 256, 64, chunk=64)
+    torch.cuda.synchronize()
+    pt_s = time.perf_counter() - t0
+    mean, m2 = mean.cpu().numpy(), m2.cpu().numpy()
+    ctx.close()
+    return {
+        "config": "c1", "workload": "256x256 view of a 128^3 fBm cloud (5 octaves, seed 1), sun 30 deg, alpha 64",
+        "fp32_plain_mpix_s": round(res["fp32_plain"]["mpix"], 2), "bf16_mpix_s": round(res["bf16"]["mpix"], 2),
+        "cpu_predictor": {"mpix_s": round(256 * 256 / cpu_s / 1e6, 5), "cores": threads, "kind": "port",
+                          "seconds": round(cpu_s, 3)},
+        "gpu_fp32_vs_cpu_oracle": _rel(res["fp32_plain"]["img"][:, 0], cpu[:, 0]),
+        "bf16_vs_fp32": _rel(res["bf16"]["img"][:, 0], res["fp32_plain"]["img"][:, 0]),
+        "pt": {"spp": 256, "max_bounce": 64, "seconds": round(pt_s, 3),
+               "fp32_in_3sigma": _band(res["fp32_plain"]["img"], mean, m2, 256),
+               "bf16_in_3sigma": _band(res["bf16"]["img"], mean, m2, 256),
+               "fp32_vs_pt": _rel(res["fp32_plain"]["img"][:, 0], mean[:, 0])},
+    }
+
+
+def c2(args):
+    import torch
+
+    import bench
+    import mrpnn_amd as M
+    import oracle as O
+    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    ctx = M.Context()
+    suns = bench.sun_dirs(16)
+    W, H = 1920, 1080
+    rng = np.random.default_rng(0)
+    acc = {p: {"vs_f64": [], "vs_f32": []} for p in ("fp32", "bf16", "fp8")}
+    frames = 0
+    t_start = time.perf_counter()
+    for seed in (1, 2, 3, 4):
+        vol = M.Volume(ctx, 512)
+        vol.fill_fbm(6, 4.0, seed)
+        vol.update()
+        for si, L in enumerate(suns):
+            v = M.orbit_view(W, H, dist=2.0, azimuth=0.6, elevation=0.25, fov_deg=40.0, light=L, alpha=64.0, g=G,
+                             albedo=0.99, seed=seed, jitter=0.5)
+            ctx.prepare_views([vol], [v])
+            feats, active, _, _ = vol.gather_features(v)
+            act = active > 0
+            f = feats[act].contiguous()
+            del feats
+            sr = np.array([0.99 / 1.001] * 3, np.float32)
+            outs = {p: ctx.predict_features(f, sr, G, M.PRECISIONS[p]).cpu().numpy() for p in ("fp32", "bf16", "fp8")}
+            idx = rng.choice(f.shape[0], size=min(2048, f.shape[0]), replace=False)
+            ref64 = O.predict_f64(f[torch.as_tensor(idx, device="cuda")].cpu().numpy().astype(np.float64),
+                                  sr.astype(np.float64), G)
+            for p in ("fp32", "bf16", "fp8"):
+                acc[p]["vs_f64"].append((outs[p][idx, 0], ref64[:, 0]))
+                acc[p]["vs_f32"].append((outs[p][:, 0], outs["fp32"][:, 0]))
+            frames += 1
+            del f
+        _log("c2: seed %d done (%.0f s)" % (seed, time.perf_counter() - t_start))
+        vol.close()
+    ctx.close()
+    res = {"config": "c2", "workload": "1920x1080 views of 512^3 fBm clouds (6 octaves, seeds 1..4), 16 suns",
+           "frames": frames, "f64_sample_per_frame": 2048}
+    for p in ("fp32", "bf16", "fp8"):
+        a64 = np.concatenate([x for x, _ in acc[p]["vs_f64"]]), np.concatenate([y for _, y in acc[p]["vs_f64"]])
+        a32 = np.concatenate([x for x, _ in acc[p]["vs_f32"]]), np.concatenate([y for _, y in acc[p]["vs_f32"]])
+        res[p] = {"vs_fp64_network": _rel(*a64), "vs_fp32_predictor": _rel(*a32) if p != "fp32" else None}
+    return res
+
+
+def c3(args):
+    import torch
+
+    import mrpnn_amd as M
+    ctx = M.Context()
+    vol = M.Volume(ctx, 512)
+    vol.fill_fbm(6, 4.0, 1)
+    vol.update()
+    R, spp, nb = args.pt_res, args.pt_spp, args.pt_bounces
+    v = M.orbit_view(R, R, dist=2.0, azimuth=0.6, elevation=0.25, fov_deg=40.0, light=_sun(30, 0), alpha=64.0,
+                     g=G, albedo=0.999, seed=1, jitter=0.5)
+    out = torch.zeros((R * R, 4), device="cuda")
+    preds = {}
+    for p in ("fp32", "bf16"):
+        ctx.prepare_views([vol], [v])
+        ctx.render_batch([vol], [v], [out], M.PRECISIONS[p])
+        torch.cuda.synchronize()
+        preds[p] = out.cpu().numpy().copy()
+    n_img = torch.zeros((R * R, 3), dtype=torch.float64, device="cuda")
+    m2 = torch.zeros_like(n_img)
+    done = 0
+    t0 = time.perf_counter()
+    chunk = 16
+    while done < spp:
+        k = min(chunk, spp - done)
+        vol.pt_render(v, k, nb, chunk=k, mean=n_img, m2=m2, spp0=done)
+        torch.cuda.synchronize()
+        done += k
+        _log("c3: %d / %d spp (%.0f s)" % (done, spp, time.perf_counter() - t0))
+    dt = time.perf_counter() - t0
+    mean, m2 = n_img.cpu().numpy(), m2.cpu().numpy()
+    ctx.close()
+    return {"config": "c3", "workload": "%dx%d view of a 512^3 fBm cloud, albedo 0.999, alpha 64" % (R, R),
+            "spp": spp, "max_bounce": nb, "seconds": round(dt, 2),
+            "samples_per_s": round(R * R * spp / dt, 1),
+            "fp32_in_3sigma": _band(preds["fp32"], mean, m2, spp),
+            "bf16_in_3sigma": _band(preds["bf16"], mean, m2, spp),
+            "fp32_vs_pt": _rel(preds["fp32"][:, 0], mean[:, 0]),
+            "bf16_vs_pt": _rel(preds["bf16"][:, 0], mean[:, 0])}
+
+
+def c4(args):
+    import torch
+
+    import mrpnn_amd as M
+    ctx = M.Context()
+    ctx.set_timing(True)
+    rng = np.random.default_rng(4)
+    vols = []
+    for k, N in enumerate((128, 128, 256, 256, 384, 384, 512, 512)):
+        v = M.Volume(ctx, N)
+        v.fill_fbm(6, 4.0, 11 + k)
+        v.update()
+        vols.append(v)
+    sizes = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    views, outs = [], []
+    for i in range(64):
+        w, h = sizes[rng.integers(0, 4)]
+        sun = rng.normal(size=3)
+        sun[1] = abs(sun[1]) + 0.15
+        views.append(M.orbit_view(w, h, dist=2.0, azimuth=float(rng.uniform(0, 2 * np.pi)),
+                                  elevation=float(rng.uniform(-0.3, 0.6)), fov_deg=40.0, light=tuple(sun),
+                                  alpha=float(rng.uniform(16, 128)), g=G, albedo=0.99, seed=1, view_id=i,
+                                  volume_index=int(rng.integers(0, 8))))
+        outs.append(torch.zeros((w * h, 4), device="cuda"))
+    pixels = sum(v.width * v.height for v in views)
+    t0 = time.perf_counter()
+    ctx.prepare_views(vols, views)  # 64 TR sets (one per sun)
+    torch.cuda.synchronize()
+    bake_ms = (time.perf_counter() - t0) * 1e3
+    best_w, best_ms = ctx.autotune(vols, views, outs, M.BF16)
+    ctx.set_option(M.OPT_GRAPH, 1)
+    res = {}
+    for p in ("bf16", "fp8"):
+        for _ in range(2):
+            ctx.render_batch(vols, views, outs, M.PRECISIONS[p])
+        torch.cuda.synchronize()
+        steps = 10
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        ev[0].record()
+        for i in range(steps):
+            ctx.render_batch(vols, views, outs, M.PRECISIONS[p])
+            ev[i + 1].record()
+        torch.cuda.synchronize()
+        lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+        res[p] = {"ms_per_batch": round(float(np.mean(lat)), 3), "mpix_s": round(pixels / np.mean(lat) / 1e3, 1),
+                  "p99_batch_ms": round(float(np.percentile(lat, 99)), 3)}
+    ctx.set_option(M.OPT_GRAPH, 0)
+    # plain per-frame FP32 launches of the same 64 views, for scale
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, v in enumerate(views):
+        vols[v.volume_index].render_plain_device(v, outs[i])
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    for v in vols:
+        v.close()
+    ctx.close()
+    return {"config": "c4", "workload": "64 views, 8 volumes (2 x 128^3, 256^3, 384^3, 512^3), mixed resolutions",
+            "pixels_per_batch": pixels, "tr_bake_ms_64_suns": round(bake_ms, 1),
+            "autotune": {"waves_per_cu": best_w, "ms": round(best_ms, 3)}, "graph": True,
+            "bf16": res["bf16"], "fp8": res["fp8"],
+            "fp32_plain_mpix_s_incl_tr_rebake": round(pixels / plain_s / 1e6, 1)}
+
+
+def c5(args):
+    import mrpnn_amd as M
+    ctx = M.Context()
+    sizes = [(512, 512), (1280, 720), (1920, 1080)]
+    rng = np.random.default_rng(5)
+    res = {"config": "c5", "workload": "Poisson arrivals over 32 fBm volumes (128^3..512^3), "
+                                       "sizes {512^2, 1280x720, 1920x1080}, random orbit / sun / alpha"}
+    for name, prec, plain, rate, dur in (("bf16", M.BF16, False, args.serve_rate, args.serve_seconds),
+                                         ("fp8", M.FP8, False, args.serve_rate, args.serve_seconds),
+                                         ("bf16_at_300rps", M.BF16, False, 300.0, args.serve_seconds),
+                                         ("fp32_plain", M.FP32, True, args.serve_rate, min(0.25, args.serve_seconds))):
+        srv = M.Server(ctx, prec, plain=plain, max_batch_views=32, max_batch_pixels=8 << 20, max_inflight=2, g=G)
+        ctx.set_option(M.OPT_TR_SLOTS, 128)
+        ids = [srv.add_fbm_volume(int(rng.choice([128, 256, 384, 512])), 6, 4.0, 100 + k) for k in range(32)]
+        # admission (fBm generation + pyramid) happens before the timed load: one tiny request per volume
+        import torch
+        tiny = torch.zeros((1, 4), device="cuda")
+        for vid in ids:
+            srv.submit(vid, M.orbit_view(1, 1, g=G), tiny)
+        srv.drain()
+        srv.poll(64)
+        t0 = time.perf_counter()
+        rep = srv.run_poisson(rate, dur, sizes, seed=7, buffers_per_size=48)
+        st = srv.stats()
+        _log("c5 %s: %s (%.1f s)" % (name, rep, time.perf_counter() - t0))
+        res[name] = {"offered_rps": rate, "window_s": dur, "requests": rep["requests"],
+                     "mpix_s": round(rep["mpix_s"], 1), "p50_ms": round(rep["p50_ms"], 2),
+                     "p99_ms": round(rep["p99_ms"], 2), "max_ms": round(rep["max_ms"], 2),
+                     "wall_s": round(rep["wall_s"], 2), "batches": st.batches,
+                     "views_per_batch": round(st.batched_views / max(st.batches, 1), 2),
+                     "admissions": st.admissions}
+        srv.close()
+    ctx.close()
+    return res
