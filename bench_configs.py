@@ -130,7 +130,8 @@ def c2(args):
         for si, L in enumerate(suns):
             v = M.orbit_view(W, H, dist=2.0, azimuth=0.6, elevation=0.25, fov_deg=40.0, light=L, alpha=64.0, g=G,
                              albedo=0.99, seed=seed, jitter=0.5)
-            ctx.prepare_views([vol], [v])
+            ctx.prepare_views([vol], [v])  # LUT for g
+            vol.update_tr(np.asarray(v.light_dir[:], np.float32), v.alpha)  # the volume's own TR mips
             feats, active, _, _ = vol.gather_features(v)
             act = active > 0
             f = feats[act].contiguous()
