@@ -41,7 +41,7 @@ extern "C" {
 
 /* Context options (mrp_ctx_set_option).  No counterpart in the reference, whose launch
  * shapes are fixed at compile time. */
-#define MRP_OPT_MFMA_WAVES_PER_CU 1 /* persistent-grid waves per CU of the fused MFMA kernel (1..32, default 8) */
+#define MRP_OPT_MFMA_WAVES_PER_CU 1 /* persistent-grid waves per CU of the fused MFMA kernel (1..32, default 12) */
 #define MRP_OPT_EXACT_GATHER 2      /* 1: the fused MFMA kernels gather features with the FP32 path's exact
                                        arithmetic (bit-identical features); 0 (default): fast gather from
                                        zero-padded pyramid copies (features within a few ulps) */

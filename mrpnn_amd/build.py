@@ -19,8 +19,10 @@ ARCH = os.environ.get("MRP_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: float expressions round as written (FMAs only where written);
 # packed fp32 ops are disabled for all device code.
+# -amdgpu-mfma-vgpr-form: MFMA results in VGPRs, so the network kernel fits three waves per
+# SIMD (accumulation registers would otherwise be allocated on top of the VGPR budget).
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form",
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 
 SOURCES = ["capi.hip", "volume_kernels.hip", "predict_plain.hip", "fused.hip", "fused_host.hip", "mlp_mfma.hip",

@@ -83,13 +83,14 @@ __device__ inline float clampf(float x, float lo, float hi) { return __builtin_a
 __device__ inline float log_ge1(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
 
 // Trilinear blend of a padded level (byte offset base, dims R, R*R) at the non-negative
-// continuous coordinate (cx, cy, cz): corner = trunc (= floor), weights = fract.  Indices stay
-// below 2^24, so the products use the 24-bit multiplier.
+// continuous coordinate (cx, cy, cz): corner = floor, weights = fract.  The linear index
+// (x*R + y)*R + z stays below 2^24, so it is formed exactly with two float FMAs and converted
+// once.
 __device__ inline float tri_taps(BufRsrc r, unsigned base, unsigned R, unsigned R2, float cx, float cy, float cz) {
-    const unsigned ix = (unsigned)cx, iy = (unsigned)cy, iz = (unsigned)cz;
     const float wx = __builtin_amdgcn_fractf(cx), wy = __builtin_amdgcn_fractf(cy), wz = __builtin_amdgcn_fractf(cz);
-    const unsigned t = __umul24(__umul24(ix, R) + iy, R) + iz;
-    const int o = (int)(t * 4u + base);
+    const float fR = (float)R;
+    const float lin = fmaf(fmaf(floorf(cx), fR, floorf(cy)), fR, floorf(cz));
+    const int o = (int)((unsigned)lin * 4u + base);
     const f32x2_t c00 = ld_pair(r, o), c01 = ld_pair(r, o + (int)(R * 4u));
     const f32x2_t c10 = ld_pair(r, o + (int)(R2 * 4u)), c11 = ld_pair(r, o + (int)((R2 + R) * 4u));
     return flerp(flerp(flerp(c00.x, c00.y, wz), flerp(c01.x, c01.y, wz), wy),

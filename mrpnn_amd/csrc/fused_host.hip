@@ -25,7 +25,7 @@ struct FusedState {
     hipEvent_t staged = nullptr;  // staging buffers free once this completes
     bool have_counts = false;
     MfmaNet net;  // packed BF16 / FP8 weights
-    int mfma_waves_per_cu = 8;
+    int mfma_waves_per_cu = 12;  // three 256-thread blocks per CU (three waves per SIMD)
     // graph mode: the batch (staging copies, counter reset, march, network) is captured once
     // per launch shape on a private stream and replayed on the context's stream
     bool use_graph = false;

@@ -873,7 +873,7 @@ __device__ inline void load_shared(float* sW1, float* sG, StencilEntry* sst, con
 }
 
 template <int PREC, bool FAST>
-__global__ __launch_bounds__(256) void k_mlp_mfma_active(MfmaNet net, const DevVolume* __restrict__ vols,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_mlp_mfma_active(MfmaNet net, const DevVolume* __restrict__ vols,
                                                          const ViewSlot* __restrict__ slots,
                                                          const StencilEntry* __restrict__ stencil,
                                                          const FastStencil* __restrict__ fast,

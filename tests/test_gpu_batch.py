@@ -149,4 +149,4 @@ def test_autotune_keeps_results(bctx, volumes):
     got = _render(bctx, volumes, views, M.BF16)
     for i in range(len(views)):
         assert np.array_equal(got[i], ref[i]), i
-    bctx.set_option(M.OPT_MFMA_WAVES_PER_CU, 8)
+    bctx.set_option(M.OPT_MFMA_WAVES_PER_CU, 12)
