@@ -92,8 +92,10 @@ def main():
     ctx = M.Context()
     ctx.set_timing(True)
     # one cached TR set, as the reference keeps (volume.cu:1592): the sun moves every frame,
-    # so every frame rebakes its TR mips (a larger cache would serve the 16 cycling suns
-    # from memory)
+    # so every frame rebakes its TR mips (a cache of 16 or more would serve the 16 cycling suns
+    # from memory).  The bake runs on the library's second stream beside the march; with two
+    # sets it could also overlap the previous frame's network, which measured slower (both are
+    # VALU-bound and share the CUs)
     ctx.set_option(M.OPT_TR_SLOTS, 1)
     W, H = args.width, args.height
     vol = M.Volume(ctx, args.res)

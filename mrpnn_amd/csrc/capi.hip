@@ -171,6 +171,7 @@ extern "C" int mrp_ctx_destroy(mrp_ctx* c) {
     for (TrSlot& t : c->tr_slots) {
         hipFree(t.d_tr);
         hipFree(t.d_ptr);
+        if (t.last_read) hipEventDestroy(t.last_read);
     }
     for (int i = 0; i < 4; ++i) hipEventDestroy(c->ev[i]);
     for (int i = 0; i < TimingRing::kSlots; ++i)
@@ -348,6 +349,7 @@ extern "C" int mrp_volume_destroy(mrp_volume* v) {
     hipFree(v->d_pmips);
     hipFree(v->d_ptr);
     hipFree(v->d_occ);
+    if (v->mips_ready) hipEventDestroy(v->mips_ready);
     delete v;
     return MRP_OK;
 }
@@ -446,6 +448,8 @@ extern "C" int mrp_volume_update(mrp_volume* v) {
         hipFree(tmp[0]);
         hipFree(tmp[1]);
     }
+    if (!v->mips_ready) HIP_TRY(hipEventCreateWithFlags(&v->mips_ready, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(v->mips_ready, s));
     v->mips_valid = true;
     v->tr_valid = false;
     mrp_tr_slots_drop(v->ctx, v);

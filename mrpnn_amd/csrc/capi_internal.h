@@ -38,6 +38,8 @@ struct TrSlot {
     float* d_ptr = nullptr;  // zero-padded TR mips 0..6 (kPadTrFloats)
     uint64_t stamp = 0;      // last prepare call that used it
     bool valid = false;
+    hipEvent_t last_read = nullptr;  // recorded after the last batch that read this set
+    bool read_pending = false;
 };
 
 struct mrp_ctx {
@@ -79,6 +81,7 @@ struct mrp_volume {
     float* mip[9]{};
     float* trm[8]{};
     bool mips_valid = false;
+    hipEvent_t mips_ready = nullptr;  // recorded after the last pyramid rebuild
     bool tr_valid = false;
     float tr_light[3]{};
     float tr_alpha = 0.0f;

@@ -177,11 +177,10 @@ extern "C" int mrp_prepare_views(mrp_ctx* c, mrp_volume* const* vols, int n_vols
     int rc = mrp_ctx_update_hg_lut(c, views[0].g, c->lut_valid ? c->lut_seed : 1u);
     if (rc) return rc;
     ++c->tr_stamp;
-    // bakes on the second stream start after all work enqueued so far on the context's stream
-    // (pyramids they read; earlier batches still reading a set that gets re-baked)
+    // a bake on the second stream waits only for what it depends on: the pyramid it reads
+    // (mips_ready of the volume) and the last batch that read the set it overwrites
+    // (last_read of the slot); earlier batches may still run meanwhile
     FusedState* f = c->fused;
-    if (hipEventRecord(f->pre, c->stream) != hipSuccess || hipStreamWaitEvent(f->aux, f->pre, 0) != hipSuccess)
-        return mrp_fail(MRP_ERR_HIP, "stream ordering failed");
     for (int i = 0; i < n_views; ++i) {
         mrp_volume* v = vols[views[i].volume_index];
         const float* L = views[i].light_dir;
@@ -193,6 +192,9 @@ extern "C" int mrp_prepare_views(mrp_ctx* c, mrp_volume* const* vols, int n_vols
                                 c->tr_slot_cap);
             TrSlot& t = c->tr_slots[s];
             t.valid = false;
+            if ((t.read_pending && hipStreamWaitEvent(f->aux, t.last_read, 0) != hipSuccess) ||
+                (v->mips_ready && hipStreamWaitEvent(f->aux, v->mips_ready, 0) != hipSuccess))
+                return mrp_fail(MRP_ERR_HIP, "stream ordering failed");
             rc = mrp_tr_bake_into(v, L, views[i].alpha, t.d_tr, t.d_ptr, f->aux);
             if (rc) return rc;
             t.vol = v;
@@ -367,6 +369,14 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         if (ev) HT(hipEventRecord(ev[2], st));
         HT(hipEventRecord(f->staged, st));
     }
+    // the TR sets this batch reads may be re-baked only after it
+    for (size_t sl = 0; sl < entry_of_slot.size(); ++sl) {
+        if (entry_of_slot[sl] < 0) continue;
+        TrSlot& t = c->tr_slots[sl];
+        if (!t.last_read) HT(hipEventCreateWithFlags(&t.last_read, hipEventDisableTiming));
+        HT(hipEventRecord(t.last_read, st));
+        t.read_pending = true;
+    }
     f->have_counts = true;
     c->stats.pixels = (int64_t)pixels;
     c->stats.active_pixels = -1;
@@ -402,9 +412,11 @@ extern "C" int mrp_ctx_set_option(mrp_ctx* c, int option, int value) {
             if (value < 1 || value > 4096) return mrp_fail(MRP_ERR_INVALID, "MRP_OPT_TR_SLOTS in [1, 4096]");
             if (value < (int)c->tr_slots.size()) {  // shrink: free the tail sets
                 if (hipStreamSynchronize(c->stream) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "sync failed");
+                if (hipStreamSynchronize(c->fused->aux) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "sync failed");
                 for (size_t i = value; i < c->tr_slots.size(); ++i) {
                     hipFree(c->tr_slots[i].d_tr);
                     hipFree(c->tr_slots[i].d_ptr);
+                    if (c->tr_slots[i].last_read) hipEventDestroy(c->tr_slots[i].last_read);
                 }
                 c->tr_slots.resize(value);
             }
