@@ -149,6 +149,7 @@ extern "C" int mrp_ctx_create(const char* weights_path, mrp_ctx** out) {
     HIP_TRY(hipMemcpy(c->d_stencil, st, sizeof st, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&c->d_lut, 512 * 512 * sizeof(float)));
     HIP_TRY(hipMalloc(&c->d_fast, kPC * sizeof(FastStencil)));
+    HIP_TRY(hipMalloc(&c->d_lut_rows, (size_t)kPC * kLutRow * sizeof(float)));
     HIP_TRY(hipMemset(c->d_lut, 0, 512 * 512 * sizeof(float)));
     for (int i = 0; i < 4; ++i) HIP_TRY(hipEventCreate(&c->ev[i]));
     for (int i = 0; i < TimingRing::kSlots; ++i)
@@ -167,6 +168,7 @@ extern "C" int mrp_ctx_destroy(mrp_ctx* c) {
     hipFree(c->d_stencil);
     hipFree(c->d_lut);
     hipFree(c->d_fast);
+    hipFree(c->d_lut_rows);
     hipFree(c->d_plain_out);
     for (TrSlot& t : c->tr_slots) {
         hipFree(t.d_tr);
@@ -251,7 +253,7 @@ extern "C" int mrp_ctx_update_hg_lut(mrp_ctx* c, float g, uint32_t seed) {
     CHECK_ARG(g > -1.0f && g < 1.0f, "g must be in (-1, 1)");
     if (c->lut_valid && c->lut_g == g && c->lut_seed == seed) return MRP_OK;  // volume.cu:1403
     HIP_TRY(launch_hg_lut(c->d_lut, g, seed, c->stream));
-    HIP_TRY(launch_fast_stencil(c->d_stencil, c->d_lut, c->d_fast, c->stream));
+    HIP_TRY(launch_fast_stencil(c->d_stencil, c->d_lut, c->d_fast, c->d_lut_rows, c->stream));
     c->lut_valid = true;
     c->lut_g = g;
     c->lut_seed = seed;
@@ -266,7 +268,7 @@ extern "C" int mrp_ctx_download_hg_lut(mrp_ctx* c, float* host) {
 extern "C" int mrp_ctx_upload_hg_lut(mrp_ctx* c, const float* host, float g) {
     CHECK_ARG(c && host, "null argument");
     HIP_TRY(hipMemcpyAsync(c->d_lut, host, 512 * 512 * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(launch_fast_stencil(c->d_stencil, c->d_lut, c->d_fast, c->stream));
+    HIP_TRY(launch_fast_stencil(c->d_stencil, c->d_lut, c->d_fast, c->d_lut_rows, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->lut_valid = true;
     c->lut_g = g;

@@ -51,6 +51,7 @@ struct mrp_ctx {
     mrp::StencilEntry* d_stencil = nullptr;
     float* d_lut = nullptr;  // 512x512
     void* d_fast = nullptr;  // FastStencil[192], rebuilt with the LUT
+    float* d_lut_rows = nullptr;  // per stencil point the LUT row at its v (kLutRow floats each)
     bool lut_valid = false;
     float lut_g = 0.0f;
     uint32_t lut_seed = 0;

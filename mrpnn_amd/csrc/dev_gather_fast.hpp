@@ -36,20 +36,28 @@ MRP_HD constexpr long long ptr_offset(int m) {  // TR mips 0..6, base 128
 constexpr long long kPadMipFloats = pmip_offset(8);
 constexpr long long kPadTrFloats = ptr_offset(7);
 
-// One stencil point of the fast gather (64 bytes).
+// One stencil point of the fast gather (80 bytes).
 struct FastStencil {
     float ddx, ddy, ddz;  // table direction * offset (sphere points)
     float cscale;         // 1 / (|table direction| * offset) (sphere points)
     float off;            // offset (light points)
-    float lut_v;          // LUT v coordinate
     float hg0;            // LUT(dx/|d| * 0.5 + 0.5, v) (sphere points)
     float dres, tres;     // density / TR mip resolution
+    float dR, tR;         // padded dims (exact small integers)
     unsigned dbase, tbase;  // padded level offsets (bytes)
-    unsigned dR, tR;        // padded dims
-    unsigned dR2, tR2;      // padded dims squared
+    unsigned dR4, dRR4;     // 4 * R, 4 * R * R (byte strides of the density level)
+    unsigned tR4, tRR4;     // same for the TR level
+    unsigned lbase;         // byte offset of this point's LUT row (LUT rows at v = lut_v)
     int kind;               // 0 centre, 1 sphere, 2 light
+    float lut_v;            // LUT v coordinate
+    int pad0;
 };
-static_assert(sizeof(FastStencil) == 64, "FastStencil must be 64 bytes");
+static_assert(sizeof(FastStencil) == 80, "FastStencil must be 80 bytes");
+
+// LUT rows: for every stencil point the 512-texel row of the HG LUT interpolated at the
+// point's v (the bilinear lookup is separable, so a lookup becomes one linear interpolation);
+// entry 512 repeats 511 so a pair load at the clamped edge stays in the row.
+constexpr int kLutRow = 516;
 
 // Buffer resource over a device array: loads are bounds-checked by the hardware (reads
 // past num_records return 0) and take a 32-bit byte offset, so a trilinear tap pair costs one
@@ -65,7 +73,7 @@ __device__ inline f32x2_t ld_pair(BufRsrc r, int byte_off) {
 
 // Per-tile (uniform) inputs.
 struct FastView {
-    BufRsrc pmips, ptr, lut;
+    BufRsrc pmips, ptr, rows;  // padded density mips, padded TR mips, LUT rows
     v3 L, LX, invLX;  // light direction as the gather uses it; LX = normalize(L); inv3(LX)
     float alpha64;    // alpha / 64
 };
@@ -82,51 +90,69 @@ __device__ inline float clampf(float x, float lo, float hi) { return __builtin_a
 // ln(x) for x >= 1 (no denormal inputs): hardware log2 times ln 2.
 __device__ inline float log_ge1(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
 
-// Trilinear blend of a padded level (byte offset base, dims R, R*R) at the non-negative
-// continuous coordinate (cx, cy, cz): corner = floor, weights = fract.  The linear index
-// (x*R + y)*R + z stays below 2^24, so it is formed exactly with two float FMAs and converted
-// once.
-__device__ inline float tri_taps(BufRsrc r, unsigned base, unsigned R, unsigned R2, float cx, float cy, float cz) {
+// Trilinear blend of a padded level at the non-negative continuous coordinate (cx, cy, cz)
+// (already clamped into the padding): corner = floor, weights = fract.  The linear index
+// (x*R + y)*R + z stays below 2^24, so it is formed exactly with two float FMAs.
+__device__ inline float tri_taps(BufRsrc r, unsigned base, float R, unsigned R4, unsigned RR4, float cx, float cy,
+                                 float cz) {
     const float wx = __builtin_amdgcn_fractf(cx), wy = __builtin_amdgcn_fractf(cy), wz = __builtin_amdgcn_fractf(cz);
-    const float fR = (float)R;
-    const float lin = fmaf(fmaf(floorf(cx), fR, floorf(cy)), fR, floorf(cz));
+    const float lin = fmaf(fmaf(floorf(cx), R, floorf(cy)), R, floorf(cz));
     const int o = (int)((unsigned)lin * 4u + base);
-    const f32x2_t c00 = ld_pair(r, o), c01 = ld_pair(r, o + (int)(R * 4u));
-    const f32x2_t c10 = ld_pair(r, o + (int)(R2 * 4u)), c11 = ld_pair(r, o + (int)((R2 + R) * 4u));
+    const f32x2_t c00 = ld_pair(r, o), c01 = ld_pair(r, o + (int)R4);
+    const f32x2_t c10 = ld_pair(r, o + (int)RR4), c11 = ld_pair(r, o + (int)(RR4 + R4));
     return flerp(flerp(flerp(c00.x, c00.y, wz), flerp(c01.x, c01.y, wz), wy),
                  flerp(flerp(c10.x, c10.y, wz), flerp(c11.x, c11.y, wz), wy), wx);
 }
-// Zero-border trilinear on a padded level: uvw = p + 0.5, data coordinate uvw*res - 0.5,
-// padded coordinate +1, clamped to [0, res+1] (both taps outside -> exact 0).
-__device__ inline float tri_border(BufRsrc r, unsigned base, unsigned R, unsigned R2, float res, v3 p) {
-    const float k = fmaf(0.5f, res, 0.5f), hi = res + 1.0f;
-    const float cx = clampf(fmaf(p.x, res, k), 0.0f, hi);
-    const float cy = clampf(fmaf(p.y, res, k), 0.0f, hi);
-    const float cz = clampf(fmaf(p.z, res, k), 0.0f, hi);
-    return tri_taps(r, base, R, R2, cx, cy, cz);
+// Padded continuous coordinate of p on a level of resolution res: data coordinate
+// (p + 0.5)*res - 0.5, plus one for the padding.
+__device__ inline v3 padded_coord(v3 p, float res) {
+    const float k = fmaf(0.5f, res, 0.5f);
+    return mk3(fmaf(p.x, res, k), fmaf(p.y, res, k), fmaf(p.z, res, k));
 }
-// Clamp-addressing trilinear (TR mips): data coordinate clamped to [0, res-1].
-__device__ inline float tri_clamp(BufRsrc r, unsigned base, unsigned R, unsigned R2, float res, v3 p) {
-    const float k = fmaf(0.5f, res, -0.5f), hi = res - 1.0f;
-    const float cx = clampf(fmaf(p.x, res, k), 0.0f, hi) + 1.0f;
-    const float cy = clampf(fmaf(p.y, res, k), 0.0f, hi) + 1.0f;
-    const float cz = clampf(fmaf(p.z, res, k), 0.0f, hi) + 1.0f;
-    return tri_taps(r, base, R, R2, cx, cy, cz);
+// Zero-border trilinear (density): padded coordinate clamped to [0, res+1] (both taps
+// outside -> exact 0).
+__device__ inline float tri_border_c(BufRsrc r, const FastStencil& s, v3 c) {
+    const float hi = s.dres + 1.0f;
+    return tri_taps(r, s.dbase, s.dR, s.dR4, s.dRR4, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi),
+                    clampf(c.z, 0.0f, hi));
 }
-// 512x512 LUT, bilinear, clamp.  At the clamped edge x0 = 511 the pair's second value belongs
-// to the next row (or past the end: 0) and has weight 0.
-__device__ inline float lut_fast(BufRsrc lut, float u, float v) {
+// Clamp-addressing trilinear (TR mips): data coordinate clamped to [0, res-1], i.e. padded
+// coordinate to [1, res].
+__device__ inline float tri_clamp_c(BufRsrc r, const FastStencil& s, v3 c) {
+    const float hi = s.tres;
+    return tri_taps(r, s.tbase, s.tR, s.tR4, s.tRR4, clampf(c.x, 1.0f, hi), clampf(c.y, 1.0f, hi),
+                    clampf(c.z, 1.0f, hi));
+}
+// Point-level LUT lookup: linear interpolation along u of the point's LUT row.
+__device__ inline float lut_row(BufRsrc rows, unsigned lbase, float u) {
+    const float px = clampf(fmaf(u, 512.0f, -0.5f), 0.0f, 511.0f);
+    const f32x2_t a = ld_pair(rows, (int)((unsigned)px * 4u + lbase));
+    return flerp(a.x, a.y, __builtin_amdgcn_fractf(px));
+}
+// Zero-border trilinear on padded level data of resolution res at byte offset base.
+__device__ inline float tri_border_lvl(BufRsrc r, unsigned base, float res, v3 p) {
+    const float R = res + 3.0f;
+    const unsigned Ru = (unsigned)R;
+    const v3 c = padded_coord(p, res);
+    const float hi = res + 1.0f;
+    return tri_taps(r, base, R, Ru * 4u, Ru * Ru * 4u, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi),
+                    clampf(c.z, 0.0f, hi));
+}
+// 512x512 LUT, bilinear, clamp (used to build the rows).
+__device__ inline float lut_bilinear(const float* __restrict__ lut, float u, float v) {
     const float px = clampf(fmaf(u, 512.0f, -0.5f), 0.0f, 511.0f);
     const float py = clampf(fmaf(v, 512.0f, -0.5f), 0.0f, 511.0f);
     const float wx = __builtin_amdgcn_fractf(px), wy = __builtin_amdgcn_fractf(py);
     const unsigned x0 = (unsigned)px, y0 = (unsigned)py;
-    const unsigned y1 = min(y0 + 1u, 511u);
-    const f32x2_t a = ld_pair(lut, (int)((y0 * 512u + x0) * 4u)), b = ld_pair(lut, (int)((y1 * 512u + x0) * 4u));
-    return flerp(flerp(a.x, a.y, wx), flerp(b.x, b.y, wx), wy);
+    const unsigned x1 = min(x0 + 1u, 511u), y1 = min(y0 + 1u, 511u);
+    return flerp(flerp(lut[y0 * 512u + x0], lut[y0 * 512u + x1], wx), flerp(lut[y1 * 512u + x0], lut[y1 * 512u + x1], wx),
+                 wy);
 }
 
-// ShadowTerm_TRTex(...).x (renderResources.cu:84-102) on the padded TR level.
-__device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3 p) {
+// ShadowTerm_TRTex(...).x (renderResources.cu:84-102) on the padded TR level.  c_same: the
+// padded density coordinate of p when the TR level has the density level's resolution
+// (stencil mips >= 1), reused for points inside the box.
+__device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3 p, v3 c_same, bool same) {
     if (p.x < -0.5f || p.y < -0.5f || p.z < -0.5f || p.x > 0.5f || p.y > 0.5f || p.z > 0.5f) {
         // RayBoxOffset(p, LX) (vector.cu:46-63) with inv(LX) precomputed
         const v3 to_axil = (-p) * fv.invLX;
@@ -138,19 +164,22 @@ __device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3
         if (!(tma >= tmi)) return 1.0f;
         const float off = fmaxf(tmi, 0.0f);
         p = mk3(fmaf(fv.LX.x, off, p.x), fmaf(fv.LX.y, off, p.y), fmaf(fv.LX.z, off, p.z));
+        return tri_clamp_c(fv.ptr, s, padded_coord(p, s.tres));
     }
-    return tri_clamp(fv.ptr, s.tbase, s.tR, s.tR2, s.tres, p);
+    return tri_clamp_c(fv.ptr, s, same ? c_same : padded_coord(p, s.tres));
 }
 
 // (X_Val, X_Val_Hg, X_Val_Sub) of one stencil point.  LIGHT: points 160..191 (along the
-// light); centre: point 0.  LIGHT is a compile-time constant at every call site.
+// light); centre: point 0; same: the point's density and TR levels have equal resolution.
+// LIGHT is a template argument and same a constant after inlining at every call site.
 template <bool LIGHT>
-__device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const FastPixel& px, bool centre) {
+__device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const FastPixel& px, bool centre,
+                                 bool same) {
     v3 p;
     float xh;
     if (LIGHT) {  // MsDir = normalize(L * off) = L
         p = mk3(fmaf(fv.L.x, s.off, px.pos.x), fmaf(fv.L.y, s.off, px.pos.y), fmaf(fv.L.z, s.off, px.pos.z));
-        const float h0 = lut_fast(fv.lut, px.uc1, s.lut_v), h1 = lut_fast(fv.lut, px.uc2, s.lut_v);
+        const float h0 = lut_row(fv.rows, s.lbase, px.uc1), h1 = lut_row(fv.rows, s.lbase, px.uc2);
         xh = log_ge1(fmaf(h0, h1, 1.0f));
     } else {
         p.x = fmaf(px.X.x, s.ddx, fmaf(px.Y.x, s.ddy, fmaf(px.Z.x, s.ddz, px.pos.x)));
@@ -158,12 +187,13 @@ __device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const
         p.z = fmaf(px.X.z, s.ddx, fmaf(px.Y.z, s.ddy, fmaf(px.Z.z, s.ddz, px.pos.z)));
         // cos between LX and the point direction: (ax*dx + ay*dy + az*dz)/|d|, ddx = dx*off
         const float c2 = fmaf(px.ax, s.ddx, fmaf(px.ay, s.ddy, px.az * s.ddz)) * s.cscale;
-        const float h1 = lut_fast(fv.lut, fmaf(c2, 0.5f, 0.5f), s.lut_v);
+        const float h1 = lut_row(fv.rows, s.lbase, fmaf(c2, 0.5f, 0.5f));
         xh = centre ? px.xh_centre : log_ge1(fmaf(s.hg0, h1, 1.0f));
     }
-    const float rho = tri_border(fv.pmips, s.dbase, s.dR, s.dR2, s.dres, p);
+    const v3 c = padded_coord(p, s.dres);
+    const float rho = tri_border_c(fv.pmips, s, c);
     const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
-    const float xs = shadow_fast(fv, s, p);
+    const float xs = shadow_fast(fv, s, p, c, same);
     return mk3(xv, xh, xs);
 }
 

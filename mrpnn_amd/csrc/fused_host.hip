@@ -324,7 +324,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
                                            (int)need, q);
         else
             e = launch_mlp_mfma_active(precision, fast, f->net, f->d_vols, f->d_slots, c->d_stencil, c->d_fast,
-                                       c->d_weights, f->d_recs, f->d_counter, grid, q);
+                                       c->d_lut_rows, c->d_weights, f->d_recs, f->d_counter, grid, q);
         if (e) return e;
         if (ev && (e = hipEventRecord(ev[2], q))) return e;
         return hipSuccess;

@@ -16,7 +16,7 @@ hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, float*
                           const float alpha_lvl[8], v3 light, hipStream_t s);
 hipError_t launch_pad_mips(const float* const mips[9], float* pmips, hipStream_t s);
 hipError_t launch_occupancy(const float* grid, int N, uint8_t* occ, int occ_n, hipStream_t s);
-hipError_t launch_fast_stencil(const StencilEntry* st, const float* lut, void* fast, hipStream_t s);
+hipError_t launch_fast_stencil(const StencilEntry* st, const float* lut, void* fast, float* lut_rows, hipStream_t s);
 hipError_t launch_hg_lut(float* lut, float g, uint32_t seed, hipStream_t s);
 
 // predict_plain.hip

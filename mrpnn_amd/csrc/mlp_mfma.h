@@ -91,8 +91,8 @@ uint8_t host_f32_to_e4m3(float f);
 
 hipError_t launch_mlp_mfma_active(int precision, bool fast, const MfmaNet& net, const DevVolume* vols,
                                   const ViewSlot* slots, const StencilEntry* stencil, const void* fast_stencil,
-                                  const float* W, const ActiveRec* recs, const unsigned* counter, int grid,
-                                  hipStream_t s);
+                                  const float* lut_rows, const float* W, const ActiveRec* recs,
+                                  const unsigned* counter, int grid, hipStream_t s);
 hipError_t launch_mlp_mfma_features(int precision, const MfmaNet& net, const float* W, const float* feats, int n,
                                     const float sr[3], float g, float* rgb, hipStream_t s);
 

@@ -495,9 +495,12 @@ struct FastGatherSrc {
             for (int i = 0; i < 4; ++i) {
                 const int k = p0 + 16 * h + 4 * lc.q + i;
                 v3 f = mk3(0.0f, 0.0f, 0.0f);
+                // density and TR levels share the resolution for stencil mips >= 1: all points
+                // of chunks 16..159 and the second half of the light points
+                const bool same = (p0 >= 16 && p0 < 160) || (p0 == 160 && h == 1);
                 if (lc.ok) {
-                    if (p0 >= 160) f = gather_fast<true>(fv, st[k], px, false);
-                    else f = gather_fast<false>(fv, st[k], px, p0 == 0 && h == 0 && i == 0 && lc.q == 0);
+                    if (p0 >= 160) f = gather_fast<true>(fv, st[k], px, false, same);
+                    else f = gather_fast<false>(fv, st[k], px, p0 == 0 && h == 0 && i == 0 && lc.q == 0, same);
                 }
                 F[0][4 * h + i] = f.x;
                 F[2][4 * h + i] = f.y;
@@ -877,6 +880,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                                                          const ViewSlot* __restrict__ slots,
                                                          const StencilEntry* __restrict__ stencil,
                                                          const FastStencil* __restrict__ fast,
+                                                         const float* __restrict__ lut_rows,
                                                          const float* __restrict__ W, const ActiveRec* __restrict__ recs,
                                                          const unsigned* __restrict__ counter) {
     __shared__ float sW1[1200];
@@ -931,7 +935,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             src.st = opaque_v(sfs);
             src.fv.pmips = make_rsrc(vol.pmips, kPadMipFloats);
             src.fv.ptr = make_rsrc(vol.ptr, kPadTrFloats);
-            src.fv.lut = make_rsrc(vol.lut, 512 * 512);
+            src.fv.rows = make_rsrc(lut_rows, (long long)kPC * kLutRow);
             src.fv.L = ps.L;
             src.fv.LX = ps.LX;
             src.fv.invLX = inv3(ps.LX);
@@ -1001,27 +1005,28 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
 
 template <int PREC>
 static void launch_active_prec(bool fast, const MfmaNet& net, const DevVolume* vols, const ViewSlot* slots,
-                               const StencilEntry* stencil, const FastStencil* fst, const float* W,
-                               const ActiveRec* recs, const unsigned* counter, int grid, hipStream_t s) {
+                               const StencilEntry* stencil, const FastStencil* fst, const float* rows,
+                               const float* W, const ActiveRec* recs, const unsigned* counter, int grid,
+                               hipStream_t s) {
     if (fast)
         hipLaunchKernelGGL((k_mlp_mfma_active<PREC, true>), dim3(grid), dim3(256), 0, s, net, vols, slots, stencil, fst,
-                           W, recs, counter);
+                           rows, W, recs, counter);
     else
         hipLaunchKernelGGL((k_mlp_mfma_active<PREC, false>), dim3(grid), dim3(256), 0, s, net, vols, slots, stencil,
-                           fst, W, recs, counter);
+                           fst, rows, W, recs, counter);
 }
 
 hipError_t launch_mlp_mfma_active(int precision, bool fast, const MfmaNet& net, const DevVolume* vols,
                                   const ViewSlot* slots, const StencilEntry* stencil, const void* fast_stencil,
-                                  const float* W, const ActiveRec* recs, const unsigned* counter, int grid,
-                                  hipStream_t s) {
+                                  const float* lut_rows, const float* W, const ActiveRec* recs,
+                                  const unsigned* counter, int grid, hipStream_t s) {
     const FastStencil* fst = (const FastStencil*)fast_stencil;
     if (precision == MRP_BF16)
-        launch_active_prec<MRP_BF16>(fast, net, vols, slots, stencil, fst, W, recs, counter, grid, s);
+        launch_active_prec<MRP_BF16>(fast, net, vols, slots, stencil, fst, lut_rows, W, recs, counter, grid, s);
     else if (precision == MRP_BF16_1T)
-        launch_active_prec<MRP_BF16_1T>(fast, net, vols, slots, stencil, fst, W, recs, counter, grid, s);
+        launch_active_prec<MRP_BF16_1T>(fast, net, vols, slots, stencil, fst, lut_rows, W, recs, counter, grid, s);
     else
-        launch_active_prec<MRP_FP8>(fast, net, vols, slots, stencil, fst, W, recs, counter, grid, s);
+        launch_active_prec<MRP_FP8>(fast, net, vols, slots, stencil, fst, lut_rows, W, recs, counter, grid, s);
     return hipGetLastError();
 }
 

@@ -154,13 +154,12 @@ __global__ __launch_bounds__(256) void k_tr_bake(TrArgs a) {
         if (a.pmips && m + 1 < 8) {
             // zero-padded copy of the density mip: no per-tap border tests
             const unsigned base = (unsigned)pmip_offset(m + 1) * 4u;
-            const unsigned R = (unsigned)pad_dim(sres);
             const float fres = (float)sres;
             const BufRsrc rs = make_rsrc(a.pmips, kPadMipFloats);
 #pragma unroll 8
             for (int i = 0; i < 128; ++i) {
                 lp = lp + a.light * st;
-                sum = sum + tri_border(rs, base, R, R * R, fres, lp);
+                sum = sum + tri_border_lvl(rs, base, fres, lp);
             }
         } else {
             const float* src = a.mips[m + 1];
@@ -318,15 +317,34 @@ __global__ void k_fast_stencil(const StencilEntry* __restrict__ st, const float*
     f.lut_v = e.lut_v;
     f.hg0 = f.kind == 1 ? sample_lut_exact(lut, (e.dx / len) * 0.5f + 0.5f, e.lut_v) : 0.0f;
     const int tm = e.mip - 1 > 0 ? e.mip - 1 : 0;
+    const unsigned dR = (unsigned)pad_dim(256 >> e.mip), tR = (unsigned)pad_dim(128 >> tm);
     f.dres = (float)(256 >> e.mip);
     f.tres = (float)(128 >> tm);
+    f.dR = (float)dR;
+    f.tR = (float)tR;
     f.dbase = (unsigned)pmip_offset(e.mip) * 4u;
     f.tbase = (unsigned)ptr_offset(tm) * 4u;
-    f.dR = (unsigned)pad_dim(256 >> e.mip);
-    f.tR = (unsigned)pad_dim(128 >> tm);
-    f.dR2 = f.dR * f.dR;
-    f.tR2 = f.tR * f.tR;
+    f.dR4 = dR * 4u;
+    f.dRR4 = dR * dR * 4u;
+    f.tR4 = tR * 4u;
+    f.tRR4 = tR * tR * 4u;
+    f.lbase = (unsigned)(i * kLutRow * 4);
+    f.pad0 = 0;
     out[i] = f;
+}
+
+// LUT rows of the fast gather: row i = the LUT interpolated along v at stencil point i's v.
+__global__ __launch_bounds__(512) void k_lut_rows(const StencilEntry* __restrict__ st, const float* __restrict__ lut,
+                                                  float* __restrict__ rows) {
+    const int i = blockIdx.x, x = threadIdx.x;
+    const float v = st[i].lut_v;
+    const float py = fminf(fmaxf(fmaf(v, 512.0f, -0.5f), 0.0f), 511.0f);
+    const unsigned y0 = (unsigned)py, y1 = min(y0 + 1u, 511u);
+    const float wy = __builtin_amdgcn_fractf(py);
+    const float r = flerp(lut[y0 * 512u + x], lut[y1 * 512u + x], wy);
+    rows[i * kLutRow + x] = r;
+    if (x == 511)
+        for (int k = 512; k < kLutRow; ++k) rows[i * kLutRow + k] = r;
 }
 
 static inline int grid_for(size_t total, int block) {
@@ -389,8 +407,9 @@ hipError_t launch_occupancy(const float* grid, int N, uint8_t* occ, int occ_n, h
                        occ_n);
     return hipGetLastError();
 }
-hipError_t launch_fast_stencil(const StencilEntry* st, const float* lut, void* fast, hipStream_t s) {
+hipError_t launch_fast_stencil(const StencilEntry* st, const float* lut, void* fast, float* lut_rows, hipStream_t s) {
     hipLaunchKernelGGL(k_fast_stencil, dim3(1), dim3(kPC), 0, s, st, lut, (FastStencil*)fast);
+    hipLaunchKernelGGL(k_lut_rows, dim3(kPC), dim3(512), 0, s, st, lut, lut_rows);
     return hipGetLastError();
 }
 
