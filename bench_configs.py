@@ -258,7 +258,8 @@ def c4(args):
     ctx.close()
     return {"config": "c4", "workload": "64 views, 8 volumes (2 x 128^3, 256^3, 384^3, 512^3), mixed resolutions",
             "pixels_per_batch": pixels, "tr_bake_ms_64_suns": round(bake_ms, 1),
-            "autotune": {"waves_per_cu": best_w, "ms": round(best_ms, 3)}, "graph": True,
+            "autotune": {"waves_per_cu": best_w, "lds_weights": ctx.get_option(M.OPT_LDS_WEIGHTS),
+                         "ms": round(best_ms, 3)}, "graph": True,
             "bf16": res["bf16"], "fp8": res["fp8"],
             "fp32_plain_mpix_s_incl_tr_rebake": round(pixels / plain_s / 1e6, 1)}
 

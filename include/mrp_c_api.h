@@ -51,6 +51,11 @@ extern "C" {
 #define MRP_OPT_TR_SLOTS 3          /* capacity of the batched path's TR cache: one set of transmittance
                                        mips (about 20 MB) per distinct (volume, light_dir, alpha) of a
                                        batch (1..4096, default 64) */
+#define MRP_OPT_LDS_WEIGHTS 5       /* 1 (default): the fused MFMA kernel runs one 768-thread workgroup
+                                       per CU holding the weight fragments in LDS (the whole network
+                                       for FP8 and single-term BF16, the tail for two-term BF16);
+                                       0: fragments are read through L1/L2 by MRP_OPT_MFMA_WAVES_PER_CU
+                                       persistent waves per CU */
 
 /* Number of features per pixel: [X_Val 0:192 | X_Val_Hg 192:384 | X_Val_Sub 384:576 | gamma]
  * (core/radiancePredict.cu:427-485, row order of PopulateInputData :562-565). */
@@ -166,9 +171,10 @@ int mrp_render_plain_device(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view,
 int mrp_render_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views,
                      int precision, float* const* dev_out4);
 /* Launch-shape autotuner (no counterpart in the reference, whose launch shapes are fixed):
- * renders the prepared batch with 4, 8, 12, 16, 24 and 32 persistent waves per CU for the
- * network kernel, keeps the fastest as MRP_OPT_MFMA_WAVES_PER_CU and reports it with its
- * time per batch.  Blocking. */
+ * renders the prepared batch with the LDS-weight kernel (12 waves per CU) and with the
+ * L1/L2-weight kernel at 4, 8, 12, 16, 24 and 32 persistent waves per CU, keeps the fastest
+ * (MRP_OPT_LDS_WEIGHTS, MRP_OPT_MFMA_WAVES_PER_CU) and reports its waves per CU and time per
+ * batch.  Blocking. */
 int mrp_autotune_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views,
                        int precision, float* const* dev_out4, int* best_waves_per_cu, float* best_ms);
 /* Brings the HG LUT (all views share g) and the batch TR cache up to date: one set of TR mips

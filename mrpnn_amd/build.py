@@ -25,13 +25,17 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contr
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form",
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 
+# Per-file additions.  mlp_mfma.hip: no input of its network kernels is NaN, and without NaN
+# semantics a ReLU / max is one v_max_f32 (IEEE mode would quiet every MFMA result first).
+FILE_FLAGS = {"mlp_mfma.hip": ["-fno-honor-nans"]}
+
 SOURCES = ["capi.hip", "volume_kernels.hip", "predict_plain.hip", "fused.hip", "fused_host.hip", "mlp_mfma.hip",
            "pathtrace.hip", "serve.cpp", "cxx_api.cpp"]
 
 
 def _compile(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", out]
+    cmd = [HIPCC] + HIP_FLAGS + FILE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))

@@ -129,6 +129,14 @@ __device__ inline float lut_row(BufRsrc rows, unsigned lbase, float u) {
     const f32x2_t a = ld_pair(rows, (int)((unsigned)px * 4u + lbase));
     return flerp(a.x, a.y, __builtin_amdgcn_fractf(px));
 }
+// HenyeyGreenstein (vector.cu:210-214) with t^1.5 formed as t * sqrt(t) (two correctly
+// rounded steps instead of a powf call).
+__device__ inline float hg_fast(float cs, float g) {
+    const float g2 = g * g;
+    const float t = 1.0f + g2 - 2.0f * g * cs;
+    return ((1.0f - g2) / (t * sqrtf(t))) * 0.25f;
+}
+
 // Zero-border trilinear on padded level data of resolution res at byte offset base.
 __device__ inline float tri_border_lvl(BufRsrc r, unsigned base, float res, v3 p) {
     const float R = res + 3.0f;

@@ -6,9 +6,24 @@
 
 namespace mrp {
 
+// View-uniform inputs of the MFMA network kernels, computed once per view on the host with
+// the device's formulas (normalize3, inv3, IEEE division) instead of once per 16-pixel tile.
+struct ViewConst {
+    v3 L;         // normalize(light_dir)                (render.cuh:812)
+    v3 LX;        // normalize(L): LXMain                (render.cuh:694-708,823)
+    v3 invLX;     // inv3(LX)                            (vector.cu:19)
+    v3 sr;        // scatter_rate / 1.001                (render.cuh:843)
+    v3 sp;        // sr^4                                (radiancePredict.cu:324)
+    float alpha64;  // alpha / 64
+    int need_y;     // channel y differs from x: a second tail evaluation
+    int need_z;     // channel z differs from x and y: a third tail evaluation
+};
+void view_const_init(const ViewDev& vw, ViewConst* c);
+
 // One view of a batch as the device sees it.
 struct ViewSlot {
     ViewDev vw;
+    ViewConst vc;
     float4* out;  // width*height (rgb, distance)
     int vol;      // index into the batch's DevVolume table
     int tile0;    // first 8x8 tile of this view in the flattened tile grid

@@ -26,12 +26,13 @@ struct FusedState {
     bool have_counts = false;
     MfmaNet net;  // packed BF16 / FP8 weights
     int mfma_waves_per_cu = 12;  // three 256-thread blocks per CU (three waves per SIMD)
+    bool lds_weights = true;     // one 768-thread block per CU with the weights in LDS
     // graph mode: the batch (staging copies, counter reset, march, network) is captured once
     // per launch shape on a private stream and replayed on the context's stream
     bool use_graph = false;
     hipStream_t cap_stream = nullptr;
     struct Graph {
-        int n_views, n_entries, n_tiles, precision, fast, grid;
+        int n_views, n_entries, n_tiles, precision, fast, grid, lds_w;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -41,6 +42,17 @@ struct FusedState {
     hipEvent_t pre = nullptr, tr_done = nullptr;
     bool tr_recorded = false;
 };
+
+void mrp::view_const_init(const ViewDev& vw, ViewConst* c) {
+    c->L = normalize3(vw.light_dir);
+    c->LX = normalize3(c->L);
+    c->invLX = inv3(c->LX);
+    c->sr = vw.scatter_rate / 1.001f;
+    c->sp = mk3(powf(c->sr.x, 4.0f), powf(c->sr.y, 4.0f), powf(c->sr.z, 4.0f));
+    c->alpha64 = vw.alpha / 64.0f;
+    c->need_y = c->sr.y != c->sr.x;
+    c->need_z = c->sr.z != c->sr.x && c->sr.z != c->sr.y;
+}
 
 static void clear_graphs(FusedState* f) {
     for (auto& g : f->graphs) hipGraphExecDestroy(g.exec);
@@ -289,6 +301,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
     for (int i = 0; i < n_views; ++i) {
         ViewSlot& s = f->h_slots[i];
         s.vw = mrp_to_dev(&views[i]);
+        view_const_init(s.vw, &s.vc);
         s.out = (float4*)dev_out4[i];
         s.vol = entry_of_slot[view_slot[i]];
         s.tile0 = t0;
@@ -302,7 +315,9 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         hipError_t e_ = (x);                                                                \
         if (e_ != hipSuccess) return mrp_fail(MRP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
     } while (0)
-    const int grid = c->props.multiProcessorCount * f->mfma_waves_per_cu / 4;
+    // LDS-weight kernel: one 768-thread workgroup per CU; else 256-thread workgroups
+    const bool lds_w = f->lds_weights && precision != MRP_FP32;
+    const int grid = lds_w ? c->props.multiProcessorCount : c->props.multiProcessorCount * f->mfma_waves_per_cu / 4;
     const bool fast = !c->exact_gather;
     // the work of one batch on stream q; ev (optional) brackets march and network
     auto enqueue = [&](hipStream_t q, hipEvent_t* ev, hipEvent_t staged, bool wait_tr) -> hipError_t {
@@ -323,7 +338,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
             e = launch_predict_active_fp32(f->d_vols, f->d_slots, c->d_stencil, c->d_weights, f->d_recs, f->d_counter,
                                            (int)need, q);
         else
-            e = launch_mlp_mfma_active(precision, fast, f->net, f->d_vols, f->d_slots, c->d_stencil, c->d_fast,
+            e = launch_mlp_mfma_active(precision, fast, lds_w, f->net, f->d_vols, f->d_slots, c->d_stencil, c->d_fast,
                                        c->d_lut_rows, c->d_weights, f->d_recs, f->d_counter, grid, q);
         if (e) return e;
         if (ev && (e = hipEventRecord(ev[2], q))) return e;
@@ -340,7 +355,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         FusedState::Graph* g = nullptr;
         for (auto& x : f->graphs)
             if (x.n_views == n_views && x.n_entries == n_entries && x.n_tiles == n_tiles && x.precision == precision &&
-                x.fast == (int)fast && x.grid == grid)
+                x.fast == (int)fast && x.grid == grid && x.lds_w == (int)lds_w)
                 g = &x;
         if (!g) {
             if (!f->cap_stream) HT(hipStreamCreateWithFlags(&f->cap_stream, hipStreamNonBlocking));
@@ -355,7 +370,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
             hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             hipGraphDestroy(graph);
             HT(ie);
-            f->graphs.push_back({n_views, n_entries, n_tiles, precision, (int)fast, grid, exec});
+            f->graphs.push_back({n_views, n_entries, n_tiles, precision, (int)fast, grid, (int)lds_w, exec});
             g = &f->graphs.back();
         }
         // graph mode times the whole replay as the network time (march time reads 0)
@@ -408,6 +423,9 @@ extern "C" int mrp_ctx_set_option(mrp_ctx* c, int option, int value) {
         case MRP_OPT_GRAPH:
             c->fused->use_graph = value != 0;
             return MRP_OK;
+        case MRP_OPT_LDS_WEIGHTS:
+            c->fused->lds_weights = value != 0;
+            return MRP_OK;
         case MRP_OPT_TR_SLOTS:
             if (value < 1 || value > 4096) return mrp_fail(MRP_ERR_INVALID, "MRP_OPT_TR_SLOTS in [1, 4096]");
             if (value < (int)c->tr_slots.size()) {  // shrink: free the tail sets
@@ -434,6 +452,7 @@ extern "C" int mrp_ctx_get_option(mrp_ctx* c, int option, int* value) {
         case MRP_OPT_EXACT_GATHER: *value = c->exact_gather ? 1 : 0; return MRP_OK;
         case MRP_OPT_TR_SLOTS: *value = c->tr_slot_cap; return MRP_OK;
         case MRP_OPT_GRAPH: *value = c->fused->use_graph ? 1 : 0; return MRP_OK;
+        case MRP_OPT_LDS_WEIGHTS: *value = c->fused->lds_weights ? 1 : 0; return MRP_OK;
         default: return mrp_fail(MRP_ERR_INVALID, "unknown option %d", option);
     }
 }
@@ -443,16 +462,21 @@ extern "C" int mrp_ctx_get_option(mrp_ctx* c, int option, int* value) {
 extern "C" int mrp_autotune_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
                                   int precision, float* const* dev_out4, int* best_waves, float* best_ms) {
     if (!c) return mrp_fail(MRP_ERR_INVALID, "ctx is null");
-    static const int cand[] = {4, 8, 12, 16, 24, 32};
+    // candidate 0: LDS-resident weights (one 768-thread workgroup per CU); then L1/L2 weights
+    // at each waves-per-CU count
+    static const int cand[] = {0, 4, 8, 12, 16, 24, 32};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return mrp_fail(MRP_ERR_HIP, "event creation failed");
     const int keep = c->fused->mfma_waves_per_cu;
+    const bool keep_lds = c->fused->lds_weights;
     int best = keep;
+    bool best_lds = keep_lds;
     float best_t = 1e30f;
     int rc = MRP_OK;
     for (int w : cand) {
-        c->fused->mfma_waves_per_cu = w;
+        c->fused->lds_weights = w == 0;
+        c->fused->mfma_waves_per_cu = w == 0 ? keep : w;
         if ((rc = mrp_render_batch(c, vols, n_vols, views, n_views, precision, dev_out4))) break;  // warm-up
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = mrp_fail(MRP_ERR_HIP, "event record failed"); break; }
         for (int r = 0; r < 3 && !rc; ++r) rc = mrp_render_batch(c, vols, n_vols, views, n_views, precision, dev_out4);
@@ -465,14 +489,16 @@ extern "C" int mrp_autotune_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vol
         }
         if (ms / 3.0f < best_t) {
             best_t = ms / 3.0f;
-            best = w;
+            best = w == 0 ? keep : w;
+            best_lds = w == 0;
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     c->fused->mfma_waves_per_cu = rc ? keep : best;
+    c->fused->lds_weights = rc ? keep_lds : best_lds;
     if (rc) return rc;
-    if (best_waves) *best_waves = best;
+    if (best_waves) *best_waves = best_lds ? 12 : best;
     if (best_ms) *best_ms = best_t;
     return MRP_OK;
 }

@@ -32,6 +32,16 @@ enum MfmaLayerId {
     ML_COUNT = 60
 };
 
+// Kernel variants: a precision id (MRP_BF16, MRP_FP8, MRP_BF16_1T) plus kLdsW when the weight
+// fragments of layers >= lds_from(precision) are staged in LDS (one 768-thread workgroup per
+// CU shares one copy) instead of being read through the L1/L2 path for every tile.
+constexpr int kLdsW = 16;
+constexpr int base_prec(int v) { return v & (kLdsW - 1); }
+constexpr bool lds_weights(int v) { return (v & kLdsW) != 0; }
+// first LDS-resident layer: the whole network for FP8 (60.5 KB) and single-term BF16
+// (121 KB); the tail (hi and lo terms, 134 KB) for two-term BF16
+constexpr int lds_from(int v) { return base_prec(v) == 1 ? 42 /* ML_FIN2 */ : 0; }
+
 // Compile-time shape of every layer: rows, inputs, first destination row inside the padded
 // 16-row M-tiles, and whether K comes in 32-wide chunks (else K <= 16, one chunk whose
 // slots 8q+4..7 are zero).
@@ -73,8 +83,17 @@ constexpr int mfma_bias(int id) {
 }
 constexpr int kMfmaFrags = mfma_frag(ML_COUNT);
 constexpr int kMfmaBias = mfma_bias(ML_COUNT);
+static_assert(ML_FIN2 == 42, "lds_from names ML_FIN2");
+// bytes of the LDS weight copy of a variant: fragments lds_from.. (1 KB each for bf16, 512 B
+// for fp8), then for two-term BF16 the same layers' lo terms
+constexpr int lds_weight_bytes(int v) {
+    return !lds_weights(v) ? 0
+           : base_prec(v) == 2 ? (kMfmaFrags - mfma_frag(lds_from(v))) * 512
+           : (kMfmaFrags - mfma_frag(lds_from(v))) * 1024 * (base_prec(v) == 1 ? 2 : 1);
+}
 
 struct MfmaNet {
+    const void* lds = nullptr;       // device: LDS copy of the fragments of layers >= lds_from
     const uint4* bf16 = nullptr;     // fragments, 16 bytes per lane: bf16(W)
     const uint4* bf16_lo = nullptr;  // bf16(W - bf16(W)): second term of the two-term weights
     const uint2* fp8 = nullptr;   // fragments, 8 bytes per lane
@@ -89,7 +108,7 @@ void mfma_net_destroy(MfmaNet* net);
 uint16_t host_f32_to_bf16(float f);
 uint8_t host_f32_to_e4m3(float f);
 
-hipError_t launch_mlp_mfma_active(int precision, bool fast, const MfmaNet& net, const DevVolume* vols,
+hipError_t launch_mlp_mfma_active(int precision, bool fast, bool lds_w, const MfmaNet& net, const DevVolume* vols,
                                   const ViewSlot* slots, const StencilEntry* stencil, const void* fast_stencil,
                                   const float* lut_rows, const float* W, const ActiveRec* recs,
                                   const unsigned* counter, int grid, hipStream_t s);

@@ -227,6 +227,7 @@ struct NetRs {
     BufRsrc wlo;  // bf16 lo fragments
     BufRsrc bias;
     const int* wexp;
+    const char* lw;  // LDS copy of the fragments of layers >= lds_from (variants with kLdsW)
 };
 // The pointers are uniform (kernel arguments); reading them through readfirstlane keeps the
 // descriptors in scalar registers also where they arrive in vector registers (call arguments).
@@ -243,6 +244,7 @@ __device__ inline NetRs make_net_rs(const MfmaNet& n, bool fp8) {
     r.wlo = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.bf16_lo), (short)0, kMfmaFrags * 1024, 0x00020000);
     r.bias = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.bias), (short)0, kMfmaBias * 4, 0x00020000);
     r.wexp = (const int*)uniform_ptr(n.wexp);
+    r.lw = (const char*)n.lds;
     return r;
 }
 
@@ -254,11 +256,16 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
     constexpr int NM = mfma_nm(ID), NK = mfma_nk(ID), FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
     // second weight term: MRP_BF16 applies it to the tail (LSEFin2, LC0..LX12), where the
     // single-term rounding error concentrates; the trunk's 8..32-wide layers use one term
-    constexpr bool TWO = PREC == MRP_BF16 && ID >= ML_FIN2;
+    constexpr int P = base_prec(PREC);
+    constexpr bool TWO = P == MRP_BF16 && ID >= ML_FIN2;
+    // weights of this layer from the LDS copy: fragment f at (f - LF) * FB + lane * (FB / 64),
+    // lo terms after the NL hi fragments
+    constexpr bool LDS = lds_weights(PREC) && ID >= lds_from(PREC);
+    constexpr int LF = mfma_frag(lds_from(PREC)), NL = kMfmaFrags - LF;
     const int q = lane >> 4;
     // Fragments of M-tile mt+1 are requested before the MFMAs of M-tile mt (two buffers), so
     // the weight latency of a multi-tile layer is paid once.
-    if constexpr (PREC == MRP_BF16 || PREC == MRP_BF16_1T) {
+    if constexpr (P == MRP_BF16 || P == MRP_BF16_1T) {
         u32x4_t b[NK];
 #pragma unroll
         for (int kc = 0; kc < NK; ++kc)
@@ -268,15 +275,24 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
         auto fetch = [&](int mt, int buf) {
 #pragma unroll
             for (int kc = 0; kc < NK; ++kc) {
-                const int fo = (FRAG + mt * NK + kc) * 1024;
-                wa[buf][kc] = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, fo, 0);
-                if constexpr (TWO) wl[buf][kc] = __builtin_amdgcn_raw_buffer_load_b128(net.wlo, lane * 16, fo, 0);
+                if constexpr (LDS) {
+                    const int lo = (FRAG - LF + mt * NK + kc) * 1024 + lane * 16;
+                    wa[buf][kc] = *(const u32x4_t*)(net.lw + lo);
+                    if constexpr (TWO) wl[buf][kc] = *(const u32x4_t*)(net.lw + NL * 1024 + lo);
+                } else {
+                    const int fo = (FRAG + mt * NK + kc) * 1024;
+                    wa[buf][kc] = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, fo, 0);
+                    if constexpr (TWO) wl[buf][kc] = __builtin_amdgcn_raw_buffer_load_b128(net.wlo, lane * 16, fo, 0);
+                }
             }
             bb[buf] = u32x4_t{0u, 0u, 0u, 0u};
             if constexpr (mfma_has_bias(ID))
                 bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
         };
         fetch(0, 0);
+        // the first M-tile's loads (both weight terms and the bias) are issued together:
+        // otherwise register reuse can serialise the second term's load behind the first MFMA
+        if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int mt = 0; mt < NM; ++mt) {
             const int cur = mt & 1;
@@ -317,14 +333,19 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
         u32x4_t bb[2];
         auto fetch = [&](int mt, int buf) {
 #pragma unroll
-            for (int kc = 0; kc < NK; ++kc)
-                wa[buf][kc] = __builtin_bit_cast(
-                    long, __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, (FRAG + mt * NK + kc) * 512, 0));
+            for (int kc = 0; kc < NK; ++kc) {
+                if constexpr (LDS)
+                    wa[buf][kc] = *(const long*)(net.lw + (FRAG - LF + mt * NK + kc) * 512 + lane * 8);
+                else
+                    wa[buf][kc] = __builtin_bit_cast(
+                        long, __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, (FRAG + mt * NK + kc) * 512, 0));
+            }
             bb[buf] = u32x4_t{0u, 0u, 0u, 0u};
             if constexpr (mfma_has_bias(ID))
                 bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
         };
         fetch(0, 0);
+        if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int mt = 0; mt < NM; ++mt) {
             const int cur = mt & 1;
@@ -361,9 +382,9 @@ __device__ inline T* opaque_v(T* p) {
     return p + opaque_zero();
 }
 
-// Gate sigmoid with the hardware exp2 and reciprocal (about 2 ulp): the gates feed BF16 / FP8
-// products whose rounding is far coarser.
-__device__ inline float sigmoidf_(float x) { return __frcp_rn(1.0f + __expf(-x)); }
+// Gate sigmoid with the hardware exp2 and reciprocal (v_rcp_f32, 1 ulp; about 2 ulp in all):
+// the gates feed BF16 / FP8 products whose rounding is far coarser.
+__device__ inline float sigmoidf_(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 
 // Per-pixel lane state shared by the tile routine.
 struct LaneCtx {
@@ -371,6 +392,9 @@ struct LaneCtx {
     bool ok;        // this column holds a pixel
     float g, gamma;
     v3 sr;          // scatterrate passed to the network (scatter_rate / 1.001)
+    v3 sp;          // sr^4 (radiancePredict.cu:324)
+    bool need_y;    // sr.y != sr.x: channel y needs its own tail
+    bool need_z;    // sr.z differs from sr.x and sr.y
 };
 
 // SE gate weights of the 12 gated blocks (trunk blocks 0..7, DI blocks 0..3) staged in LDS:
@@ -605,32 +629,113 @@ __device__ inline void di_block(const NetRs& net, const float* __restrict__ sG, 
         for (int i = 0; i < 8; ++i) R[b][i] = (lo && i < 4) ? O[b][i] + G[b][i] : 0.0f;
 }
 
-template <int PREC, int K>  // LX residual pair k: x += ReLU(LX_{2k+1} ReLU(LX_{2k} x))
-__device__ inline void lx_pair(const NetRs& net, const LaneCtx& lc, float (&x)[1][4]) {
-    float xi[1][8], y[1][4], yi[1][8], z[1][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        xi[0][i] = x[0][i];
-        xi[0][4 + i] = 0.0f;
+// Fragments of a layer with one M-tile and one K-chunk of at most 16 inputs (the LX chain),
+// loaded ahead of the layer that uses them.  FP8 keeps its 8 weight bytes in w[0..1].
+struct Frag1 {
+    u32x4_t w, wl, b;
+};
+template <int PREC, int ID>
+__device__ inline void frag1_fetch(const NetRs& net, int lane, Frag1& f) {
+    static_assert(mfma_nm(ID) == 1 && mfma_nk(ID) == 1 && !mfma_shape(ID).k32, "single-fragment layer");
+    constexpr int P = base_prec(PREC);
+    constexpr bool TWO = P == MRP_BF16 && ID >= ML_FIN2;
+    constexpr int FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
+    constexpr bool LDS = lds_weights(PREC) && ID >= lds_from(PREC);
+    constexpr int LF = mfma_frag(lds_from(PREC)), NL = kMfmaFrags - LF;
+    if constexpr (P == MRP_FP8) {
+        if constexpr (LDS) {
+            const uint2 v = *(const uint2*)(net.lw + (FRAG - LF) * 512 + lane * 8);
+            f.w = u32x4_t{v.x, v.y, 0u, 0u};
+        } else {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, FRAG * 512, 0);
+            f.w = u32x4_t{v[0], v[1], 0u, 0u};
+        }
+    } else if constexpr (LDS) {
+        f.w = *(const u32x4_t*)(net.lw + (FRAG - LF) * 1024 + lane * 16);
+        if constexpr (TWO) f.wl = *(const u32x4_t*)(net.lw + (NL + FRAG - LF) * 1024 + lane * 16);
+    } else {
+        f.w = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, FRAG * 1024, 0);
+        if constexpr (TWO) f.wl = __builtin_amdgcn_raw_buffer_load_b128(net.wlo, lane * 16, FRAG * 1024, 0);
     }
-    dense<PREC, ML_LXK + 2 * K>(net, xi, y, lc.lane, true);
+    f.b = __builtin_amdgcn_raw_buffer_load_b128(net.bias, (lane >> 4) * 16, BIAS * 4, 0);
+}
+// out = act(W in + b) for such a layer; in = the 4 rows the lane holds (K slots 0..3), the
+// same arithmetic as dense().
+template <int PREC, int ID>
+__device__ inline void frag1_apply(const NetRs& net, const Frag1& f, const float (&in)[4], float (&out)[4],
+                                   bool relu) {
+    constexpr int P = base_prec(PREC);
+    constexpr bool TWO = P == MRP_BF16 && ID >= ML_FIN2;
+    if constexpr (P == MRP_BF16 || P == MRP_BF16_1T) {
+        const u32x4_t b = u32x4_t{pack_bf16x2(in[0], in[1]), pack_bf16x2(in[2], in[3]), 0u, 0u};
+        f32x4_t acc = f32x4_t{__uint_as_float(f.b[0]), __uint_as_float(f.b[1]), __uint_as_float(f.b[2]),
+                              __uint_as_float(f.b[3])};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, f.w), __builtin_bit_cast(bf16x8_t, b),
+                                                      acc, 0, 0, 0);
+        if constexpr (TWO)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, f.wl),
+                                                          __builtin_bit_cast(bf16x8_t, b), acc, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        yi[0][i] = y[0][i];
-        yi[0][4 + i] = 0.0f;
+        for (int i = 0; i < 4; ++i) out[i] = relu ? fmaxf(acc[i], 0.0f) : acc[i];
+    } else {
+        float m = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) m = fmaxf(m, fabsf(in[s]));
+        m = q_max(m);
+        const int ea = fp8_exp_for(m);
+        const float sc = ldexpf(1.0f, ea);
+        const long b = (long)(uint64_t)pack_fp8x4(in[0] * sc, in[1] * sc, in[2] * sc, in[3] * sc);
+        const long w = (long)(((uint64_t)f.w[1] << 32) | f.w[0]);
+        const float ds = ldexpf(1.0f, -(ea + net.wexp[ID]));
+        const f32x4_t acc =
+            __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(w, b, f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float v = fmaf(acc[i], ds, __uint_as_float(f.b[i]));
+            out[i] = relu ? fmaxf(v, 0.0f) : v;
+        }
     }
-    dense<PREC, ML_LXK + 2 * K + 1>(net, yi, z, lc.lane, true);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) x[0][i] = x[0][i] + z[0][i];
+}
+// Layer ID on fragments f, then f <- fragments of layer NEXT (two layers ahead).
+template <int PREC, int ID, int NEXT>
+__device__ inline void frag1_step(const NetRs& net, int lane, Frag1& f, const float (&in)[4], float (&out)[4]) {
+    frag1_apply<PREC, ID>(net, f, in, out, true);
+    if constexpr (NEXT >= 0) frag1_fetch<PREC, NEXT>(net, lane, f);
+    __builtin_amdgcn_sched_barrier(0);
+}
+// The residual chain after LX (radiancePredict.cu:392-409): six pairs
+// x += ReLU(LX_{2k+1} ReLU(LX_{2k} x)), then LX12.  Returns LX12's output row 0.
+template <int PREC>
+__device__ inline float lx_chain(const NetRs& net, int lane, float (&x)[4]) {
+    Frag1 fa, fb;
+    frag1_fetch<PREC, ML_LXK + 0>(net, lane, fa);
+    frag1_fetch<PREC, ML_LXK + 1>(net, lane, fb);
+    __builtin_amdgcn_sched_barrier(0);
+    float y[4], z[4];
+#define MRP_LX_PAIR(K, NA, NB)                                         \
+    frag1_step<PREC, ML_LXK + 2 * (K), NA>(net, lane, fa, x, y);        \
+    frag1_step<PREC, ML_LXK + 2 * (K) + 1, NB>(net, lane, fb, y, z);    \
+    for (int i = 0; i < 4; ++i) x[i] = x[i] + z[i];
+    MRP_LX_PAIR(0, ML_LXK + 2, ML_LXK + 3)
+    MRP_LX_PAIR(1, ML_LXK + 4, ML_LXK + 5)
+    MRP_LX_PAIR(2, ML_LXK + 6, ML_LXK + 7)
+    MRP_LX_PAIR(3, ML_LXK + 8, ML_LXK + 9)
+    MRP_LX_PAIR(4, ML_LXK + 10, ML_LXK + 11)
+    MRP_LX_PAIR(5, ML_LX12, -1)
+#undef MRP_LX_PAIR
+    float p[4];
+    frag1_step<PREC, ML_LX12, -1>(net, lane, fa, x, p);
+    return p[0];  // row 0: lanes q == 0
 }
 
 // Tail SC(p) for one channel (radiancePredict.cu:375-409); returns P in lanes q == 0.
 template <int PREC>
-__device__ __attribute__((noinline)) float tail(MfmaNet nt, const float* __restrict__ W, const float* __restrict__ sW1,
-                             const LaneCtx& lc, float srp, const float (&hacc)[4], const float (&comb)[3][8],
-                             const float (&tA)[4], const float (&tB)[4]) {
+__device__ __attribute__((always_inline)) inline float tail(const NetRs& net, const float* __restrict__ W,
+                                                             const float* __restrict__ sW1, const LaneCtx& lc,
+                                                             float srp, const float (&hacc)[4],
+                                                             const float (&comb)[3][8], const float (&tA)[4],
+                                                             const float (&tB)[4]) {
     using namespace net;
-    const NetRs net = make_net_rs(nt, PREC == MRP_FP8);
     const int q = lc.q;
     const bool lo = q < 2;
     // LSEFin1 rows 4q+i: add the g, gamma, srp columns, ReLU
@@ -716,20 +821,18 @@ __device__ __attribute__((noinline)) float tail(MfmaNet nt, const float* __restr
     }
     float x[1][4];
     dense<PREC, ML_LX>(net, in3, x, lc.lane, true);
-    lx_pair<PREC, 0>(net, lc, x);
-    lx_pair<PREC, 1>(net, lc, x);
-    lx_pair<PREC, 2>(net, lc, x);
-    lx_pair<PREC, 3>(net, lc, x);
-    lx_pair<PREC, 4>(net, lc, x);
-    lx_pair<PREC, 5>(net, lc, x);
-    float xi[1][8], pout[1][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        xi[0][i] = x[0][i];
-        xi[0][4 + i] = 0.0f;
-    }
-    dense<PREC, ML_LX12>(net, xi, pout, lc.lane, true);
-    return pout[0][0];  // row 0: lanes q == 0
+    return lx_chain<PREC>(net, lc.lane, x[0]);
+}
+
+// Out-of-line tail for the second and third distinct channels.  The call also shapes the
+// kernel's register allocation: with it, the inputs it takes by reference stay in scratch
+// and the trunk keeps to the three-waves-per-SIMD register budget without spills.
+template <int PREC>
+__device__ __attribute__((noinline)) float tail_call(MfmaNet nt, const float* __restrict__ W,
+                                                     const float* __restrict__ sW1, const LaneCtx& lc, float srp,
+                                                     const float (&hacc)[4], const float (&comb)[3][8],
+                                                     const float (&tA)[4], const float (&tB)[4]) {
+    return tail<PREC>(make_net_rs(nt, base_prec(PREC) == MRP_FP8), W, sW1, lc, srp, hacc, comb, tA, tB);
 }
 
 // Full network on one 16-pixel tile; returns, in lanes q == 0, max(e^P - 1, 0) * srp per
@@ -739,7 +842,7 @@ __device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, co
                               const float* __restrict__ sW1,
                               const LaneCtx& lc, const Src& src) {
     using namespace net;
-    const NetRs net = make_net_rs(nt, PREC == MRP_FP8);
+    const NetRs net = make_net_rs(nt, base_prec(PREC) == MRP_FP8);
     const bool lo = lc.q < 2;
     float F[3][8], G[3][8], R[3][8], O[3][8], s[3];
     float hacc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -845,21 +948,20 @@ __device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, co
             tB[i] = o2[0][i];
         }
     }
-    // ---- tail per distinct colour channel (radiancePredict.cu:411-421)
-    const bool need_y = __any(lc.ok && lc.sr.y != lc.sr.x);
-    const bool need_z = __any(lc.ok && lc.sr.z != lc.sr.x && lc.sr.z != lc.sr.y);
-    const float sp0 = powf(lc.sr.x, 4.0f), sp1 = powf(lc.sr.y, 4.0f), sp2 = powf(lc.sr.z, 4.0f);
-    float P0 = tail<PREC>(nt, W, sW1, lc, sp0, hacc, comb, tA, tB);
+    // ---- tail per distinct colour channel (radiancePredict.cu:411-421).  scatterrate is
+    // uniform over the tile (one view): the first channel's tail is inlined, a second or third
+    // distinct channel runs out of line
+    float P0 = tail<PREC>(net, W, sW1, lc, lc.sp.x, hacc, comb, tA, tB);
     float P1 = P0, P2 = P0;
-    if (need_y) P1 = tail<PREC>(nt, W, sW1, lc, sp1, hacc, comb, tA, tB);
-    if (need_z) P2 = tail<PREC>(nt, W, sW1, lc, sp2, hacc, comb, tA, tB);
+    if (lc.need_y) P1 = tail_call<PREC>(nt, W, sW1, lc, lc.sp.y, hacc, comb, tA, tB);
+    if (lc.need_z) P2 = tail_call<PREC>(nt, W, sW1, lc, lc.sp.z, hacc, comb, tA, tB);
     if (lc.sr.y == lc.sr.x) P1 = P0;
     if (lc.sr.z == lc.sr.x) P2 = P0;
     else if (lc.sr.z == lc.sr.y) P2 = P1;
     v3 r;
-    r.x = fmaxf(expf(P0) - 1.0f, 0.0f) * sp0;
-    r.y = fmaxf(expf(P1) - 1.0f, 0.0f) * sp1;
-    r.z = fmaxf(expf(P2) - 1.0f, 0.0f) * sp2;
+    r.x = fmaxf(expf(P0) - 1.0f, 0.0f) * lc.sp.x;
+    r.y = fmaxf(expf(P1) - 1.0f, 0.0f) * lc.sp.y;
+    r.z = fmaxf(expf(P2) - 1.0f, 0.0f) * lc.sp.z;
     return r;
 }
 
@@ -875,8 +977,26 @@ __device__ inline void load_shared(float* sW1, float* sG, StencilEntry* sst, con
     __syncthreads();
 }
 
+// Copies the fragments of layers >= lds_from into LDS (variants with kLdsW).
+template <int PREC>
+__device__ inline void stage_weights(const MfmaNet& net, u32x4_t* sWf) {
+    constexpr int LF = mfma_frag(lds_from(PREC));
+    constexpr int n = lds_weight_bytes(PREC) / 16;
+    if constexpr (base_prec(PREC) == MRP_FP8) {
+        const u32x4_t* src = (const u32x4_t*)(net.fp8 + LF * 64);  // 64 lanes x 8 bytes per fragment
+        for (int i = threadIdx.x; i < n; i += blockDim.x) sWf[i] = src[i];
+    } else {
+        constexpr int nh = (kMfmaFrags - LF) * 64;  // hi fragments: 64 lanes x 16 bytes
+        const u32x4_t* hi = (const u32x4_t*)(net.bf16 + LF * 64);
+        const u32x4_t* lo = (const u32x4_t*)(net.bf16_lo + LF * 64);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) sWf[i] = i < nh ? hi[i] : lo[i - nh];
+    }
+}
+
+// PREC: a precision id, plus kLdsW for the variant with LDS-resident weights (one 768-thread
+// workgroup per CU); FAST: the fast gather of dev_gather_fast.hpp.
 template <int PREC, bool FAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_mlp_mfma_active(MfmaNet net, const DevVolume* __restrict__ vols,
+__global__ __launch_bounds__(lds_weights(PREC) ? 768 : 256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_mlp_mfma_active(MfmaNet net, const DevVolume* __restrict__ vols,
                                                          const ViewSlot* __restrict__ slots,
                                                          const StencilEntry* __restrict__ stencil,
                                                          const FastStencil* __restrict__ fast,
@@ -887,6 +1007,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     __shared__ float sG[kGates * kGateFloats];
     __shared__ StencilEntry sst[FAST ? 1 : kPC];
     __shared__ FastStencil sfs[FAST ? kPC : 1];
+    __shared__ u32x4_t sWf[lds_weights(PREC) ? lds_weight_bytes(PREC) / 16 : 1];
+    if constexpr (lds_weights(PREC)) stage_weights<PREC>(net, sWf);
     if (FAST) {
         for (int i = threadIdx.x; i < kPC; i += blockDim.x) sfs[i] = fast[i];
         load_shared(sW1, sG, nullptr, W, nullptr);
@@ -908,26 +1030,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         const ActiveRec r = recs[idx];
         lc.ok = r.pixel >= 0;
         const ViewSlot& sl = slots[view];
-        const ViewDev vw = sl.vw;
+        const ViewDev& vw = sl.vw;
+        const ViewConst& vc = sl.vc;
         PixelState ps;
         ps.pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
         ps.X = mk3(r.X[0], r.X[1], r.X[2]);
         ps.Y = mk3(r.Y[0], r.Y[1], r.Y[2]);
         ps.Z = mk3(r.Z[0], r.Z[1], r.Z[2]);
-        const v3 light = normalize3(vw.light_dir);
-        ps.L = light;
-        ps.LX = normalize3(light);
+        ps.L = vc.L;
+        ps.LX = vc.LX;
         ps.dist = r.dist;
         ps.active = 1;
         ps.taps = 0;
         lc.g = vw.g;
         lc.gamma = gamma_of(ps);
-        lc.sr = vw.scatter_rate / 1.001f;
+        lc.sr = vc.sr;
+        lc.sp = vc.sp;
+        lc.need_y = vc.need_y != 0;
+        lc.need_z = vc.need_z != 0;
         MfmaNet nt = net;
         nt.bf16 = opaque(net.bf16);
         nt.bf16_lo = opaque(net.bf16_lo);
         nt.fp8 = opaque(net.fp8);
         nt.bias = opaque(net.bias);
+        nt.lds = opaque_v(sWf);
         v3 rgb;
         if (FAST) {
             const DevVolume& vol = vols[sl.vol];
@@ -938,8 +1064,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             src.fv.rows = make_rsrc(lut_rows, (long long)kPC * kLutRow);
             src.fv.L = ps.L;
             src.fv.LX = ps.LX;
-            src.fv.invLX = inv3(ps.LX);
-            src.fv.alpha64 = vw.alpha / 64.0f;
+            src.fv.invLX = vc.invLX;
+            src.fv.alpha64 = vc.alpha64;
             src.px.pos = ps.pos;
             src.px.X = ps.X;
             src.px.Y = ps.Y;
@@ -949,7 +1075,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             src.px.az = dot3(ps.LX, ps.Z);
             src.px.uc1 = fmaf(dot3(ps.X, ps.LX), 0.5f, 0.5f);
             src.px.uc2 = fmaf(dot3(ps.LX, ps.LX), 0.5f, 0.5f);
-            src.px.xh_centre = logf(henyey_greenstein(dot3(ps.X, ps.L), vw.g) + 1.0f);
+            src.px.xh_centre = log_ge1(hg_fast(dot3(ps.X, ps.L), vw.g) + 1.0f);
             rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sG), opaque_v(sW1), lc, src);
         } else {
             GatherSrc src;
@@ -974,6 +1100,7 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
     __shared__ float sW1[1200];
     __shared__ float sG[kGates * kGateFloats];
     load_shared(sW1, sG, nullptr, W, nullptr);
+    const v3 sp = mk3(powf(sr.x, 4.0f), powf(sr.y, 4.0f), powf(sr.z, 4.0f));
     const int lane = threadIdx.x & 63;
     const int wave = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * (blockDim.x >> 6);
@@ -989,6 +1116,9 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
         lc.g = g;
         lc.gamma = lc.ok ? src.row[3 * kPC] : 0.0f;
         lc.sr = sr;
+        lc.sp = sp;
+        lc.need_y = sr.y != sr.x;
+        lc.need_z = sr.z != sr.x && sr.z != sr.y;
         MfmaNet nt = net;
         nt.bf16 = opaque(net.bf16);
         nt.bf16_lo = opaque(net.bf16_lo);
@@ -1003,30 +1133,39 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
     }
 }
 
+// grid: workgroups of 256 threads (L2 weights); the LDS-weight variants take `grid` as the
+// number of 768-thread workgroups (one per CU).
 template <int PREC>
 static void launch_active_prec(bool fast, const MfmaNet& net, const DevVolume* vols, const ViewSlot* slots,
                                const StencilEntry* stencil, const FastStencil* fst, const float* rows,
                                const float* W, const ActiveRec* recs, const unsigned* counter, int grid,
                                hipStream_t s) {
+    const int block = lds_weights(PREC) ? 768 : 256;
     if (fast)
-        hipLaunchKernelGGL((k_mlp_mfma_active<PREC, true>), dim3(grid), dim3(256), 0, s, net, vols, slots, stencil, fst,
-                           rows, W, recs, counter);
+        hipLaunchKernelGGL((k_mlp_mfma_active<PREC, true>), dim3(grid), dim3(block), 0, s, net, vols, slots, stencil,
+                           fst, rows, W, recs, counter);
     else
-        hipLaunchKernelGGL((k_mlp_mfma_active<PREC, false>), dim3(grid), dim3(256), 0, s, net, vols, slots, stencil,
+        hipLaunchKernelGGL((k_mlp_mfma_active<PREC, false>), dim3(grid), dim3(block), 0, s, net, vols, slots, stencil,
                            fst, rows, W, recs, counter);
 }
 
-hipError_t launch_mlp_mfma_active(int precision, bool fast, const MfmaNet& net, const DevVolume* vols,
+hipError_t launch_mlp_mfma_active(int precision, bool fast, bool lds_w, const MfmaNet& net, const DevVolume* vols,
                                   const ViewSlot* slots, const StencilEntry* stencil, const void* fast_stencil,
                                   const float* lut_rows, const float* W, const ActiveRec* recs,
                                   const unsigned* counter, int grid, hipStream_t s) {
     const FastStencil* fst = (const FastStencil*)fast_stencil;
-    if (precision == MRP_BF16)
-        launch_active_prec<MRP_BF16>(fast, net, vols, slots, stencil, fst, lut_rows, W, recs, counter, grid, s);
-    else if (precision == MRP_BF16_1T)
-        launch_active_prec<MRP_BF16_1T>(fast, net, vols, slots, stencil, fst, lut_rows, W, recs, counter, grid, s);
-    else
-        launch_active_prec<MRP_FP8>(fast, net, vols, slots, stencil, fst, lut_rows, W, recs, counter, grid, s);
+#define MRP_LAUNCH(P) launch_active_prec<P>(fast, net, vols, slots, stencil, fst, lut_rows, W, recs, counter, grid, s)
+    if (precision == MRP_BF16) {
+        if (lds_w) MRP_LAUNCH(MRP_BF16 | kLdsW);
+        else MRP_LAUNCH(MRP_BF16);
+    } else if (precision == MRP_BF16_1T) {
+        if (lds_w) MRP_LAUNCH(MRP_BF16_1T | kLdsW);
+        else MRP_LAUNCH(MRP_BF16_1T);
+    } else {
+        if (lds_w) MRP_LAUNCH(MRP_FP8 | kLdsW);
+        else MRP_LAUNCH(MRP_FP8);
+    }
+#undef MRP_LAUNCH
     return hipGetLastError();
 }
 

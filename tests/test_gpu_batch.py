@@ -145,8 +145,34 @@ def test_autotune_keeps_results(bctx, volumes):
     bctx.prepare_views(volumes, views)
     w, ms = bctx.autotune(volumes, views, outs, M.BF16)
     assert w in (4, 8, 12, 16, 24, 32) and ms > 0
-    assert bctx.get_option(M.OPT_MFMA_WAVES_PER_CU) == w
+    lds = bctx.get_option(M.OPT_LDS_WEIGHTS)
+    assert lds in (0, 1)
+    if not lds:
+        assert bctx.get_option(M.OPT_MFMA_WAVES_PER_CU) == w
     got = _render(bctx, volumes, views, M.BF16)
     for i in range(len(views)):
         assert np.array_equal(got[i], ref[i]), i
     bctx.set_option(M.OPT_MFMA_WAVES_PER_CU, 12)
+    bctx.set_option(M.OPT_LDS_WEIGHTS, 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["bf16", "fp8", "bf16_1t"])
+def test_lds_weights_equal_l2_weights(bctx, volumes, prec):
+    """The LDS-resident-weight kernel (one 768-thread workgroup per CU) and the L1/L2-weight
+    kernel evaluate the same fragments in the same order: identical images."""
+    import mrpnn_amd as M
+    views = _views()
+    p = M.PRECISIONS[prec]
+    try:
+        bctx.set_option(M.OPT_LDS_WEIGHTS, 1)
+        a = _render(bctx, volumes, views, p)
+        bctx.set_option(M.OPT_LDS_WEIGHTS, 0)
+        for w in (4, 12):
+            bctx.set_option(M.OPT_MFMA_WAVES_PER_CU, w)
+            b = _render(bctx, volumes, views, p)
+            for i in range(len(views)):
+                assert np.array_equal(a[i], b[i]), (prec, w, i)
+    finally:
+        bctx.set_option(M.OPT_LDS_WEIGHTS, 1)
+        bctx.set_option(M.OPT_MFMA_WAVES_PER_CU, 12)
