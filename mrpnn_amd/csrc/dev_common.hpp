@@ -169,9 +169,59 @@ __device__ inline float sample_lut_exact(const float* __restrict__ lut, float u,
     return lerp_ref(a, b, wy);
 }
 
+// Full-resolution zero-border trilinear with the value of sample_border_exact(v.grid, N, p+0.5),
+// bit for bit (same coordinates, weights and lerp order), fetched as four (z, z+1) pairs:
+// 64-bit global loads at clamped in-grid indices, one 64-bit slab address per x tap and
+// 32-bit offsets inside a slab, then the border taps are zeroed by selects (no branches).
+// Needs N >= 2.
+typedef float f32x2u_t __attribute__((ext_vector_type(2), aligned(4)));
+typedef __attribute__((address_space(1))) const f32x2u_t* gpair_ptr;
+struct FullTap {  // integer corner and weights of one sample
+    int x, y, z;
+    v3 w;
+};
+__device__ inline FullTap full_tap(int N, v3 p) {
+    const v3 uv = p + 0.5f;
+    const v3 pos = uv * (float)N - 0.5f;
+    FullTap t;
+    t.x = (int)floorf(pos.x);
+    t.y = (int)floorf(pos.y);
+    t.z = (int)floorf(pos.z);
+    t.w = pos - mk3((float)t.x, (float)t.y, (float)t.z);
+    return t;
+}
+__device__ inline float density_full_at(const DevVolume& v, const FullTap& t) {
+    const int N = v.N;
+    const int z0 = min(max(t.z, 0), N - 2);  // pair (z0, z0 + 1) lies inside the row
+    const bool zlo = t.z >= 0, zhi = t.z + 1 < N;
+    // t.z = -1: wanted (0, v[0]) = (0, pair.x); t.z = N-1: wanted (v[N-1], 0) = (pair.y, 0)
+    const bool shift = t.z == N - 1;
+    float c[2][2][2];
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+        const int xi = t.x + dx;
+        const bool xok = (unsigned)xi < (unsigned)N;
+        const float* slab = v.grid + (size_t)min(max(xi, 0), N - 1) * N * N;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const int yi = t.y + dy;
+            const bool ok = xok && (unsigned)yi < (unsigned)N;
+            const int off = min(max(yi, 0), N - 1) * N + z0;
+            const f32x2u_t pr = *(gpair_ptr)(slab + off);
+            const float a = shift ? pr.y : pr.x;
+            c[dx][dy][0] = (ok && zlo) ? a : 0.0f;
+            c[dx][dy][1] = (ok && zhi) ? (t.z < 0 ? pr.x : pr.y) : 0.0f;
+        }
+    }
+    const v3 w = t.w;
+    return lerp_ref(lerp_ref(lerp_ref(c[0][0][0], c[0][0][1], w.z), lerp_ref(c[0][1][0], c[0][1][1], w.z), w.y),
+                    lerp_ref(lerp_ref(c[1][0][0], c[1][0][1], w.z), lerp_ref(c[1][1][0], c[1][1][1], w.z), w.y),
+                    w.x);
+}
+
 // Density() full-res border (render.cuh:34-38); MipDensityDynamic (renderResources.cu:32-57);
 // MipTrDynamic (:59-82); ShadowTerm_TRTex(...).x (:84-102).
-__device__ inline float density_full(const DevVolume& v, v3 p) { return sample_border_exact(v.grid, v.N, p + 0.5f); }
+__device__ inline float density_full(const DevVolume& v, v3 p) { return density_full_at(v, full_tap(v.N, p)); }
 __device__ inline float mip_density(const DevVolume& v, int mip, v3 p) {
     return sample_border_exact(v.mips[mip], 256 >> mip, p + 0.5f);
 }
@@ -196,12 +246,22 @@ __device__ inline float shadow_term_tr(const DevVolume& v, v3 ori, v3 light, int
 // reads texels x0, x0+1 in [8b-1, 8b+7].  occ[b] = 1 when every texel of that range (inside
 // the grid) is exactly zero, so such a sample is exactly 0 and its fetch can be skipped
 // without changing any result.
-__device__ inline bool sample_in_empty_brick(const DevVolume& v, v3 p) {
-    const v3 c = (p + 0.5f) * (float)v.N - 0.5f;  // as sample_border_exact
-    const int bx = ((int)floorf(c.x) + 1) >> 3, by = ((int)floorf(c.y) + 1) >> 3, bz = ((int)floorf(c.z) + 1) >> 3;
+// The march keeps the last brick it looked up (key, emptiness) in registers: a step stays in
+// one brick for several samples, so most steps need no occupancy load.
+struct BrickCache {
+    int key = -2;  // -1: outside the brick grid
+    bool empty = false;
+};
+__device__ inline bool tap_in_empty_brick(const DevVolume& v, const FullTap& t, BrickCache& bc) {
     const int n = v.occ_n;
-    if ((unsigned)bx >= (unsigned)n || (unsigned)by >= (unsigned)n || (unsigned)bz >= (unsigned)n) return false;
-    return v.occ[((size_t)bx * n + by) * n + bz] != 0;
+    const int bx = (t.x + 1) >> 3, by = (t.y + 1) >> 3, bz = (t.z + 1) >> 3;
+    const bool in = (unsigned)bx < (unsigned)n && (unsigned)by < (unsigned)n && (unsigned)bz < (unsigned)n;
+    const int key = in ? (bx * n + by) * n + bz : -1;
+    if (key != bc.key) {
+        bc.key = key;
+        bc.empty = in && *(const __attribute__((address_space(1))) uint8_t*)(v.occ + key) != 0;
+    }
+    return bc.empty;
 }
 
 __device__ inline bool estimated_hit(const DevVolume& v, float alpha, v3 pos, v3 dir, float dis, v3* next, int* taps) {
@@ -212,6 +272,7 @@ __device__ inline bool estimated_hit(const DevVolume& v, float alpha, v3 pos, v3
     float inv_res = 1.0f / (float)v.N;
     int loop = 0;
     float last_dens = 0.0f;
+    BrickCache bc;
     while (loop++ < 10000) {
         if (t > dis) {
             if (last_non_empty_t != 0.0f) { t = last_non_empty_t; break; }
@@ -219,7 +280,8 @@ __device__ inline bool estimated_hit(const DevVolume& v, float alpha, v3 pos, v3
             return false;
         }
         const v3 sp = pos + (dir * t);
-        float d = (v.occ && sample_in_empty_brick(v, sp)) ? 0.0f : density_full(v, sp);
+        const FullTap tp = full_tap(v.N, sp);
+        float d = (v.occ && tap_in_empty_brick(v, tp, bc)) ? 0.0f : density_full_at(v, tp);
         ++*taps;
         if (d > 0.0f) {
             float S = d * alpha;

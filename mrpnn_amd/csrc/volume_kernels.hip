@@ -133,7 +133,7 @@ struct TrArgs {
     long long start[9];  // flattened texel offsets of each level
     v3 light;
 };
-__global__ __launch_bounds__(256) void k_tr_bake(TrArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_tr_bake(TrArgs a) {
     const long long total = a.start[8];
     for (long long g = (long long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long long)gridDim.x * 256) {
         int m = 0;
@@ -152,14 +152,21 @@ __global__ __launch_bounds__(256) void k_tr_bake(TrArgs a) {
         float sum = 0.0f;
         const int sres = 256 >> (m + 1);
         if (a.pmips && m + 1 < 8) {
-            // zero-padded copy of the density mip: no per-tap border tests
+            // zero-padded copy of the density mip: no per-tap border tests.  Every sample lies
+            // on the segment from the texel centre to the box exit, so its padded coordinate
+            // is in [0.5, sres + 0.5] and both taps per axis are inside the padded level:
+            // the zero-border clamp of tri_border_lvl never changes a coordinate and is left out
             const unsigned base = (unsigned)pmip_offset(m + 1) * 4u;
             const float fres = (float)sres;
+            const float R = fres + 3.0f;
+            const unsigned Ru = (unsigned)R;
+            const unsigned R4 = Ru * 4u, RR4 = Ru * Ru * 4u;
             const BufRsrc rs = make_rsrc(a.pmips, kPadMipFloats);
-#pragma unroll 8
+#pragma unroll 4
             for (int i = 0; i < 128; ++i) {
                 lp = lp + a.light * st;
-                sum = sum + tri_border_lvl(rs, base, fres, lp);
+                const v3 c = padded_coord(lp, fres);
+                sum = sum + tri_taps(rs, base, R, R4, RR4, c.x, c.y, c.z);
             }
         } else {
             const float* src = a.mips[m + 1];
