@@ -133,7 +133,7 @@ struct TrArgs {
     long long start[9];  // flattened texel offsets of each level
     v3 light;
 };
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_tr_bake(TrArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_tr_bake(TrArgs a) {
     const long long total = a.start[8];
     for (long long g = (long long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long long)gridDim.x * 256) {
         int m = 0;
@@ -162,11 +162,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             const unsigned Ru = (unsigned)R;
             const unsigned R4 = Ru * 4u, RR4 = Ru * Ru * 4u;
             const BufRsrc rs = make_rsrc(a.pmips, kPadMipFloats);
-#pragma unroll 4
-            for (int i = 0; i < 128; ++i) {
-                lp = lp + a.light * st;
-                const v3 c = padded_coord(lp, fres);
-                sum = sum + tri_taps(rs, base, R, R4, RR4, c.x, c.y, c.z);
+            // four samples per round: positions (the reference's running sum), then all 16
+            // tap-pair loads, then the blends, added to the sum in step order
+            for (int i = 0; i < 128; i += 4) {
+                float wx[4], wy[4], wz[4];
+                int o[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    lp = lp + a.light * st;
+                    const v3 c = padded_coord(lp, fres);
+                    wx[k] = __builtin_amdgcn_fractf(c.x);
+                    wy[k] = __builtin_amdgcn_fractf(c.y);
+                    wz[k] = __builtin_amdgcn_fractf(c.z);
+                    const float lin = fmaf(fmaf(floorf(c.x), R, floorf(c.y)), R, floorf(c.z));
+                    o[k] = (int)((unsigned)lin * 4u + base);
+                }
+                f32x2_t t[4][4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    t[k][0] = ld_pair(rs, o[k]);
+                    t[k][1] = ld_pair(rs, o[k] + (int)R4);
+                    t[k][2] = ld_pair(rs, o[k] + (int)RR4);
+                    t[k][3] = ld_pair(rs, o[k] + (int)(RR4 + R4));
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float v = flerp(flerp(flerp(t[k][0].x, t[k][0].y, wz[k]), flerp(t[k][1].x, t[k][1].y, wz[k]), wy[k]),
+                                          flerp(flerp(t[k][2].x, t[k][2].y, wz[k]), flerp(t[k][3].x, t[k][3].y, wz[k]), wy[k]),
+                                          wx[k]);
+                    sum = sum + v;
+                }
             }
         } else {
             const float* src = a.mips[m + 1];
