@@ -90,18 +90,32 @@ __device__ inline float clampf(float x, float lo, float hi) { return __builtin_a
 // ln(x) for x >= 1 (no denormal inputs): hardware log2 times ln 2.
 __device__ inline float log_ge1(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
 
-// Trilinear blend of a padded level at the non-negative continuous coordinate (cx, cy, cz)
-// (already clamped into the padding): corner = floor, weights = fract.  The linear index
-// (x*R + y)*R + z stays below 2^24, so it is formed exactly with two float FMAs.
-__device__ inline float tri_taps(BufRsrc r, unsigned base, float R, unsigned R4, unsigned RR4, float cx, float cy,
-                                 float cz) {
-    const float wx = __builtin_amdgcn_fractf(cx), wy = __builtin_amdgcn_fractf(cy), wz = __builtin_amdgcn_fractf(cz);
-    const float lin = fmaf(fmaf(floorf(cx), R, floorf(cy)), R, floorf(cz));
-    const int o = (int)((unsigned)lin * 4u + base);
+// Corner and weights of a trilinear blend on a padded level at the non-negative continuous
+// coordinate (cx, cy, cz) (already inside the padding): corner = floor, weights = fract.  The
+// linear index (x*R + y)*R + z stays below 2^24, so it is formed exactly with two float FMAs;
+// o4 = 4 * index (bytes, without the level base).  Levels of equal resolution share it.
+struct TapGeo {
+    unsigned o4;
+    float wx, wy, wz;
+};
+__device__ inline TapGeo tap_geo(float R, float cx, float cy, float cz) {
+    TapGeo g;
+    g.wx = __builtin_amdgcn_fractf(cx);
+    g.wy = __builtin_amdgcn_fractf(cy);
+    g.wz = __builtin_amdgcn_fractf(cz);
+    g.o4 = (unsigned)fmaf(fmaf(floorf(cx), R, floorf(cy)), R, floorf(cz)) * 4u;
+    return g;
+}
+__device__ inline float tri_geo(BufRsrc r, unsigned base, unsigned R4, unsigned RR4, const TapGeo& g) {
+    const int o = (int)(g.o4 + base);
     const f32x2_t c00 = ld_pair(r, o), c01 = ld_pair(r, o + (int)R4);
     const f32x2_t c10 = ld_pair(r, o + (int)RR4), c11 = ld_pair(r, o + (int)(RR4 + R4));
-    return flerp(flerp(flerp(c00.x, c00.y, wz), flerp(c01.x, c01.y, wz), wy),
-                 flerp(flerp(c10.x, c10.y, wz), flerp(c11.x, c11.y, wz), wy), wx);
+    return flerp(flerp(flerp(c00.x, c00.y, g.wz), flerp(c01.x, c01.y, g.wz), g.wy),
+                 flerp(flerp(c10.x, c10.y, g.wz), flerp(c11.x, c11.y, g.wz), g.wy), g.wx);
+}
+__device__ inline float tri_taps(BufRsrc r, unsigned base, float R, unsigned R4, unsigned RR4, float cx, float cy,
+                                 float cz) {
+    return tri_geo(r, base, R4, RR4, tap_geo(R, cx, cy, cz));
 }
 // Padded continuous coordinate of p on a level of resolution res: data coordinate
 // (p + 0.5)*res - 0.5, plus one for the padding.
@@ -157,24 +171,40 @@ __device__ inline float lut_bilinear(const float* __restrict__ lut, float u, flo
                  wy);
 }
 
-// ShadowTerm_TRTex(...).x (renderResources.cu:84-102) on the padded TR level.  c_same: the
-// padded density coordinate of p when the TR level has the density level's resolution
-// (stencil mips >= 1), reused for points inside the box.
-__device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3 p, v3 c_same, bool same) {
-    if (p.x < -0.5f || p.y < -0.5f || p.z < -0.5f || p.x > 0.5f || p.y > 0.5f || p.z > 0.5f) {
-        // RayBoxOffset(p, LX) (vector.cu:46-63) with inv(LX) precomputed
-        const v3 to_axil = (-p) * fv.invLX;
-        const v3 a2f = mk3(0.4999f, 0.4999f, 0.4999f) * fv.invLX;
-        const v3 d0 = to_axil + a2f, d1 = to_axil - a2f;
-        const v3 tmin = vmin3(d0, d1), tmax = vmax3(d0, d1);
-        const float tmi = fmaxf(tmin.x, fmaxf(tmin.y, tmin.z));
-        const float tma = fminf(tmax.x, fminf(tmax.y, tmax.z));
-        if (!(tma >= tmi)) return 1.0f;
-        const float off = fmaxf(tmi, 0.0f);
-        p = mk3(fmaf(fv.LX.x, off, p.x), fmaf(fv.LX.y, off, p.y), fmaf(fv.LX.z, off, p.z));
-        return tri_clamp_c(fv.ptr, s, padded_coord(p, s.tres));
+// ShadowTerm_TRTex(...).x (renderResources.cu:84-102) on the padded TR level (replicated
+// edges).  g_same: the density blend's corner and weights at p when the TR level has the
+// density level's resolution (stencil mips >= 1); for p inside the box that coordinate lies in
+// [0.5, res + 0.5], where the replicated edges give the clamp-addressed value, so the TR
+// blend reuses it.  Other points take their own clamped coordinate (the box entry along the
+// light for points outside); one blend serves every lane.
+// Outside the box [-0.5, 0.5]^3 (the test of renderResources.cu:86-87): one max3 of |p|.
+__device__ inline bool outside_box(v3 p) { return fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))) > 0.5f; }
+__device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3 p, const TapGeo& g_same,
+                                    bool same, bool outside) {
+    TapGeo g = g_same;
+    bool miss = false;
+    // wave-uniform test first: when every lane of a `same` point is inside the box (the common
+    // case) the own-coordinate path is skipped as a whole
+    if (!same || __builtin_amdgcn_ballot_w64(outside) != 0)
+    if (outside || !same) {
+        v3 q = p;
+        if (outside) {
+            // RayBoxOffset(p, LX) (vector.cu:46-63) with inv(LX) precomputed
+            const v3 to_axil = (-p) * fv.invLX;
+            const v3 a2f = mk3(0.4999f, 0.4999f, 0.4999f) * fv.invLX;
+            const v3 d0 = to_axil + a2f, d1 = to_axil - a2f;
+            const v3 tmin = vmin3(d0, d1), tmax = vmax3(d0, d1);
+            const float tmi = fmaxf(tmin.x, fmaxf(tmin.y, tmin.z));
+            const float tma = fminf(tmax.x, fminf(tmax.y, tmax.z));
+            miss = !(tma >= tmi);
+            const float off = fmaxf(tmi, 0.0f);
+            q = mk3(fmaf(fv.LX.x, off, p.x), fmaf(fv.LX.y, off, p.y), fmaf(fv.LX.z, off, p.z));
+        }
+        const v3 c = padded_coord(q, s.tres);
+        g = tap_geo(s.tR, clampf(c.x, 1.0f, s.tres), clampf(c.y, 1.0f, s.tres), clampf(c.z, 1.0f, s.tres));
     }
-    return tri_clamp_c(fv.ptr, s, same ? c_same : padded_coord(p, s.tres));
+    const float T = tri_geo(fv.ptr, s.tbase, s.tR4, s.tRR4, g);
+    return miss ? 1.0f : T;
 }
 
 // (X_Val, X_Val_Hg, X_Val_Sub) of one stencil point.  LIGHT: points 160..191 (along the
@@ -199,9 +229,17 @@ __device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const
         xh = centre ? px.xh_centre : log_ge1(fmaf(s.hg0, h1, 1.0f));
     }
     const v3 c = padded_coord(p, s.dres);
-    const float rho = tri_border_c(fv.pmips, s, c);
+    const bool outside = outside_box(p);
+    // inside the box the padded coordinate is in [0.5, res + 0.5]: no clamp needed; lanes
+    // outside clamp to the zero border [0, res + 1] (skipped when no lane of the wave is outside)
+    TapGeo g = tap_geo(s.dR, c.x, c.y, c.z);
+    if (__builtin_amdgcn_ballot_w64(outside) != 0 && outside) {
+        const float hi = s.dres + 1.0f;
+        g = tap_geo(s.dR, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi), clampf(c.z, 0.0f, hi));
+    }
+    const float rho = tri_geo(fv.pmips, s.dbase, s.dR4, s.dRR4, g);
     const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
-    const float xs = shadow_fast(fv, s, p, c, same);
+    const float xs = shadow_fast(fv, s, p, g, same, outside);
     return mk3(xv, xh, xs);
 }
 

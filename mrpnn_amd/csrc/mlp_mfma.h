@@ -39,8 +39,8 @@ constexpr int kLdsW = 16;
 constexpr int base_prec(int v) { return v & (kLdsW - 1); }
 constexpr bool lds_weights(int v) { return (v & kLdsW) != 0; }
 // first LDS-resident layer: the whole network for FP8 (60.5 KB) and single-term BF16
-// (121 KB); the tail (hi and lo terms, 134 KB) for two-term BF16
-constexpr int lds_from(int v) { return base_prec(v) == 1 ? 42 /* ML_FIN2 */ : 0; }
+// (121 KB); for two-term BF16 the tail from LC0 on (hi and lo terms, 132 KB, and its biases)
+constexpr int lds_from(int v) { return base_prec(v) == 1 ? 43 /* ML_LC0 */ : 0; }
 
 // Compile-time shape of every layer: rows, inputs, first destination row inside the padded
 // 16-row M-tiles, and whether K comes in 32-wide chunks (else K <= 16, one chunk whose
@@ -83,13 +83,24 @@ constexpr int mfma_bias(int id) {
 }
 constexpr int kMfmaFrags = mfma_frag(ML_COUNT);
 constexpr int kMfmaBias = mfma_bias(ML_COUNT);
-static_assert(ML_FIN2 == 42, "lds_from names ML_FIN2");
+static_assert(ML_LC0 == 43, "lds_from names ML_LC0");
 // bytes of the LDS weight copy of a variant: fragments lds_from.. (1 KB each for bf16, 512 B
-// for fp8), then for two-term BF16 the same layers' lo terms
-constexpr int lds_weight_bytes(int v) {
+// for fp8), then for two-term BF16 the same layers' lo terms, then their biases
+constexpr int lds_frag_bytes(int v) {
     return !lds_weights(v) ? 0
            : base_prec(v) == 2 ? (kMfmaFrags - mfma_frag(lds_from(v))) * 512
            : (kMfmaFrags - mfma_frag(lds_from(v))) * 1024 * (base_prec(v) == 1 ? 2 : 1);
+}
+// The biases of those layers follow when the whole workgroup allocation (fragments, biases and
+// the kernel's other 24,384 bytes) stays within kLdsBudget: a workgroup asking for 163,392 of
+// the CU's 163,840 bytes measured at about half speed (as if only every other CU could host
+// it), one with 161,600 bytes at full speed.
+constexpr int kLdsOther = 24384, kLdsBudget = 161792;
+constexpr bool lds_bias(int v) {
+    return lds_weights(v) && lds_frag_bytes(v) + (kMfmaBias - mfma_bias(lds_from(v))) * 4 + kLdsOther <= kLdsBudget;
+}
+constexpr int lds_weight_bytes(int v) {
+    return lds_frag_bytes(v) + (lds_bias(v) ? (kMfmaBias - mfma_bias(lds_from(v))) * 4 : 0);
 }
 
 struct MfmaNet {

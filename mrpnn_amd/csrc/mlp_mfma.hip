@@ -261,7 +261,9 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
     // weights of this layer from the LDS copy: fragment f at (f - LF) * FB + lane * (FB / 64),
     // lo terms after the NL hi fragments
     constexpr bool LDS = lds_weights(PREC) && ID >= lds_from(PREC);
-    constexpr int LF = mfma_frag(lds_from(PREC)), NL = kMfmaFrags - LF;
+    // (constexpr variables: these table walks must fold at compile time)
+    constexpr int LF = mfma_frag(lds_from(PREC)), NL = kMfmaFrags - LF, BL = mfma_bias(lds_from(PREC));
+    constexpr int LFB = lds_frag_bytes(PREC);
     const int q = lane >> 4;
     // Fragments of M-tile mt+1 are requested before the MFMAs of M-tile mt (two buffers), so
     // the weight latency of a multi-tile layer is paid once.
@@ -286,8 +288,12 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
                 }
             }
             bb[buf] = u32x4_t{0u, 0u, 0u, 0u};
-            if constexpr (mfma_has_bias(ID))
-                bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
+            if constexpr (mfma_has_bias(ID)) {
+                if constexpr (LDS && lds_bias(PREC))
+                    bb[buf] = *(const u32x4_t*)(net.lw + LFB + (BIAS - BL + 16 * mt) * 4 + q * 16);
+                else
+                    bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
+            }
         };
         fetch(0, 0);
         // the first M-tile's loads (both weight terms and the bias) are issued together:
@@ -341,8 +347,12 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
                         long, __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, (FRAG + mt * NK + kc) * 512, 0));
             }
             bb[buf] = u32x4_t{0u, 0u, 0u, 0u};
-            if constexpr (mfma_has_bias(ID))
-                bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
+            if constexpr (mfma_has_bias(ID)) {
+                if constexpr (LDS && lds_bias(PREC))
+                    bb[buf] = *(const u32x4_t*)(net.lw + LFB + (BIAS - BL + 16 * mt) * 4 + q * 16);
+                else
+                    bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
+            }
         };
         fetch(0, 0);
         if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
@@ -641,7 +651,8 @@ __device__ inline void frag1_fetch(const NetRs& net, int lane, Frag1& f) {
     constexpr bool TWO = P == MRP_BF16 && ID >= ML_FIN2;
     constexpr int FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
     constexpr bool LDS = lds_weights(PREC) && ID >= lds_from(PREC);
-    constexpr int LF = mfma_frag(lds_from(PREC)), NL = kMfmaFrags - LF;
+    constexpr int LF = mfma_frag(lds_from(PREC)), NL = kMfmaFrags - LF, BL = mfma_bias(lds_from(PREC));
+    constexpr int LFB = lds_frag_bytes(PREC);
     if constexpr (P == MRP_FP8) {
         if constexpr (LDS) {
             const uint2 v = *(const uint2*)(net.lw + (FRAG - LF) * 512 + lane * 8);
@@ -657,7 +668,10 @@ __device__ inline void frag1_fetch(const NetRs& net, int lane, Frag1& f) {
         f.w = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, FRAG * 1024, 0);
         if constexpr (TWO) f.wl = __builtin_amdgcn_raw_buffer_load_b128(net.wlo, lane * 16, FRAG * 1024, 0);
     }
-    f.b = __builtin_amdgcn_raw_buffer_load_b128(net.bias, (lane >> 4) * 16, BIAS * 4, 0);
+    if constexpr (LDS && lds_bias(PREC))
+        f.b = *(const u32x4_t*)(net.lw + LFB + (BIAS - BL) * 4 + (lane >> 4) * 16);
+    else
+        f.b = __builtin_amdgcn_raw_buffer_load_b128(net.bias, (lane >> 4) * 16, BIAS * 4, 0);
 }
 // out = act(W in + b) for such a layer; in = the 4 rows the lane holds (K slots 0..3), the
 // same arithmetic as dense().
@@ -981,7 +995,7 @@ __device__ inline void load_shared(float* sW1, float* sG, StencilEntry* sst, con
 template <int PREC>
 __device__ inline void stage_weights(const MfmaNet& net, u32x4_t* sWf) {
     constexpr int LF = mfma_frag(lds_from(PREC));
-    constexpr int n = lds_weight_bytes(PREC) / 16;
+    constexpr int n = lds_frag_bytes(PREC) / 16;
     if constexpr (base_prec(PREC) == MRP_FP8) {
         const u32x4_t* src = (const u32x4_t*)(net.fp8 + LF * 64);  // 64 lanes x 8 bytes per fragment
         for (int i = threadIdx.x; i < n; i += blockDim.x) sWf[i] = src[i];
@@ -990,6 +1004,12 @@ __device__ inline void stage_weights(const MfmaNet& net, u32x4_t* sWf) {
         const u32x4_t* hi = (const u32x4_t*)(net.bf16 + LF * 64);
         const u32x4_t* lo = (const u32x4_t*)(net.bf16_lo + LF * 64);
         for (int i = threadIdx.x; i < n; i += blockDim.x) sWf[i] = i < nh ? hi[i] : lo[i - nh];
+    }
+    // biases of the same layers (16 floats per M-tile) after the fragments
+    if constexpr (lds_bias(PREC)) {
+        constexpr int BF = mfma_bias(lds_from(PREC));
+        float* sb = (float*)(sWf + n);
+        for (int i = threadIdx.x; i < kMfmaBias - BF; i += blockDim.x) sb[i] = net.bias[BF + i];
     }
 }
 
@@ -1008,6 +1028,8 @@ __global__ __launch_bounds__(lds_weights(PREC) ? 768 : 256) __attribute__((amdgp
     __shared__ StencilEntry sst[FAST ? 1 : kPC];
     __shared__ FastStencil sfs[FAST ? kPC : 1];
     __shared__ u32x4_t sWf[lds_weights(PREC) ? lds_weight_bytes(PREC) / 16 : 1];
+    static_assert(!lds_weights(PREC) || !FAST || lds_weight_bytes(PREC) + kLdsOther <= kLdsBudget,
+                  "LDS budget of the LDS-weight kernel");
     if constexpr (lds_weights(PREC)) stage_weights<PREC>(net, sWf);
     if (FAST) {
         for (int i = threadIdx.x; i < kPC; i += blockDim.x) sfs[i] = fast[i];

@@ -203,8 +203,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         const float T = expf(-sum * a.alpha_lvl[m] * st) * 1.0f;
         a.tr[m][l] = T;
         if (a.tr_pad && m < 7) {
+            // padded copy with replicated edges: padded index i + 1 holds texel i, indices 0
+            // and res + 1, res + 2 repeat the first and last texel of each axis, so a blend at
+            // any coordinate in [0.5, res + 0.5] equals the clamp-addressed one (a blend of two
+            // equal taps is exact)
             const int R = pad_dim(res);
-            a.tr_pad[ptr_offset(m) + ((long long)(x + 1) * R + (y + 1)) * R + (z + 1)] = T;
+            const int lo[3] = {x == 0 ? 0 : x + 1, y == 0 ? 0 : y + 1, z == 0 ? 0 : z + 1};
+            const int hi[3] = {x == res - 1 ? res + 2 : x + 1, y == res - 1 ? res + 2 : y + 1,
+                               z == res - 1 ? res + 2 : z + 1};
+            float* dst = a.tr_pad + ptr_offset(m);
+            for (int px = lo[0]; px <= hi[0]; ++px)
+                for (int py = lo[1]; py <= hi[1]; ++py)
+                    for (int pz = lo[2]; pz <= hi[2]; ++pz) dst[((long long)px * R + py) * R + pz] = T;
         }
     }
 }
