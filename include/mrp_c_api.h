@@ -34,7 +34,8 @@ extern "C" {
 
 /* MLP arithmetic of the fused predictor. */
 #define MRP_FP32 0    /* scalar fp32, the reference's arithmetic */
-#define MRP_BF16 1    /* bf16 MFMA, fp32 accumulation; tail-layer weights as two bf16 terms (hi + lo) */
+#define MRP_BF16 1    /* bf16 MFMA, fp32 accumulation; the tail layers after LC0 (LC1, LC2, LX, LX0..LX12)
+                         with weights as two bf16 terms (hi + lo) */
 #define MRP_FP8 2     /* fp8 e4m3 MFMA, per-layer max-abs weight scales, per-pixel activation
                          scales, fp32 accumulation */
 #define MRP_BF16_1T 3 /* bf16 MFMA with single-term bf16 weights (fastest, least accurate) */

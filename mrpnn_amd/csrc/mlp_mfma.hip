@@ -166,6 +166,33 @@ int mfma_net_create(const float* W, MfmaNet* out) {
     out->fp8 = (const uint2*)df8;
     out->bias = (const float*)db;
     out->wexp = (const int*)dw;
+    // LDS images of the LDS-weight kernels: hi fragments of every layer, lo fragments of the
+    // two-term layers, biases (offsets lds_hi_off / lds_lo_off / lds_frag_bytes)
+    const int precs[3] = {MRP_BF16, MRP_FP8, MRP_BF16_1T};
+    for (int pi = 0; pi < 3; ++pi) {
+        const int v = precs[pi] | kLdsW;
+        std::vector<uint8_t> img(lds_weight_bytes(v), 0);
+        for (int id = 0; id < ML_COUNT; ++id) {
+            const int nf = mfma_nm(id) * mfma_nk(id), fb = frag_bytes(v, id), lb = fb / 64;
+            for (int f = 0; f < nf; ++f)
+                for (int lane = 0; lane < 64; ++lane) {
+                    const size_t g = (size_t)(mfma_frag(id) + f) * 64 + lane;  // 8 values per lane
+                    uint8_t* dst = &img[lds_hi_off(v, id) + f * fb + lane * lb];
+                    if (precs[pi] == MRP_FP8) {
+                        memcpy(dst, &f8[g * 8], 8);
+                    } else {
+                        memcpy(dst, &bf[g * 8], lb);  // slots 0..3 (compact) or 0..7
+                        if (two_term(v, id)) memcpy(&img[lds_lo_off(v, id) + f * fb + lane * lb], &bflo[g * 8], lb);
+                    }
+                }
+        }
+        memcpy(&img[lds_frag_bytes(v)], bias.data(), bias.size() * 4);
+        void* d = nullptr;
+        if (hipMalloc(&d, img.size()) != hipSuccess) return mrp_fail(MRP_ERR_NOMEM, "LDS image allocation failed");
+        if (hipMemcpy(d, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return mrp_fail(MRP_ERR_HIP, "LDS image upload failed");
+        out->img[precs[pi]] = (const uint4*)d;
+    }
     return MRP_OK;
 }
 
@@ -175,6 +202,10 @@ void mfma_net_destroy(MfmaNet* net) {
     hipFree((void*)net->fp8);
     hipFree((void*)net->bias);
     hipFree((void*)net->wexp);
+    for (int i = 0; i < 4; ++i) {
+        hipFree((void*)net->img[i]);
+        net->img[i] = nullptr;
+    }
     net->wexp = nullptr;
     net->bf16 = nullptr;
     net->fp8 = nullptr;
@@ -227,7 +258,7 @@ struct NetRs {
     BufRsrc wlo;  // bf16 lo fragments
     BufRsrc bias;
     const int* wexp;
-    const char* lw;  // LDS copy of the fragments of layers >= lds_from (variants with kLdsW)
+    const char* lw;  // LDS copy of the weight image (variants with kLdsW)
 };
 // The pointers are uniform (kernel arguments); reading them through readfirstlane keeps the
 // descriptors in scalar registers also where they arrive in vector registers (call arguments).
@@ -248,6 +279,18 @@ __device__ inline NetRs make_net_rs(const MfmaNet& n, bool fp8) {
     return r;
 }
 
+// A bf16 weight fragment of one lane from the LDS image: 16 bytes, or 8 (slots 0..3) for a
+// compact fragment whose slots 4..7 are zero.
+template <bool CMP>
+__device__ inline u32x4_t lds_frag(const char* p) {
+    if constexpr (CMP) {
+        const uint2 v = *(const uint2*)p;
+        return u32x4_t{v.x, v.y, 0u, 0u};
+    } else {
+        return *(const u32x4_t*)p;
+    }
+}
+
 // out[NM][4] = act(W_ID * in + b): in[NK][8] in the K-slot order of mlp_mfma.h.  One M-tile
 // at a time behind a scheduling barrier, so at most NK weight fragments are in flight.
 template <int PREC, int ID, bool SB = true>
@@ -257,13 +300,13 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
     // second weight term: MRP_BF16 applies it to the tail (LSEFin2, LC0..LX12), where the
     // single-term rounding error concentrates; the trunk's 8..32-wide layers use one term
     constexpr int P = base_prec(PREC);
-    constexpr bool TWO = P == MRP_BF16 && ID >= ML_FIN2;
-    // weights of this layer from the LDS copy: fragment f at (f - LF) * FB + lane * (FB / 64),
-    // lo terms after the NL hi fragments
-    constexpr bool LDS = lds_weights(PREC) && ID >= lds_from(PREC);
-    // (constexpr variables: these table walks must fold at compile time)
-    constexpr int LF = mfma_frag(lds_from(PREC)), NL = kMfmaFrags - LF, BL = mfma_bias(lds_from(PREC));
-    constexpr int LFB = lds_frag_bytes(PREC);
+    constexpr bool TWO = two_term(PREC, ID);
+    // LDS image offsets of this layer (constexpr variables: the table walks fold at compile
+    // time); compact fragments hold slots 0..3 of each lane (8 bytes)
+    constexpr bool LDS = lds_weights(PREC);
+    constexpr int HO = lds_hi_off(PREC, ID), LO = lds_lo_off(PREC, ID), FB = frag_bytes(PREC, ID);
+    constexpr bool CMP = compact_frag(PREC, ID);
+    constexpr int BO = lds_frag_bytes(PREC) + BIAS * 4;
     const int q = lane >> 4;
     // Fragments of M-tile mt+1 are requested before the MFMAs of M-tile mt (two buffers), so
     // the weight latency of a multi-tile layer is paid once.
@@ -278,9 +321,9 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
 #pragma unroll
             for (int kc = 0; kc < NK; ++kc) {
                 if constexpr (LDS) {
-                    const int lo = (FRAG - LF + mt * NK + kc) * 1024 + lane * 16;
-                    wa[buf][kc] = *(const u32x4_t*)(net.lw + lo);
-                    if constexpr (TWO) wl[buf][kc] = *(const u32x4_t*)(net.lw + NL * 1024 + lo);
+                    const int fo = (mt * NK + kc) * FB + lane * (FB / 64);
+                    wa[buf][kc] = lds_frag<CMP>(net.lw + HO + fo);
+                    if constexpr (TWO) wl[buf][kc] = lds_frag<CMP>(net.lw + LO + fo);
                 } else {
                     const int fo = (FRAG + mt * NK + kc) * 1024;
                     wa[buf][kc] = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, fo, 0);
@@ -289,8 +332,8 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
             }
             bb[buf] = u32x4_t{0u, 0u, 0u, 0u};
             if constexpr (mfma_has_bias(ID)) {
-                if constexpr (LDS && lds_bias(PREC))
-                    bb[buf] = *(const u32x4_t*)(net.lw + LFB + (BIAS - BL + 16 * mt) * 4 + q * 16);
+                if constexpr (LDS)
+                    bb[buf] = *(const u32x4_t*)(net.lw + BO + 16 * mt * 4 + q * 16);
                 else
                     bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
             }
@@ -303,19 +346,25 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
         for (int mt = 0; mt < NM; ++mt) {
             const int cur = mt & 1;
             if (mt + 1 < NM) fetch(mt + 1, cur ^ 1);
-            // bias first, then the products (the reference's LinearReLU_SFL order)
+            // bias first, then the products (the reference's LinearReLU_SFL order); the lo terms
+            // accumulate in a second, independent chain added at the end, so the two MFMA chains
+            // overlap instead of forming one chain twice as long
             f32x4_t acc = f32x4_t{__uint_as_float(bb[cur][0]), __uint_as_float(bb[cur][1]),
                                   __uint_as_float(bb[cur][2]), __uint_as_float(bb[cur][3])};
+            f32x4_t acl = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int kc = 0; kc < NK; ++kc) {
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, wa[cur][kc]),
                                                               __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
                 if constexpr (TWO)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, wl[cur][kc]),
-                                                                  __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
+                    acl = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, wl[cur][kc]),
+                                                                  __builtin_bit_cast(bf16x8_t, b[kc]), acl, 0, 0, 0);
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) out[mt][i] = relu ? fmaxf(acc[i], 0.0f) : acc[i];
+            for (int i = 0; i < 4; ++i) {
+                const float v = TWO ? acc[i] + acl[i] : acc[i];
+                out[mt][i] = relu ? fmaxf(v, 0.0f) : v;
+            }
             if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
         }
     } else {
@@ -341,15 +390,15 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
 #pragma unroll
             for (int kc = 0; kc < NK; ++kc) {
                 if constexpr (LDS)
-                    wa[buf][kc] = *(const long*)(net.lw + (FRAG - LF + mt * NK + kc) * 512 + lane * 8);
+                    wa[buf][kc] = *(const long*)(net.lw + HO + (mt * NK + kc) * 512 + lane * 8);
                 else
                     wa[buf][kc] = __builtin_bit_cast(
                         long, __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, (FRAG + mt * NK + kc) * 512, 0));
             }
             bb[buf] = u32x4_t{0u, 0u, 0u, 0u};
             if constexpr (mfma_has_bias(ID)) {
-                if constexpr (LDS && lds_bias(PREC))
-                    bb[buf] = *(const u32x4_t*)(net.lw + LFB + (BIAS - BL + 16 * mt) * 4 + q * 16);
+                if constexpr (LDS)
+                    bb[buf] = *(const u32x4_t*)(net.lw + BO + 16 * mt * 4 + q * 16);
                 else
                     bb[buf] = __builtin_amdgcn_raw_buffer_load_b128(net.bias, q * 16, (BIAS + 16 * mt) * 4, 0);
             }
@@ -648,28 +697,29 @@ template <int PREC, int ID>
 __device__ inline void frag1_fetch(const NetRs& net, int lane, Frag1& f) {
     static_assert(mfma_nm(ID) == 1 && mfma_nk(ID) == 1 && !mfma_shape(ID).k32, "single-fragment layer");
     constexpr int P = base_prec(PREC);
-    constexpr bool TWO = P == MRP_BF16 && ID >= ML_FIN2;
+    constexpr bool TWO = two_term(PREC, ID);
     constexpr int FRAG = mfma_frag(ID), BIAS = mfma_bias(ID);
-    constexpr bool LDS = lds_weights(PREC) && ID >= lds_from(PREC);
-    constexpr int LF = mfma_frag(lds_from(PREC)), NL = kMfmaFrags - LF, BL = mfma_bias(lds_from(PREC));
-    constexpr int LFB = lds_frag_bytes(PREC);
+    constexpr bool LDS = lds_weights(PREC);
+    constexpr int HO = lds_hi_off(PREC, ID), LO = lds_lo_off(PREC, ID), FB = frag_bytes(PREC, ID);
+    constexpr bool CMP = compact_frag(PREC, ID);
+    constexpr int BO = lds_frag_bytes(PREC) + BIAS * 4;
     if constexpr (P == MRP_FP8) {
         if constexpr (LDS) {
-            const uint2 v = *(const uint2*)(net.lw + (FRAG - LF) * 512 + lane * 8);
+            const uint2 v = *(const uint2*)(net.lw + HO + lane * 8);
             f.w = u32x4_t{v.x, v.y, 0u, 0u};
         } else {
             const auto v = __builtin_amdgcn_raw_buffer_load_b64(net.w, lane * 8, FRAG * 512, 0);
             f.w = u32x4_t{v[0], v[1], 0u, 0u};
         }
     } else if constexpr (LDS) {
-        f.w = *(const u32x4_t*)(net.lw + (FRAG - LF) * 1024 + lane * 16);
-        if constexpr (TWO) f.wl = *(const u32x4_t*)(net.lw + (NL + FRAG - LF) * 1024 + lane * 16);
+        f.w = lds_frag<CMP>(net.lw + HO + lane * (FB / 64));
+        if constexpr (TWO) f.wl = lds_frag<CMP>(net.lw + LO + lane * (FB / 64));
     } else {
         f.w = __builtin_amdgcn_raw_buffer_load_b128(net.w, lane * 16, FRAG * 1024, 0);
         if constexpr (TWO) f.wl = __builtin_amdgcn_raw_buffer_load_b128(net.wlo, lane * 16, FRAG * 1024, 0);
     }
-    if constexpr (LDS && lds_bias(PREC))
-        f.b = *(const u32x4_t*)(net.lw + LFB + (BIAS - BL) * 4 + (lane >> 4) * 16);
+    if constexpr (LDS)
+        f.b = *(const u32x4_t*)(net.lw + BO + (lane >> 4) * 16);
     else
         f.b = __builtin_amdgcn_raw_buffer_load_b128(net.bias, (lane >> 4) * 16, BIAS * 4, 0);
 }
@@ -679,18 +729,22 @@ template <int PREC, int ID>
 __device__ inline void frag1_apply(const NetRs& net, const Frag1& f, const float (&in)[4], float (&out)[4],
                                    bool relu) {
     constexpr int P = base_prec(PREC);
-    constexpr bool TWO = P == MRP_BF16 && ID >= ML_FIN2;
+    constexpr bool TWO = two_term(PREC, ID);
     if constexpr (P == MRP_BF16 || P == MRP_BF16_1T) {
         const u32x4_t b = u32x4_t{pack_bf16x2(in[0], in[1]), pack_bf16x2(in[2], in[3]), 0u, 0u};
         f32x4_t acc = f32x4_t{__uint_as_float(f.b[0]), __uint_as_float(f.b[1]), __uint_as_float(f.b[2]),
                               __uint_as_float(f.b[3])};
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, f.w), __builtin_bit_cast(bf16x8_t, b),
                                                       acc, 0, 0, 0);
+        f32x4_t acl = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};  // lo terms: an independent MFMA (as dense())
         if constexpr (TWO)
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, f.wl),
-                                                          __builtin_bit_cast(bf16x8_t, b), acc, 0, 0, 0);
+            acl = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, f.wl),
+                                                          __builtin_bit_cast(bf16x8_t, b), acl, 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) out[i] = relu ? fmaxf(acc[i], 0.0f) : acc[i];
+        for (int i = 0; i < 4; ++i) {
+            const float v = TWO ? acc[i] + acl[i] : acc[i];
+            out[i] = relu ? fmaxf(v, 0.0f) : v;
+        }
     } else {
         float m = 0.0f;
 #pragma unroll
@@ -991,26 +1045,12 @@ __device__ inline void load_shared(float* sW1, float* sG, StencilEntry* sst, con
     __syncthreads();
 }
 
-// Copies the fragments of layers >= lds_from into LDS (variants with kLdsW).
+// Copies the variant's LDS weight image (built by mfma_net_create) into LDS.
 template <int PREC>
 __device__ inline void stage_weights(const MfmaNet& net, u32x4_t* sWf) {
-    constexpr int LF = mfma_frag(lds_from(PREC));
-    constexpr int n = lds_frag_bytes(PREC) / 16;
-    if constexpr (base_prec(PREC) == MRP_FP8) {
-        const u32x4_t* src = (const u32x4_t*)(net.fp8 + LF * 64);  // 64 lanes x 8 bytes per fragment
-        for (int i = threadIdx.x; i < n; i += blockDim.x) sWf[i] = src[i];
-    } else {
-        constexpr int nh = (kMfmaFrags - LF) * 64;  // hi fragments: 64 lanes x 16 bytes
-        const u32x4_t* hi = (const u32x4_t*)(net.bf16 + LF * 64);
-        const u32x4_t* lo = (const u32x4_t*)(net.bf16_lo + LF * 64);
-        for (int i = threadIdx.x; i < n; i += blockDim.x) sWf[i] = i < nh ? hi[i] : lo[i - nh];
-    }
-    // biases of the same layers (16 floats per M-tile) after the fragments
-    if constexpr (lds_bias(PREC)) {
-        constexpr int BF = mfma_bias(lds_from(PREC));
-        float* sb = (float*)(sWf + n);
-        for (int i = threadIdx.x; i < kMfmaBias - BF; i += blockDim.x) sb[i] = net.bias[BF + i];
-    }
+    constexpr int n = lds_weight_bytes(PREC) / 16;
+    const u32x4_t* src = (const u32x4_t*)net.img[base_prec(PREC)];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sWf[i] = src[i];
 }
 
 // PREC: a precision id, plus kLdsW for the variant with LDS-resident weights (one 768-thread
@@ -1028,7 +1068,7 @@ __global__ __launch_bounds__(lds_weights(PREC) ? 768 : 256) __attribute__((amdgp
     __shared__ StencilEntry sst[FAST ? 1 : kPC];
     __shared__ FastStencil sfs[FAST ? kPC : 1];
     __shared__ u32x4_t sWf[lds_weights(PREC) ? lds_weight_bytes(PREC) / 16 : 1];
-    static_assert(!lds_weights(PREC) || !FAST || lds_weight_bytes(PREC) + kLdsOther <= kLdsBudget,
+    static_assert(!lds_weights(PREC) || lds_weight_bytes(PREC) + kLdsOther <= kLdsBudget,
                   "LDS budget of the LDS-weight kernel");
     if constexpr (lds_weights(PREC)) stage_weights<PREC>(net, sWf);
     if (FAST) {
