@@ -53,8 +53,7 @@ extern "C" {
                                        mips (about 20 MB) per distinct (volume, light_dir, alpha) of a
                                        batch (1..4096, default 64) */
 #define MRP_OPT_LDS_WEIGHTS 5       /* 1 (default): the fused MFMA kernel runs one 768-thread workgroup
-                                       per CU holding the weight fragments in LDS (the whole network
-                                       for FP8 and single-term BF16, the tail for two-term BF16);
+                                       per CU holding all weight fragments and biases in LDS;
                                        0: fragments are read through L1/L2 by MRP_OPT_MFMA_WAVES_PER_CU
                                        persistent waves per CU */
 
