@@ -10,7 +10,9 @@ c4  64 views over 8 resident volumes (two each of 128^3, 256^3, 384^3, 512^3), r
     from {256^2, 512^2, 1280x720, 1920x1080}, one graph-replayed BF16 batch per step with
     the autotuned launch shape;
 c5  serving loop: Poisson arrivals over 32 resident volumes, BF16 / FP8 batched against
-    the plain per-frame FP32 launch.
+    the plain per-frame FP32 launch;
+tile  tile feature gather (the training-input path, MRPNNInputData.fetch_input) at 1920x1080
+    over a 512^3 cloud, with the oracle's CPU gather of the same frame as baseline and check.
 
 All volumes are the seeded fBm clouds of the library (mrp_volume_fill_fbm); the oracle is the
 CPU restatement under oracle/ and is used here only as the timed CPU baseline and as the FP64
@@ -297,5 +299,62 @@ def c5(args):
                      "views_per_batch": round(st.batched_views / max(st.batches, 1), 2),
                      "admissions": st.admissions}
         srv.close()
+    ctx.close()
+    return res
+
+
+def tile(args):
+    """Tile feature gather, the training-input path (MRPNNInputData.fetch_input at 1920x1080:
+    a 240x135 grid of pixels with 64 sub-rays each over a 512^3 cloud): fetches per second,
+    sub-ray marches per second, the oracle on the host cores over the same frame, and parity
+    of every row against it."""
+    import torch
+
+    import mrpnn_amd as M
+    import oracle as O
+    ctx = M.Context()
+    N = 512
+    vol = M.Volume(ctx, N)
+    vol.fill_fbm(6, 4.0, 1)
+    vol.update()
+    data = M.MRPNNInputData(1920, 1080, vol)
+    rows = data.w * data.h
+    res = {"config": "tile", "workload": "fetch_input at 1920x1080 (240x135 pixels x 64 sub-rays, 773 floats "
+                                         "per pixel) over a 512^3 fBm cloud, default camera and light, "
+                                         "all 64 sub-pixel offsets"}
+    for alpha in (1.0, 4.0, 16.0):
+        data.fetch_input(alpha=alpha)  # LUT and TR mips for this (g, light, alpha)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(64):
+            data.fetch_input(alpha=alpha, pixelOffset=(k % 8, k // 8))
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 64
+        act = float((data.tensor[..., 0] != -1000.0).float().mean())
+        res["alpha_%g" % alpha] = {"ms_per_fetch": round(dt * 1e3, 4), "fetches_s": round(1.0 / dt, 1),
+                                   "rows_s": round(rows / dt, 1), "subrays_s": round(64 * rows / dt, 1),
+                                   "active_frac": round(act, 4)}
+        _log("tile alpha %g: %.3f ms per fetch, active %.3f" % (alpha, dt * 1e3, act))
+    if args.no_cpu:
+        ctx.close()
+        return res
+    # CPU baseline and parity: the oracle's GatherData restatement on the host cores, one frame
+    alpha = 4.0
+    got = data.fetch_input(alpha=alpha, pixelOffset=(3, 5)).reshape(rows, -1).cpu().numpy()
+    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    sc = O.Scene.from_arrays(vol.get_data(), N, 1.0, vol.download_mips(), vol.download_tr(), ctx.download_hg_lut())
+    v = M.camera_view(data.w, data.h, (0.67085, -0.03808, -0.04856), (0.34281, 0.70711, 0.61845), alpha=alpha, g=G)
+    t0 = time.perf_counter()
+    want, active = sc.tile_gather(O.View.from_buffer_copy(bytes(v)), 3, 5)
+    cpu_s = time.perf_counter() - t0
+    act = active > 0
+    rel = np.abs(got[act, :581] - want[act, :581]) / np.maximum(np.abs(want[act, :581]), 1e-3)
+    rel[:, 577:579] = np.abs(got[act, 577:579] - want[act, 577:579]) * (2e-4 / 1e-6)
+    res["cpu_baseline"] = {"rows_s": round(rows / cpu_s, 1), "seconds": round(cpu_s, 3), "cores": threads,
+                           "kind": "port", "sample": "one full 240x135 frame, alpha 4, offset (3, 5)"}
+    res["parity_vs_oracle"] = {"active_equal": bool(np.array_equal(got[:, 0] != -1000.0, act)),
+                               "t_grid_max_abs": float(np.abs(got[:, 581:] - want[:, 581:]).max()),
+                               "rows_within_2e-4": round(float(np.mean(rel.max(axis=1) <= 2e-4)), 5)}
     ctx.close()
     return res

@@ -195,6 +195,21 @@ int mrp_gather_features(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, flo
 int mrp_predict_features(mrp_ctx* ctx, const float* dev_feats, int n, const float scatterrate[3], float g, int precision,
                          float* dev_rgb);
 
+/* Tile feature gather, the training-data path: VolumeRender::GatherData (core/volume.cu:630-708)
+ * -> GatherSamples (core/volume.cu:257-283) -> NNGatherSamples / PrepareRay / GetSampleT
+ * (core/render.cuh:1006-1142, 464-486, 199-397) -> SetupInputData2 (core/radiancePredict.cu:487-565),
+ * called by the reference's MRPNNInputData.fetch_input (mrpnn_input_data_wrapper.cpp:72-84).
+ * Brings the LUT (view->g) and the volume's TR mips (light_dir, alpha) up to date, then writes
+ * width*height rows of MRP_TILE_ROW floats to dev_out: [X_Val 0:192 | X_Val_Hg 192:384 |
+ * X_Val_Sub 384:576 | gamma, d_gamma_right, d_gamma_up, horizontal t diff, vertical t diff |
+ * per sub-ray of the 8x8 grid (mid t, t range) 581:709 | per sub-ray opacity 709:773]
+ * (mrpnn_helpers.py unpack_features).  pixel_offset_x/y shift the pixel grid by eighths of a
+ * pixel.  A pixel none of whose 64 sub-rays hits the medium gets row[0] = -1000, zeros in
+ * 1:581 (the reference leaves them unwritten) and its t grid.  Asynchronous. */
+#define MRP_TILE_ROW 773
+int mrp_gather_tiles(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int pixel_offset_x, int pixel_offset_y,
+                     float* dev_out);
+
 /* Delta-tracking path tracer (judge): RenderCamera<false> / CalculateRadiance
  * (core/volume.cu:198-228, core/render.cuh:535-597) with Philox streams keyed
  * (seed, frame) / (pixel, sample) and FP64 per-pixel Welford accumulators.

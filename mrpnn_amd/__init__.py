@@ -16,6 +16,7 @@ FP32, BF16, FP8, BF16_1T = 0, 1, 2, 3
 PRECISIONS = {"fp32": FP32, "bf16": BF16, "fp8": FP8, "bf16_1t": BF16_1T}
 OPT_MFMA_WAVES_PER_CU, OPT_EXACT_GATHER, OPT_TR_SLOTS, OPT_GRAPH, OPT_LDS_WEIGHTS = 1, 2, 3, 4, 5
 NFEAT = 577
+TILE_ROW = 773  # tile feature gather row (mrp_gather_tiles)
 MIP_TOTAL = sum((256 >> m) ** 3 for m in range(9))
 TR_TOTAL = sum((128 >> m) ** 3 for m in range(8))
 
@@ -135,6 +136,7 @@ def lib():
         L.mrp_render_batch.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View), ctypes.c_int,
                                        ctypes.c_int, ctypes.POINTER(_vp)]
         L.mrp_gather_features.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp, _vp, _vp, _vp]
+        L.mrp_gather_tiles.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _vp]
         L.mrp_predict_features.argtypes = [_vp, _vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.c_int, _vp]
         L.mrp_pt_render.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_float, _vp, _vp]
@@ -313,6 +315,17 @@ class Volume:
                                         _ptr(taps)))
         return feats, active, pos, taps
 
+    def gather_tiles(self, view, off_x=0, off_y=0, out=None):
+        """Tile feature gather (mrp_gather_tiles): (height*width, 773) float32 rows on the GPU."""
+        import torch
+        n = view.width * view.height
+        if out is None:
+            out = torch.empty((n, TILE_ROW), dtype=torch.float32, device="cuda")
+        if out.numel() < n * TILE_ROW or not out.is_contiguous() or out.dtype != torch.float32:
+            raise MrpError("gather_tiles: out must be a contiguous float32 tensor of %d x %d" % (n, TILE_ROW))
+        check(lib().mrp_gather_tiles(self.ctx.h, self.h, ctypes.byref(view), int(off_x), int(off_y), _ptr(out)))
+        return out
+
     def pt_render(self, view, spp, max_bounce, nee_baked=0, tr_scale=1.0, chunk=None, mean=None, m2=None, spp0=0):
         import torch
         n = view.width * view.height
@@ -352,6 +365,56 @@ def orbit_view(width, height, dist=2.0, azimuth=0.0, elevation=0.0, fov_deg=40.0
     v.width, v.height = width, height
     v.seed, v.frame, v.view_id, v.volume_index = seed, frame, view_id, volume_index
     return v
+
+
+def camera_view(width, height, position, light_dir, alpha=1.0, g=0.857, fov_deg=90.0):
+    """View of Camera(VolumeRender&, name, position) after SetPosition (core/camera.cu:166-192):
+    looking at the origin, right = normalize(cross(y, forward)), up = cross(forward, right); the
+    reference's camera has a fixed 90 degree field of view."""
+    ori = np.asarray(position, np.float32)
+    fwd = -ori / np.linalg.norm(ori)
+    right = np.cross(np.array([0, 1, 0], np.float32), fwd)
+    right = right / np.linalg.norm(right)
+    up = np.cross(fwd, right)
+    v = View()
+    v.ori[:] = ori.tolist()
+    v.up[:] = up.astype(np.float32).tolist()
+    v.right[:] = right.astype(np.float32).tolist()
+    L = np.asarray(light_dir, np.float64)
+    v.light_dir[:] = (L / np.linalg.norm(L)).astype(np.float32).tolist()
+    v.light_color[:] = [1.0, 1.0, 1.0]
+    v.scatter_rate[:] = [1.0, 1.0, 1.0]
+    v.alpha, v.g = float(alpha), float(g)
+    v.tan_half_fov = float(np.tan(np.radians(fov_deg) / 2))
+    v.jitter = 0.5
+    v.width, v.height = int(width), int(height)
+    v.seed, v.frame, v.view_id, v.volume_index = 1, 0, 0, 0
+    return v
+
+
+class MRPNNInputData:
+    """The reference's training-input fetcher (mrpnn_input_data_wrapper.cpp:34-103): a volume,
+    a camera at 1/8 of the image resolution and a (height/8, width/8, 773) device tensor that
+    fetch_input refills with mrp_gather_tiles.  The reference takes a volume file path; here a
+    resident `Volume` (swap_data replaces it).  Unlike the reference's square camera
+    (resolution = width/8 for both axes), the tile grid is width/8 x height/8."""
+
+    def __init__(self, width, height, volume):
+        import torch
+        self.volume = volume
+        self.w, self.h = int(width) // 8, int(height) // 8
+        if self.w < 1 or self.h < 1:
+            raise MrpError("MRPNNInputData: width and height must be at least 8")
+        self.tensor = torch.empty((self.h, self.w, TILE_ROW), dtype=torch.float32, device="cuda")
+
+    def fetch_input(self, camPos=(0.67085, -0.03808, -0.04856), lightDir=(0.34281, 0.70711, 0.61845), alpha=1.0,
+                    g=0.857, pixelOffset=(0, 0)):
+        v = camera_view(self.w, self.h, camPos, lightDir, alpha, g)
+        self.volume.gather_tiles(v, int(pixelOffset[0]), int(pixelOffset[1]), out=self.tensor)
+        return self.tensor
+
+    def swap_data(self, volume):
+        self.volume = volume
 
 
 class Server:

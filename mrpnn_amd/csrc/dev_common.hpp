@@ -317,6 +317,115 @@ __device__ inline bool estimated_hit(const DevVolume& v, float alpha, v3 pos, v3
     return true;
 }
 
+// ---------------------------------------------------------------------------------------
+// Optical-depth march of the tile feature gather: DeterminateEstimatedHit float4 version with
+// CheckExitCondition (render.cuh:199-397) and GetSampleT (render.cuh:464-486).  Trapezoid
+// optical depth along the ray; ts = (t at survival 0.75, t at survival 0.25, first non-empty t,
+// opacity).  Same arithmetic as the oracle's estimated_hit_tau.
+// ---------------------------------------------------------------------------------------
+// -logf(0.75f), -logf(0.76f), -logf(0.25f), correctly rounded
+constexpr float kTauFirst = 0x1.269622p-2f, kTauStepMax = 0x1.1905fap-2f, kTauSecond = 0x1.62e43p+0f;
+struct Ts4 {
+    float x, y, z, w;
+};
+__device__ inline float trapezoid_offset(float A, float B, float C, float dt, float S, float lastS, float delta_tau) {
+    if (fabsf(A) < 1e-6f) return delta_tau / (dt * (S + lastS) * 0.5f);  // linear fallback
+    float D = B * B - 4.0f * A * C;
+    D = fmaxf(D, 0.0f);
+    const float sqrtD = sqrtf(D);
+    const float q = -0.5f * (B + sqrtD);
+    const float u1 = q / A;
+    const float u2 = C / q;
+    return u1 == fminf(fmaxf(u1, 0.0f), 1.0f) ? u1 : u2;
+}
+__device__ inline bool check_exit(float S, float lastS, float dt, float t, float& tau, float& tau_target, Ts4& ts,
+                                  bool& first_hit) {
+    const float tau_next = tau + (S + lastS) * 0.5f * dt;
+    if (tau_next >= tau_target) {
+        const float delta_tau = tau_target - tau;
+        const float A = 0.5f * dt * (S - lastS);
+        const float B = dt * lastS;
+        const float C = -delta_tau;
+        float offset = trapezoid_offset(A, B, C, dt, S, lastS, delta_tau);
+        offset = fminf(fmaxf(offset, 0.0f), 1.0f);
+        const float hit_t = t - dt * (1.0f - offset);
+        if (first_hit) {
+            ts.x = hit_t;
+            tau_target = kTauSecond;
+            first_hit = false;
+            if (tau_next >= tau_target) {
+                const float delta_tau2 = tau_target - tau;
+                const float offset2 = trapezoid_offset(A, B, -delta_tau2, dt, S, lastS, delta_tau2);  // not clamped
+                ts.y = t - dt * (1.0f - offset2);
+                return true;
+            }
+        } else {
+            ts.y = hit_t;
+            return true;
+        }
+    }
+    tau = tau_next;
+    return false;
+}
+__device__ inline Ts4 estimated_hit_tau(const DevVolume& v, float alpha, v3 pos, v3 dir, float dis) {
+    const float smax = v.max_density * alpha;
+    float t = 0.0f;
+    Ts4 ts = {0.0f, 0.0f, 3.402823466e38f, 0.0f};
+    float last_non_empty_t = 0.0f;
+    float t_step = 1.0f / (float)v.N;
+    if (smax * t_step > kTauStepMax) t_step = kTauStepMax / smax;
+    float tau = 0.0f, tau_target = kTauFirst;
+    bool first_hit = true;
+    int loop = 0;
+    float last_dens = 0.0f;
+    BrickCache bc;
+    while (loop++ < 10000) {
+        if (t > dis) {
+            if (first_hit) ts.x = last_non_empty_t;
+            else ts.y = last_non_empty_t;
+            break;
+        }
+        const FullTap tp = full_tap(v.N, pos + (dir * t));
+        const float d = (v.occ && tap_in_empty_brick(v, tp, bc)) ? 0.0f : density_full_at(v, tp);
+        if (d > 0.0f) {
+            const float S = d * alpha;
+            if (check_exit(S, last_dens, t_step, t, tau, tau_target, ts, first_hit)) break;
+            ts.z = fminf(t, ts.z);
+            t += t_step;
+            last_dens = S;
+        } else {
+            if (last_dens > 0.0f) {
+                const float st = t_step + d;  // a negative SDF value shortens the step
+                if (st > 0.0f) {
+                    if (check_exit(0.0f, last_dens, st, t, tau, tau_target, ts, first_hit)) break;
+                    last_non_empty_t = t + st;
+                } else {
+                    last_non_empty_t = t;
+                }
+            }
+            t += fmaxf(-d * 1.0f, 0.5f / (float)v.N);
+            last_dens = 0.0f;
+        }
+    }
+    if (ts.z == 3.402823466e38f) ts.z = 0.0f;
+    const float survival = expf(-tau);
+    ts.w = fminf(fmaxf(1.0f - survival, 0.0f) / 0.75f, 1.0f);
+    return ts;
+}
+__device__ inline Ts4 get_sample_t(const DevVolume& v, v3 ori, v3 dir, float alpha) {
+    const float dis = ray_box_offset(ori, dir);
+    if (dis < 0.0f) return Ts4{0.0f, 0.0f, 0.0f, 0.0f};
+    const v3 sp = ori + dir * dis;
+    const float max_dis = ray_box_distance(sp, dir);
+    Ts4 ts = estimated_hit_tau(v, alpha, sp, dir, max_dis);
+    if (ts.x > 0.0f) {
+        ts.x += dis;
+        ts.z += dis;
+    }
+    if (ts.y > 0.0f) ts.y += dis;
+    return ts;
+}
+
 // XCD-aware block numbering.  Workgroups are dispatched round-robin over the 8 XCDs of an
 // MI355X (block b runs on XCD b % 8) and each XCD has its own L2: renumbering blocks so that
 // every XCD owns one contiguous range of work items keeps neighbouring pixels (which read

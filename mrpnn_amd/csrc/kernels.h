@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/mrp_c_api.h"
+
 #include "dev_common.hpp"
 #include "dev_gather_fast.hpp"
 
@@ -26,5 +28,9 @@ hipError_t launch_gather_features(const DevVolume& vol, const StencilEntry* sten
                                   int* active, float* pos3, int* taps, hipStream_t s);
 hipError_t launch_predict_features_fp32(const float* feats, int n, const float* W, v3 sr, float g, float* rgb,
                                         hipStream_t s);
+
+// tile_gather.hip
+hipError_t launch_tile_gather(const DevVolume& vol, const StencilEntry* stencil, const mrp_view* v, int off_x,
+                              int off_y, float* out, hipStream_t s);
 
 }  // namespace mrp

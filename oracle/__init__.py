@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
 WEIGHTS = os.path.join(HERE, "..", "mrpnn_amd", "assets", "mrpnn_weights.bin")
 
 NFEAT = 577
+TILE_ROW = 773  # tile feature gather row: 577 features + 4 gamma extras + 192 t-grid values
 PC = 192
 MIP_TOTAL = sum((256 >> m) ** 3 for m in range(9))
 TR_TOTAL = sum((128 >> m) ** 3 for m in range(8))
@@ -69,6 +70,8 @@ def lib():
         L.orc_render_predict_rows.argtypes = vol + [ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _f]
         L.orc_render_predict_list.argtypes = vol + [ctypes.POINTER(View), _i, ctypes.c_int, _f]
         L.orc_march.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, _f, ctypes.c_float, _f, _i, _i]
+        L.orc_tile_gather.argtypes = vol + [ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _f, _i]
+        L.orc_sample_t.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, _f, ctypes.c_float, _f]
         L.orc_gather_state.argtypes = vol + [ctypes.c_int, _f, _f, _f, _f, _f, ctypes.c_float, ctypes.c_float, _f]
         L.orc_basis.argtypes = [_f, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _f]
         L.orc_pt_render.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, ctypes.POINTER(View), ctypes.c_int,
@@ -243,6 +246,14 @@ class Scene:
         sc.lut = np.ascontiguousarray(lut, np.float32)
         return sc
 
+    def tile_gather(self, view, off_x=0, off_y=0):
+        """GatherData (core/volume.cu:630-708): (H*W, 773) rows and the active flags."""
+        W, H = view.width, view.height
+        out = np.zeros((H * W, TILE_ROW), np.float32)
+        active = np.zeros(H * W, np.int32)
+        lib().orc_tile_gather(*self._vol(), ctypes.byref(view), int(off_x), int(off_y), _p(out), _p(active, _i))
+        return out, active
+
     def gather_state(self, active, pos, X, Y, Z, light, g, alpha):
         out = np.zeros(NFEAT, np.float32)
         a = [np.asarray(v, np.float32) for v in (pos, X, Y, Z, light)]
@@ -257,6 +268,15 @@ class Scene:
         lib().orc_pt_render(_p(self.grid), self.N, self.max_density, _p(self.tr), ctypes.byref(view), spp,
                             max_bounce, nee_baked, tr_scale, _p(mean, _d), _p(m2, _d), ctypes.byref(taps))
         return mean, m2, taps.value
+
+
+def sample_t(grid, N, max_density, ori, d, alpha):
+    """GetSampleT (core/render.cuh:464-486): (t at survival 0.75, t at 0.25, first t, opacity)."""
+    grid = np.ascontiguousarray(grid, np.float32)
+    ts = np.zeros(4, np.float32)
+    o, dd = np.asarray(ori, np.float32), np.asarray(d, np.float32)
+    lib().orc_sample_t(_p(grid), N, max_density, _p(o), _p(dd), float(alpha), _p(ts))
+    return ts
 
 
 def march(grid, N, max_density, ori, d, alpha):

@@ -1,6 +1,7 @@
 // CPU oracle implementation.  TEST INFRASTRUCTURE ONLY (see mrp_oracle.hpp).
 #include "mrp_oracle.hpp"
 
+#include <float.h>
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -632,6 +633,251 @@ void pt_pixel(const Volume& v, const View& view, const PTParams& pp, int row, in
         }
     }
     for (int c = 0; c < 3; ++c) { mean3[c] = mean[c]; m2_3[c] = m2[c]; }
+}
+
+// ---------------------------------------------------------------------------------------
+// Tile feature gather
+// ---------------------------------------------------------------------------------------
+// One root of the trapezoid optical depth inside a step (render.cuh:223-241): A u^2 + B u + C = 0
+static float trapezoid_offset(float A, float B, float C, float dt, float S, float lastS, float delta_tau) {
+    if (fabsf(A) < 1e-6f) return delta_tau / (dt * (S + lastS) * 0.5f);  // linear fallback
+    float D = B * B - 4.0f * A * C;
+    D = fmaxf(D, 0.0f);
+    float sqrtD = sqrtf(D);
+    float q = -0.5f * (B + sqrtD);  // the reference's double -0.5 scales exactly
+    float u1 = q / A;
+    float u2 = C / q;
+    return u1 == fminf(fmaxf(u1, 0.0f), 1.0f) ? u1 : u2;  // saturate
+}
+// CheckExitCondition (render.cuh:199-288)
+static bool check_exit(float S, float lastS, float dt, float t, float& tau, float& tau_target, Ts4& ts,
+                       bool& first_hit) {
+    float tau_next = tau + (S + lastS) * 0.5f * dt;
+    if (tau_next >= tau_target) {
+        float delta_tau = tau_target - tau;
+        float A = 0.5f * dt * (S - lastS);
+        float B = dt * lastS;
+        float C = -delta_tau;
+        float offset = trapezoid_offset(A, B, C, dt, S, lastS, delta_tau);
+        offset = fminf(fmaxf(offset, 0.0f), 1.0f);
+        float hit_t = t - dt * (1.0f - offset);
+        if (first_hit) {
+            ts.x = hit_t;
+            tau_target = kTauSecond;
+            first_hit = false;
+            if (tau_next >= tau_target) {
+                float delta_tau2 = tau_target - tau;
+                float offset2 = trapezoid_offset(A, B, -delta_tau2, dt, S, lastS, delta_tau2);  // not clamped
+                ts.y = t - dt * (1.0f - offset2);
+                return true;
+            }
+        } else {
+            ts.y = hit_t;
+            return true;
+        }
+    }
+    tau = tau_next;
+    return false;
+}
+// DeterminateEstimatedHit, float4 version (render.cuh:290-397)
+Ts4 estimated_hit_tau(const Volume& v, float alpha, f3 pos, f3 dir, float dis) {
+    float smax = v.max_density * alpha;
+    float t = 0.0f;
+    Ts4 ts = {0.0f, 0.0f, FLT_MAX, 0.0f};
+    float last_non_empty_t = 0.0f;
+    float t_step = 1.0f / (float)v.N;
+    if (smax * t_step > kTauStepMax) t_step = kTauStepMax / smax;
+    float tau = 0.0f, tau_target = kTauFirst;
+    bool first_hit = true;
+    int loop = 0;
+    float last_dens = 0.0f;
+    while (loop++ < 10000) {
+        if (t > dis) {
+            if (first_hit) ts.x = last_non_empty_t;
+            else ts.y = last_non_empty_t;
+            break;
+        }
+        float d = density(v, pos + (dir * t));
+        if (d > 0.0f) {
+            float S = d * alpha;
+            if (check_exit(S, last_dens, t_step, t, tau, tau_target, ts, first_hit)) break;
+            ts.z = fminf(t, ts.z);
+            t += t_step;
+            last_dens = S;
+        } else {
+            if (last_dens > 0.0f) {
+                float st = t_step + d;  // the negative SDF value shortens the step
+                if (st > 0.0f) {
+                    if (check_exit(0.0f, last_dens, st, t, tau, tau_target, ts, first_hit)) break;
+                    last_non_empty_t = t + st;
+                } else {
+                    last_non_empty_t = t;
+                }
+            }
+            t += fmaxf(-d * 1.0f, 0.5f / (float)v.N);
+            last_dens = 0.0f;
+        }
+    }
+    if (ts.z == FLT_MAX) ts.z = 0.0f;
+    float survival = expf(-tau);
+    ts.w = fminf(fmaxf(1.0f - survival, 0.0f) / 0.75f, 1.0f);
+    return ts;
+}
+// GetSampleT (render.cuh:464-486)
+Ts4 get_sample_t(const Volume& v, f3 ori, f3 dir, float alpha) {
+    float dis = ray_box_offset(ori, dir);
+    if (dis < 0.0f) return Ts4{0.0f, 0.0f, 0.0f, 0.0f};
+    f3 sp = ori + dir * dis;
+    float max_dis = ray_box_distance(sp, dir);
+    Ts4 ts = estimated_hit_tau(v, alpha, sp, dir, max_dis);
+    if (ts.x > 0.0f) {
+        ts.x += dis;
+        ts.z += dis;
+    }
+    if (ts.y > 0.0f) ts.y += dis;
+    return ts;
+}
+
+bool tile_gather_pixel(const Volume& v, const View& view, int row, int col, int off_x, int off_y, float* out) {
+    const int W = view.width, H = view.height;
+    // GatherSamples (volume.cu:257-283): sub-pixel offset in eighths, centre of the pixel
+    float u = 1.0f - ((float)col + (float)off_x / 8.0f + 0.5f) / (float)W;
+    float vv = 1.0f - ((float)row + (float)off_y / 8.0f + 0.5f) / (float)H;
+    float sy = view.tan_half_fov;
+    float sx = view.tan_half_fov * ((float)W / (float)H);
+    f3 fwd = normalize(-view.ori);
+    f3 dir = fwd + (view.right * ((u * 2.0f - 1.0f) * sx)) + (view.up * ((vv * 2.0f - 1.0f) * sy));
+    f3 right_off = (-(view.right * sx)) / ((float)W * 4.0f);
+    f3 up_off = (-(view.up * sy)) / ((float)H * 4.0f);
+    f3 L = normalize(view.light_dir);
+    float* tgrid = out + 3 * PC + 5;
+    for (int i = 0; i < TILE_ROW; ++i) out[i] = 0.0f;
+    // NNGatherSamples (render.cuh:1049-1103): 16 groups of 2x2 sub-rays
+    struct Sample {
+        f3 d;
+        float w;
+    } samples[16];
+    f3 avg = mk(0, 0, 0);
+    float minx = FLT_MAX, miny = FLT_MAX, maxx = -FLT_MAX, maxy = -FLT_MAX;
+    float hor = 0.0f, vert = 0.0f;
+    auto prepare = [&](float ox, float oy, bool do25, int gid, float& h, float& vt) {  // PrepareRay
+        f3 sd = dir + right_off * ox + up_off * oy;
+        sd = normalize(sd);
+        Ts4 ts = get_sample_t(v, view.ori, sd, view.alpha);
+        Sample r{sd, do25 ? ts.y : ts.x};
+        const float rough = 1.0f / 3.0f;
+        float min_t = ts.z * rough;
+        float max_t = fmaxf(ts.x, ts.y) * rough;
+        tgrid[gid] = (min_t + max_t) * 0.5f;
+        tgrid[gid + 1] = max_t - min_t;
+        tgrid[gid / 2 + 128] = ts.w;
+        if (r.w != 0.0f) {
+            minx = fminf(minx, ox);
+            miny = fminf(miny, oy);
+            maxx = fmaxf(maxx, ox);
+            maxy = fmaxf(maxy, oy);
+            avg = avg + ((ts.y > 0.0f) ? sd * 2.0f : sd);
+            float sum_t = ts.x + ts.y;
+            h += sum_t;
+            vt += sum_t;
+        }
+        return r;
+    };
+    int samp = 0, gid = 0, cb = 0;
+    for (int y = -3; y <= 3; y += 2, cb = (cb + 1) & 1, gid += 16) {
+        for (int x = -3; x <= 3; x += 2, ++samp, gid += 4) {
+            bool do25 = ((samp + cb) & 1) != 0;
+            float h = 0.0f, vt = 0.0f;
+            Sample t = prepare((float)x - 0.5f, (float)y - 0.5f, do25, gid, h, vt);
+            const float ox[3] = {(float)x + 0.5f, (float)x - 0.5f, (float)x + 0.5f};
+            const float oy[3] = {(float)y - 0.5f, (float)y + 0.5f, (float)y + 0.5f};
+            const int go[3] = {2, 16, 18};
+            for (int k = 0; k < 3; ++k) {
+                Sample c = prepare(ox[k], oy[k], do25, gid + go[k], h, vt);
+                if ((c.w != 0.0f && ((c.w < t.w) != do25)) || t.w == 0.0f) t = c;
+            }
+            hor += (x > 0) ? h : -h;
+            vert += (y > 0) ? vt : -vt;
+            int fill = samp / 2 + (do25 ? 8 : 0);
+            samples[fill] = t;
+        }
+    }
+    if (minx == FLT_MAX) {
+        out[0] = -1000.0f;
+        return false;
+    }
+    // GetMatrixFromNormal(seed, avgDir, 0) (render.cuh:694-708): with zero spread the cap
+    // direction is UniformSampleSphere(0, 0) = (0, 0, 1); the reference loops forever when
+    // |dot| < 1e-5, here the basis then keeps (0, 1, 0), (0, 0, 1)
+    f3 X = normalize(avg), Y = mk(0, 1, 0), Z = mk(0, 0, 1);
+    {
+        f3 r = uniform_sample_sphere(0.0f, 0.0f);
+        if (!((double)fabsf(dot(r, X)) < 0.00001)) {
+            Y = normalize(cross(X, r));
+            Z = cross(Y, X);
+        }
+    }
+    // gamma outputs (render.cuh:1114-1134)
+    {
+        minx -= 0.5f;
+        miny -= 0.5f;
+        maxx += 0.5f;
+        maxy += 0.5f;
+        f3 bl = normalize(dir + right_off * minx + up_off * miny);
+        f3 rc = normalize(dir + right_off * maxx);
+        f3 uc = normalize(dir + up_off * maxy);
+        auto sc = [](float x) { return fmaxf(-1.0f, fminf(1.0f, x)); };
+        float* g = out + 3 * PC;
+        g[0] = acosf(sc(dot(X, L)));
+        float gbl = acosf(sc(dot(bl, L)));
+        g[1] = acosf(sc(dot(rc, L))) - gbl;
+        g[2] = acosf(sc(dot(uc, L))) - gbl;
+        g[3] = hor;
+        g[4] = vert;
+    }
+    // SetupInputData2 (radiancePredict.cu:487-547)
+    float* Xv = out;
+    float* Xh = out + PC;
+    float* Xs = out + 2 * PC;
+    for (int i = 0; i < 16; ++i) {
+        const Sample& sm = samples[i];
+        if (sm.w != 0.0f) {
+            f3 sp = view.ori + sm.d * sm.w;
+            Xv[i] = logf(1.0f + view.alpha * mip_density(v, 0, sp) / 64.0f);
+            Xh[i] = logf(henyey_greenstein(dot(sm.d, L), view.g) + 1.0f);
+            Xs[i] = shadow_term_tr(v, sp, L, 0);
+        } else {
+            Xv[i] = Xh[i] = Xs[i] = 0.0f;
+        }
+    }
+    int sid = 0;
+    for (int i = 16; i < PC; ++i) {
+        while (samples[sid].w == 0.0f) sid = (sid + 1) & 15;
+        const Sample& sm = samples[sid];
+        sid = (sid + 1) & 15;
+        f3 sp = view.ori + sm.d * sm.w;
+        StencilPoint s = get_samples23(i);
+        float layer = (float)s.mip + 0.1f;
+        int mip = (int)layer;
+        f3 p;
+        if (s.type >= 4) {
+            f3 Ld = stencil_dir(i);
+            p = sp + ((X * Ld.x + Y * Ld.y + Z * Ld.z) * s.offset);
+        } else {
+            p = sp + L * s.offset;
+        }
+        Xv[i] = logf(1.0f + view.alpha * mip_density(v, mip, p) / 64.0f);
+        f3 ms = normalize(p - sp);
+        float radius = (float)(1 << mip) / 256.0f;
+        float angle = atanf(0.5f * radius / s.offset);
+        float c1 = dot(X, ms), c2 = dot(L, ms);
+        float uu = c1 * 0.5f + 0.5f, u2 = c2 * 0.5f + 0.5f;
+        float lv = angle / (3.1415926535f * 60.0f / 180.0f);
+        float h0 = sample_lut(v.lut, uu, lv), h1 = sample_lut(v.lut, u2, lv);
+        Xh[i] = logf(h0 * h1 + 1.0f);
+        Xs[i] = shadow_term_tr(v, p, L, mip - 1 > 0 ? mip - 1 : 0);
+    }
+    return true;
 }
 
 }  // namespace oracle

@@ -421,6 +421,26 @@ PixelState pixel_state(const Volume& v, const View& view, int row, int col);
 void predict_pixel(const Volume& v, const Net& net, const View& view, int row, int col, float out4[4],
                    float* feats_out = nullptr, int* steps_out = nullptr);
 
+// ---------------------------------------------------------------------------------------
+// Tile feature gather (training-data path): GatherSamples (volume.cu:257-283) ->
+// NNGatherSamples (render.cuh:1049-1142) -> PrepareRay (render.cuh:1006-1047) -> GetSampleT
+// (render.cuh:464-486) -> DeterminateEstimatedHit float4 version with CheckExitCondition
+// (render.cuh:199-340) -> PopulateInputData / SetupInputData2 (radiancePredict.cu:487-565).
+// One output row of 773 floats per pixel: [X_Val 0:192 | X_Val_Hg 192:384 | X_Val_Sub 384:576 |
+// gamma 576:581 | t grid 581:773] (mrpnn_helpers.py unpack_features).
+// ---------------------------------------------------------------------------------------
+constexpr int TILE_ROW = 773;
+// -logf(0.75f), -logf(0.76f), -logf(0.25f), correctly rounded (the reference evaluates logf)
+constexpr float kTauFirst = 0x1.269622p-2f, kTauStepMax = 0x1.1905fap-2f, kTauSecond = 0x1.62e43p+0f;
+struct Ts4 {
+    float x, y, z, w;  // t at survival 0.75, t at survival 0.25, first non-empty t, opacity
+};
+Ts4 estimated_hit_tau(const Volume& v, float alpha, f3 pos, f3 dir, float dis);
+Ts4 get_sample_t(const Volume& v, f3 ori, f3 dir, float alpha);
+// out: 773 floats.  Returns false for a pixel none of whose 64 sub-rays hits the medium (the
+// reference then writes only out[0] = -1000 and the t grid; here the rest of the row is 0).
+bool tile_gather_pixel(const Volume& v, const View& view, int row, int col, int off_x, int off_y, float* out);
+
 // Preparation steps
 void build_pyramid(const float* grid, int N, float* const mips[9]);                 // volume.cu:1474-1515
 void bake_tr(const float* const mips[9], f3 light_dir, float alpha, float* const tr[8]);  // volume.cu:1535-1560,1636-1653

@@ -163,6 +163,31 @@ int orc_render_predict(const float* grid, int N, float max_density, const float*
     return 0;
 }
 
+// Tile feature gather (GatherData, volume.cu:630-708): W*H rows of 773 floats; active[p] = 1
+// when some sub-ray of pixel p hits the medium.
+int orc_tile_gather(const float* grid, int N, float max_density, const float* mips, const float* tr,
+                    const float* lut, const orc_view* ov, int off_x, int off_y, float* out, int32_t* active) {
+    Volume v = to_volume(grid, N, max_density, mips, tr, lut);
+    View view = to_view(ov);
+    int W = view.width, H = view.height;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int p = 0; p < W * H; ++p) {
+        bool a = tile_gather_pixel(v, view, p / W, p % W, off_x, off_y, out + (size_t)p * TILE_ROW);
+        if (active) active[p] = a ? 1 : 0;
+    }
+    return 0;
+}
+// One sub-ray's GetSampleT (render.cuh:464-486): ts = (t75, t25, first non-empty t, opacity).
+void orc_sample_t(const float* grid, int N, float max_density, const float* ori3, const float* dir3, float alpha,
+                  float* ts4) {
+    Volume v = to_volume(grid, N, max_density, nullptr, nullptr, nullptr);
+    Ts4 t = get_sample_t(v, mk(ori3[0], ori3[1], ori3[2]), mk(dir3[0], dir3[1], dir3[2]), alpha);
+    ts4[0] = t.x;
+    ts4[1] = t.y;
+    ts4[2] = t.z;
+    ts4[3] = t.w;
+}
+
 // Time-bounded CPU baseline: renders the first n_pixels pixels in row-major order.
 int orc_render_predict_rows(const float* grid, int N, float max_density, const float* mips, const float* tr,
                             const float* lut, const orc_view* ov, int p0, int n_pixels, float* out4) {
