@@ -12,7 +12,9 @@ c4  64 views over 8 resident volumes (two each of 128^3, 256^3, 384^3, 512^3), r
 c5  serving loop: Poisson arrivals over 32 resident volumes, BF16 / FP8 batched against
     the plain per-frame FP32 launch;
 tile  tile feature gather (the training-input path, MRPNNInputData.fetch_input) at 1920x1080
-    over a 512^3 cloud, with the oracle's CPU gather of the same frame as baseline and check.
+    over a 512^3 cloud, with the oracle's CPU gather of the same frame as baseline and check;
+interactive  the GUI's progressive renderer at 1024x1024 over a 512^3 cloud (MRPNN BF16 / FP32,
+    path tracing with and without the histogram filter).
 
 All volumes are the seeded fBm clouds of the library (mrp_volume_fill_fbm); the oracle is the
 CPU restatement under oracle/ and is used here only as the timed CPU baseline and as the FP64
@@ -356,5 +358,70 @@ def tile(args):
     res["parity_vs_oracle"] = {"active_equal": bool(np.array_equal(got[:, 0] != -1000.0, act)),
                                "t_grid_max_abs": float(np.abs(got[:, 581:] - want[:, 581:]).max()),
                                "rows_within_2e-4": round(float(np.mean(rel.max(axis=1) <= 2e-4)), 5)}
+    ctx.close()
+    return res
+
+
+def interactive(args):
+    """Interactive progressive renderer (the GUI path, VolumeRender::Render into caller-owned
+    target / histogram / RGBA8 buffers) at 1024x1024 over a 512^3 cloud: frames per second of
+    the MRPNN predictor (BF16 and FP32, with and without the checkerboard) and of the
+    delta-tracking path (512 bounces, albedo 1, the GUI defaults) with and without the
+    histogram-similarity filter; the oracle's filter on the host cores over the same frame as
+    baseline and check."""
+    import torch
+
+    import mrpnn_amd as M
+    import oracle as O
+    ctx = M.Context()
+    N, W = 512, 1024
+    vol = M.Volume(ctx, N)
+    vol.fill_fbm(6, 4.0, 1)
+    vol.update()
+    it = M.Interactive(ctx, W, W)
+    target, histo, rgba = it.buffers()
+    v = M.camera_view(W, W, (0.67085, -0.03808, -0.04856), (0.34281, 0.70711, 0.61845), alpha=8.0, g=G)
+    v.jitter = -1.0
+    res = {"config": "interactive", "workload": "1024x1024 frames over a 512^3 fBm cloud (6 octaves), GUI camera "
+                                                "and sun, alpha 8, ACES; PT 512 bounces, albedo 1"}
+    cases = (("mrpnn_bf16", M.RT_MRPNN, M.BF16, 1, False, 32), ("mrpnn_bf16_checker", M.RT_MRPNN, M.BF16, 1, True, 32),
+             ("mrpnn_fp32", M.RT_MRPNN, M.FP32, 1, False, 8), ("pt_no_denoise", M.RT_PT, M.FP32, 512, False, 8),
+             ("pt_denoise", M.RT_PT, M.FP32, 512, True, 8))
+    calls = 0  # the renderer's checkerboard phase flips on every call
+    for name, rt, prec, mb, chk, frames in cases:
+        it.set_checkerboard(chk)
+        denoise = name == "pt_denoise"
+        for f in range(2):
+            it.render(vol, v, target, histo, rgba, rt, prec, mb, f, M.TONE_ACES, denoise)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(2, 2 + frames):
+            it.render(vol, v, target, histo, rgba, rt, prec, mb, f, M.TONE_ACES, denoise)
+        torch.cuda.synchronize()
+        calls += 2 + frames
+        last_frame, last_flip = 1 + frames, (calls - 1) % 2
+        dt = (time.perf_counter() - t0) / frames
+        res[name] = {"ms_per_frame": round(dt * 1e3, 3), "fps": round(1.0 / dt, 1)}
+        _log("interactive %s: %.3f ms per frame" % (name, dt * 1e3))
+    # the filter alone on the last PT state: the frame's denoise minus the plain resolve
+    t = target.cpu().numpy()
+    h = histo.cpu().numpy()
+    res["pt_filter_ms"] = round(res["pt_denoise"]["ms_per_frame"] - res["pt_no_denoise"]["ms_per_frame"], 3)
+    var = np.abs(h[:, 31] - h[:, 32] ** 2)
+    res["filtered_pixel_frac"] = round(float(np.mean(var > 0.01)), 4)
+    if args.no_cpu:
+        ctx.close()
+        return res
+    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    got = rgba.cpu().numpy().view(np.uint32)
+    t0 = time.perf_counter()
+    want = O.denoise(t, h, W, W, last_frame, last_flip, 1, 1.0, M.TONE_ACES)
+    cpu_s = time.perf_counter() - t0
+    d = np.abs(got.view(np.uint8).reshape(-1, 4).astype(int) - want.view(np.uint8).reshape(-1, 4).astype(int))
+    res["cpu_baseline"] = {"filter_ms_per_frame": round(cpu_s * 1e3, 1), "cores": threads, "kind": "port",
+                           "sample": "the histogram filter and ACES over one 1024x1024 PT frame"}
+    res["parity_vs_oracle"] = {"rgba_equal_frac": round(float(np.mean(d.max(axis=1) == 0)), 5),
+                               "rgba_max_step": int(d.max())}
     ctx.close()
     return res

@@ -221,6 +221,59 @@ int mrp_pt_render(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int spp0,
                   int nee_baked, float tr_scale, double* dev_mean3, double* dev_m2_3);
 
 /* ---------------------------------------------------------------------------------------
+ * Interactive progressive rendering: VolumeRender::Render(float3* target, Histogram*,
+ * unsigned* target2, size, ori, up, right, lightDir, lightColor, alpha, multiScatter, g,
+ * randseed, rt, toneType, denoise) (core/volume.cu:550-628), called per GUI frame by
+ * Camera::Render (core/camera.cu:259-261, core/GUI.hpp:1020-1027).
+ *
+ * Buffers are caller-owned device memory in the reference layout: dev_target holds 3 floats
+ * per pixel (the float3 accumulation image), dev_histo one mrp_histogram per pixel, dev_rgba
+ * one 0xAARRGGBB word per pixel; pixel (i, j) is at i*width + j, row i seeing
+ * v = (i + jitter) / height (row 0 at the bottom, unlike mrp_render_plain).
+ * --------------------------------------------------------------------------------------- */
+#define MRP_HISTO_SIZE 10
+typedef struct mrp_histogram { /* Histogram (core/volume.hpp:14-19): 132 bytes */
+    float bin[3 * MRP_HISTO_SIZE]; /* per channel, 10 bins of the tone-mapped sample c / (c + 1) */
+    float total_sample_num;
+    float x2; /* running mean of the squared luminance sum */
+    float x;  /* running mean of the luminance sum */
+} mrp_histogram;
+#define MRP_RT_PT 0    /* VolumeRender::RenderType (core/volume.hpp:98-100) */
+#define MRP_RT_RPNN 1
+#define MRP_RT_MRPNN 2
+#define MRP_TONE_NONE 0 /* toneType of Denoise / ReprojectionDenoise (core/volume.cu:332-335) */
+#define MRP_TONE_GAMMA 1
+#define MRP_TONE_ACES 2
+typedef struct mrp_interactive mrp_interactive;
+/* The renderer's frame state (VolumeRender's checkboard, last_predict and flip counter, the
+ * device exposure, core/volume.hpp:50-51, core/volume.cu:52,481-482) plus a width*height
+ * radiance buffer for the frame. */
+int mrp_interactive_create(mrp_ctx* ctx, int width, int height, mrp_interactive** out);
+void mrp_interactive_destroy(mrp_interactive* it);
+int mrp_interactive_set_checkerboard(mrp_interactive* it, int on); /* SetCheckboard; default 1 */
+int mrp_interactive_set_exposure(mrp_interactive* it, float exposure); /* SetExposure (volume.cu:1758-1761) */
+/* One frame.  render_type MRP_RT_MRPNN: the predictor in `precision` (MRP_FP32 = the
+ * reference's arithmetic), max(0, rgb) into the target (on half the pixels in checkerboard
+ * mode), then ReprojectionDenoise: the histogram's (total, x, x2) take the target's (r, g, b)
+ * and the target is tone-mapped.  MRP_RT_PT: one delta-tracking sample per pixel (baked
+ * TR NEE, max_bounce = multiScatter), blended into the target with weight 1 / (1 + n),
+ * n = randseed (randseed / 2 with checkerboard), and counted into the histogram; then
+ * Denoise: at randseed 0 the checkerboard's other half is filled with the mean of its four
+ * neighbours, later frames with denoise on replace pixels whose luminance variance exceeds
+ * 0.01 by an 11x11 average weighted by histogram similarity.  Tone mapping: exposure, then
+ * none / Gamma 1/2.2 / ACES (core/render.cuh:1144-1236), saturate, 8 bits per channel.
+ * The view's light, alpha and g update the LUT and TR mips as the reference does; the
+ * jitter stream is keyed by (view->seed, randseed).  Asynchronous.  MRP_RT_RPNN is not
+ * supported (MRP_ERR_INVALID). */
+int mrp_render_interactive(mrp_interactive* it, mrp_volume* vol, const mrp_view* view, int render_type, int precision,
+                           int max_bounce, int randseed, int tone_type, int denoise, float* dev_target,
+                           mrp_histogram* dev_histo, uint32_t* dev_rgba);
+/* Copies the last frame's radiance, max(0, rgb) and the distance (0 for PT), per pixel in the
+ * target layout into dev_out4 (width*height*4 floats); w = -2 for pixels the frame did not
+ * render (checkerboard).  No counterpart in the reference.  Asynchronous. */
+int mrp_interactive_radiance(mrp_interactive* it, float* dev_out4);
+
+/* ---------------------------------------------------------------------------------------
  * Serving loop (no counterpart in the reference, which renders one blocking view per
  * Render call, core/volume.cu:487-548).  Volumes are registered with the server and made
  * device-resident on demand when their grid + pyramids + padded copies (mrp_volume_bytes)

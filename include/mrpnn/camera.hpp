@@ -34,6 +34,10 @@ public:
                                      float alpha = 1, int multiScatter = 1, float g = 0, int sampleNum = 1,
                                      ToneMapFunction tone = None,
                                      VolumeRender::RenderType rt = VolumeRender::RenderType::PT, float exp = 1.0f);
+    // Interactive frame of the GUI (core/camera.cu:259-261): VolumeRender's interactive Render.
+    void Render(float3* target, Histogram* histo_buffer, unsigned int* target2, int2 size, int frame,
+                float3 lightDir, float3 lightColor = {1, 1, 1}, float alpha = 1, int multiScatter = 1, float g = 0,
+                int tone = 2, VolumeRender::RenderType rt = VolumeRender::RenderType::PT, bool denoise = false);
 };
 
 // Minimal PNG writer (stored deflate blocks), RGBA8, row-major.

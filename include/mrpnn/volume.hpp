@@ -28,6 +28,10 @@ struct Error : std::runtime_error {
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// core/volume.hpp:12-19: per-pixel histogram of the interactive renderer (132 bytes)
+#define HISTO_SIZE MRP_HISTO_SIZE
+typedef mrp_histogram Histogram;
+
 // omp.hpp:8 (x, y, z integer coordinates; u, v, w = (i + 0.5) / resolution)
 typedef std::function<float(int x, int y, int z, float u, float v, float w)> FillFunc;
 
@@ -64,6 +68,17 @@ public:
     std::vector<float4> Render(int2 size, float3 ori, float3 up, float3 right, float3 lightDir,
                                RenderType rt = RenderType::PT, float g = 0.857f, float alpha = 1,
                                float3 lightColor = {1, 1, 1}, int multiScatter = 512, int sampleNum = 1024);
+    // Interactive progressive frame into caller-owned device buffers (core/volume.cu:550-628):
+    // target size.x*size.y float3, histo_buffer one Histogram per pixel, target2 0xAARRGGBB.
+    // Synchronises the device before returning, as the reference.
+    void Render(float3* target, Histogram* histo_buffer, unsigned int* target2, int2 size, float3 ori, float3 up,
+                float3 right, float3 lightDir, float3 lightColor = {1, 1, 1}, float alpha = 1, int multiScatter = 1,
+                float g = 0, int randseed = 0, RenderType rt = RenderType::PT, int toneType = 2, bool denoise = false);
+    void SetCheckboard(bool checkboard);
+    void SetExposure(float exp);
+    // Extension: network arithmetic of the interactive MRPNN frames (MRP_FP32 = the reference's,
+    // MRP_BF16 / MRP_FP8 / MRP_BF16_1T the MFMA forms).
+    void SetInteractivePrecision(int precision);
 
     int resolution() const { return resolution_; }
     mrp_volume* handle() const { return vol_; }
@@ -77,6 +92,11 @@ private:
     float tr_scale_ = 1.0f;
     float tan_half_fov_ = 1.0f;
     uint32_t frame_counter_ = 0;
+    mrp_interactive* interactive_ = nullptr;
+    bool checkboard_ = true;  // core/volume.hpp:50
+    float exposure_ = 1.0f;   // core/volume.cu:52
+    int interactive_precision_ = MRP_FP32;
+    int iw_ = 0, ih_ = 0;
     void create(int resolution);
 };
 

@@ -140,6 +140,14 @@ def lib():
         L.mrp_predict_features.argtypes = [_vp, _vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.c_int, _vp]
         L.mrp_pt_render.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_float, _vp, _vp]
+        L.mrp_interactive_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.mrp_interactive_destroy.argtypes = [_vp]
+        L.mrp_interactive_destroy.restype = None
+        L.mrp_interactive_set_checkerboard.argtypes = [_vp, ctypes.c_int]
+        L.mrp_interactive_set_exposure.argtypes = [_vp, ctypes.c_float]
+        L.mrp_interactive_radiance.argtypes = [_vp, _vp]
+        L.mrp_render_interactive.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -390,6 +398,66 @@ def camera_view(width, height, position, light_dir, alpha=1.0, g=0.857, fov_deg=
     v.width, v.height = int(width), int(height)
     v.seed, v.frame, v.view_id, v.volume_index = 1, 0, 0, 0
     return v
+
+
+RT_PT, RT_RPNN, RT_MRPNN = 0, 1, 2  # VolumeRender::RenderType
+TONE_NONE, TONE_GAMMA, TONE_ACES = 0, 1, 2
+HISTO_FLOATS = 33  # mrp_histogram: 30 bins, total sample count, x2, x
+
+
+class Interactive:
+    """Interactive progressive renderer (mrp_render_interactive): VolumeRender::Render(float3*
+    target, Histogram*, unsigned* target2, ...) (core/volume.cu:550-628).  The caller owns the
+    device buffers: target (height*width, 3) float32, histo (height*width, 33) float32 and
+    rgba (height*width,) int32 holding 0xAARRGGBB words."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx, self.width, self.height = ctx, int(width), int(height)
+        self.h = _vp()
+        check(lib().mrp_interactive_create(ctx.h, self.width, self.height, ctypes.byref(self.h)))
+        ctx._children.add(self)
+
+    def close(self):
+        if self.h:
+            lib().mrp_interactive_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def buffers(self):
+        import torch
+        n = self.width * self.height
+        return (torch.zeros((n, 3), dtype=torch.float32, device="cuda"),
+                torch.zeros((n, HISTO_FLOATS), dtype=torch.float32, device="cuda"),
+                torch.zeros((n,), dtype=torch.int32, device="cuda"))
+
+    def set_checkerboard(self, on):
+        check(lib().mrp_interactive_set_checkerboard(self.h, 1 if on else 0))
+
+    def set_exposure(self, exposure):
+        check(lib().mrp_interactive_set_exposure(self.h, float(exposure)))
+
+    def render(self, volume, view, target, histo, rgba, render_type=RT_MRPNN, precision=FP32, max_bounce=1,
+               randseed=0, tone=TONE_ACES, denoise=False):
+        n = self.width * self.height
+        for t, k, name in ((target, 3 * n, "target"), (histo, HISTO_FLOATS * n, "histo"), (rgba, n, "rgba")):
+            if t.numel() < k or not t.is_contiguous() or t.element_size() != 4:
+                raise MrpError("render_interactive: %s must be a contiguous 4-byte tensor of %d elements" % (name, k))
+        check(lib().mrp_render_interactive(self.h, volume.h, ctypes.byref(view), int(render_type), int(precision),
+                                           int(max_bounce), int(randseed), int(tone), 1 if denoise else 0,
+                                           _ptr(target), _ptr(histo), _ptr(rgba)))
+
+    def radiance(self):
+        """The last frame's radiance (height*width, 4): rgb after max(0), distance; w = -2 where the
+        frame did not render (checkerboard)."""
+        import torch
+        out = torch.empty((self.width * self.height, 4), dtype=torch.float32, device="cuda")
+        check(lib().mrp_interactive_radiance(self.h, _ptr(out)))
+        return out
 
 
 class MRPNNInputData:

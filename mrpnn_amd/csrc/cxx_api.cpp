@@ -38,6 +38,7 @@ VolumeRender::VolumeRender(int resolution) { create(resolution); }
 VolumeRender::VolumeRender(std::string path, bool cacheTR) { LoadData(path, cacheTR); }
 
 VolumeRender::~VolumeRender() {
+    if (interactive_) mrp_interactive_destroy(interactive_);
     if (vol_) mrp_volume_destroy(vol_);
 }
 
@@ -153,6 +154,48 @@ std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float
     return out;
 }
 
+void VolumeRender::SetCheckboard(bool checkboard) { checkboard_ = checkboard; }
+void VolumeRender::SetExposure(float exp) { exposure_ = exp; }
+void VolumeRender::SetInteractivePrecision(int precision) {
+    if (precision < MRP_FP32 || precision > MRP_BF16_1T) throw Error(MRP_ERR_INVALID, "unknown precision");
+    interactive_precision_ = precision;
+}
+
+void VolumeRender::Render(float3* target, Histogram* histo_buffer, unsigned int* target2, int2 size, float3 ori,
+                          float3 up, float3 right, float3 lightDir, float3 lightColor, float alpha, int multiScatter,
+                          float g, int randseed, RenderType rt, int toneType, bool denoise) {
+    if (rt == RenderType::RPNN) throw Error(MRP_ERR_INVALID, "RPNN is not part of this library");
+    // the renderer keeps its frame state (flip counter, last mode) across calls of one size
+    if (!interactive_ || size.x != iw_ || size.y != ih_) {
+        if (interactive_) mrp_interactive_destroy(interactive_);
+        interactive_ = nullptr;
+        check(mrp_interactive_create(default_context(), size.x, size.y, &interactive_));
+        iw_ = size.x;
+        ih_ = size.y;
+    }
+    check(mrp_interactive_set_checkerboard(interactive_, checkboard_ ? 1 : 0));
+    check(mrp_interactive_set_exposure(interactive_, exposure_));
+    mrp_view v;
+    memset(&v, 0, sizeof v);
+    const float3 o[6] = {ori, up, right, lightDir, lightColor, scatter_rate_};
+    float* dst[6] = {v.ori, v.up, v.right, v.light_dir, v.light_color, v.scatter_rate};
+    for (int i = 0; i < 6; ++i) {
+        dst[i][0] = o[i].x;
+        dst[i][1] = o[i].y;
+        dst[i][2] = o[i].z;
+    }
+    v.alpha = alpha;
+    v.g = g;
+    v.tan_half_fov = tan_half_fov_;
+    v.jitter = -1.0f;
+    v.width = size.x;
+    v.height = size.y;
+    check(mrp_render_interactive(interactive_, vol_, &v, rt == RenderType::PT ? MRP_RT_PT : MRP_RT_MRPNN,
+                                 interactive_precision_, multiScatter, randseed, toneType, denoise ? 1 : 0,
+                                 reinterpret_cast<float*>(target), histo_buffer, target2));
+    if (hipDeviceSynchronize() != hipSuccess) throw Error(MRP_ERR_HIP, "interactive frame failed");
+}
+
 // ------------------------------------------------------------------------------------------
 Camera::Camera(VolumeRender& vol, std::string nm, float3 position, int res) {
     volume = &vol;
@@ -181,6 +224,13 @@ void Camera::SetPosition(float3 position) {  // camera.cu:176-192
 }
 
 void Camera::SetVolume(VolumeRender& vol) { volume = &vol; }
+
+void Camera::Render(float3* target, Histogram* histo_buffer, unsigned int* target2, int2 size, int frame,
+                    float3 lightDir, float3 lightColor, float alpha, int multiScatter, float g, int tone,
+                    VolumeRender::RenderType rt, bool denoise) {  // camera.cu:259-261
+    volume->Render(target, histo_buffer, target2, size, ori, up, right, lightDir, lightColor, alpha, multiScatter, g,
+                   frame, rt, tone, denoise);
+}
 
 std::vector<float4> Camera::RenderToFile(std::string path, float3 lightDir, float3 lightColor, float alpha,
                                          int multiScatter, float g, int sampleNum, ToneMapFunction tone,

@@ -8,63 +8,9 @@
 // {seed, frame} / {pixel, RNG_PT, sample, *}; per-pixel mean and M2 accumulate in FP64
 // (Welford), so a render can be split into sample chunks and merged exactly.
 #include "capi_internal.h"
-#include "dev_common.hpp"
+#include "dev_pathtrace.hpp"
 
 namespace mrp {
-
-__device__ inline float tr_stochastic(const DevVolume& v, PhiloxStream& rs, v3 pos, v3 dir, float dis, float alpha,
-                                      float tr_scale, unsigned& taps) {
-    const float smax = v.max_density * alpha;
-    float tr = 1.0f, t = 0.0f;
-    int loop = 0;
-    while (loop++ < 10000 && tr > 0.5f) {
-        float rk = rs.next();
-        t -= logf(1.0f - rk) / smax * tr_scale;
-        if (t > dis) break;
-        float d = density_full(v, pos + (dir * t));
-        ++taps;
-        float S = d * alpha;
-        tr *= 1.0f - fmaxf(0.0f, S / smax);
-        if (d < 0.0f) t -= d;
-    }
-    while (loop++ < 10000 && (double)tr > 0.001) {
-        float rk = rs.next();
-        t -= logf(1.0f - rk) / smax * tr_scale;
-        if (t > dis) break;
-        float d = density_full(v, pos + (dir * t));
-        ++taps;
-        float S = d * alpha;
-        if (rs.next() < fmaxf(0.0f, S / smax)) {
-            tr = 0.0f;
-            break;
-        }
-        if (d < 0.0f) t -= d;
-    }
-    return tr;
-}
-
-__device__ inline bool next_vertex(const DevVolume& v, PhiloxStream& rs, float alpha, float g, v3 pos, v3 dir,
-                                   float dis, v3* np, v3* nd, unsigned& taps) {
-    const float smax = v.max_density * alpha;
-    float t = 0.0f;
-    int loop = 0;
-    while (loop++ < 10000) {
-        float rk = rs.next();
-        t -= logf(1.0f - rk) / smax;
-        if (t > dis) return false;
-        rk = rs.next();
-        float d = density_full(v, pos + (dir * t));
-        ++taps;
-        float S = d * alpha;
-        if (S / smax > rk) break;
-        if (d < 0.0f) t -= d;
-    }
-    float e0 = rs.next();
-    float e1 = rs.next();
-    *nd = sample_hg(e0, e1, dir, g);
-    *np = (dir * t) + pos;
-    return true;
-}
 
 __global__ __launch_bounds__(64) void k_pathtrace(DevVolume vol, ViewDev vw, int spp0, int spp, int max_bounce,
                                                   int nee_baked, float tr_scale, double* __restrict__ mean3,
@@ -84,35 +30,7 @@ __global__ __launch_bounds__(64) void k_pathtrace(DevVolume vol, ViewDev vw, int
         const v3 fwd = normalize3(-vw.ori);
         const float sy = vw.tan_half_fov, sx = vw.tan_half_fov * ((float)vw.width / (float)vw.height);
         for (int s = spp0; s < spp0 + spp; ++s) {
-            PhiloxStream rs(vw.seed, vw.frame, pix, RNG_PT, (uint32_t)s);
-            float xu = vw.jitter, xv = vw.jitter;
-            if (vw.jitter < 0.0f) {
-                xu = rs.next();
-                xv = rs.next();
-            }
-            float u = 1.0f - ((float)col + xu) / (float)vw.width;
-            float vv = 1.0f - ((float)row + xv) / (float)vw.height;
-            v3 dir = fwd + (vw.right * ((u * 2.0f - 1.0f) * sx)) + (vw.up * ((vv * 2.0f - 1.0f) * sy));
-            dir = normalize3(dir);  // volume.cu:216
-            dir = normalize3(dir);  // render.cuh:540
-            v3 res = mk3(0, 0, 0);
-            float dis = ray_box_offset(vw.ori, dir);
-            if (dis >= 0.0f) {
-                v3 spos = vw.ori + dir * dis, rdir = dir, csr = mk3(1, 1, 1);
-                for (int b = 0; b < max_bounce; ++b) {
-                    v3 npos, ndir;
-                    float maxd = ray_box_distance(spos, rdir);
-                    if (!next_vertex(vol, rs, vw.alpha, vw.g, spos, rdir, maxd, &npos, &ndir, taps)) break;  // sky 0
-                    spos = npos;
-                    csr = csr * vw.scatter_rate;
-                    float ld = ray_box_distance(spos, light);
-                    float phase = henyey_greenstein(dot3(rdir, light), vw.g);
-                    float tr = nee_baked ? mip_tr(vol, 0, spos)
-                                         : tr_stochastic(vol, rs, spos, light, ld, vw.alpha, tr_scale, taps);
-                    res = res + vw.light_color * csr * (tr * phase);
-                    rdir = ndir;
-                }
-            }
+            const v3 res = pt_sample(vol, vw, light, fwd, sx, sy, row, col, pix, s, max_bounce, nee_baked, tr_scale, taps);
             const double x[3] = {(double)res.x, (double)res.y, (double)res.z};
             for (int c = 0; c < 3; ++c) {
                 double d = x[c] - mean[c];

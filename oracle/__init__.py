@@ -81,6 +81,16 @@ def lib():
                                         ctypes.c_float, ctypes.c_uint32, ctypes.c_int, _f]
         L.orc_fbm_volume.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, _f]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        _v = ctypes.c_void_p
+        L.orc_pt_accumulate.argtypes = [_f, _v, _f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int]
+        L.orc_predict_commit.argtypes = [_f, _f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_denoise.argtypes = [_f, _v, _u, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_float, ctypes.c_int]
+        L.orc_reprojection_denoise.argtypes = [_f, _v, _u, ctypes.c_int, ctypes.c_float, ctypes.c_int]
+        L.orc_tonemap.argtypes = [_f, ctypes.c_int, ctypes.c_float, ctypes.c_int, _u, _f]
+        L.orc_histo_compare.argtypes = [_v, _v]
+        L.orc_histo_compare.restype = ctypes.c_float
         w = np.fromfile(WEIGHTS, dtype="<f4")
         if L.orc_load_weights(w.ctypes.data_as(_f), int(w.size)) != 0:
             raise RuntimeError("oracle: weight asset does not match the network layout")
@@ -328,3 +338,58 @@ def orbit_view(width, height, dist=2.0, azimuth=0.0, elevation=0.0, fov_deg=40.0
     v.width, v.height = width, height
     v.seed, v.frame, v.view_id, v.volume_index = seed, frame, view_id, volume_index
     return v
+
+
+# ---------------------------------------------------------------------------------------
+# Interactive progressive path (core/volume.cu:57-372, core/render.cuh:1144-1236).
+# Images are (height*width, 3) float32 targets, (height*width, 33) float32 histograms
+# (30 bins, total, x2, x) and (height*width, 4) float32 frame radiance; all in place.
+# ---------------------------------------------------------------------------------------
+HISTO_FLOATS = 33
+
+
+def _c(a, dt):
+    assert a.dtype == dt and a.flags["C_CONTIGUOUS"], "contiguous %s array required" % dt
+    return a
+
+
+def pt_accumulate(target, histo, rad4, W, H, checker, flip, fnum):
+    _c(target, np.float32), _c(histo, np.float32), _c(rad4, np.float32)
+    lib().orc_pt_accumulate(_p(target), histo.ctypes.data, _p(rad4), W, H, int(checker), int(flip), int(fnum))
+
+
+def predict_commit(target, rad4, W, H, checker, flip):
+    _c(target, np.float32), _c(rad4, np.float32)
+    lib().orc_predict_commit(_p(target), _p(rad4), W, H, int(checker), int(flip))
+
+
+def denoise(target, histo, W, H, frame_num, flip, denoise_on, exposure, tone):
+    _c(target, np.float32), _c(histo, np.float32)
+    rgba = np.zeros(W * H, np.uint32)
+    lib().orc_denoise(_p(target), histo.ctypes.data, _p(rgba, _u), W, H, int(frame_num), int(flip), int(denoise_on),
+                      float(exposure), int(tone))
+    return rgba
+
+
+def reprojection_denoise(target, histo, exposure, tone):
+    _c(target, np.float32), _c(histo, np.float32)
+    n = target.shape[0]
+    rgba = np.zeros(n, np.uint32)
+    lib().orc_reprojection_denoise(_p(target), histo.ctypes.data, _p(rgba, _u), n, float(exposure), int(tone))
+    return rgba
+
+
+def tonemap(rgb, exposure, tone):
+    """(packed 0xAARRGGBB, the tone-mapped float rgb before saturation)."""
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+    n = rgb.shape[0]
+    rgba = np.zeros(n, np.uint32)
+    mapped = np.zeros((n, 3), np.float32)
+    lib().orc_tonemap(_p(rgb), n, float(exposure), int(tone), _p(rgba, _u), _p(mapped))
+    return rgba, mapped
+
+
+def histo_compare(x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return float(lib().orc_histo_compare(x.ctypes.data, y.ctypes.data))

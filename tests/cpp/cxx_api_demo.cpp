@@ -5,6 +5,9 @@
 #include <stdio.h>
 
 #include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "mrpnn/camera.hpp"
 
@@ -39,7 +42,34 @@ int main(int argc, char** argv) {
                                    VolumeRender::PT);
         double sp = 0;
         for (auto& p : pt) sp += p.x;
-        printf("active %d mean_pred %.6f mean_pt %.6f\n", act, s / img.size(), sp / pt.size());
+        // the GUI loop (core/GUI.hpp:1020-1027): progressive frames into device buffers
+        const int W = 64;
+        float3* accum = nullptr;
+        Histogram* histo = nullptr;
+        unsigned int* rgba = nullptr;
+        if (hipMalloc((void**)&accum, W * W * sizeof(float3)) != hipSuccess ||
+            hipMalloc((void**)&histo, W * W * sizeof(Histogram)) != hipSuccess ||
+            hipMalloc((void**)&rgba, W * W * sizeof(unsigned int)) != hipSuccess)
+            return 4;
+        if (hipMemset(accum, 0, W * W * sizeof(float3)) != hipSuccess) return 4;
+        v.SetCheckboard(false);
+        v.SetExposure(1.0f);
+        int frame = 0;
+        for (; frame < 3; ++frame)
+            cam.Render(accum, histo, rgba, int2{W, W}, frame, sun, float3{1, 1, 1}, 32.0f, 64, 0.857f, 2,
+                       VolumeRender::PT, true);
+        std::vector<unsigned int> px(W * W);
+        if (hipMemcpy(px.data(), rgba, px.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return 4;
+        int lit_pt = 0;
+        for (unsigned int p : px) lit_pt += (p & 0xffffffu) != 0 && (p >> 24) == 0xffu;
+        cam.Render(accum, histo, rgba, int2{W, W}, frame, sun, float3{1, 1, 1}, 32.0f, 1, 0.857f, 2,
+                   VolumeRender::MRPNN, true);
+        if (hipMemcpy(px.data(), rgba, px.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return 4;
+        int lit_pred = 0;
+        for (unsigned int p : px) lit_pred += (p & 0xffffffu) != 0 && (p >> 24) == 0xffu;
+        if (hipFree(accum) != hipSuccess || hipFree(histo) != hipSuccess || hipFree(rgba) != hipSuccess) return 4;
+        printf("active %d mean_pred %.6f mean_pt %.6f lit_pt %d lit_pred %d\n", act, s / img.size(),
+               sp / pt.size(), lit_pt, lit_pred);
         return act > 0 ? 0 : 3;
     } catch (const Error& e) {
         fprintf(stderr, "mrpnn error %d: %s\n", e.code, e.what());
