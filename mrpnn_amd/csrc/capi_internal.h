@@ -100,6 +100,9 @@ int mrp_tr_slot_find(const mrp_ctx* c, const mrp_volume* v, const float L[3], fl
 
 // fused.hip
 int mrp_fused_init(mrp_ctx* c);
+// mrp_render_batch with a checkerboard per view (ViewSlot::checker; nullptr = every pixel)
+int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
+                             int precision, float* const* dev_out4, const int* checker);
 void mrp_fused_free(mrp_ctx* c);
 int mrp_fused_collect_stats(mrp_ctx* c);
 int mrp_fused_join_aux(mrp_ctx* c);

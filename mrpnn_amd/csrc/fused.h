@@ -28,7 +28,7 @@ struct ViewSlot {
     int vol;      // index into the batch's DevVolume table
     int tile0;    // first 8x8 tile of this view in the flattened tile grid
     int tiles_x;  // tiles per row
-    int pad;
+    int checker;  // 0: every pixel; 1 + p: only pixels with (row + col + p) even, the others untouched
 };
 
 // One active pixel after the march: 64 bytes.

@@ -28,7 +28,7 @@ __global__ __launch_bounds__(64) void k_march_compact(const DevVolume* __restric
     const int tx = lt % sl.tiles_x, ty = lt / sl.tiles_x;
     const int col = tx * 8 + (threadIdx.x & 7);
     const int row = ty * 8 + (threadIdx.x >> 3);
-    const bool valid = row < vw.height && col < vw.width;
+    const bool valid = row < vw.height && col < vw.width && (sl.checker == 0 || ((row + col + sl.checker - 1) & 1) == 0);
     PixelState st;
     st.active = 0;
     st.taps = 0;

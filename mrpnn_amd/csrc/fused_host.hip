@@ -223,6 +223,11 @@ extern "C" int mrp_prepare_views(mrp_ctx* c, mrp_volume* const* vols, int n_vols
 
 extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
                                 int precision, float* const* dev_out4) {
+    return mrp_render_batch_checker(c, vols, n_vols, views, n_views, precision, dev_out4, nullptr);
+}
+
+int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
+                             int precision, float* const* dev_out4, const int* checker) {
     if (!c || !vols || !views || !dev_out4 || n_vols < 1 || n_views < 1)
         return mrp_fail(MRP_ERR_INVALID, "bad batch arguments");
     if (precision != MRP_FP32 && precision != MRP_BF16 && precision != MRP_FP8 && precision != MRP_BF16_1T)
@@ -306,7 +311,7 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
         s.vol = entry_of_slot[view_slot[i]];
         s.tile0 = t0;
         s.tiles_x = (views[i].width + 7) / 8;
-        s.pad = 0;
+        s.checker = checker ? checker[i] : 0;
         t0 += s.tiles_x * ((views[i].height + 7) / 8);
     }
     hipStream_t st = c->stream;

@@ -7,10 +7,9 @@
 //
 // The per-frame radiance comes from the predictor kernels of this library (MRPNN) or the
 // path sample shared with the judge (PT); the blend, the histogram update, the filter and the
-// tone map are elementwise or stencil work over 144 bytes per pixel.  The filter stages a
-// 42x26 halo of histograms (30 bins and the sample count, structure of arrays) and targets
-// in LDS per 32x16 tile, so each of the 121 comparisons per pixel reads LDS instead of
-// 132-byte records from L2.
+// tone map are elementwise or stencil work over 144 bytes per pixel.  The filter lists the
+// noisy pixels and spreads them over a persistent grid, one pixel's 121 histogram comparisons
+// per workgroup, so its cost follows the noisy pixels, not the image.
 #include "../../include/mrp_c_api.h"
 #include "capi_internal.h"
 #include "dev_pathtrace.hpp"
@@ -220,65 +219,88 @@ __global__ __launch_bounds__(256) void k_resolve_plain(const float* __restrict__
 // Denoise<true> after frame 0 (core/volume.cu:304-327): where the luminance variance of a pixel
 // exceeds 0.01, the average of its 11x11 neighbourhood (clamped at the borders) weighted by
 // max(0, 1 - 1.2 Compare(centre, neighbour)) (Compare: core/volume.cu:236-255).
-constexpr int kDTX = 32, kDTY = 16, kDR = 5;
-constexpr int kDHX = kDTX + 2 * kDR, kDHY = kDTY + 2 * kDR, kDHN = kDHX * kDHY;
+//
+// Noisy pixels are few and clustered, so a pass over all pixels tone-maps the quiet ones and
+// lists the noisy ones; a persistent grid then takes the listed pixels one per 128-thread
+// workgroup: 121 threads each compare one neighbour's histogram with the centre's (30 bins,
+// read through L1 from the 132-byte records) and stage its weight and target in LDS, and one
+// thread sums them in the reference's order, so the result is the same float for float.
+constexpr int kDR = 5, kDW = 2 * kDR + 1, kDN = kDW * kDW;
 
-__global__ __launch_bounds__(kDTX* kDTY) void k_denoise_filter(const float* __restrict__ target,
-                                                               const mrp_histogram* __restrict__ histo,
-                                                               uint32_t* __restrict__ rgba, int W, int H,
-                                                               float exposure, int tone) {
-    __shared__ float s_bin[kBins + 1][kDHN];  // 30 bins, then the sample count
-    __shared__ float s_t[3][kDHN];
-    const int c0 = blockIdx.x * kDTX, r0 = blockIdx.y * kDTY;
-    for (int k = threadIdx.x; k < kDHN; k += kDTX * kDTY) {
-        const int hr = k / kDHX, hc = k % kDHX;
-        const int r = clampi(r0 - kDR + hr, 0, H - 1), c = clampi(c0 - kDR + hc, 0, W - 1);
-        const size_t id = (size_t)r * W + c;
-        const float* h = reinterpret_cast<const float*>(histo + id);
-        for (int b = 0; b <= kBins; ++b) s_bin[b][k] = h[b];
-        s_t[0][k] = target[3 * id];
-        s_t[1][k] = target[3 * id + 1];
-        s_t[2][k] = target[3 * id + 2];
+__global__ __launch_bounds__(256) void k_denoise_mark(const float* __restrict__ target,
+                                                      const mrp_histogram* __restrict__ histo,
+                                                      uint32_t* __restrict__ rgba, int n, float exposure, int tone,
+                                                      int* __restrict__ list, unsigned* __restrict__ count) {
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    bool noisy = false;
+    if (id < n) {
+        const float hx2 = histo[id].x2, hxm = histo[id].x;
+        noisy = fabsf(hx2 - hxm * hxm) > 0.01f;
+        if (!noisy) rgba[id] = tonemap_pack(ld3(target + 3 * (size_t)id), exposure, tone);
     }
-    __syncthreads();
-    const int tx = threadIdx.x % kDTX, ty = threadIdx.x / kDTX;
-    const int r = r0 + ty, c = c0 + tx;
-    if (r >= H || c >= W) return;
-    const size_t id = (size_t)r * W + c;
-    const int ck = (ty + kDR) * kDHX + tx + kDR;
-    v3 res = mk3(s_t[0][ck], s_t[1][ck], s_t[2][ck]);
-    const float hx2 = histo[id].x2, hxm = histo[id].x;
-    const float variance = fabsf(hx2 - hxm * hxm);
-    if (variance > 0.01f) {
-        float cb[kBins];
-        for (int b = 0; b < kBins; ++b) cb[b] = s_bin[b][ck];
-        const float nx = s_bin[kBins][ck];
-        v3 acc = mk3(0.0f, 0.0f, 0.0f);
-        float ws = 0.0f;
-        for (int di = -kDR; di <= kDR; ++di) {
-            for (int dj = -kDR; dj <= kDR; ++dj) {
-                const int k = (ty + kDR + di) * kDHX + tx + kDR + dj;
-                const float ny = s_bin[kBins][k];
-                const float sqrt_y_x = sqrtf(ny / nx), sqrt_x_y = sqrtf(nx / ny);
-                float p = 0.0f, sum = 0.0f;
-                for (int b = 0; b < kBins; ++b) {
-                    const float hx = cb[b], hy = s_bin[b][k];
-                    if (hx != 0.0f || hy != 0.0f) {
-                        p += 1.0f;
-                        const float t = sqrt_y_x * hx - sqrt_x_y * hy;
-                        sum += t * t / (hx + hy);
-                    }
-                }
-                const float cmp = p == 0.0f ? 0.0f : sum / p;
-                const float w = (float)fmax(0.0, 1.0 - 1.2 * (double)cmp);
-                acc = acc + mk3(s_t[0][k], s_t[1][k], s_t[2][k]) * w;
+    const unsigned long long mask = __ballot(noisy);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    unsigned base = 0;
+    if (lane == __ffsll((long long)mask) - 1) base = atomicAdd(count, (unsigned)__popcll(mask));
+    base = __shfl(base, __ffsll((long long)mask) - 1);
+    if (noisy) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = id;
+}
+
+__global__ __launch_bounds__(128) void k_denoise_pixels(const float* __restrict__ target,
+                                                        const mrp_histogram* __restrict__ histo,
+                                                        uint32_t* __restrict__ rgba, int W, int H, float exposure,
+                                                        int tone, const int* __restrict__ list,
+                                                        const unsigned* __restrict__ count) {
+    __shared__ float s_w[kDN];
+    __shared__ float s_t[3][kDN];
+    const int tid = threadIdx.x;
+    const int n = (int)*count;
+    for (int q = blockIdx.x; q < n; q += gridDim.x) {
+        const int id = list[q];
+        const int r = id / W, c = id % W;
+        if (tid < kDN) {
+            const int nr = clampi(r + tid / kDW - kDR, 0, H - 1), nc = clampi(c + tid % kDW - kDR, 0, W - 1);
+            const size_t nid = (size_t)nr * W + nc;
+            const float* hx = reinterpret_cast<const float*>(histo + id);
+            const float* hy = reinterpret_cast<const float*>(histo + nid);
+            float bx[kBins], by[kBins];
+            for (int b = 0; b < kBins; ++b) {
+                bx[b] = hx[b];
+                by[b] = hy[b];
+            }
+            const float nx = hx[kBins], ny = hy[kBins];
+            const float sqrt_y_x = sqrtf(ny / nx), sqrt_x_y = sqrtf(nx / ny);
+            float p = 0.0f, sum = 0.0f;
+            for (int b = 0; b < kBins; ++b) {  // the skip of empty bin pairs as selects of exact zeros
+                const bool used = bx[b] != 0.0f || by[b] != 0.0f;
+                const float t = sqrt_y_x * bx[b] - sqrt_x_y * by[b];
+                const float qv = t * t / (bx[b] + by[b]);
+                p += used ? 1.0f : 0.0f;
+                sum += used ? qv : 0.0f;
+            }
+            const float cmp = p == 0.0f ? 0.0f : sum / p;
+            s_w[tid] = (float)fmax(0.0, 1.0 - 1.2 * (double)cmp);
+            s_t[0][tid] = target[3 * nid];
+            s_t[1][tid] = target[3 * nid + 1];
+            s_t[2][tid] = target[3 * nid + 2];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            v3 acc = mk3(0.0f, 0.0f, 0.0f);
+            float ws = 0.0f;
+            for (int nb = 0; nb < kDN; ++nb) {  // i outer, j inner as the reference
+                const float w = s_w[nb];
+                acc = acc + mk3(s_t[0][nb], s_t[1][nb], s_t[2][nb]) * w;
                 ws += w;
             }
+            rgba[id] = tonemap_pack(acc / ws, exposure, tone);
         }
-        res = acc / ws;
+        __syncthreads();
     }
-    rgba[id] = tonemap_pack(res, exposure, tone);
 }
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
 }  // namespace mrp
 
@@ -293,6 +315,9 @@ struct mrp_interactive {
     float exposure = 1.0f;   // the device global exposure (core/volume.cu:52)
     float4* d_rad = nullptr;   // the frame's radiance, target layout
     float4* d_pred = nullptr;  // predictor output, render layout
+    int* d_list = nullptr;        // noisy pixels of the filter
+    unsigned* d_count = nullptr;  // their number
+    int filter_grid = 0;          // persistent workgroups of the filter: 16 per CU
 };
 
 extern "C" int mrp_interactive_create(mrp_ctx* c, int width, int height, mrp_interactive** out) {
@@ -304,20 +329,27 @@ extern "C" int mrp_interactive_create(mrp_ctx* c, int width, int height, mrp_int
     it->W = width;
     it->H = height;
     const size_t bytes = (size_t)width * height * sizeof(float4);
-    if (hipMalloc(&it->d_rad, bytes) != hipSuccess || hipMalloc(&it->d_pred, bytes) != hipSuccess) {
-        hipFree(it->d_rad);
-        hipFree(it->d_pred);
+    if (hipMalloc(&it->d_rad, bytes) != hipSuccess || hipMalloc(&it->d_pred, bytes) != hipSuccess ||
+        hipMalloc(&it->d_list, (size_t)width * height * sizeof(int)) != hipSuccess ||
+        hipMalloc(&it->d_count, sizeof(unsigned)) != hipSuccess) {
+        (void)hipFree(it->d_rad);
+        (void)hipFree(it->d_pred);
+        (void)hipFree(it->d_list);
+        (void)hipFree(it->d_count);
         delete it;
         return mrp_fail(MRP_ERR_NOMEM, "interactive buffers (%zu bytes)", 2 * bytes);
     }
+    it->filter_grid = 16 * (c->props.multiProcessorCount > 0 ? c->props.multiProcessorCount : 256);
     *out = it;
     return MRP_OK;
 }
 
 extern "C" void mrp_interactive_destroy(mrp_interactive* it) {
     if (!it) return;
-    hipFree(it->d_rad);
-    hipFree(it->d_pred);
+    (void)hipFree(it->d_rad);
+    (void)hipFree(it->d_pred);
+    (void)hipFree(it->d_list);
+    (void)hipFree(it->d_count);
     delete it;
 }
 
@@ -377,14 +409,14 @@ extern "C" int mrp_render_interactive(mrp_interactive* it, mrp_volume* v, const 
     it->last_predict = predict;
     if (e != hipSuccess) return mrp_fail(MRP_ERR_HIP, "interactive: %s", hipGetErrorString(e));
     if (predict) {
+        // the batched predictor (its FP32 form equals the plain per-frame kernel bit for bit),
+        // marching only this frame's checkerboard half: target row i is render row H-1-i, so
+        // the set ((i + flip) + j) even is (row + col + H - 1 + flip) even
         float* pred = reinterpret_cast<float*>(it->d_pred);
-        if (precision == MRP_FP32) {
-            rc = mrp_render_plain_device(c, v, &view, pred);
-        } else {
-            mrp_volume* vols[1] = {v};
-            rc = mrp_prepare_views(c, vols, 1, &view, 1);
-            if (!rc) rc = mrp_render_batch(c, vols, 1, &view, 1, precision, &pred);
-        }
+        mrp_volume* vols[1] = {v};
+        const int checker = it->checkerboard ? 1 + ((H - 1 + flip) & 1) : 0;
+        rc = mrp_prepare_views(c, vols, 1, &view, 1);
+        if (!rc) rc = mrp_render_batch_checker(c, vols, 1, &view, 1, precision, &pred, &checker);
         if (rc) return rc;
         hipLaunchKernelGGL(k_commit_predict, dim3(blocks), dim3(256), 0, c->stream, it->d_pred, W, H,
                            it->checkerboard, flip, dev_target, it->d_rad);
@@ -404,9 +436,11 @@ extern "C" int mrp_render_interactive(mrp_interactive* it, mrp_volume* v, const 
         e = hipGetLastError();
         if (e == hipSuccess) {
             if (denoise && randseed != 0) {
-                dim3 grid((W + kDTX - 1) / kDTX, (H + kDTY - 1) / kDTY);
-                hipLaunchKernelGGL(k_denoise_filter, grid, dim3(kDTX * kDTY), 0, c->stream, dev_target, dev_histo,
-                                   dev_rgba, W, H, it->exposure, tone_type);
+                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c->stream, it->d_count);
+                hipLaunchKernelGGL(k_denoise_mark, dim3(blocks), dim3(256), 0, c->stream, dev_target, dev_histo,
+                                   dev_rgba, n, it->exposure, tone_type, it->d_list, it->d_count);
+                hipLaunchKernelGGL(k_denoise_pixels, dim3(it->filter_grid), dim3(128), 0, c->stream, dev_target,
+                                   dev_histo, dev_rgba, W, H, it->exposure, tone_type, it->d_list, it->d_count);
             } else {
                 hipLaunchKernelGGL(k_resolve_plain, dim3(blocks), dim3(256), 0, c->stream, dev_target, dev_histo,
                                    dev_rgba, W, H, randseed == 0 ? 1 : 0, flip, it->exposure, tone_type);
