@@ -14,7 +14,8 @@ c5  serving loop: Poisson arrivals over 32 resident volumes, BF16 / FP8 batched 
 tile  tile feature gather (the training-input path, MRPNNInputData.fetch_input) at 1920x1080
     over a 512^3 cloud, with the oracle's CPU gather of the same frame as baseline and check;
 interactive  the GUI's progressive renderer at 1024x1024 over a 512^3 cloud (MRPNN BF16 / FP32,
-    path tracing with and without the histogram filter).
+    path tracing with and without the histogram filter);
+ingest  loaders (.vox / .txt / .bin), the RGBE reader and the cost of an environment map.
 
 All volumes are the seeded fBm clouds of the library (mrp_volume_fill_fbm); the oracle is the
 CPU restatement under oracle/ and is used here only as the timed CPU baseline and as the FP64
@@ -424,4 +425,116 @@ def interactive(args):
     res["parity_vs_oracle"] = {"rgba_equal_frac": round(float(np.mean(d.max(axis=1) == 0)), 5),
                                "rgba_max_step": int(d.max())}
     ctx.close()
+    return res
+
+
+def ingest(args):
+    """Volume ingest and environment lighting: parse rates of the loaders (a 256^3 .vox and a
+    128^3 .txt written here), LoadData of a 512^3 .bin including upload and pyramid, the RGBE
+    reader on a 2048x1024 RLE image, and the cost of the environment map on the 1080p BF16
+    batched frame over a 512^3 cloud.  numpy's text parser is the CPU comparison."""
+    import struct
+    import tempfile
+
+    import torch
+
+    import mrpnn_amd as M
+    res = {"config": "ingest", "workload": "256^3 .vox and 128^3 .txt parse, 512^3 .bin LoadData, 2048x1024 RGBE read, "
+                                           "1080p BF16 frame with and without an environment map"}
+    rng = np.random.default_rng(1)
+    with tempfile.TemporaryDirectory() as d:
+        vals = rng.integers(0, 256, 256 ** 3).astype(np.uint8)
+        lut = [b"%d" % i for i in range(256)]
+        lines = [b" ".join(lut[v] for v in vals[i:i + 64]) for i in range(0, vals.size, 64)]
+        vox = os.path.join(d, "model.vox")
+        with open(vox, "wb") as f:
+            f.write(b"width 256\nheight 256\ndepth 256\n" + b"\n".join(lines) + b"\n")
+        mb = os.path.getsize(vox) / 1e6
+        t0 = time.perf_counter()
+        g, n, md = M.read_volume(vox)
+        dt = time.perf_counter() - t0
+        assert np.array_equal(g, vals.astype(np.float32) * np.float32(64.0 / 255.0))
+        res["vox_256"] = {"file_mb": round(mb, 1), "seconds": round(dt, 3), "mb_s": round(mb / dt, 1)}
+        txt = os.path.join(d, "grid.txt")
+        gt = rng.uniform(0, 1, 128 ** 3).astype(np.float32)
+        with open(txt, "w") as f:
+            f.write("128\n")
+            np.savetxt(f, gt.reshape(-1, 8), fmt="%.9g")
+        mb = os.path.getsize(txt) / 1e6
+        t0 = time.perf_counter()
+        g, n, md = M.read_volume(txt)
+        dt = time.perf_counter() - t0
+        assert np.array_equal(g, gt)
+        t0 = time.perf_counter()
+        gn = np.loadtxt(txt, skiprows=1, dtype=np.float32).ravel()
+        dn = time.perf_counter() - t0
+        assert np.array_equal(gn, gt)
+        res["txt_128"] = {"file_mb": round(mb, 1), "seconds": round(dt, 3), "mb_s": round(mb / dt, 1),
+                          "numpy_loadtxt_seconds": round(dn, 3)}
+        ctx = M.Context()
+        big = os.path.join(d, "CLOUD0")
+        grid = rng.uniform(0, 1, 512 ** 3).astype(np.float32)
+        with open(big + ".bin", "wb") as f:
+            f.write(struct.pack("<i", 512))
+            grid.tofile(f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        vol = M.Volume.load(ctx, big)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        vol2 = M.Volume.load(ctx, big)  # the named dataset's host grid is cached
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res["bin_512_load"] = {"first_s": round(t1 - t0, 3), "cached_s": round(t2 - t1, 3),
+                               "max_density": vol.max_density}
+        vol2.close()
+        vol.close()
+        # RGBE: 2048x1024, RLE scanlines of a smooth sky
+        W, H = 2048, 1024
+        hdr = os.path.join(d, "sky.hdr")
+        yy = np.linspace(0, 1, H)[:, None]
+        xx = np.linspace(0, 1, W)[None, :]
+        img = np.stack([0.5 + 0.5 * yy * np.ones_like(xx), 0.7 * np.ones((H, W)), 1.0 + 0 * xx * yy], -1)
+        m = img.max(-1)
+        e = np.ceil(np.log2(m)).astype(int)
+        rgbe = np.concatenate([np.clip(img / np.ldexp(1.0, e)[..., None] * 256, 0, 255).astype(np.uint8),
+                               (e + 128).astype(np.uint8)[..., None]], -1)
+        with open(hdr, "wb") as f:
+            f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n" % (H, W))
+            for j in range(H):
+                f.write(bytes([2, 2, W >> 8, W & 255]))
+                for c in range(4):
+                    comp = rgbe[j, :, c]
+                    i = 0
+                    while i < W:
+                        r = 1
+                        while i + r < W and comp[i + r] == comp[i] and r < 127:
+                            r += 1
+                        if r >= 2:
+                            f.write(bytes([128 + r, comp[i]]))
+                            i += r
+                        else:
+                            f.write(bytes([1, comp[i]]))
+                            i += 1
+        t0 = time.perf_counter()
+        env = M.read_hdr(hdr)
+        res["hdr_2048x1024_s"] = round(time.perf_counter() - t0, 4)
+        # the environment map on the batched 1080p frame
+        vol = M.Volume(ctx, 512)
+        vol.fill_fbm(6, 4.0, 1)
+        vol.update()
+        v = M.orbit_view(1920, 1080, dist=1.6, azimuth=0.5, elevation=0.3, fov_deg=45.0, light=_sun(40, 30),
+                         alpha=64.0, g=G)
+        out = torch.zeros((1920 * 1080, 4), device="cuda")
+        for name, e_map in (("no_env", None), ("env", env)):
+            ctx.set_env_map(e_map)
+            ctx.prepare_views([vol], [v])
+            ctx.render_batch([vol], [v], [out], M.BF16)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(20):
+                ctx.render_batch([vol], [v], [out], M.BF16)
+            torch.cuda.synchronize()
+            res["frame_1080p_bf16_ms_" + name] = round((time.perf_counter() - t0) / 20 * 1e3, 3)
+        ctx.close()
     return res

@@ -210,6 +210,47 @@ int mrp_predict_features(mrp_ctx* ctx, const float* dev_feats, int n, const floa
 int mrp_gather_tiles(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int pixel_offset_x, int pixel_offset_y,
                      float* dev_out);
 
+/* ---------------------------------------------------------------------------------------
+ * Volume ingest (VolumeRender::LoadData and its DataCache, core/volume.cu:970-1149).
+ * A path containing CLOUD0, CLOUD1, MODEL0 or MODEL1 reads <path>.bin (int32 N, then N^3
+ * float32) with that dataset's fixed max density (510.416656, 505.490356, 64, 23.8960018);
+ * a path ending in "txt" (N, then N^3 floats) or "vox" (256^3 integers as text, 64 per line,
+ * header lines starting with w / h / d, value * 64 / 255) is parsed and written back as
+ * <path>.bin (best effort); a path ending in "bin" is read as such.  max_density is then
+ * max(grid) (at least 1e-5; the reference leaves it unchanged for txt / vox).
+ * --------------------------------------------------------------------------------------- */
+/* Host only (no device work): *host_grid is malloc'd, N^3 floats; free with mrp_host_free. */
+int mrp_volume_read(const char* path, float** host_grid, int* resolution, float* max_density);
+void mrp_host_free(void* p);
+/* mrp_volume_read + mrp_volume_create / set_data / update.  The four named datasets' host
+ * grids stay cached in the context (the reference's 4-slot DataCache): loading one again
+ * reads no file. */
+int mrp_volume_load(mrp_ctx* ctx, const char* path, mrp_volume** out, float* max_density);
+
+/* ---------------------------------------------------------------------------------------
+ * Environment lighting (SetHDRI / SetEnvExp, core/volume.cu:1698-1736; SkyBox,
+ * core/render.cuh:24-27).  With a map set, rays that leave the volume (sky pixels of the
+ * predictor, escaping paths of the path tracer) see the map at (atan2(-z, x) / 2pi + 0.5,
+ * acos(y) / pi), bilinear, wrapped, times the exposure; without one they see black.
+ * --------------------------------------------------------------------------------------- */
+/* Radiance RGBE .hdr reader (flat and RLE scanlines, as core/hdr_loader.h): width*height
+ * RGBA floats (alpha 0), rows in file order; malloc'd, free with mrp_host_free.  Host only. */
+int mrp_hdr_read(const char* path, float** host_rgba, int* width, int* height);
+/* host_rgba NULL removes the map. */
+int mrp_ctx_set_env_map(mrp_ctx* ctx, const float* host_rgba, int width, int height);
+int mrp_ctx_load_hdri(mrp_ctx* ctx, const char* path);
+int mrp_ctx_set_env_exp(mrp_ctx* ctx, float exposure); /* default 1 */
+/* The reference's per-frame light choice with a map set (core/volume.cu:494-509, 552-568):
+ * with probability rate = 4 e / (4 e + max(light_color)) (u < rate, u the caller's uniform
+ * draw in [0, 1)) the light becomes direction Roberts2(counter) on the sphere with colour
+ * map(direction) * 4 e / rate (clamped bilinear of the host image); otherwise the sun with
+ * colour light_color / (1 - rate).  Returns 1 when the environment was chosen, 0 otherwise
+ * (and copies the light through when there is no map).  Host only. */
+int mrp_env_light(const float* host_rgba, int width, int height, float env_exp, const float light_dir[3],
+                  const float light_color[3], int counter, float u, float out_dir[3], float out_color[3]);
+int mrp_ctx_env_light(mrp_ctx* ctx, const float light_dir[3], const float light_color[3], int counter, float u,
+                      float out_dir[3], float out_color[3]);
+
 /* Delta-tracking path tracer (judge): RenderCamera<false> / CalculateRadiance
  * (core/volume.cu:198-228, core/render.cuh:535-597) with Philox streams keyed
  * (seed, frame) / (pixel, sample) and FP64 per-pixel Welford accumulators.

@@ -46,7 +46,8 @@ public:
     float min_val = 0.0f;
 
     explicit VolumeRender(int resolution);
-    // .bin file: int32 N, then N^3 float32 in (x*N+y)*N+z order (core/volume.cu:1011-1035)
+    // LoadData (core/volume.cu:970-1149): a named dataset (CLOUD0/1, MODEL0/1) from <path>.bin
+    // with its fixed max density, or a .txt / .vox grid (written back as <path>.bin) / .bin file
     explicit VolumeRender(std::string path, bool cacheTR = false);
     ~VolumeRender();
     VolumeRender(const VolumeRender&) = delete;
@@ -58,6 +59,12 @@ public:
     void Update(bool process_mips = true, bool need_reset_datas_dev = true);
     void Update_TR(float3 lightDir, float alpha = 64.0f, bool CPU = false);
     void UpdateHGLut(float g);
+
+    // Environment map (core/volume.cu:1698-1736): a Radiance .hdr file; sky pixels and escaping
+    // paths see it, and with it set each MRPNN frame lights the cloud either by the sun or by one
+    // direction of the map (core/volume.cu:494-509, 552-568)
+    void SetHDRI(std::string path);
+    void SetEnvExp(float exp);
 
     void SetTrScale(float scale);
     void SetScatterRate(float rate);
@@ -97,6 +104,11 @@ private:
     float exposure_ = 1.0f;   // core/volume.cu:52
     int interactive_precision_ = MRP_FP32;
     int iw_ = 0, ih_ = 0;
+    bool has_env_ = false;
+    int rand_cpu_ = 0;           // core/volume.cu:482
+    uint32_t env_draws_ = 0;     // the stream of the sun / environment choice
+    float env_draw();            // uniform [0, 1) in place of rand() / RAND_MAX
+    void env_light(float3& lightDir, float3& lightColor);
     void create(int resolution);
 };
 

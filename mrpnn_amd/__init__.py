@@ -140,6 +140,19 @@ def lib():
         L.mrp_predict_features.argtypes = [_vp, _vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.c_int, _vp]
         L.mrp_pt_render.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_float, _vp, _vp]
+        L.mrp_volume_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(_fp), ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_float)]
+        L.mrp_host_free.argtypes = [_vp]
+        L.mrp_host_free.restype = None
+        L.mrp_volume_load.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float)]
+        L.mrp_hdr_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(_fp), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]
+        L.mrp_ctx_set_env_map.argtypes = [_vp, _fp, ctypes.c_int, ctypes.c_int]
+        L.mrp_ctx_load_hdri.argtypes = [_vp, ctypes.c_char_p]
+        L.mrp_ctx_set_env_exp.argtypes = [_vp, ctypes.c_float]
+        L.mrp_env_light.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _fp, _fp, ctypes.c_int,
+                                    ctypes.c_float, _fp, _fp]
+        L.mrp_ctx_env_light.argtypes = [_vp, _fp, _fp, ctypes.c_int, ctypes.c_float, _fp, _fp]
         L.mrp_interactive_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
         L.mrp_interactive_destroy.argtypes = [_vp]
         L.mrp_interactive_destroy.restype = None
@@ -199,6 +212,20 @@ class Context:
     def set_option(self, option, value):
         check(lib().mrp_ctx_set_option(self.h, int(option), int(value)))
 
+    def set_env_map(self, rgba):
+        """Environment map (height, width, 4) float32, or None to remove it (SetHDRI)."""
+        if rgba is None:
+            check(lib().mrp_ctx_set_env_map(self.h, None, 0, 0))
+            return
+        img = np.ascontiguousarray(rgba, np.float32)
+        check(lib().mrp_ctx_set_env_map(self.h, img.ctypes.data_as(_fp), int(img.shape[1]), int(img.shape[0])))
+
+    def load_hdri(self, path):
+        check(lib().mrp_ctx_load_hdri(self.h, os.fsencode(path)))
+
+    def set_env_exp(self, exposure):
+        check(lib().mrp_ctx_set_env_exp(self.h, float(exposure)))
+
     def get_option(self, option):
         v = ctypes.c_int()
         check(lib().mrp_ctx_get_option(self.h, int(option), ctypes.byref(v)))
@@ -255,12 +282,28 @@ class Context:
 
 
 class Volume:
-    def __init__(self, ctx, N):
+    def __init__(self, ctx, N, _handle=None):
         self.ctx = ctx
         self.N = N
         self.h = _vp()
-        check(lib().mrp_volume_create(ctx.h, int(N), ctypes.byref(self.h)))
+        if _handle is not None:
+            self.h = _handle
+        else:
+            check(lib().mrp_volume_create(ctx.h, int(N), ctypes.byref(self.h)))
         ctx._children.add(self)
+
+    @classmethod
+    def load(cls, ctx, path):
+        """VolumeRender::LoadData (core/volume.cu:970-1149) through mrp_volume_load: a named
+        dataset (CLOUD0/1, MODEL0/1) from <path>.bin, or a .txt / .vox / .bin grid, uploaded and
+        its pyramid built."""
+        h = _vp()
+        md = ctypes.c_float()
+        check(lib().mrp_volume_load(ctx.h, os.fsencode(path), ctypes.byref(h), ctypes.byref(md)))
+        vol = cls(ctx, 0, _handle=h)
+        vol.max_density = md.value
+        vol.path = str(path)
+        return vol
 
     def close(self):
         if self.h:
@@ -460,6 +503,47 @@ class Interactive:
         return out
 
 
+def read_volume(path):
+    """Host-side grid of `path` as LoadData reads it: (float32 N^3 array (x slowest), N,
+    max_density).  Writes <path>.bin after a .txt / .vox parse, as the reference."""
+    p = _fp()
+    n = ctypes.c_int()
+    md = ctypes.c_float()
+    check(lib().mrp_volume_read(os.fsencode(path), ctypes.byref(p), ctypes.byref(n), ctypes.byref(md)))
+    try:
+        grid = np.ctypeslib.as_array(p, shape=(n.value ** 3,)).copy()
+    finally:
+        lib().mrp_host_free(p)
+    return grid, n.value, md.value
+
+
+def read_hdr(path):
+    """Radiance RGBE image as (height, width, 4) float32 (alpha 0), rows in file order."""
+    p = _fp()
+    w, h = ctypes.c_int(), ctypes.c_int()
+    check(lib().mrp_hdr_read(os.fsencode(path), ctypes.byref(p), ctypes.byref(w), ctypes.byref(h)))
+    try:
+        img = np.ctypeslib.as_array(p, shape=(h.value * w.value * 4,)).copy()
+    finally:
+        lib().mrp_host_free(p)
+    return img.reshape(h.value, w.value, 4)
+
+
+def env_light(rgba, env_exp, light_dir, light_color, counter, u):
+    """The per-frame choice between the sun and an environment direction (mrp_env_light):
+    (chose_env, direction, colour)."""
+    img = None if rgba is None else np.ascontiguousarray(rgba, np.float32)
+    d = np.asarray(light_dir, np.float32)
+    c = np.asarray(light_color, np.float32)
+    od = np.zeros(3, np.float32)
+    oc = np.zeros(3, np.float32)
+    h, w = (img.shape[0], img.shape[1]) if img is not None else (0, 0)
+    r = lib().mrp_env_light(img.ctypes.data_as(_fp) if img is not None else None, w, h, float(env_exp),
+                            d.ctypes.data_as(_fp), c.ctypes.data_as(_fp), int(counter), float(u),
+                            od.ctypes.data_as(_fp), oc.ctypes.data_as(_fp))
+    return bool(r), od, oc
+
+
 class MRPNNInputData:
     """The reference's training-input fetcher (mrpnn_input_data_wrapper.cpp:34-103): a volume,
     a camera at 1/8 of the image resolution and a (height/8, width/8, 773) device tensor that
@@ -469,6 +553,8 @@ class MRPNNInputData:
 
     def __init__(self, width, height, volume):
         import torch
+        if isinstance(volume, (str, bytes, os.PathLike)):
+            raise MrpError("MRPNNInputData: pass a Volume, or use MRPNNInputData.from_path(width, height, ctx, path)")
         self.volume = volume
         self.w, self.h = int(width) // 8, int(height) // 8
         if self.w < 1 or self.h < 1:
@@ -481,7 +567,19 @@ class MRPNNInputData:
         self.volume.gather_tiles(v, int(pixelOffset[0]), int(pixelOffset[1]), out=self.tensor)
         return self.tensor
 
+    @classmethod
+    def from_path(cls, width, height, ctx, vol_path):
+        """The reference's constructor MRPNNInputData(width, height, vol_path)
+        (mrpnn_input_data_wrapper.cpp:43-67): the volume loaded as LoadData does."""
+        return cls(width, height, Volume.load(ctx, vol_path))
+
     def swap_data(self, volume):
+        """A resident Volume, or a path loaded as LoadData when it differs from the current
+        one (mrpnn_input_data_wrapper.cpp:94-101)."""
+        if isinstance(volume, (str, os.PathLike)):
+            if getattr(self.volume, "path", None) == str(volume):
+                return
+            volume = Volume.load(self.volume.ctx, volume)
         self.volume = volume
 
 

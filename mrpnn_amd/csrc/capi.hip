@@ -170,6 +170,7 @@ extern "C" int mrp_ctx_destroy(mrp_ctx* c) {
     hipFree(c->d_fast);
     hipFree(c->d_lut_rows);
     hipFree(c->d_plain_out);
+    hipFree(c->d_env);
     for (TrSlot& t : c->tr_slots) {
         hipFree(t.d_tr);
         hipFree(t.d_ptr);
@@ -553,6 +554,10 @@ DevVolume mrp_dev_volume(const mrp_volume* v) {
     d.occ_n = v->occ_n;
     d.N = v->N;
     d.max_density = v->max_density;
+    d.env = v->ctx->d_env;
+    d.env_w = v->ctx->env_w;
+    d.env_h = v->ctx->env_h;
+    d.env_exp = v->ctx->env_exp;
     return d;
 }
 

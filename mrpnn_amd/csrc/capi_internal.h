@@ -66,6 +66,15 @@ struct mrp_ctx {
     std::vector<TrSlot> tr_slots;  // TR cache of the batched path
     int tr_slot_cap = 64;
     uint64_t tr_stamp = 0;
+    // environment map (SetHDRI / SetEnvExp): device texels, and the host copy the light
+    // sampling reads
+    float4* d_env = nullptr;
+    std::vector<float> env_host;
+    int env_w = 0, env_h = 0;
+    float env_exp = 1.0f;
+    // LoadData's DataCache: host grids of the four named datasets, read once per context
+    std::vector<float> data_cache[4];
+    int data_cache_n[4] = {0, 0, 0, 0};
 };
 
 struct mrp_volume {

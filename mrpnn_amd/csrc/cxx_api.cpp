@@ -42,24 +42,54 @@ VolumeRender::~VolumeRender() {
     if (vol_) mrp_volume_destroy(vol_);
 }
 
-// .bin loader (core/volume.cu:1011-1035): int32 N, then N^3 floats; max_density = max(grid).
+// LoadData (core/volume.cu:970-1149) through mrp_volume_read: the named datasets' fixed max
+// densities, .txt / .vox parsing with the <path>.bin write-back, plain .bin files.
 void VolumeRender::LoadData(std::string path, bool /*cacheTR*/) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) throw Error(MRP_ERR_IO, "File not found: " + path);
-    int32_t n = 0;
-    f.read((char*)&n, 4);
-    if (!f || n < 2 || n > 2048) throw Error(MRP_ERR_IO, "bad volume header in " + path);
+    float* grid = nullptr;
+    int n = 0;
+    float md = 0.0f;
+    check(mrp_volume_read(path.c_str(), &grid, &n, &md));
     if (vol_) {
         mrp_volume_destroy(vol_);
         vol_ = nullptr;
     }
-    create(n);
-    f.read((char*)datas_.data(), (std::streamsize)(datas_.size() * sizeof(float)));
-    if (!f) throw Error(MRP_ERR_IO, "truncated volume file " + path);
-    float m = 0.00001f;
-    for (float v : datas_) m = v > m ? v : m;
-    max_density = m;
+    try {
+        create(n);
+    } catch (...) {
+        mrp_host_free(grid);
+        throw;
+    }
+    memcpy(datas_.data(), grid, datas_.size() * sizeof(float));
+    mrp_host_free(grid);
+    max_density = md;
     Update();
+}
+
+void VolumeRender::SetHDRI(std::string path) {
+    check(mrp_ctx_load_hdri(default_context(), path.c_str()));
+    has_env_ = true;
+}
+
+void VolumeRender::SetEnvExp(float exp) { check(mrp_ctx_set_env_exp(default_context(), exp)); }
+
+float VolumeRender::env_draw() {  // splitmix32-style hash of a counter: a uniform in [0, 1)
+    uint32_t x = 0x9e3779b9u * (++env_draws_);
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return (float)(x >> 8) * (1.0f / 16777216.0f);
+}
+
+void VolumeRender::env_light(float3& lightDir, float3& lightColor) {
+    if (!has_env_) return;
+    const float d[3] = {lightDir.x, lightDir.y, lightDir.z}, c[3] = {lightColor.x, lightColor.y, lightColor.z};
+    float od[3], oc[3];
+    int rc = mrp_ctx_env_light(default_context(), d, c, rand_cpu_, env_draw(), od, oc);
+    if (rc < 0) throw Error(MRP_ERR_INVALID, "environment light");
+    lightDir = float3{od[0], od[1], od[2]};
+    lightColor = float3{oc[0], oc[1], oc[2]};
 }
 
 void VolumeRender::SetData(int x, int y, int z, float value) {
@@ -127,8 +157,44 @@ std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float
     frame_counter_ += (uint32_t)(sampleNum > 0 ? sampleNum : 1);
     std::vector<float4> out((size_t)size.x * size.y);
     if (rt == RenderType::MRPNN) {
-        check(mrp_render_plain(default_context(), vol_, &v, sampleNum, (float*)out.data()));
-        return out;
+        if (!has_env_) {
+            check(mrp_render_plain(default_context(), vol_, &v, sampleNum, (float*)out.data()));
+            return out;
+        }
+        // with an environment map every frame draws its own light (core/volume.cu:494-509);
+        // Roberts2 uses the same counter through one Render call, as the reference
+        const int frames = sampleNum > 0 ? sampleNum : 1;
+        float* dev = nullptr;
+        if (hipMalloc((void**)&dev, out.size() * sizeof(float4)) != hipSuccess)
+            throw Error(MRP_ERR_NOMEM, "frame allocation failed");
+        std::vector<float4> acc(out.size());
+        int rc = MRP_OK;
+        for (int f = 0; f < frames && rc == MRP_OK; ++f) {
+            mrp_view vf = v;
+            vf.frame = v.frame + (uint32_t)f;
+            float3 ld = lightDir, lc = lightColor;
+            env_light(ld, lc);
+            vf.light_dir[0] = ld.x;
+            vf.light_dir[1] = ld.y;
+            vf.light_dir[2] = ld.z;
+            vf.light_color[0] = lc.x;
+            vf.light_color[1] = lc.y;
+            vf.light_color[2] = lc.z;
+            rc = mrp_render_plain_device(default_context(), vol_, &vf, dev);
+            if (rc == MRP_OK && hipMemcpy(out.data(), dev, out.size() * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess)
+                rc = MRP_ERR_HIP;
+            const float rate = 1.0f / (1.0f + (float)f);  // the progressive average (volume.cu:226-227)
+            for (size_t i = 0; rc == MRP_OK && i < out.size(); ++i) {
+                float4& a = acc[i];
+                const float4& b = out[i];
+                a = f == 0 ? b
+                           : float4{a.x * (1.0f - rate) + b.x * rate, a.y * (1.0f - rate) + b.y * rate,
+                                    a.z * (1.0f - rate) + b.z * rate, a.w * (1.0f - rate) + b.w * rate};
+            }
+        }
+        (void)hipFree(dev);
+        check(rc);
+        return acc;
     }
     // PT: sampleNum frames of 32 paths (core/volume.cu:222), NEE through the TR mips
     check(mrp_ctx_update_hg_lut(default_context(), g, 1u));
@@ -184,6 +250,17 @@ void VolumeRender::Render(float3* target, Histogram* histo_buffer, unsigned int*
         dst[i][1] = o[i].y;
         dst[i][2] = o[i].z;
     }
+    if (rt != RenderType::PT && has_env_) {
+        float3 ld = lightDir, lc = lightColor;
+        env_light(ld, lc);
+        v.light_dir[0] = ld.x;
+        v.light_dir[1] = ld.y;
+        v.light_dir[2] = ld.z;
+        v.light_color[0] = lc.x;
+        v.light_color[1] = lc.y;
+        v.light_color[2] = lc.z;
+    }
+    ++rand_cpu_;  // core/volume.cu:579
     v.alpha = alpha;
     v.g = g;
     v.tan_half_fov = tan_half_fov_;

@@ -108,7 +108,32 @@ struct DevVolume {
     int occ_n;
     int N;
     float max_density;
+    const float4* env;  // environment map, env_w * env_h RGBA texels (row-major), or null
+    int env_w, env_h;
+    float env_exp;      // enviroment_exp (core/render.cuh:15)
 };
+
+// SkyBox (core/render.cuh:24-27): the environment map at (atan2(-z, x) / 2pi + 0.5,
+// acos(y) / pi) with bilinear filtering and wrap addressing, times the exposure; zero radiance
+// without a map (the reference's unbound texture).
+__device__ inline v3 sky_box(const DevVolume& v, v3 dir) {
+    if (!v.env) return {0.0f, 0.0f, 0.0f};
+    const float u = atan2f(-dir.z, dir.x) * (float)(0.5 / 3.1415926) + 0.5f;
+    const float t = acosf(fmaxf(fminf(dir.y, 1.0f), -1.0f)) * (float)(1.0 / 3.1415926);
+    const float px = (u - floorf(u)) * (float)v.env_w - 0.5f, py = (t - floorf(t)) * (float)v.env_h - 0.5f;
+    const float fx = floorf(px), fy = floorf(py);
+    const float wx = px - fx, wy = py - fy;
+    int x0 = (int)fx, y0 = (int)fy;
+    x0 = x0 < 0 ? x0 + v.env_w : x0;
+    y0 = y0 < 0 ? y0 + v.env_h : y0;
+    const int x1 = x0 + 1 == v.env_w ? 0 : x0 + 1, y1 = y0 + 1 == v.env_h ? 0 : y0 + 1;
+    const float4 a = v.env[(size_t)y0 * v.env_w + x0], b = v.env[(size_t)y0 * v.env_w + x1];
+    const float4 c = v.env[(size_t)y1 * v.env_w + x0], d = v.env[(size_t)y1 * v.env_w + x1];
+    auto bl = [&](float p, float q, float r, float s) {
+        return lerp_ref(lerp_ref(p, q, wx), lerp_ref(r, s, wx), wy);
+    };
+    return mk3(bl(a.x, b.x, c.x, d.x), bl(a.y, b.y, c.y, d.y), bl(a.z, b.z, c.z, d.z)) * v.env_exp;
+}
 
 // Stencil entry, host-precomputed from GetSamples23_ (vector.cu:304-405) and the
 // direction tables (vector.cu:82-177).

@@ -79,12 +79,16 @@ __device__ inline v3 pt_sample(const DevVolume& vol, const ViewDev& vw, v3 light
     dir = normalize3(dir);  // render.cuh:540
     v3 res = mk3(0, 0, 0);
     float dis = ray_box_offset(vw.ori, dir);
-    if (dis >= 0.0f) {
+    if (dis < 0.0f) return sky_box(vol, dir);  // render.cuh:545-546
+    {
         v3 spos = vw.ori + dir * dis, rdir = dir, csr = mk3(1, 1, 1);
         for (int b = 0; b < max_bounce; ++b) {
             v3 npos, ndir;
             float maxd = ray_box_distance(spos, rdir);
-            if (!next_vertex(vol, rs, vw.alpha, vw.g, spos, rdir, maxd, &npos, &ndir, taps)) break;  // sky 0
+            if (!next_vertex(vol, rs, vw.alpha, vw.g, spos, rdir, maxd, &npos, &ndir, taps)) {
+                res = res + sky_box(vol, rdir) * csr;  // render.cuh:569-573
+                break;
+            }
             spos = npos;
             csr = csr * vw.scatter_rate;
             float ld = ray_box_distance(spos, light);

@@ -68,8 +68,14 @@ __global__ __launch_bounds__(64) void k_march_compact(const DevVolume* __restric
         r.dist = st.dist;
         r.pad = 0;
         recs[my] = r;
-    } else if (valid) {
-        sl.out[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else if (valid) {  // sky: SkyBox(dir), zero without an environment map (render.cuh:829,874)
+        const DevVolume& vol = vols[sl.vol];
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (vol.env) {
+            const v3 sky = sky_box(vol, normalize3(pixel_ray(vw, row, col, (uint32_t)pix)));
+            o = make_float4(fmaxf(0.0f, sky.x), fmaxf(0.0f, sky.y), fmaxf(0.0f, sky.z), 0.0f);
+        }
+        sl.out[pix] = o;
     }
     unsigned long long t = (unsigned long long)st.taps;
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);

@@ -18,7 +18,11 @@ __global__ __launch_bounds__(64) void k_predict_plain(DevVolume vol, const Stenc
     const int row = blockIdx.y * 8 + (threadIdx.x >> 3);
     if (row >= vw.height || col >= vw.width) return;
     PixelState st = pixel_state(vol, vw, row, col);
-    float4 res = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // sky = 0, distance -1 -> 0 (volume.cu:224)
+    float4 res = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // sky, distance -1 -> 0 (volume.cu:224)
+    if (!st.active && vol.env) {  // SkyBox(dir) (render.cuh:829,874)
+        const v3 sky = sky_box(vol, normalize3(pixel_ray(vw, row, col, (uint32_t)(row * vw.width + col))));
+        res = make_float4(fmaxf(0.0f, sky.x), fmaxf(0.0f, sky.y), fmaxf(0.0f, sky.z), 0.0f);
+    }
     if (st.active) {
         float X[kPC], Hh[kPC], S[kPC];
         for (int i = 0; i < kPC; ++i) {

@@ -82,6 +82,10 @@ def lib():
         L.orc_fbm_volume.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, _f]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _v = ctypes.c_void_p
+        L.orc_set_env.argtypes = [_f, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+        L.orc_set_env.restype = None
+        L.orc_sky_box.argtypes = [_f, _f]
+        L.orc_sky_box.restype = None
         L.orc_pt_accumulate.argtypes = [_f, _v, _f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int]
         L.orc_predict_commit.argtypes = [_f, _f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -393,3 +397,29 @@ def histo_compare(x, y):
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(y, np.float32)
     return float(lib().orc_histo_compare(x.ctypes.data, y.ctypes.data))
+
+
+# ---------------------------------------------------------------------------------------
+# Environment map (SetHDRI / SetEnvExp, SkyBox core/render.cuh:24-27): applies to every later
+# oracle call until removed with set_env(None).
+# ---------------------------------------------------------------------------------------
+_env_keep = None
+
+
+def set_env(rgba, env_exp=1.0):
+    global _env_keep
+    if rgba is None:
+        _env_keep = None
+        lib().orc_set_env(None, 0, 0, float(env_exp))
+        return
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    h, w = rgba.shape[:2]
+    _env_keep = rgba
+    lib().orc_set_env(_p(rgba), int(w), int(h), float(env_exp))
+
+
+def sky_box(d):
+    out = np.zeros(3, np.float32)
+    dd = np.ascontiguousarray(d, np.float32)
+    lib().orc_sky_box(_p(dd), _p(out))
+    return out

@@ -320,8 +320,12 @@ void predict_pixel(const Volume& v, const Net& net, const View& view, int row, i
     gather_features(v, st.active, st.pos, st.XMain, st.YMain, st.ZMain, st.LXMain, st.LightDir, view.g, view.alpha,
                     feats);
     if (feats_out) memcpy(feats_out, feats, sizeof feats);
-    if (!st.active) {  // sky = 0, distance -1 -> max(0,.) = 0 (volume.cu:224)
-        out4[0] = out4[1] = out4[2] = out4[3] = 0.0f;
+    if (!st.active) {  // SkyBox(dir), distance -1 -> max(0,.) = 0 (render.cuh:829,874; volume.cu:224)
+        const f3 sky = sky_box(v, st.dir);
+        out4[0] = fmaxf(0.0f, sky.x);
+        out4[1] = fmaxf(0.0f, sky.y);
+        out4[2] = fmaxf(0.0f, sky.z);
+        out4[3] = 0.0f;
         return;
     }
     f3 sr = view.scatter_rate / 1.001f;  // render.cuh:842
@@ -609,13 +613,17 @@ void pt_pixel(const Volume& v, const View& view, const PTParams& pp, int row, in
         dir = normalize(dir);
         f3 res = mk(0, 0, 0);
         float dis = ray_box_offset(view.ori, dir);
+        if (dis < 0) res = sky_box(v, dir);  // render.cuh:545-546
         if (dis >= 0) {
             f3 spos = view.ori + dir * dis, rdir = dir, csr = mk(1, 1, 1);
             for (int b = 0; b < pp.max_bounce; ++b) {
                 f3 npos, ndir;
                 float maxd = ray_box_distance(spos, rdir);
                 bool in = next_vertex(v, rs, view.alpha, view.g, spos, rdir, maxd, &npos, &ndir, taps);
-                if (!in) break;  // sky = 0
+                if (!in) {  // render.cuh:569-573
+                    res = res + sky_box(v, rdir) * csr;
+                    break;
+                }
                 spos = npos;
                 csr = csr * view.scatter_rate;
                 float ld = ray_box_distance(spos, light);

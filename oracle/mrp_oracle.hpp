@@ -174,7 +174,31 @@ struct Volume {
     const float* mips[9] = {nullptr};      // (256>>m)^3
     const float* tr[8] = {nullptr};        // (128>>m)^3
     const float* lut = nullptr;            // 512*512, lut[j*512+i], i <-> cos, j <-> cone angle
+    const float* env = nullptr;            // environment map, env_w*env_h RGBA, or null
+    int env_w = 0, env_h = 0;
+    float env_exp = 1.0f;
 };
+
+// SkyBox (render.cuh:24-27): tex2D of the environment map at (atan2(-z,x)/2pi + 0.5, acos(y)/pi),
+// bilinear with wrap addressing (exact fp32 weights), times the exposure; black without a map.
+inline f3 sky_box(const Volume& v, f3 dir) {
+    if (!v.env) return f3{0.0f, 0.0f, 0.0f};
+    const float u = atan2f(-dir.z, dir.x) * (float)(0.5 / 3.1415926) + 0.5f;
+    const float t = acosf(fmaxf(fminf(dir.y, 1.0f), -1.0f)) * (float)(1.0 / 3.1415926);
+    const float px = (u - floorf(u)) * (float)v.env_w - 0.5f, py = (t - floorf(t)) * (float)v.env_h - 0.5f;
+    const float fx = floorf(px), fy = floorf(py);
+    const float wx = px - fx, wy = py - fy;
+    const int x0 = ((int)fx % v.env_w + v.env_w) % v.env_w, y0 = ((int)fy % v.env_h + v.env_h) % v.env_h;
+    const int x1 = (x0 + 1) % v.env_w, y1 = (y0 + 1) % v.env_h;
+    f3 r;
+    float* o = &r.x;
+    for (int c = 0; c < 3; ++c) {
+        const float a = v.env[((size_t)y0 * v.env_w + x0) * 4 + c], b = v.env[((size_t)y0 * v.env_w + x1) * 4 + c];
+        const float cc = v.env[((size_t)y1 * v.env_w + x0) * 4 + c], d = v.env[((size_t)y1 * v.env_w + x1) * 4 + c];
+        o[c] = lerpf_(lerpf_(a, b, wx), lerpf_(cc, d, wx), wy);
+    }
+    return r * v.env_exp;
+}
 
 // Zero-border trilinear as the host Sample() (volume.cu:835-863), optional max(0,.) clamp.
 inline float fetch_border(const float* d, int res, int x, int y, int z, bool clamp0) {

@@ -54,9 +54,17 @@ static size_t tr_offset(int m) {
     for (int i = 0; i < m; ++i) o += (size_t)(128 >> i) * (128 >> i) * (128 >> i);
     return o;
 }
+static std::vector<float> g_env;
+static int g_env_w = 0, g_env_h = 0;
+static float g_env_exp = 1.0f;
+
 static Volume to_volume(const float* grid, int N, float max_density, const float* mips, const float* tr,
                         const float* lut) {
     Volume v;
+    v.env = g_env.empty() ? nullptr : g_env.data();
+    v.env_w = g_env_w;
+    v.env_h = g_env_h;
+    v.env_exp = g_env_exp;
     v.N = N;
     v.grid = grid;
     v.max_density = max_density;
@@ -67,6 +75,26 @@ static Volume to_volume(const float* grid, int N, float max_density, const float
 }
 
 extern "C" {
+
+// Environment map of every later call (SetHDRI / SetEnvExp); rgba NULL removes it.
+void orc_set_env(const float* rgba, int w, int h, float env_exp) {
+    g_env.clear();
+    g_env_w = g_env_h = 0;
+    g_env_exp = env_exp;
+    if (rgba && w > 0 && h > 0) {
+        g_env.assign(rgba, rgba + (size_t)w * h * 4);
+        g_env_w = w;
+        g_env_h = h;
+    }
+}
+
+void orc_sky_box(const float* dir3, float* out3) {
+    Volume v = to_volume(nullptr, 0, 1.0f, nullptr, nullptr, nullptr);
+    f3 r = sky_box(v, mk(dir3[0], dir3[1], dir3[2]));
+    out3[0] = r.x;
+    out3[1] = r.y;
+    out3[2] = r.z;
+}
 
 int orc_set_threads(int n) {
     if (n > 0) omp_set_num_threads(n);

@@ -1,0 +1,204 @@
+"""Volume ingest and environment lighting on the host (no GPU): the loaders of
+VolumeRender::LoadData (core/volume.cu:970-1149), the RGBE reader (core/hdr_loader.h), the
+per-frame light choice (core/volume.cu:494-509) and the oracle's SkyBox, each against an
+independent numpy reading of the same synthetic files or a closed form."""
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mrpnn_amd
+    return mrpnn_amd
+
+
+def test_txt_grid_and_bin_cache(M, tmp_path):
+    N = 6
+    rng = np.random.default_rng(1)
+    g = rng.uniform(-0.1, 3.0, N ** 3).astype(np.float32)
+    p = tmp_path / "ball.txt"
+    with open(p, "w") as f:
+        f.write("%d\n" % N)
+        for i in range(0, N ** 3, 8):
+            f.write(" ".join("%.9g" % v for v in g[i:i + 8]) + "\n")
+    grid, n, md = M.read_volume(str(p))
+    assert n == N and np.array_equal(grid, g) and md == g.max()
+    raw = open(str(p) + ".bin", "rb").read()
+    assert struct.unpack("<i", raw[:4])[0] == N
+    assert np.array_equal(np.frombuffer(raw[4:], "<f4"), g)
+    grid2, n2, md2 = M.read_volume(str(p) + ".bin")
+    assert n2 == N and np.array_equal(grid2, g) and md2 == md
+
+
+def test_vox_grid(M, tmp_path):
+    total = 256 ** 3
+    lines = total // 64
+    rng = np.random.default_rng(2)
+    special = {0: rng.integers(0, 256, 64), 1234: rng.integers(0, 256, 64), lines - 1: rng.integers(0, 256, 64)}
+    zero = b" ".join([b"0"] * 64) + b"\n"
+    body = bytearray(b"width 256\nheight 256\ndepth 256\n")
+    body += zero * lines
+    p = tmp_path / "model.vox"
+    with open(p, "wb") as f:
+        f.write(bytes(body))
+    # rewrite the special lines in place (each zero line is 128 bytes)
+    text = bytearray(open(p, "rb").read())
+    hdr = len(b"width 256\nheight 256\ndepth 256\n")
+    pieces = [bytes(text[:hdr])]
+    for ln in range(lines):
+        pieces.append(b" ".join(b"%d" % v for v in special[ln]) + b"\n" if ln in special else zero)
+    with open(p, "wb") as f:
+        f.write(b"".join(pieces))
+    grid, n, md = M.read_volume(str(p))
+    assert n == 256
+    inv = np.float32(64.0 / 255.0)
+    want = np.zeros(total, np.float32)
+    for ln, vals in special.items():
+        want[64 * ln:64 * ln + 64] = vals.astype(np.float32) * inv
+    assert np.array_equal(grid, want)
+    assert md == want.max()
+
+
+def test_named_dataset_max_density(M, tmp_path):
+    N = 4
+    g = np.linspace(0, 2, N ** 3, dtype=np.float32)
+    base = tmp_path / "TestCase_CLOUD1"
+    with open(str(base) + ".bin", "wb") as f:
+        f.write(struct.pack("<i", N) + g.tobytes())
+    grid, n, md = M.read_volume(str(base))
+    assert n == N and np.array_equal(grid, g) and md == np.float32(505.490356)
+
+
+def test_unknown_format_and_missing_file(M, tmp_path):
+    with pytest.raises(M.MrpError):
+        M.read_volume(str(tmp_path / "grid.raw"))
+    with pytest.raises(M.MrpError):
+        M.read_volume(str(tmp_path / "missing.txt"))
+    p = tmp_path / "short.txt"
+    p.write_text("4\n1 2 3\n")
+    with pytest.raises(M.MrpError):
+        M.read_volume(str(p))
+
+
+def _rgbe_bytes(rgbe, rle):
+    """Radiance file from (h, w, 4) uint8 RGBE; new-style RLE scanlines when rle."""
+    h, w, _ = rgbe.shape
+    out = bytearray(b"#?RADIANCE\n# test\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0\n\n-Y %d +X %d\n" % (h, w))
+    for j in range(h):
+        row = rgbe[j]
+        if not rle:
+            out += row.tobytes()
+            continue
+        out += bytes([2, 2, w >> 8, w & 255])
+        for c in range(4):
+            comp = row[:, c]
+            i = 0
+            while i < w:  # runs of equal bytes >= 3 as (128 + n, v), the rest as literal blocks
+                r = 1
+                while i + r < w and comp[i + r] == comp[i] and r < 127:
+                    r += 1
+                if r >= 3:
+                    out += bytes([128 + r, comp[i]])
+                    i += r
+                    continue
+                start = i
+                while i < w and i - start < 128:
+                    r = 1
+                    while i + r < w and comp[i + r] == comp[i] and r < 3:
+                        r += 1
+                    if r >= 3:
+                        break
+                    i += 1
+                out += bytes([i - start]) + bytes(comp[start:i])
+    return bytes(out)
+
+
+def _rgbe_decode(rgbe):
+    e = rgbe[..., 3].astype(np.int64)
+    f = np.where(e > 0, np.ldexp(np.float32(1.0), (e - 136).astype(np.int32)), 0.0).astype(np.float32)
+    rgb = (rgbe[..., :3].astype(np.float32) + np.float32(0.5)) * f[..., None]
+    return np.concatenate([np.where(e[..., None] > 0, rgb, 0.0), np.zeros(rgbe.shape[:2] + (1,))], -1).astype(np.float32)
+
+
+@pytest.mark.parametrize("w,h,rle", [(5, 3, False), (16, 4, True), (40, 7, True), (12, 2, False)])
+def test_hdr_reader(M, tmp_path, w, h, rle):
+    rng = np.random.default_rng(w * 100 + h)
+    rgbe = rng.integers(0, 256, (h, w, 4)).astype(np.uint8)
+    rgbe[..., 3] = rng.integers(120, 140, (h, w))
+    rgbe[0, : min(w, 6), :] = [10, 20, 30, 128]  # a run
+    rgbe[-1, -1, 3] = 0  # a black pixel
+    p = tmp_path / "env.hdr"
+    p.write_bytes(_rgbe_bytes(rgbe, rle))
+    img = M.read_hdr(str(p))
+    assert img.shape == (h, w, 4)
+    assert np.array_equal(img, _rgbe_decode(rgbe))
+
+
+def test_hdr_reader_rejects_bad_files(M, tmp_path):
+    p = tmp_path / "bad.hdr"
+    p.write_bytes(b"P6\n2 2\n255\n")
+    with pytest.raises(M.MrpError):
+        M.read_hdr(str(p))
+    p.write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 2 +X 4\n" + bytes(12))
+    with pytest.raises(M.MrpError):
+        M.read_hdr(str(p))
+
+
+def test_env_light_choice(M):
+    c = np.float32(0.37)
+    img = np.full((8, 16, 4), c, np.float32)
+    L = np.array([0.0, 1.0, 0.0], np.float32)
+    sun = np.array([2.0, 1.0, 0.5], np.float32)
+    e = 1.5
+    rate = np.float32(e * 4) / (np.float32(e * 4) + np.float32(2.0))
+    for n in (0, 1, 7, 1000):
+        chose, d, col = M.env_light(img, e, L, sun, n, 0.0)
+        assert chose
+        # Roberts2 in float32 as the reference computes it: frac((1/g, 1/g^2) * n + 0.5), g the
+        # plastic number (the real root of x^3 = x + 1)
+        g = np.float32(1.32471795724474602596)
+        assert abs(float(g) ** 3 - float(g) - 1) < 1e-6
+        f = np.float32
+        ex = float(f(f(f(1) / g) * f(n)) + f(0.5)) % 1.0
+        ey = float(f(f(f(1) / f(g * g)) * f(n)) + f(0.5)) % 1.0
+        ct = 1 - 2 * ey
+        st = math.sqrt(1 - ct * ct)
+        np.testing.assert_allclose(d, [st * math.cos(2 * math.pi * ex), st * math.sin(2 * math.pi * ex), ct],
+                                   atol=2e-5)
+        np.testing.assert_allclose(col, c * np.float32(e * 4) / rate, rtol=1e-6)
+    chose, d, col = M.env_light(img, e, L, sun, 3, 0.999)
+    assert not chose and np.array_equal(d, L)
+    np.testing.assert_allclose(col, sun / (1 - rate), rtol=1e-6)
+    # the choice is unbiased: over uniform draws the mean light is the sun plus 4 e times the map
+    mean = np.mean([M.env_light(img, e, L, sun, 5, u)[2] for u in (np.arange(4000) + 0.5) / 4000], axis=0)
+    np.testing.assert_allclose(mean, 4 * e * c + sun, rtol=2e-3)
+    chose, d, col = M.env_light(None, e, L, sun, 3, 0.0)
+    assert not chose and np.array_equal(col, sun) and np.array_equal(d, L)
+
+
+def test_oracle_sky_box_is_wrapped_bilinear(orc):
+    rng = np.random.default_rng(9)
+    h, w = 6, 10
+    img = rng.uniform(0, 2, (h, w, 4)).astype(np.float32)
+    orc.set_env(img, 1.7)
+    try:
+        for _ in range(64):
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            u = math.atan2(-d[2], d[0]) / (2 * math.pi) + 0.5
+            v = math.acos(max(min(d[1], 1.0), -1.0)) / math.pi
+            px, py = (u % 1.0) * w - 0.5, (v % 1.0) * h - 0.5
+            x0, y0 = math.floor(px), math.floor(py)
+            fx, fy = px - x0, py - y0
+            t = lambda x, y: img[y % h, x % w, :3].astype(np.float64)
+            want = ((t(x0, y0) * (1 - fx) + t(x0 + 1, y0) * fx) * (1 - fy) +
+                    (t(x0, y0 + 1) * (1 - fx) + t(x0 + 1, y0 + 1) * fx) * fy) * 1.7
+            np.testing.assert_allclose(orc.sky_box(d.astype(np.float32)), want, rtol=2e-5, atol=1e-6)
+        orc.set_env(None)
+        assert np.all(orc.sky_box(np.array([0, 0, 1], np.float32)) == 0)
+    finally:
+        orc.set_env(None)
