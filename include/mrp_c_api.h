@@ -211,6 +211,26 @@ int mrp_gather_tiles(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int pi
                      float* dev_out);
 
 /* ---------------------------------------------------------------------------------------
+ * RPNN, the paper's comparison predictor: RenderCamera<true, RPNN> -> NNPredict<RPNN> ->
+ * RadiancePredict_RPNN (core/volume.cu:198-228, core/render.cuh:808-875,
+ * core/radiancePredict.cu:567-722), weights of core/NNWeight_RPNN.cuh as the asset
+ * rpnn_weights.bin (1,296,401 floats, FNV-1a checked).
+ * --------------------------------------------------------------------------------------- */
+/* Loads the asset (path NULL: assets/rpnn_weights.bin next to the library); the render calls
+ * load it on first use. */
+int mrp_ctx_load_rpnn(mrp_ctx* ctx, const char* path);
+/* One frame into dev_out4 (width*height float4: max(0, radiance * light_color * scatter_rate),
+ * distance; sky = SkyBox).  precision MRP_FP32: the reference's arithmetic (every output sums
+ * its products from input 0 upwards, bias last); MRP_BF16: bf16 MFMA, fp32 accumulation.
+ * Synchronises with the march (the active-pixel count sizes the network launches). */
+int mrp_render_rpnn(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int precision, float* dev_out4);
+/* Per pixel the network's input row of 2251 floats (10 levels x 225 density samples, gamma;
+ * zeros for sky pixels) into dev_rows, and the sky pixels of dev_out4.  Parity tooling. */
+int mrp_rpnn_features(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, float* dev_rows, float* dev_out4);
+/* The network alone on n given rows of 2251 floats: dev_y[i] = max(exp(y) - 1, 0). */
+int mrp_rpnn_predict_rows(mrp_ctx* ctx, const float* dev_rows, int n, int precision, float* dev_y);
+
+/* ---------------------------------------------------------------------------------------
  * Volume ingest (VolumeRender::LoadData and its DataCache, core/volume.cu:970-1149).
  * A path containing CLOUD0, CLOUD1, MODEL0 or MODEL1 reads <path>.bin (int32 N, then N^3
  * float32) with that dataset's fixed max density (510.416656, 505.490356, 64, 23.8960018);

@@ -153,6 +153,10 @@ def lib():
         L.mrp_env_light.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _fp, _fp, ctypes.c_int,
                                     ctypes.c_float, _fp, _fp]
         L.mrp_ctx_env_light.argtypes = [_vp, _fp, _fp, ctypes.c_int, ctypes.c_float, _fp, _fp]
+        L.mrp_ctx_load_rpnn.argtypes = [_vp, ctypes.c_char_p]
+        L.mrp_render_rpnn.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, _vp]
+        L.mrp_rpnn_features.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp, _vp]
+        L.mrp_rpnn_predict_rows.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]
         L.mrp_interactive_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
         L.mrp_interactive_destroy.argtypes = [_vp]
         L.mrp_interactive_destroy.restype = None
@@ -211,6 +215,17 @@ class Context:
 
     def set_option(self, option, value):
         check(lib().mrp_ctx_set_option(self.h, int(option), int(value)))
+
+    def load_rpnn(self, path=None):
+        check(lib().mrp_ctx_load_rpnn(self.h, os.fsencode(path) if path else None))
+
+    def rpnn_predict_rows(self, rows, precision=FP32):
+        """The RPNN network on (n, 2251) float32 device rows: (n,) outputs."""
+        import torch
+        n = rows.shape[0]
+        y = torch.empty(n, dtype=torch.float32, device="cuda")
+        check(lib().mrp_rpnn_predict_rows(self.h, _ptr(rows.contiguous()), int(n), int(precision), _ptr(y)))
+        return y
 
     def set_env_map(self, rgba):
         """Environment map (height, width, 4) float32, or None to remove it (SetHDRI)."""
@@ -350,6 +365,23 @@ class Volume:
         out = np.zeros((view.height * view.width, 4), np.float32)
         check(lib().mrp_render_plain(self.ctx.h, self.h, ctypes.byref(view), int(n_frames), out.ctypes.data_as(_fp)))
         return out
+
+    def render_rpnn(self, view, precision=FP32, out=None):
+        """RPNN frame (mrp_render_rpnn): (height*width, 4) float32 on the GPU."""
+        import torch
+        if out is None:
+            out = torch.zeros((view.width * view.height, 4), dtype=torch.float32, device="cuda")
+        check(lib().mrp_render_rpnn(self.ctx.h, self.h, ctypes.byref(view), int(precision), _ptr(out)))
+        return out
+
+    def rpnn_features(self, view):
+        """(rows (height*width, 2251), out4 with the sky pixels) as the RPNN path computes them."""
+        import torch
+        n = view.width * view.height
+        rows = torch.empty((n, 2251), dtype=torch.float32, device="cuda")
+        out = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+        check(lib().mrp_rpnn_features(self.ctx.h, self.h, ctypes.byref(view), _ptr(rows), _ptr(out)))
+        return rows, out
 
     def render_plain_device(self, view, out):
         check(lib().mrp_render_plain_device(self.ctx.h, self.h, ctypes.byref(view), _ptr(out)))

@@ -30,7 +30,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contr
 FILE_FLAGS = {"mlp_mfma.hip": ["-fno-honor-nans"]}
 
 SOURCES = ["capi.hip", "volume_kernels.hip", "predict_plain.hip", "fused.hip", "fused_host.hip", "mlp_mfma.hip",
-           "pathtrace.hip", "tile_gather.hip", "interactive.hip", "ingest.cpp", "serve.cpp", "cxx_api.cpp"]
+           "pathtrace.hip", "tile_gather.hip", "interactive.hip", "rpnn.hip", "ingest.cpp", "serve.cpp", "cxx_api.cpp"]
 
 
 def _compile(src):

@@ -171,6 +171,7 @@ extern "C" int mrp_ctx_destroy(mrp_ctx* c) {
     hipFree(c->d_lut_rows);
     hipFree(c->d_plain_out);
     hipFree(c->d_env);
+    mrp_rpnn_free(c);
     for (TrSlot& t : c->tr_slots) {
         hipFree(t.d_tr);
         hipFree(t.d_ptr);

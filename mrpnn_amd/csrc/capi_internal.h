@@ -42,6 +42,8 @@ struct TrSlot {
     bool read_pending = false;
 };
 
+struct RpnnState;
+
 struct mrp_ctx {
     int device = 0;
     hipDeviceProp_t props{};
@@ -72,6 +74,8 @@ struct mrp_ctx {
     std::vector<float> env_host;
     int env_w = 0, env_h = 0;
     float env_exp = 1.0f;
+    // RPNN comparison network: weights and its working buffers, loaded on first use
+    struct RpnnState* rpnn = nullptr;
     // LoadData's DataCache: host grids of the four named datasets, read once per context
     std::vector<float> data_cache[4];
     int data_cache_n[4] = {0, 0, 0, 0};
@@ -106,6 +110,9 @@ int mrp_volume_update_tr_impl(mrp_volume* v, const float L[3], float alpha);
 int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, float* d_ptr, hipStream_t s);
 void mrp_tr_slots_drop(mrp_ctx* c, const mrp_volume* v);  // invalidate a volume's cached sets
 int mrp_tr_slot_find(const mrp_ctx* c, const mrp_volume* v, const float L[3], float alpha);
+
+// rpnn.hip
+void mrp_rpnn_free(mrp_ctx* c);
 
 // fused.hip
 int mrp_fused_init(mrp_ctx* c);

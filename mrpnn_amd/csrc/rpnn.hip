@@ -1,0 +1,466 @@
+// RPNN, the paper's comparison predictor: RenderCamera<true, RPNN> (core/volume.cu:198-228) ->
+// NNPredict<RPNN> (core/render.cuh:808-875) -> RadiancePredict_RPNN
+// (core/radiancePredict.cu:567-722) with the weights of core/NNWeight_RPNN.cuh
+// (assets/rpnn_weights.bin, 1,296,401 floats).
+//
+// Per active pixel a descriptor of 10 levels x 225 density samples (5 x 5 x 9 grid in the light
+// frame, spacing doubling per level) feeds a recurrent network: per level B = relu(V_l [226] +
+// W_l C + b), C = relu(C + U_l B + b) with 200-wide states, then F0, F1 and F2; the radiance is
+// max(exp(y) - 1, 0) for every channel.  1.29 M multiply-adds per pixel, 27x MRPNN's network.
+//
+// FP32 path (the reference's arithmetic): the march lists the active pixels; per chunk of them
+// each level's 226 inputs are written to HBM and each layer is a GEMM over (pixels x 200
+// outputs) whose every output sums its products from input 0 upwards, as Linear does, with the
+// residual adds and ReLUs in the epilogues.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mrp_c_api.h"
+#include "capi_internal.h"
+#include "dev_common.hpp"
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return mrp_fail(MRP_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace mrp {
+
+constexpr int kRL = 10, kRS = 225, kRIn = 226, kRB = 200;
+constexpr int kRTotal = 1296401;
+constexpr unsigned long long kRFnv = 0xad1ea655797fc91cull;
+
+// element offsets of the asset (header order: V0..V9, W1..W9 with biases, U0..U9 with biases,
+// F0, F1, F2 with biases)
+struct RpnnLayout {
+    size_t V[kRL], W[kRL], Wb[kRL], U[kRL], Ub[kRL], F0, F0b, F1, F1b, F2, F2b;
+};
+static RpnnLayout rpnn_layout() {
+    RpnnLayout o{};
+    size_t off = 0;
+    for (int l = 0; l < kRL; ++l, off += (size_t)kRB * kRIn) o.V[l] = off;
+    for (int l = 1; l < kRL; ++l) {
+        o.W[l] = off;
+        off += (size_t)kRB * kRB;
+        o.Wb[l] = off;
+        off += kRB;
+    }
+    for (int l = 0; l < kRL; ++l) {
+        o.U[l] = off;
+        off += (size_t)kRB * kRB;
+        o.Ub[l] = off;
+        off += kRB;
+    }
+    o.F0 = off;
+    off += (size_t)kRB * kRB;
+    o.F0b = off;
+    off += kRB;
+    o.F1 = off;
+    off += (size_t)kRB * kRB;
+    o.F1b = off;
+    off += kRB;
+    o.F2 = off;
+    off += kRB;
+    o.F2b = off;
+    return o;
+}
+
+// One active pixel for the RPNN network: 32 bytes.
+struct RpnnRec {
+    int pixel;
+    float pos[3];
+    float X[3];
+    float dist;
+};
+
+// RenderCamera<true, RPNN> up to the network: the same ray, march and basis as the MRPNN path
+// (pixel_state); sky pixels are written at once, active ones listed (wave-aggregated).
+__global__ __launch_bounds__(64) void k_rpnn_march(DevVolume vol, ViewDev vw, float4* __restrict__ out,
+                                                   RpnnRec* __restrict__ recs, unsigned* __restrict__ count) {
+    const int col = blockIdx.x * 8 + (threadIdx.x & 7);
+    const int row = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const bool valid = row < vw.height && col < vw.width;
+    PixelState st;
+    st.active = 0;
+    if (valid) st = pixel_state(vol, vw, row, col);
+    const bool act = valid && st.active;
+    const unsigned long long mask = __ballot(act);
+    const int lane = threadIdx.x & 63;
+    const size_t pix = (size_t)row * vw.width + col;
+    if (valid && !act) {
+        const v3 sky = sky_box(vol, normalize3(pixel_ray(vw, row, col, (uint32_t)pix)));
+        out[pix] = make_float4(fmaxf(0.0f, sky.x), fmaxf(0.0f, sky.y), fmaxf(0.0f, sky.z), 0.0f);
+    }
+    if (!mask) return;
+    const int lead = __ffsll((long long)mask) - 1;
+    unsigned base = 0;
+    if (lane == lead) base = atomicAdd(count, (unsigned)__popcll(mask));
+    base = __shfl(base, lead);
+    if (act) {
+        RpnnRec r;
+        r.pixel = (int)pix;
+        r.pos[0] = st.pos.x;
+        r.pos[1] = st.pos.y;
+        r.pos[2] = st.pos.z;
+        r.X[0] = st.X.x;
+        r.X[1] = st.X.y;
+        r.X[2] = st.X.z;
+        r.dist = st.dist;
+        recs[base + __popcll(mask & ((1ull << lane) - 1ull))] = r;
+    }
+}
+
+// Sample k of level l of the descriptor (radiancePredict.cu:580-611): k = (z+2)*25 + (y+2)*5 + (x+2)
+// on the 5 x 5 x 9 grid (x, y in [-2, 2], z in [-2, 6]), scaled by 2^(l-12), at mip
+// max(min(l - 2, 9), 0); value density * alpha / 64.
+__device__ inline float rpnn_sample(const DevVolume& vol, v3 pos, v3 eX, v3 eY, v3 LX, int l, int k, float alpha) {
+    const int z = k / 25 - 2, y = (k / 5) % 5 - 2, x = k % 5 - 2;
+    float scale = 0.5f * (0.5f / 1024.0f);
+    for (int i = 0; i < l; ++i) scale *= 2.0f;
+    const float cur = fmaxf(fminf((float)l - 2.0f, 9.0f), 0.0f);
+    const int mip = (int)(cur + 0.001f);
+    const v3 offset = (eX * (float)x + eY * (float)y + LX * (float)z) * scale;
+    return mip_density(vol, mip, pos + offset) * alpha / 64.0f;
+}
+
+__device__ inline void rpnn_frame(v3 X, v3 LX, v3* eX, v3* eY) {
+    *eX = normalize3(cross3(LX, X));
+    *eY = normalize3(cross3(*eX, LX));
+}
+
+// level l's 226 inputs of rows [0, m): 225 samples, gamma = acos(dot(X, LX))
+__global__ __launch_bounds__(256) void k_rpnn_input(DevVolume vol, const RpnnRec* __restrict__ recs, int m, int l,
+                                                    v3 LX, float alpha, float* __restrict__ A) {
+    const int p = blockIdx.x;
+    if (p >= m) return;
+    const RpnnRec r = recs[p];
+    const v3 X = mk3(r.X[0], r.X[1], r.X[2]);
+    v3 eX, eY;
+    rpnn_frame(X, LX, &eX, &eY);
+    const v3 pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
+    for (int k = threadIdx.x; k < kRIn; k += 256)
+        A[(size_t)p * kRIn + k] = k < kRS ? rpnn_sample(vol, pos, eX, eY, LX, l, k, alpha) : acosf(dot3(X, LX));
+}
+
+// level l's 226 inputs from given rows (2250 + gamma)
+__global__ __launch_bounds__(256) void k_rpnn_input_rows(const float* __restrict__ rows, int m, int l,
+                                                         float* __restrict__ A) {
+    const int p = blockIdx.x;
+    if (p >= m) return;
+    const float* row = rows + (size_t)p * (kRL * kRS + 1);
+    for (int k = threadIdx.x; k < kRIn; k += 256) A[(size_t)p * kRIn + k] = k < kRS ? row[l * kRS + k] : row[kRL * kRS];
+}
+
+// Full descriptor rows (2250 + gamma) of all pixels of a view, zeros for sky (parity tooling)
+__global__ __launch_bounds__(256) void k_rpnn_rows(DevVolume vol, const RpnnRec* __restrict__ recs, int m, v3 LX,
+                                                   float alpha, float* __restrict__ rows) {
+    const int p = blockIdx.x;
+    if (p >= m) return;
+    const RpnnRec r = recs[p];
+    const v3 X = mk3(r.X[0], r.X[1], r.X[2]);
+    v3 eX, eY;
+    rpnn_frame(X, LX, &eX, &eY);
+    const v3 pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
+    float* row = rows + (size_t)r.pixel * (kRL * kRS + 1);
+    for (int k = threadIdx.x; k < kRL * kRS + 1; k += 256)
+        row[k] = k < kRL * kRS ? rpnn_sample(vol, pos, eX, eY, LX, k / kRS, k % kRS, alpha) : acosf(dot3(X, LX));
+}
+
+// out[p][i] = sum_j A[p][j] W[i][j] (j ascending from 0), then the epilogue:
+enum RpnnEpi {
+    kEpiRelu = 0,        // relu(acc)                          V_0 A
+    kEpiBias = 1,        // acc + b                            W_l C + b
+    kEpiAddRelu = 2,     // relu(acc + add)                    V_l A + (W_l C + b)
+    kEpiBiasRelu = 3,    // relu(acc + b)                      U_0 B + b, F0, F1
+    kEpiResBiasRelu = 4, // relu(add + (acc + b)), in place    C + (U_l B + b)
+    kEpiOut = 5          // max(exp(relu(acc + b)) - 1, 0)     F2 (exp in double, rounded once)
+};
+constexpr int kGT = 64, kGK = 32;
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_rpnn_gemm(const float* __restrict__ A, int K, const float* __restrict__ W,
+                                                   const float* __restrict__ bias, const float* add, float* out,
+                                                   int m, int To) {
+    __shared__ float sA[kGK][kGT + 1];
+    __shared__ float sW[kGK][kGT + 1];
+    const int p0 = blockIdx.x * kGT, o0 = blockIdx.y * kGT;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    float acc[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += kGK) {
+        for (int e = threadIdx.x; e < kGT * kGK; e += 256) {
+            const int r = e / kGK, kk = e % kGK;
+            const int k = k0 + kk;
+            sA[kk][r] = (p0 + r < m && k < K) ? A[(size_t)(p0 + r) * K + k] : 0.0f;
+            sW[kk][r] = (o0 + r < To && k < K) ? W[(size_t)(o0 + r) * K + k] : 0.0f;
+        }
+        __syncthreads();
+        const int kn = min(kGK, K - k0);
+        for (int kk = 0; kk < kn; ++kk) {
+            float a[4], w[4];
+            for (int i = 0; i < 4; ++i) a[i] = sA[kk][ty * 4 + i];
+            for (int j = 0; j < 4; ++j) w[j] = sW[kk][tx * 4 + j];
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + a[i] * w[j];
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 4; ++i) {
+        const int p = p0 + ty * 4 + i;
+        if (p >= m) continue;
+        for (int j = 0; j < 4; ++j) {
+            const int o = o0 + tx * 4 + j;
+            if (o >= To) continue;
+            const size_t idx = (size_t)p * To + o;
+            float v = acc[i][j];
+            if (EPI == kEpiRelu) {
+                v = v >= 0.0f ? v : 0.0f;
+            } else if (EPI == kEpiBias) {
+                v = v + bias[o];
+            } else if (EPI == kEpiAddRelu) {
+                v = v + add[idx];
+                v = v >= 0.0f ? v : 0.0f;
+            } else if (EPI == kEpiBiasRelu) {
+                v = v + bias[o];
+                v = v >= 0.0f ? v : 0.0f;
+            } else if (EPI == kEpiResBiasRelu) {
+                v = add[idx] + (v + bias[o]);
+                v = v >= 0.0f ? v : 0.0f;
+            } else {
+                v = v + bias[o];
+                v = v >= 0.0f ? v : 0.0f;
+                const float e = (float)exp((double)v) - 1.0f;
+                v = e > 0.0f ? e : 0.0f;
+            }
+            out[idx] = v;
+        }
+    }
+}
+
+// pixel = max(0, (y * light_color) * scatter_rate), distance (render.cuh:874, volume.cu:224)
+__global__ __launch_bounds__(256) void k_rpnn_compose(const RpnnRec* __restrict__ recs, int m,
+                                                      const float* __restrict__ y, v3 lc, v3 sr,
+                                                      float4* __restrict__ out) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= m) return;
+    const float v = y[p];
+    const v3 r = (mk3(v, v, v) * lc) * sr;
+    out[recs[p].pixel] = make_float4(fmaxf(0.0f, r.x), fmaxf(0.0f, r.y), fmaxf(0.0f, r.z), fmaxf(0.0f, recs[p].dist));
+}
+
+}  // namespace mrp
+
+using namespace mrp;
+
+struct RpnnState {
+    float* d_w = nullptr;  // fp32 asset, header order
+    RpnnLayout lay;
+    size_t rec_cap = 0;
+    RpnnRec* d_recs = nullptr;
+    unsigned* d_count = nullptr;
+    int chunk = 0;
+    float *d_A = nullptr, *d_B = nullptr, *d_C = nullptr, *d_T = nullptr, *d_y = nullptr;
+};
+
+void mrp_rpnn_free(mrp_ctx* c) {
+    RpnnState* s = c->rpnn;
+    if (!s) return;
+    (void)hipFree(s->d_w);
+    (void)hipFree(s->d_recs);
+    (void)hipFree(s->d_count);
+    (void)hipFree(s->d_A);
+    (void)hipFree(s->d_B);
+    (void)hipFree(s->d_C);
+    (void)hipFree(s->d_T);
+    (void)hipFree(s->d_y);
+    delete s;
+    c->rpnn = nullptr;
+}
+
+static std::string default_rpnn_path() {
+    Dl_info info;
+    if (dladdr((void*)&mrp_ctx_load_rpnn, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t s = p.find_last_of('/');
+        return (s == std::string::npos ? std::string(".") : p.substr(0, s)) + "/assets/rpnn_weights.bin";
+    }
+    return "assets/rpnn_weights.bin";
+}
+
+extern "C" int mrp_ctx_load_rpnn(mrp_ctx* c, const char* path) {
+    if (!c) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    if (c->rpnn) return MRP_OK;
+    const std::string p = path ? path : default_rpnn_path();
+    FILE* f = fopen(p.c_str(), "rb");
+    if (!f) return mrp_fail(MRP_ERR_IO, "cannot open RPNN weight asset %s", p.c_str());
+    std::vector<float> w(kRTotal);
+    size_t got = fread(w.data(), sizeof(float), w.size(), f);
+    char extra;
+    size_t more = fread(&extra, 1, 1, f);
+    fclose(f);
+    if (got != w.size() || more) return mrp_fail(MRP_ERR_IO, "RPNN weight asset %s: expected %d floats", p.c_str(), kRTotal);
+    uint64_t h = 0xcbf29ce484222325ull;
+    const unsigned char* b = (const unsigned char*)w.data();
+    for (size_t i = 0; i < w.size() * 4; ++i) {
+        h ^= b[i];
+        h *= 0x100000001b3ull;
+    }
+    if (h != kRFnv) return mrp_fail(MRP_ERR_IO, "RPNN weight asset %s: checksum mismatch", p.c_str());
+    RpnnState* s = new RpnnState();
+    s->lay = rpnn_layout();
+    if (hipMalloc(&s->d_w, w.size() * sizeof(float)) != hipSuccess ||
+        hipMemcpy(s->d_w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->d_count, sizeof(unsigned)) != hipSuccess) {
+        c->rpnn = s;
+        mrp_rpnn_free(c);
+        return mrp_fail(MRP_ERR_NOMEM, "RPNN weights upload failed");
+    }
+    c->rpnn = s;
+    return MRP_OK;
+}
+
+// active-pixel list of a view (march) and the sky pixels of out4; returns the count on the host
+static int rpnn_march(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, float* dev_out4, int* n_active) {
+    RpnnState* s = c->rpnn;
+    const size_t n = (size_t)vw->width * vw->height;
+    if (n > s->rec_cap) {
+        (void)hipFree(s->d_recs);
+        s->d_recs = nullptr;
+        if (hipMalloc(&s->d_recs, n * sizeof(RpnnRec)) != hipSuccess) {
+            s->rec_cap = 0;
+            return mrp_fail(MRP_ERR_NOMEM, "RPNN record list");
+        }
+        s->rec_cap = n;
+    }
+    HIP_TRY(hipMemsetAsync(s->d_count, 0, sizeof(unsigned), c->stream));
+    dim3 grid((vw->width + 7) / 8, (vw->height + 7) / 8);
+    hipLaunchKernelGGL(k_rpnn_march, grid, dim3(64), 0, c->stream, mrp_dev_volume(v), mrp_to_dev(vw),
+                       (float4*)dev_out4, s->d_recs, s->d_count);
+    HIP_TRY(hipGetLastError());
+    unsigned cnt = 0;
+    HIP_TRY(hipMemcpyAsync(&cnt, s->d_count, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *n_active = (int)cnt;
+    return MRP_OK;
+}
+
+static int rpnn_buffers(RpnnState* s, int chunk) {
+    if (s->chunk >= chunk) return MRP_OK;
+    (void)hipFree(s->d_A);
+    (void)hipFree(s->d_B);
+    (void)hipFree(s->d_C);
+    (void)hipFree(s->d_T);
+    (void)hipFree(s->d_y);
+    s->d_A = s->d_B = s->d_C = s->d_T = s->d_y = nullptr;
+    s->chunk = 0;
+    if (hipMalloc(&s->d_A, (size_t)chunk * kRIn * 4) != hipSuccess ||
+        hipMalloc(&s->d_B, (size_t)chunk * kRB * 4) != hipSuccess ||
+        hipMalloc(&s->d_C, (size_t)chunk * kRB * 4) != hipSuccess ||
+        hipMalloc(&s->d_T, (size_t)chunk * kRB * 4) != hipSuccess || hipMalloc(&s->d_y, (size_t)chunk * 4) != hipSuccess)
+        return mrp_fail(MRP_ERR_NOMEM, "RPNN activations");
+    s->chunk = chunk;
+    return MRP_OK;
+}
+
+template <int EPI>
+static hipError_t gemm(hipStream_t st, const float* A, int K, const float* W, const float* bias, const float* add,
+                       float* out, int m, int To) {
+    dim3 grid((m + kGT - 1) / kGT, (To + kGT - 1) / kGT);
+    hipLaunchKernelGGL(k_rpnn_gemm<EPI>, grid, dim3(256), 0, st, A, K, W, bias, add, out, m, To);
+    return hipGetLastError();
+}
+
+// The FP32 network over m inputs in chunks: from the march's records (view set) or from given
+// rows (rows set); the outputs are composed into out4 (records) or copied to y_out (rows).
+static int rpnn_fp32(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int m, float* dev_out4, const float* rows,
+                     float* y_out) {
+    RpnnState* s = c->rpnn;
+    const int chunk = 65536;
+    int rc = rpnn_buffers(s, chunk);
+    if (rc) return rc;
+    DevVolume dv{};
+    v3 LX = mk3(0, 0, 1), lc = mk3(1, 1, 1), sr = mk3(1, 1, 1);
+    if (vw) {
+        dv = mrp_dev_volume(v);
+        LX = normalize3(mk3(vw->light_dir[0], vw->light_dir[1], vw->light_dir[2]));
+        lc = mk3(vw->light_color[0], vw->light_color[1], vw->light_color[2]);
+        sr = mk3(vw->scatter_rate[0], vw->scatter_rate[1], vw->scatter_rate[2]);
+    }
+    const float* w = s->d_w;
+    const RpnnLayout& L = s->lay;
+    hipStream_t st = c->stream;
+    for (int off = 0; off < m; off += chunk) {
+        const int n = m - off < chunk ? m - off : chunk;
+        const RpnnRec* recs = s->d_recs + off;
+        for (int l = 0; l < kRL; ++l) {
+            if (rows)
+                hipLaunchKernelGGL(k_rpnn_input_rows, dim3(n), dim3(256), 0, st, rows + (size_t)off * (kRL * kRS + 1),
+                                   n, l, s->d_A);
+            else
+                hipLaunchKernelGGL(k_rpnn_input, dim3(n), dim3(256), 0, st, dv, recs, n, l, LX, vw->alpha, s->d_A);
+            HIP_TRY(hipGetLastError());
+            if (l == 0) {
+                HIP_TRY(gemm<kEpiRelu>(st, s->d_A, kRIn, w + L.V[0], nullptr, nullptr, s->d_B, n, kRB));
+                HIP_TRY(gemm<kEpiBiasRelu>(st, s->d_B, kRB, w + L.U[0], w + L.Ub[0], nullptr, s->d_C, n, kRB));
+            } else {
+                HIP_TRY(gemm<kEpiBias>(st, s->d_C, kRB, w + L.W[l], w + L.Wb[l], nullptr, s->d_T, n, kRB));
+                HIP_TRY(gemm<kEpiAddRelu>(st, s->d_A, kRIn, w + L.V[l], nullptr, s->d_T, s->d_B, n, kRB));
+                HIP_TRY(gemm<kEpiResBiasRelu>(st, s->d_B, kRB, w + L.U[l], w + L.Ub[l], s->d_C, s->d_C, n, kRB));
+            }
+        }
+        HIP_TRY(gemm<kEpiBiasRelu>(st, s->d_C, kRB, w + L.F0, w + L.F0b, nullptr, s->d_T, n, kRB));
+        HIP_TRY(gemm<kEpiBiasRelu>(st, s->d_T, kRB, w + L.F1, w + L.F1b, nullptr, s->d_B, n, kRB));
+        HIP_TRY(gemm<kEpiOut>(st, s->d_B, kRB, w + L.F2, w + L.F2b, nullptr, rows ? y_out + off : s->d_y, n, 1));
+        if (!rows) {
+            hipLaunchKernelGGL(k_rpnn_compose, dim3((n + 255) / 256), dim3(256), 0, st, recs, n, s->d_y, lc, sr,
+                               (float4*)dev_out4);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    return MRP_OK;
+}
+
+extern "C" int mrp_render_rpnn(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int precision, float* dev_out4) {
+    if (!c || !v || !vw || !dev_out4) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    int rc = mrp_check_view(vw);
+    if (rc) return rc;
+    if (precision != MRP_FP32) return mrp_fail(MRP_ERR_INVALID, "RPNN precision %d not available", precision);
+    if (!c->rpnn && (rc = mrp_ctx_load_rpnn(c, nullptr))) return rc;
+    if (!v->mips_valid) return mrp_fail(MRP_ERR_STATE, "mrp_volume_update must run before rendering");
+    int m = 0;
+    rc = rpnn_march(c, v, vw, dev_out4, &m);
+    if (rc || m == 0) return rc;
+    return rpnn_fp32(c, v, vw, m, dev_out4, nullptr, nullptr);
+}
+
+extern "C" int mrp_rpnn_predict_rows(mrp_ctx* c, const float* dev_rows, int n, int precision, float* dev_y) {
+    if (!c || !dev_rows || !dev_y || n < 0) return mrp_fail(MRP_ERR_INVALID, "bad arguments");
+    if (precision != MRP_FP32) return mrp_fail(MRP_ERR_INVALID, "RPNN precision %d not available", precision);
+    int rc;
+    if (!c->rpnn && (rc = mrp_ctx_load_rpnn(c, nullptr))) return rc;
+    if (n == 0) return MRP_OK;
+    return rpnn_fp32(c, nullptr, nullptr, n, nullptr, dev_rows, dev_y);
+}
+
+extern "C" int mrp_rpnn_features(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, float* dev_rows, float* dev_out4) {
+    if (!c || !v || !vw || !dev_rows || !dev_out4) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    int rc = mrp_check_view(vw);
+    if (rc) return rc;
+    if (!c->rpnn && (rc = mrp_ctx_load_rpnn(c, nullptr))) return rc;
+    if (!v->mips_valid) return mrp_fail(MRP_ERR_STATE, "mrp_volume_update must run before rendering");
+    const size_t n = (size_t)vw->width * vw->height;
+    HIP_TRY(hipMemsetAsync(dev_rows, 0, n * (kRL * kRS + 1) * sizeof(float), c->stream));
+    int m = 0;
+    rc = rpnn_march(c, v, vw, dev_out4, &m);
+    if (rc || m == 0) return rc;
+    const v3 LX = normalize3(mk3(vw->light_dir[0], vw->light_dir[1], vw->light_dir[2]));
+    hipLaunchKernelGGL(k_rpnn_rows, dim3(m), dim3(256), 0, c->stream, mrp_dev_volume(v), c->rpnn->d_recs, m, LX,
+                       vw->alpha, dev_rows);
+    HIP_TRY(hipGetLastError());
+    return MRP_OK;
+}

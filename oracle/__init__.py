@@ -12,6 +12,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
 WEIGHTS = os.path.join(HERE, "..", "mrpnn_amd", "assets", "mrpnn_weights.bin")
+RPNN_WEIGHTS = os.path.join(HERE, "..", "mrpnn_amd", "assets", "rpnn_weights.bin")
+RPNN_ROW = 2251  # 10 levels x 225 density samples, then gamma
 
 NFEAT = 577
 TILE_ROW = 773  # tile feature gather row: 577 features + 4 gamma extras + 192 t-grid values
@@ -82,6 +84,12 @@ def lib():
         L.orc_fbm_volume.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, _f]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _v = ctypes.c_void_p
+        L.orc_rpnn_load.argtypes = [_f, ctypes.c_int]
+        L.orc_rpnn_predict_f32.argtypes = [_f, ctypes.c_int, _f]
+        L.orc_rpnn_predict_f64.argtypes = [_f, ctypes.c_int, _d]
+        L.orc_rpnn_descriptor.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, _f, _f, _f, ctypes.c_float, _f, _f]
+        L.orc_rpnn_descriptor.restype = None
+        L.orc_render_rpnn.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, ctypes.POINTER(View), _f, _i, _f]
         L.orc_set_env.argtypes = [_f, ctypes.c_int, ctypes.c_int, ctypes.c_float]
         L.orc_set_env.restype = None
         L.orc_sky_box.argtypes = [_f, _f]
@@ -423,3 +431,53 @@ def sky_box(d):
     dd = np.ascontiguousarray(d, np.float32)
     lib().orc_sky_box(_p(dd), _p(out))
     return out
+
+
+# ---------------------------------------------------------------------------------------
+# RPNN comparison predictor (core/radiancePredict.cu:567-722)
+# ---------------------------------------------------------------------------------------
+_rpnn_loaded = False
+
+
+def _rpnn():
+    global _rpnn_loaded
+    L = lib()
+    if not _rpnn_loaded:
+        w = np.fromfile(RPNN_WEIGHTS, dtype="<f4")
+        if L.orc_rpnn_load(w.ctypes.data_as(_f), int(w.size)) != 0:
+            raise RuntimeError("oracle: RPNN weight asset does not match the network layout")
+        _rpnn_loaded = True
+    return L
+
+
+def rpnn_predict_f32(rows):
+    rows = np.ascontiguousarray(rows, np.float32).reshape(-1, RPNN_ROW)
+    out = np.zeros(rows.shape[0], np.float32)
+    _rpnn().orc_rpnn_predict_f32(_p(rows), rows.shape[0], _p(out))
+    return out
+
+
+def rpnn_predict_f64(rows):
+    rows = np.ascontiguousarray(rows, np.float32).reshape(-1, RPNN_ROW)
+    out = np.zeros(rows.shape[0], np.float64)
+    _rpnn().orc_rpnn_predict_f64(_p(rows), rows.shape[0], _p(out, _d))
+    return out
+
+
+def rpnn_descriptor(scene, pos, X, LX, alpha):
+    desc = np.zeros(2250, np.float32)
+    gamma = np.zeros(1, np.float32)
+    a = [np.ascontiguousarray(v, np.float32) for v in (pos, X, LX)]
+    _rpnn().orc_rpnn_descriptor(_p(scene.grid), scene.N, scene.max_density, _p(scene.mips), *[_p(x) for x in a],
+                                float(alpha), _p(desc), _p(gamma))
+    return desc, float(gamma[0])
+
+
+def render_rpnn(scene, view):
+    W, H = view.width, view.height
+    rows = np.zeros((H * W, RPNN_ROW), np.float32)
+    active = np.zeros(H * W, np.int32)
+    out = np.zeros((H * W, 4), np.float32)
+    _rpnn().orc_render_rpnn(_p(scene.grid), scene.N, scene.max_density, _p(scene.mips), ctypes.byref(view),
+                            _p(rows), _p(active, _i), _p(out))
+    return dict(rows=rows, active=active, out=out)

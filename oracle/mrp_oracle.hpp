@@ -465,6 +465,11 @@ Ts4 get_sample_t(const Volume& v, f3 ori, f3 dir, float alpha);
 // reference then writes only out[0] = -1000 and the t grid; here the rest of the row is 0).
 bool tile_gather_pixel(const Volume& v, const View& view, int row, int col, int off_x, int off_y, float* out);
 
+// RPNN comparison predictor (rpnn.cpp): RadiancePredict_RPNN (radiancePredict.cu:567-722)
+void rpnn_descriptor(const Volume& v, f3 pos, f3 X, f3 LX, float alpha, float* desc /* 2250 */, float* gamma);
+bool rpnn_ready();
+float rpnn_predict_f32(const float* desc, float gamma);
+
 // Preparation steps
 void build_pyramid(const float* grid, int N, float* const mips[9]);                 // volume.cu:1474-1515
 void bake_tr(const float* const mips[9], f3 light_dir, float alpha, float* const tr[8]);  // volume.cu:1535-1560,1636-1653

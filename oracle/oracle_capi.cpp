@@ -191,6 +191,48 @@ int orc_render_predict(const float* grid, int N, float max_density, const float*
     return 0;
 }
 
+void orc_rpnn_descriptor(const float* grid, int N, float max_density, const float* mips, const float* pos3,
+                         const float* X3, const float* LX3, float alpha, float* desc, float* gamma) {
+    Volume v = to_volume(grid, N, max_density, mips, nullptr, nullptr);
+    rpnn_descriptor(v, mk(pos3[0], pos3[1], pos3[2]), mk(X3[0], X3[1], X3[2]), mk(LX3[0], LX3[1], LX3[2]), alpha,
+                    desc, gamma);
+}
+
+// RenderCamera<true, RPNN> (volume.cu:198-228 -> render.cuh:808-875): per pixel the row of 2251
+// floats the network reads (descriptor, gamma; zeros for inactive pixels), the active flag and
+// the stored pixel max(0, (predict * light_color * scatter_rate, distance)), sky = SkyBox.
+int orc_render_rpnn(const float* grid, int N, float max_density, const float* mips, const orc_view* ov,
+                    float* rows, int32_t* active, float* out4) {
+    if (!rpnn_ready()) return 1;
+    Volume v = to_volume(grid, N, max_density, mips, nullptr, nullptr);
+    View view = to_view(ov);
+    const int W = view.width, H = view.height;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int p = 0; p < W * H; ++p) {
+        PixelState st = pixel_state(v, view, p / W, p % W);
+        float* row = rows + (size_t)p * 2251;
+        float* o = out4 + 4 * (size_t)p;
+        active[p] = st.active ? 1 : 0;
+        if (!st.active) {
+            memset(row, 0, 2251 * sizeof(float));
+            const f3 sky = sky_box(v, st.dir);
+            o[0] = fmaxf(0.0f, sky.x);
+            o[1] = fmaxf(0.0f, sky.y);
+            o[2] = fmaxf(0.0f, sky.z);
+            o[3] = 0.0f;
+            continue;
+        }
+        rpnn_descriptor(v, st.pos, st.XMain, st.LXMain, view.alpha, row, row + 2250);
+        const float y = rpnn_predict_f32(row, row[2250]);
+        const f3 r = (mk(y, y, y) * view.light_color) * view.scatter_rate;  // render.cuh:874
+        o[0] = fmaxf(0.0f, r.x);
+        o[1] = fmaxf(0.0f, r.y);
+        o[2] = fmaxf(0.0f, r.z);
+        o[3] = fmaxf(0.0f, length(view.ori - st.pos));
+    }
+    return 0;
+}
+
 // Tile feature gather (GatherData, volume.cu:630-708): W*H rows of 773 floats; active[p] = 1
 // when some sub-ray of pixel p hits the medium.
 int orc_tile_gather(const float* grid, int N, float max_density, const float* mips, const float* tr,

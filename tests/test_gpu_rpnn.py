@@ -1,0 +1,72 @@
+"""GPU parity of the RPNN comparison predictor (core/radiancePredict.cu:567-722) against the
+oracle: the network's input rows (same march, light frame and mip samples; the basis and
+gamma through device vs libm sinf / cosf / acosf: 1e-4 relative), the FP32 network on
+identical rows (the reference's summation order on both sides: 1e-5 relative), and whole
+FP32 frames including sky pixels and composition."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rscene(ctx, orc):
+    import mrpnn_amd as M
+    N = 48
+    vol = M.Volume(ctx, N)
+    vol.fill_fbm(5, 4.0, 2)
+    vol.update()
+    sc = orc.Scene(orc.fbm_volume(N, 5, 4.0, 2), N)
+    ctx.load_rpnn()
+    return vol, sc
+
+
+def _view(orc, M, w=28, h=22):
+    v = orc.orbit_view(w, h, alpha=32.0, light=(0.3, 0.9, 0.2), azimuth=0.7, elevation=0.25)
+    return v, M.View.from_buffer_copy(bytes(v))
+
+
+def test_rpnn_rows(ctx, orc, rscene):
+    import mrpnn_amd as M
+    vol, sc = rscene
+    v, vm = _view(orc, M)
+    rows, out = vol.rpnn_features(vm)
+    ref = orc.render_rpnn(sc, v)
+    act = ref["active"] > 0
+    assert act.sum() > 100 and (~act).sum() > 50
+    got = rows.cpu().numpy()
+    assert np.all(got[~act] == 0)
+    # the basis' device vs libm sinf / cosf move far samples by ulps of their position
+    np.testing.assert_allclose(got[act, :2250], ref["rows"][act, :2250], rtol=1e-4, atol=2e-6)
+    d = np.abs(got[act, :2250] - ref["rows"][act, :2250])
+    assert np.median(d[d > 0] / np.abs(ref["rows"][act, :2250])[d > 0]) < 1e-6  # ulp-sized where they differ
+    np.testing.assert_allclose(got[act, 2250], ref["rows"][act, 2250], rtol=0, atol=2e-6)
+
+
+def test_rpnn_fp32_network_on_identical_rows(ctx, orc, rscene):
+    import torch
+    vol, sc = rscene
+    import mrpnn_amd as M
+    v, _ = _view(orc, M)
+    ref = orc.render_rpnn(sc, v)
+    rows = ref["rows"][ref["active"] > 0]
+    want = orc.rpnn_predict_f32(rows)
+    got = ctx.rpnn_predict_rows(torch.tensor(rows, device="cuda")).cpu().numpy()
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-30)
+    print("rpnn fp32 vs oracle: max rel %.2e, equal %.4f" % (rel.max(), np.mean(got == want)))
+    assert rel.max() <= 1e-5
+
+
+def test_rpnn_fp32_frame(ctx, orc, rscene):
+    import mrpnn_amd as M
+    vol, sc = rscene
+    v, vm = _view(orc, M)
+    got = vol.render_rpnn(vm, M.FP32).cpu().numpy()
+    ref = orc.render_rpnn(sc, v)
+    act = ref["active"] > 0
+    assert np.all(got[~act] == 0)
+    # inputs differ by ulps (basis sinf / cosf); the recurrent network carries them to the output
+    rel = np.abs(got[act, 0] - ref["out"][act, 0]) / np.maximum(ref["out"][act, 0], 1e-6)
+    print("rpnn fp32 frame vs oracle: median rel %.2e max %.2e" % (np.median(rel), rel.max()))
+    assert np.median(rel) < 1e-5 and rel.max() < 1e-3
+    np.testing.assert_allclose(got[act, 3], ref["out"][act, 3], rtol=1e-6)
