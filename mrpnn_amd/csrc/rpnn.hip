@@ -11,7 +11,7 @@
 // FP32 path (the reference's arithmetic): the march lists the active pixels; per chunk of them
 // each level's 226 inputs are written to HBM and each layer is a GEMM over (pixels x 200
 // outputs) whose every output sums its products from input 0 upwards, as Linear does, with the
-// residual adds and ReLUs in the epilogues.
+// residual adds and ReLUs in the epilogues.  BF16 path: one fused MFMA kernel (below).
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
@@ -253,13 +253,249 @@ __global__ __launch_bounds__(256) void k_rpnn_compose(const RpnnRec* __restrict_
     out[recs[p].pixel] = make_float4(fmaxf(0.0f, r.x), fmaxf(0.0f, r.y), fmaxf(0.0f, r.z), fmaxf(0.0f, recs[p].dist));
 }
 
+// ------------------------------------------------------------------------------------------
+// BF16 MFMA path: one wavefront = 16 pixels (lane = c + 16q), weights as A fragments (16
+// output rows per M-tile, 13 tiles for the 200-wide states), activations as B fragments in the
+// K-slot order where slot 8q+i <-> input 32k+4q+i and slot 8q+4+i <-> input 32k+16+4q+i, so a
+// layer's D registers are the next layer's B operand.  A 512-thread workgroup (128 pixels)
+// stages one layer's fragments (<= 104 KB) in LDS at a time; the descriptor samples are taken
+// by the lanes that feed them to the first layer of each level.
+// ------------------------------------------------------------------------------------------
+typedef __bf16 rbf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 rbf16x2_t __attribute__((ext_vector_type(2)));
+typedef float rf32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned ru32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kRNM = 13;                    // M-tiles of a 200-wide layer
+constexpr int kRKV = 8, kRKS = 7;           // 32-wide K chunks of the level input (226) and a state (200)
+constexpr int kRStages = kRL * 3 - 1 + 3;   // V, W (levels 1..9), U per level; F0, F1, F2
+constexpr int kRImgMax = kRNM * kRKV * 64;  // uint4 of the largest layer image (104 KB)
+
+struct RpnnStage {
+    int frag;  // first uint4 of the layer image
+    int nk;    // K chunks
+    int nm;    // M-tiles
+    int bias;  // first float of the padded bias (16 * nm), -1 without
+};
+struct RpnnStages {
+    RpnnStage s[kRStages];
+};
+
+__device__ inline uint32_t rpack_bf16x2(float a, float b) {  // round to nearest even
+    const rbf16x2_t v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(uint32_t, v);
+}
+__device__ inline ru32x4_t rpack8(const float* v) {
+    return ru32x4_t{rpack_bf16x2(v[0], v[1]), rpack_bf16x2(v[2], v[3]), rpack_bf16x2(v[4], v[5]),
+                    rpack_bf16x2(v[6], v[7])};
+}
+__device__ inline rf32x4_t rmfma(ru32x4_t a, ru32x4_t b, rf32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(rbf16x8_t, a), __builtin_bit_cast(rbf16x8_t, b),
+                                                   c, 0, 0, 0);
+}
+// B fragments of a 200-wide state held in D layout (13 tiles): chunk k = tiles 2k, 2k+1
+__device__ inline void rstate_frags(const rf32x4_t (&d)[kRNM], ru32x4_t (&f)[kRKS]) {
+    for (int k = 0; k < kRKS; ++k) {
+        const rf32x4_t a = d[2 * k];
+        const rf32x4_t b = 2 * k + 1 < kRNM ? d[2 * k + 1] : rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        f[k] = ru32x4_t{rpack_bf16x2(a[0], a[1]), rpack_bf16x2(a[2], a[3]), rpack_bf16x2(b[0], b[1]),
+                        rpack_bf16x2(b[2], b[3])};
+    }
+}
+__device__ inline void rstage_load(ru32x4_t* sW, const ru32x4_t* __restrict__ img, const RpnnStage& st) {
+    __syncthreads();  // the previous layer's readers are done
+    const int n = st.nm * st.nk * 64;
+    for (int i = threadIdx.x; i < n; i += 512) sW[i] = img[st.frag + i];
+    __syncthreads();
+}
+// acc[t] += W(stage) x in for the 13 M-tiles, K chunks from registers
+// (one K chunk's 13 fragments in flight at a time: a scheduling barrier per chunk keeps the
+// compiler from hoisting all 91 LDS reads of the layer)
+__device__ inline void rlayer_state(const ru32x4_t* sW, const ru32x4_t (&in)[kRKS], rf32x4_t (&acc)[kRNM], int lane) {
+    for (int k = 0; k < kRKS; ++k) {
+        for (int t = 0; t < kRNM; ++t) acc[t] = rmfma(sW[(t * kRKS + k) * 64 + lane], in[k], acc[t]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+__device__ inline void rbias_relu(rf32x4_t (&acc)[kRNM], const float* __restrict__ bias, int q, bool relu) {
+    for (int t = 0; t < kRNM; ++t) {
+        const float4 b = bias ? *(const float4*)(bias + 16 * t + 4 * q) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rf32x4_t v = acc[t] + rf32x4_t{b.x, b.y, b.z, b.w};
+        if (relu)
+            for (int i = 0; i < 4; ++i) v[i] = v[i] >= 0.0f ? v[i] : 0.0f;
+        acc[t] = v;
+    }
+}
+
+// The descriptor as the first layer's B fragments, bf16, in HBM: item ((t * 10 + l) * 8 + k) * 64
+// + lane holds the 8 inputs of K chunk k of level l for pixel 16t + (lane & 15), slots of
+// q = lane >> 4.  A separate, fully occupied kernel hides the latency of the 2250 trilinear
+// samples per pixel that the MFMA kernel would otherwise wait for.
+__global__ __launch_bounds__(256) void k_rpnn_desc(DevVolume vol, const RpnnRec* __restrict__ recs, int m, v3 LX,
+                                                   float alpha, ru32x4_t* __restrict__ desc) {
+    const long long item = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int tiles = (m + 15) / 16;
+    if (item >= (long long)tiles * kRL * kRKV * 64) return;
+    const int lane = (int)(item & 63), k = (int)((item >> 6) % kRKV), l = (int)((item / (64 * kRKV)) % kRL);
+    const int t = (int)(item / (64 * kRKV * kRL));
+    const int p = 16 * t + (lane & 15), q = lane >> 4;
+    float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (p < m) {
+        const RpnnRec r = recs[p];
+        const v3 pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
+        const v3 X = mk3(r.X[0], r.X[1], r.X[2]);
+        v3 eX, eY;
+        rpnn_frame(X, LX, &eX, &eY);
+        for (int j = 0; j < 8; ++j) {
+            const int idx = 32 * k + (j < 4 ? 4 * q + j : 16 + 4 * q + j - 4);
+            v[j] = idx < kRS ? rpnn_sample(vol, pos, eX, eY, LX, l, idx, alpha) : (idx == kRS ? acosf(dot3(X, LX)) : 0.0f);
+        }
+    }
+    desc[item] = rpack8(v);
+}
+
+__global__ __launch_bounds__(512) void k_rpnn_mfma(const RpnnRec* __restrict__ recs, int m,
+                                                   const ru32x4_t* __restrict__ desc,
+                                                   const ru32x4_t* __restrict__ img, const float* __restrict__ bias,
+                                                   const RpnnStage* __restrict__ stages, v3 lc, v3 sr,
+                                                   float4* __restrict__ out) {
+    __shared__ ru32x4_t sW[kRImgMax];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, q = lane >> 4;
+    const int tiles = (m + 127) / 128;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int t16 = tile * 8 + wave;  // this wave's 16 pixels
+        const int p = 16 * t16 + c;
+        const bool valid = p < m;
+        const ru32x4_t* dsc = desc + (size_t)t16 * kRL * kRKV * 64 + lane;
+        // live across the levels: the state C (D layout); fragments are made just before use
+        rf32x4_t C[kRNM], acc[kRNM];
+        int si = 0;
+        for (int l = 0; l < kRL; ++l) {
+            // B = relu(V_l [desc_l, gamma] (+ W_l C + b))
+            rstage_load(sW, img, stages[si]);
+            for (int t = 0; t < kRNM; ++t) acc[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int k = 0; k < kRKV; ++k) {
+                const ru32x4_t b = (16 * t16 < m) ? dsc[(l * kRKV + k) * 64] : ru32x4_t{0u, 0u, 0u, 0u};
+                for (int t = 0; t < kRNM; ++t) acc[t] = rmfma(sW[(t * kRKV + k) * 64 + lane], b, acc[t]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            ++si;
+            if (l > 0) {
+                rstage_load(sW, img, stages[si]);
+                {
+                    ru32x4_t Cf[kRKS];
+                    rstate_frags(C, Cf);
+                    rlayer_state(sW, Cf, acc, lane);
+                }
+                rbias_relu(acc, bias + stages[si].bias, q, true);
+                ++si;
+            } else {
+                rbias_relu(acc, nullptr, q, true);
+            }
+            // C = relu((C +) U_l B + b), accumulated in place
+            rstage_load(sW, img, stages[si]);
+            {
+                ru32x4_t Bf[kRKS];
+                rstate_frags(acc, Bf);
+                if (l == 0)
+                    for (int t = 0; t < kRNM; ++t) C[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+                rlayer_state(sW, Bf, C, lane);
+            }
+            rbias_relu(C, bias + stages[si].bias, q, true);
+            ++si;
+        }
+        // F0, F1 (ReLU), F2 (one output row), radiance max(exp(y) - 1, 0)
+        rstage_load(sW, img, stages[si]);
+        for (int t = 0; t < kRNM; ++t) acc[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        {
+            ru32x4_t Cf[kRKS];
+            rstate_frags(C, Cf);
+            rlayer_state(sW, Cf, acc, lane);
+        }
+        rbias_relu(acc, bias + stages[si].bias, q, true);
+        ++si;
+        rstage_load(sW, img, stages[si]);
+        for (int t = 0; t < kRNM; ++t) C[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        {
+            ru32x4_t Bf[kRKS];
+            rstate_frags(acc, Bf);
+            rlayer_state(sW, Bf, C, lane);
+        }
+        rbias_relu(C, bias + stages[si].bias, q, true);
+        ++si;
+        rstage_load(sW, img, stages[si]);
+        rf32x4_t y = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        {
+            ru32x4_t Cf[kRKS];
+            rstate_frags(C, Cf);
+            for (int k = 0; k < kRKS; ++k) y = rmfma(sW[k * 64 + lane], Cf[k], y);
+        }
+        if (valid && q == 0) {
+            float v = y[0] + bias[stages[si].bias];
+            v = v >= 0.0f ? v : 0.0f;
+            float o = (float)exp((double)v) - 1.0f;
+            o = o > 0.0f ? o : 0.0f;
+            const v3 rr = (mk3(o, o, o) * lc) * sr;
+            const RpnnRec& r = recs[p];
+            out[r.pixel] = make_float4(fmaxf(0.0f, rr.x), fmaxf(0.0f, rr.y), fmaxf(0.0f, rr.z), fmaxf(0.0f, r.dist));
+        }
+    }
+}
+
 }  // namespace mrp
 
 using namespace mrp;
 
+namespace mrp {
+uint16_t host_f32_to_bf16(float f);
+}
+
+// Fragment images and padded biases of the MFMA path, in stage order (rpnn_stage_list).
+static void rpnn_pack(const std::vector<float>& w, const RpnnLayout& L, std::vector<ru32x4_t>& img,
+                      std::vector<float>& bias, RpnnStages& st) {
+    int si = 0;
+    auto add = [&](size_t wo, long bo, int M, int K) {
+        const int nm = (M + 15) / 16, nk = (K + 31) / 32;
+        RpnnStage& s = st.s[si++];
+        s.frag = (int)img.size();
+        s.nk = nk;
+        s.nm = nm;
+        s.bias = bo < 0 ? -1 : (int)bias.size();
+        for (int mt = 0; mt < nm; ++mt)
+            for (int kc = 0; kc < nk; ++kc)
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int qq = lane >> 4, row = 16 * mt + (lane & 15);
+                    uint16_t h[8];
+                    for (int slot = 0; slot < 8; ++slot) {
+                        const int k = 32 * kc + (slot < 4 ? 4 * qq + slot : 16 + 4 * qq + slot - 4);
+                        h[slot] = host_f32_to_bf16(row < M && k < K ? w[wo + (size_t)row * K + k] : 0.0f);
+                    }
+                    img.push_back(ru32x4_t{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16),
+                                           (uint32_t)h[4] | ((uint32_t)h[5] << 16), (uint32_t)h[6] | ((uint32_t)h[7] << 16)});
+                }
+        if (bo >= 0)
+            for (int r = 0; r < 16 * nm; ++r) bias.push_back(r < M ? w[(size_t)bo + r] : 0.0f);
+    };
+    for (int l = 0; l < kRL; ++l) {
+        add(L.V[l], -1, kRB, kRIn);
+        if (l > 0) add(L.W[l], (long)L.Wb[l], kRB, kRB);
+        add(L.U[l], (long)L.Ub[l], kRB, kRB);
+    }
+    add(L.F0, (long)L.F0b, kRB, kRB);
+    add(L.F1, (long)L.F1b, kRB, kRB);
+    add(L.F2, (long)L.F2b, 1, kRB);
+}
+
 struct RpnnState {
     float* d_w = nullptr;  // fp32 asset, header order
     RpnnLayout lay;
+    ru32x4_t* d_img = nullptr;  // bf16 fragments of the MFMA path
+    float* d_bias = nullptr;
+    RpnnStages stages;
+    RpnnStage* d_stages = nullptr;
+    ru32x4_t* d_desc = nullptr;  // descriptor fragments of one chunk of pixels
+    int desc_cap = 0;            // pixels
     size_t rec_cap = 0;
     RpnnRec* d_recs = nullptr;
     unsigned* d_count = nullptr;
@@ -271,6 +507,10 @@ void mrp_rpnn_free(mrp_ctx* c) {
     RpnnState* s = c->rpnn;
     if (!s) return;
     (void)hipFree(s->d_w);
+    (void)hipFree(s->d_img);
+    (void)hipFree(s->d_bias);
+    (void)hipFree(s->d_desc);
+    (void)hipFree(s->d_stages);
     (void)hipFree(s->d_recs);
     (void)hipFree(s->d_count);
     (void)hipFree(s->d_A);
@@ -313,8 +553,17 @@ extern "C" int mrp_ctx_load_rpnn(mrp_ctx* c, const char* path) {
     if (h != kRFnv) return mrp_fail(MRP_ERR_IO, "RPNN weight asset %s: checksum mismatch", p.c_str());
     RpnnState* s = new RpnnState();
     s->lay = rpnn_layout();
+    std::vector<ru32x4_t> img;
+    std::vector<float> bias;
+    rpnn_pack(w, s->lay, img, bias, s->stages);
     if (hipMalloc(&s->d_w, w.size() * sizeof(float)) != hipSuccess ||
         hipMemcpy(s->d_w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->d_img, img.size() * sizeof(ru32x4_t)) != hipSuccess ||
+        hipMemcpy(s->d_img, img.data(), img.size() * sizeof(ru32x4_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->d_bias, bias.size() * sizeof(float)) != hipSuccess ||
+        hipMemcpy(s->d_bias, bias.data(), bias.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->d_stages, sizeof(RpnnStages)) != hipSuccess ||
+        hipMemcpy(s->d_stages, &s->stages, sizeof(RpnnStages), hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s->d_count, sizeof(unsigned)) != hipSuccess) {
         c->rpnn = s;
         mrp_rpnn_free(c);
@@ -429,13 +678,42 @@ extern "C" int mrp_render_rpnn(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, in
     if (!c || !v || !vw || !dev_out4) return mrp_fail(MRP_ERR_INVALID, "null argument");
     int rc = mrp_check_view(vw);
     if (rc) return rc;
-    if (precision != MRP_FP32) return mrp_fail(MRP_ERR_INVALID, "RPNN precision %d not available", precision);
+    if (precision != MRP_FP32 && precision != MRP_BF16)
+        return mrp_fail(MRP_ERR_INVALID, "RPNN precision %d not available (MRP_FP32, MRP_BF16)", precision);
     if (!c->rpnn && (rc = mrp_ctx_load_rpnn(c, nullptr))) return rc;
     if (!v->mips_valid) return mrp_fail(MRP_ERR_STATE, "mrp_volume_update must run before rendering");
     int m = 0;
     rc = rpnn_march(c, v, vw, dev_out4, &m);
     if (rc || m == 0) return rc;
-    return rpnn_fp32(c, v, vw, m, dev_out4, nullptr, nullptr);
+    if (precision == MRP_FP32) return rpnn_fp32(c, v, vw, m, dev_out4, nullptr, nullptr);
+    RpnnState* s = c->rpnn;
+    const v3 LX = normalize3(mk3(vw->light_dir[0], vw->light_dir[1], vw->light_dir[2]));
+    const v3 lc = mk3(vw->light_color[0], vw->light_color[1], vw->light_color[2]);
+    const v3 sr = mk3(vw->scatter_rate[0], vw->scatter_rate[1], vw->scatter_rate[2]);
+    // descriptor fragments of up to 256K pixels at a time: 4.5 KB per pixel (bf16)
+    const int chunk = 262144;
+    if (s->desc_cap < chunk) {
+        (void)hipFree(s->d_desc);
+        s->d_desc = nullptr;
+        s->desc_cap = 0;
+        if (hipMalloc(&s->d_desc, (size_t)chunk / 16 * kRL * kRKV * 64 * sizeof(ru32x4_t)) != hipSuccess)
+            return mrp_fail(MRP_ERR_NOMEM, "RPNN descriptor buffer");
+        s->desc_cap = chunk;
+    }
+    const int cus = c->props.multiProcessorCount > 0 ? c->props.multiProcessorCount : 256;
+    const DevVolume dv = mrp_dev_volume(v);
+    for (int off = 0; off < m; off += chunk) {
+        const int n = m - off < chunk ? m - off : chunk;
+        const long long items = (long long)((n + 15) / 16) * kRL * kRKV * 64;
+        hipLaunchKernelGGL(k_rpnn_desc, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, c->stream, dv,
+                           s->d_recs + off, n, LX, vw->alpha, s->d_desc);
+        HIP_TRY(hipGetLastError());
+        const int tiles = (n + 127) / 128;
+        hipLaunchKernelGGL(k_rpnn_mfma, dim3(tiles < cus ? tiles : cus), dim3(512), 0, c->stream, s->d_recs + off, n,
+                           s->d_desc, s->d_img, s->d_bias, s->d_stages, lc, sr, (float4*)dev_out4);
+        HIP_TRY(hipGetLastError());
+    }
+    return MRP_OK;
 }
 
 extern "C" int mrp_rpnn_predict_rows(mrp_ctx* c, const float* dev_rows, int n, int precision, float* dev_y) {

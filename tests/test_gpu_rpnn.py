@@ -70,3 +70,20 @@ def test_rpnn_fp32_frame(ctx, orc, rscene):
     print("rpnn fp32 frame vs oracle: median rel %.2e max %.2e" % (np.median(rel), rel.max()))
     assert np.median(rel) < 1e-5 and rel.max() < 1e-3
     np.testing.assert_allclose(got[act, 3], ref["out"][act, 3], rtol=1e-6)
+
+
+def test_rpnn_bf16_frame_close_to_fp32(ctx, orc, rscene):
+    """The BF16 MFMA network (fused, descriptor fragments from HBM) against the FP32 path on the
+    same frame: identical sky pixels and distances, radiance within a few percent on average
+    (ten recurrent levels of bf16 activations and weights)."""
+    import mrpnn_amd as M
+    vol, sc = rscene
+    v, vm = _view(orc, M, 40, 32)
+    a = vol.render_rpnn(vm, M.FP32).cpu().numpy()
+    b = vol.render_rpnn(vm, M.BF16).cpu().numpy()
+    sky = a[:, 3] == 0
+    assert np.array_equal(a[sky], b[sky]) and np.array_equal(a[:, 3], b[:, 3])
+    m = a[:, 0] > 1e-3 * a[:, 0].max()
+    rel = np.abs(b[m, 0] - a[m, 0]) / a[m, 0]
+    print("rpnn bf16 vs fp32: mean rel %.4f p99 %.4f over %d pixels" % (rel.mean(), np.quantile(rel, 0.99), m.sum()))
+    assert rel.mean() < 0.05
