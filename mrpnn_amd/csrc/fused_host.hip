@@ -431,6 +431,11 @@ extern "C" int mrp_ctx_set_option(mrp_ctx* c, int option, int value) {
         case MRP_OPT_LDS_WEIGHTS:
             c->fused->lds_weights = value != 0;
             return MRP_OK;
+        case MRP_OPT_RPNN_CHUNK:
+            if (value < 4096 || value > 1048576 || value % 128)
+                return mrp_fail(MRP_ERR_INVALID, "MRP_OPT_RPNN_CHUNK in [4096, 1048576], a multiple of 128");
+            c->rpnn_chunk = value;
+            return MRP_OK;
         case MRP_OPT_TR_SLOTS:
             if (value < 1 || value > 4096) return mrp_fail(MRP_ERR_INVALID, "MRP_OPT_TR_SLOTS in [1, 4096]");
             if (value < (int)c->tr_slots.size()) {  // shrink: free the tail sets
@@ -458,6 +463,7 @@ extern "C" int mrp_ctx_get_option(mrp_ctx* c, int option, int* value) {
         case MRP_OPT_TR_SLOTS: *value = c->tr_slot_cap; return MRP_OK;
         case MRP_OPT_GRAPH: *value = c->fused->use_graph ? 1 : 0; return MRP_OK;
         case MRP_OPT_LDS_WEIGHTS: *value = c->fused->lds_weights ? 1 : 0; return MRP_OK;
+        case MRP_OPT_RPNN_CHUNK: *value = c->rpnn_chunk; return MRP_OK;
         default: return mrp_fail(MRP_ERR_INVALID, "unknown option %d", option);
     }
 }

@@ -331,8 +331,115 @@ __device__ inline void rbias_relu(rf32x4_t (&acc)[kRNM], const float* __restrict
 // + lane holds the 8 inputs of K chunk k of level l for pixel 16t + (lane & 15), slots of
 // q = lane >> 4.  A separate, fully occupied kernel hides the latency of the 2250 trilinear
 // samples per pixel that the MFMA kernel would otherwise wait for.
+//
+// One workgroup per (16-pixel tile, level).  A level's 225 samples form a lattice of spacing
+// 1/16, 1/8, then 1/4 texel of its mip (levels 0, 1, then 2..9 on mip l-2), so they reach at
+// most 5 texels per axis: each pixel's 5 x 5 x 5 box of the mip (zeros outside the grid,
+// pre-multiplied by alpha / 64) is staged in LDS once, then every sample is an LDS trilinear
+// at texel coordinates u0 + x a + y b + z c (fused multiply-adds; the values differ from the
+// FP32 path's by float rounding, far below the bf16 step they are stored at).
+constexpr int kRBox = 125;       // 5 x 5 x 5 texels
+constexpr int kRDescStride = 260;  // floats per pixel of the staged level input (256 slots + pad)
 __global__ __launch_bounds__(256) void k_rpnn_desc(DevVolume vol, const RpnnRec* __restrict__ recs, int m, v3 LX,
                                                    float alpha, ru32x4_t* __restrict__ desc) {
+    __shared__ float sBox[16 * kRBox];
+    __shared__ __align__(16) float sIn[16 * kRDescStride];
+    __shared__ float sPix[16][16];  // u0, a, b, c (texel units), box origin, gamma
+    const int t = blockIdx.x / kRL, l = blockIdx.x % kRL;
+    const int mip = l < 2 ? 0 : l - 2;
+    const int res = 256 >> mip;
+    const float* __restrict__ grid = vol.mips[mip];
+    if (threadIdx.x < 16) {
+        const int p = 16 * t + threadIdx.x;
+        float* P = sPix[threadIdx.x];
+        if (p < m) {
+            const RpnnRec r = recs[p];
+            const v3 X = mk3(r.X[0], r.X[1], r.X[2]);
+            v3 eX, eY;
+            rpnn_frame(X, LX, &eX, &eY);
+            const float sp = ldexpf(1.0f, l - 12) * (float)res;  // lattice spacing in texels
+            const v3 u0 = (mk3(r.pos[0], r.pos[1], r.pos[2]) + 0.5f) * (float)res - 0.5f;
+            const v3 a = eX * sp, b = eY * sp, c = LX * sp;
+            const float u[3] = {u0.x, u0.y, u0.z}, A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, C[3] = {c.x, c.y, c.z};
+            for (int i = 0; i < 3; ++i) {
+                P[i] = u[i];
+                P[3 + i] = A[i];
+                P[6 + i] = B[i];
+                P[9 + i] = C[i];
+                // lowest coordinate of the lattice on this axis (x, y in [-2, 2], z in [-2, 6])
+                const float lo = u[i] - 2.0f * fabsf(A[i]) - 2.0f * fabsf(B[i]) + fminf(-2.0f * C[i], 6.0f * C[i]);
+                P[12 + i] = floorf(lo);
+            }
+            P[15] = acosf(dot3(X, LX));
+        } else {
+            for (int i = 0; i < 16; ++i) P[i] = 0.0f;
+            P[12] = P[13] = P[14] = -1.0e6f;  // box entirely outside the grid: zeros
+        }
+    }
+    __syncthreads();
+    const float scale = alpha / 64.0f;
+    for (int i = threadIdx.x; i < 16 * kRBox; i += 256) {
+        const int px = i / kRBox, rr = i - px * kRBox;
+        const int ix = rr / 25, iy = (rr / 5) % 5, iz = rr % 5;
+        const int x = (int)sPix[px][12] + ix, y = (int)sPix[px][13] + iy, z = (int)sPix[px][14] + iz;
+        float v = 0.0f;
+        if (x >= 0 && y >= 0 && z >= 0 && x < res && y < res && z < res)
+            v = grid[((size_t)x * res + y) * res + z] * scale;
+        sBox[i] = v;
+    }
+    // slots 225 (gamma) .. 255 of the level input
+    for (int i = threadIdx.x; i < 16 * 31; i += 256) {
+        const int px = i / 31, s = kRS + i % 31;
+        sIn[px * kRDescStride + s] = s == kRS ? sPix[px][15] : 0.0f;
+    }
+    __syncthreads();
+    // 16 pixels x 45 lattice rows (y, z) of 5 samples along x
+    for (int i = threadIdx.x; i < 16 * 45; i += 256) {
+        const int px = i / 45, row = i - px * 45;
+        const float* P = sPix[px];
+        const float fy = (float)(row % 5 - 2), fz = (float)(row / 5 - 2);
+        float base[3], lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            base[a] = fmaf(fz, P[9 + a], fmaf(fy, P[6 + a], P[a]));
+            lo[a] = P[12 + a];
+            hi[a] = P[12 + a] + 3.99f;
+        }
+        const float* box = sBox + px * kRBox;
+        float* dst = sIn + px * kRDescStride + row * 5;
+#pragma unroll
+        for (int xi = 0; xi < 5; ++xi) {
+            const float fx = (float)(xi - 2);
+            float w[3];
+            int o[3];
+            for (int a = 0; a < 3; ++a) {
+                // the clamp only absorbs rounding at the box's lower face (the lattice spans
+                // < 3.5 texels per axis from its lowest coordinate)
+                const float u = __builtin_amdgcn_fmed3f(fmaf(fx, P[3 + a], base[a]), lo[a], hi[a]);
+                const float f = floorf(u);
+                w[a] = u - f;
+                o[a] = (int)(f - lo[a]);
+            }
+            const float* b0 = box + o[0] * 25 + o[1] * 5 + o[2];
+            const float c00 = fmaf(w[2], b0[1] - b0[0], b0[0]), c01 = fmaf(w[2], b0[6] - b0[5], b0[5]);
+            const float c10 = fmaf(w[2], b0[26] - b0[25], b0[25]), c11 = fmaf(w[2], b0[31] - b0[30], b0[30]);
+            const float c0 = fmaf(w[1], c01 - c00, c00), c1 = fmaf(w[1], c11 - c10, c10);
+            dst[xi] = fmaf(w[0], c1 - c0, c0);
+        }
+    }
+    __syncthreads();
+    // two fragments per thread: item k * 64 + lane of this (tile, level)
+    for (int j = threadIdx.x; j < kRKV * 64; j += 256) {
+        const int k = j >> 6, lane = j & 63, c = lane & 15, q = lane >> 4;
+        const float* src = sIn + c * kRDescStride + 32 * k + 4 * q;
+        const float4 lo4 = *(const float4*)src, hi4 = *(const float4*)(src + 16);
+        const float v[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
+        desc[(size_t)blockIdx.x * (kRKV * 64) + j] = 16 * t + c < m ? rpack8(v) : ru32x4_t{0u, 0u, 0u, 0u};
+    }
+}
+
+// The same fragments from given rows (2250 descriptor values, gamma): the BF16 network alone
+__global__ __launch_bounds__(256) void k_rpnn_desc_rows(const float* __restrict__ rows, int m,
+                                                        ru32x4_t* __restrict__ desc) {
     const long long item = (long long)blockIdx.x * 256 + threadIdx.x;
     const int tiles = (m + 15) / 16;
     if (item >= (long long)tiles * kRL * kRKV * 64) return;
@@ -341,14 +448,10 @@ __global__ __launch_bounds__(256) void k_rpnn_desc(DevVolume vol, const RpnnRec*
     const int p = 16 * t + (lane & 15), q = lane >> 4;
     float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (p < m) {
-        const RpnnRec r = recs[p];
-        const v3 pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
-        const v3 X = mk3(r.X[0], r.X[1], r.X[2]);
-        v3 eX, eY;
-        rpnn_frame(X, LX, &eX, &eY);
+        const float* row = rows + (size_t)p * (kRL * kRS + 1);
         for (int j = 0; j < 8; ++j) {
             const int idx = 32 * k + (j < 4 ? 4 * q + j : 16 + 4 * q + j - 4);
-            v[j] = idx < kRS ? rpnn_sample(vol, pos, eX, eY, LX, l, idx, alpha) : (idx == kRS ? acosf(dot3(X, LX)) : 0.0f);
+            v[j] = idx < kRS ? row[l * kRS + idx] : (idx == kRS ? row[kRL * kRS] : 0.0f);
         }
     }
     desc[item] = rpack8(v);
@@ -358,7 +461,7 @@ __global__ __launch_bounds__(512) void k_rpnn_mfma(const RpnnRec* __restrict__ r
                                                    const ru32x4_t* __restrict__ desc,
                                                    const ru32x4_t* __restrict__ img, const float* __restrict__ bias,
                                                    const RpnnStage* __restrict__ stages, v3 lc, v3 sr,
-                                                   float4* __restrict__ out) {
+                                                   float4* __restrict__ out, float* __restrict__ y_out) {
     __shared__ ru32x4_t sW[kRImgMax];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, q = lane >> 4;
@@ -372,12 +475,15 @@ __global__ __launch_bounds__(512) void k_rpnn_mfma(const RpnnRec* __restrict__ r
         rf32x4_t C[kRNM], acc[kRNM];
         int si = 0;
         for (int l = 0; l < kRL; ++l) {
-            // B = relu(V_l [desc_l, gamma] (+ W_l C + b))
+            // B = relu(V_l [desc_l, gamma] (+ W_l C + b)); the level's descriptor fragments are
+            // requested before the stage load so that their HBM latency overlaps it
+            ru32x4_t dfr[kRKV];
+            for (int k = 0; k < kRKV; ++k)
+                dfr[k] = (16 * t16 < m) ? dsc[(l * kRKV + k) * 64] : ru32x4_t{0u, 0u, 0u, 0u};
             rstage_load(sW, img, stages[si]);
             for (int t = 0; t < kRNM; ++t) acc[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
             for (int k = 0; k < kRKV; ++k) {
-                const ru32x4_t b = (16 * t16 < m) ? dsc[(l * kRKV + k) * 64] : ru32x4_t{0u, 0u, 0u, 0u};
-                for (int t = 0; t < kRNM; ++t) acc[t] = rmfma(sW[(t * kRKV + k) * 64 + lane], b, acc[t]);
+                for (int t = 0; t < kRNM; ++t) acc[t] = rmfma(sW[(t * kRKV + k) * 64 + lane], dfr[k], acc[t]);
                 __builtin_amdgcn_sched_barrier(0);
             }
             ++si;
@@ -436,9 +542,14 @@ __global__ __launch_bounds__(512) void k_rpnn_mfma(const RpnnRec* __restrict__ r
             v = v >= 0.0f ? v : 0.0f;
             float o = (float)exp((double)v) - 1.0f;
             o = o > 0.0f ? o : 0.0f;
-            const v3 rr = (mk3(o, o, o) * lc) * sr;
-            const RpnnRec& r = recs[p];
-            out[r.pixel] = make_float4(fmaxf(0.0f, rr.x), fmaxf(0.0f, rr.y), fmaxf(0.0f, rr.z), fmaxf(0.0f, r.dist));
+            if (y_out) {
+                y_out[p] = o;
+            } else {
+                const v3 rr = (mk3(o, o, o) * lc) * sr;
+                const RpnnRec& r = recs[p];
+                out[r.pixel] =
+                    make_float4(fmaxf(0.0f, rr.x), fmaxf(0.0f, rr.y), fmaxf(0.0f, rr.z), fmaxf(0.0f, r.dist));
+            }
         }
     }
 }
@@ -674,6 +785,51 @@ static int rpnn_fp32(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int m, float
     return MRP_OK;
 }
 
+// The BF16 network over m inputs in chunks: descriptor fragments from the march's records (view
+// set) or from given rows (rows set), then the fused MFMA network; outputs composed into out4
+// (records) or written to y_out (rows).
+static int rpnn_bf16(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int m, float* dev_out4, const float* rows,
+                     float* y_out) {
+    RpnnState* s = c->rpnn;
+    v3 LX = mk3(0, 0, 1), lc = mk3(1, 1, 1), sr = mk3(1, 1, 1);
+    DevVolume dv{};
+    if (vw) {
+        dv = mrp_dev_volume(v);
+        LX = normalize3(mk3(vw->light_dir[0], vw->light_dir[1], vw->light_dir[2]));
+        lc = mk3(vw->light_color[0], vw->light_color[1], vw->light_color[2]);
+        sr = mk3(vw->scatter_rate[0], vw->scatter_rate[1], vw->scatter_rate[2]);
+    }
+    // descriptor fragments of c->rpnn_chunk pixels at a time (5 KB per pixel, bf16)
+    const int chunk = c->rpnn_chunk;
+    if (s->desc_cap < chunk) {
+        (void)hipFree(s->d_desc);
+        s->d_desc = nullptr;
+        s->desc_cap = 0;
+        if (hipMalloc(&s->d_desc, (size_t)chunk / 16 * kRL * kRKV * 64 * sizeof(ru32x4_t)) != hipSuccess)
+            return mrp_fail(MRP_ERR_NOMEM, "RPNN descriptor buffer");
+        s->desc_cap = chunk;
+    }
+    const int cus = c->props.multiProcessorCount > 0 ? c->props.multiProcessorCount : 256;
+    for (int off = 0; off < m; off += chunk) {
+        const int n = m - off < chunk ? m - off : chunk;
+        if (rows) {
+            const long long items = (long long)((n + 15) / 16) * kRL * kRKV * 64;
+            hipLaunchKernelGGL(k_rpnn_desc_rows, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, c->stream,
+                               rows + (size_t)off * (kRL * kRS + 1), n, s->d_desc);
+        } else {
+            hipLaunchKernelGGL(k_rpnn_desc, dim3((unsigned)((n + 15) / 16 * kRL)), dim3(256), 0, c->stream, dv,
+                               s->d_recs + off, n, LX, vw->alpha, s->d_desc);
+        }
+        HIP_TRY(hipGetLastError());
+        const int tiles = (n + 127) / 128;
+        hipLaunchKernelGGL(k_rpnn_mfma, dim3(tiles < cus ? tiles : cus), dim3(512), 0, c->stream,
+                           rows ? nullptr : s->d_recs + off, n, s->d_desc, s->d_img, s->d_bias, s->d_stages, lc, sr,
+                           (float4*)dev_out4, rows ? y_out + off : nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    return MRP_OK;
+}
+
 extern "C" int mrp_render_rpnn(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int precision, float* dev_out4) {
     if (!c || !v || !vw || !dev_out4) return mrp_fail(MRP_ERR_INVALID, "null argument");
     int rc = mrp_check_view(vw);
@@ -686,42 +842,17 @@ extern "C" int mrp_render_rpnn(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, in
     rc = rpnn_march(c, v, vw, dev_out4, &m);
     if (rc || m == 0) return rc;
     if (precision == MRP_FP32) return rpnn_fp32(c, v, vw, m, dev_out4, nullptr, nullptr);
-    RpnnState* s = c->rpnn;
-    const v3 LX = normalize3(mk3(vw->light_dir[0], vw->light_dir[1], vw->light_dir[2]));
-    const v3 lc = mk3(vw->light_color[0], vw->light_color[1], vw->light_color[2]);
-    const v3 sr = mk3(vw->scatter_rate[0], vw->scatter_rate[1], vw->scatter_rate[2]);
-    // descriptor fragments of up to 256K pixels at a time: 4.5 KB per pixel (bf16)
-    const int chunk = 262144;
-    if (s->desc_cap < chunk) {
-        (void)hipFree(s->d_desc);
-        s->d_desc = nullptr;
-        s->desc_cap = 0;
-        if (hipMalloc(&s->d_desc, (size_t)chunk / 16 * kRL * kRKV * 64 * sizeof(ru32x4_t)) != hipSuccess)
-            return mrp_fail(MRP_ERR_NOMEM, "RPNN descriptor buffer");
-        s->desc_cap = chunk;
-    }
-    const int cus = c->props.multiProcessorCount > 0 ? c->props.multiProcessorCount : 256;
-    const DevVolume dv = mrp_dev_volume(v);
-    for (int off = 0; off < m; off += chunk) {
-        const int n = m - off < chunk ? m - off : chunk;
-        const long long items = (long long)((n + 15) / 16) * kRL * kRKV * 64;
-        hipLaunchKernelGGL(k_rpnn_desc, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, c->stream, dv,
-                           s->d_recs + off, n, LX, vw->alpha, s->d_desc);
-        HIP_TRY(hipGetLastError());
-        const int tiles = (n + 127) / 128;
-        hipLaunchKernelGGL(k_rpnn_mfma, dim3(tiles < cus ? tiles : cus), dim3(512), 0, c->stream, s->d_recs + off, n,
-                           s->d_desc, s->d_img, s->d_bias, s->d_stages, lc, sr, (float4*)dev_out4);
-        HIP_TRY(hipGetLastError());
-    }
-    return MRP_OK;
+    return rpnn_bf16(c, v, vw, m, dev_out4, nullptr, nullptr);
 }
 
 extern "C" int mrp_rpnn_predict_rows(mrp_ctx* c, const float* dev_rows, int n, int precision, float* dev_y) {
     if (!c || !dev_rows || !dev_y || n < 0) return mrp_fail(MRP_ERR_INVALID, "bad arguments");
-    if (precision != MRP_FP32) return mrp_fail(MRP_ERR_INVALID, "RPNN precision %d not available", precision);
+    if (precision != MRP_FP32 && precision != MRP_BF16)
+        return mrp_fail(MRP_ERR_INVALID, "RPNN precision %d not available (MRP_FP32, MRP_BF16)", precision);
     int rc;
     if (!c->rpnn && (rc = mrp_ctx_load_rpnn(c, nullptr))) return rc;
     if (n == 0) return MRP_OK;
+    if (precision == MRP_BF16) return rpnn_bf16(c, nullptr, nullptr, n, nullptr, dev_rows, dev_y);
     return rpnn_fp32(c, nullptr, nullptr, n, nullptr, dev_rows, dev_y);
 }
 

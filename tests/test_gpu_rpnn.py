@@ -87,3 +87,44 @@ def test_rpnn_bf16_frame_close_to_fp32(ctx, orc, rscene):
     rel = np.abs(b[m, 0] - a[m, 0]) / a[m, 0]
     print("rpnn bf16 vs fp32: mean rel %.4f p99 %.4f over %d pixels" % (rel.mean(), np.quantile(rel, 0.99), m.sum()))
     assert rel.mean() < 0.05
+
+
+def test_rpnn_bf16_network_on_identical_rows(ctx, orc, rscene):
+    """The BF16 MFMA network alone (fragments packed from the oracle's rows) against the
+    oracle's FP32 network."""
+    import torch
+    import mrpnn_amd as M
+    vol, sc = rscene
+    v, _ = _view(orc, M)
+    ref = orc.render_rpnn(sc, v)
+    rows = ref["rows"][ref["active"] > 0]
+    want = orc.rpnn_predict_f32(rows)
+    got = ctx.rpnn_predict_rows(torch.tensor(rows, device="cuda"), M.BF16).cpu().numpy()
+    m = want > 1e-3 * want.max()
+    rel = np.abs(got[m] - want[m]) / want[m]
+    print("rpnn bf16 rows vs oracle fp32: mean rel %.4f p99 %.4f" % (rel.mean(), np.quantile(rel, 0.99)))
+    assert rel.mean() < 0.05 and np.all(np.isfinite(got)) and np.all(got >= 0)
+
+
+def test_rpnn_bf16_descriptor_matches_rows(ctx, orc, rscene):
+    """The BF16 render's own descriptor (LDS boxes, fused lattice coordinates) against the exact
+    FP32 rows of the same view fed to the same BF16 network: the two inputs differ by float
+    rounding only, so after bf16 rounding the outputs agree to a small fraction of a percent."""
+    import torch
+    import mrpnn_amd as M
+    vol, sc = rscene
+    for w, h, azi in ((40, 32, 0.7), (36, 30, 2.3)):
+        v = orc.orbit_view(w, h, alpha=32.0, light=(-0.5, 0.4, 0.77), azimuth=azi, elevation=0.4)
+        vm = M.View.from_buffer_copy(bytes(v))
+        rows, _ = vol.rpnn_features(vm)
+        got4 = vol.render_rpnn(vm, M.BF16).cpu().numpy()
+        act = got4[:, 3] > 0  # distance: zero for sky pixels
+        assert act.sum() > 300
+        idx = torch.from_numpy(np.flatnonzero(act)).cuda()
+        y = ctx.rpnn_predict_rows(rows[idx].contiguous(), M.BF16)
+        want = (y.cpu().numpy() * np.float32(v.light_color[0])) * np.float32(v.scatter_rate[0])
+        got = got4[act, 0]
+        m = want > 1e-3 * want.max()
+        rel = np.abs(got[m] - want[m]) / want[m]
+        print("rpnn bf16 render vs bf16(rows): median %.2e p99 %.2e max %.2e" % (np.median(rel), np.quantile(rel, 0.99), rel.max()))
+        assert np.median(rel) < 1e-3 and np.quantile(rel, 0.99) < 1e-2
