@@ -58,8 +58,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=4)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device-to-device (scale reference for HBM counters)")
-    ap.add_argument("--config", default="headline", choices=["headline", "c1", "c2", "c3", "c4", "c5", "tile", "interactive", "ingest"],
-                    help="headline = the metric line; c1..c5, tile, interactive, ingest = the other workloads (one JSON line each)")
+    ap.add_argument("--config", default="headline", choices=["headline", "c1", "c2", "c3", "c4", "c5", "tile", "interactive", "ingest", "rpnn"],
+                    help="headline = the metric line; c1..c5, tile, interactive, ingest, rpnn = the other workloads (one JSON line each)")
     ap.add_argument("--pt-res", type=int, default=1024, help="c3: image side")
     ap.add_argument("--pt-spp", type=int, default=1024, help="c3: samples per pixel")
     ap.add_argument("--pt-bounces", type=int, default=512, help="c3: maximum bounces")

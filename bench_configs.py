@@ -15,7 +15,9 @@ tile  tile feature gather (the training-input path, MRPNNInputData.fetch_input) 
     over a 512^3 cloud, with the oracle's CPU gather of the same frame as baseline and check;
 interactive  the GUI's progressive renderer at 1024x1024 over a 512^3 cloud (MRPNN BF16 / FP32,
     path tracing with and without the histogram filter);
-ingest  loaders (.vox / .txt / .bin), the RGBE reader and the cost of an environment map.
+ingest  loaders (.vox / .txt / .bin), the RGBE reader and the cost of an environment map;
+rpnn  the RPNN comparison predictor at 1920x1080 over a 512^3 cloud (FP32 and BF16 MFMA paths),
+    against the MRPNN BF16 frame of the same view and the oracle's RPNN on the host cores.
 
 All volumes are the seeded fBm clouds of the library (mrp_volume_fill_fbm); the oracle is the
 CPU restatement under oracle/ and is used here only as the timed CPU baseline and as the FP64
@@ -537,4 +539,87 @@ def ingest(args):
             torch.cuda.synchronize()
             res["frame_1080p_bf16_ms_" + name] = round((time.perf_counter() - t0) / 20 * 1e3, 3)
         ctx.close()
+    return res
+
+
+def rpnn(args):
+    """RPNN, the paper's comparison predictor (RenderCamera<true, RPNN>), at 1920x1080 over a
+    512^3 cloud with the headline camera and sun: frames per second of the FP32 path (the
+    reference's arithmetic) and of the BF16 MFMA path, the BF16 error against FP32, the
+    MRPNN BF16 batched frame of the same view for comparison, and the oracle's RPNN on the
+    host cores over a 960x540 frame of the same camera as baseline and check."""
+    import torch
+
+    import mrpnn_amd as M
+    import oracle as O
+    ctx = M.Context()
+    N, W, H = 512, 1920, 1080
+    vol = M.Volume(ctx, N)
+    vol.fill_fbm(6, 4.0, 1)
+    vol.update()
+    ctx.load_rpnn()
+    sun = _sun(30.0, 40.0)
+
+    def view(w, h):
+        return M.orbit_view(w, h, dist=2.0, azimuth=0.6, elevation=0.25, fov_deg=40.0, light=sun, alpha=64.0, g=G,
+                            albedo=0.99, seed=1, jitter=0.5)
+
+    v = view(W, H)
+    out = torch.zeros((W * H, 4), device="cuda")
+    res = {"config": "rpnn", "workload": "RPNN frame at 1920x1080 over a 512^3 fBm cloud (6 octaves), orbit camera "
+                                         "(distance 2, fov 40), alpha 64, one sun",
+           "network_flop_per_pixel": 2 * 1292200}
+    frames = {}
+    for name, prec, reps in (("bf16", M.BF16, 10), ("fp32", M.FP32, 3)):
+        vol.render_rpnn(v, prec, out)
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            vol.render_rpnn(v, prec, out)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        frames[name] = out[:, 0].cpu().numpy().copy()
+        active = int((out[:, 3] > 0).sum())
+        dt = float(np.mean(lat))
+        res[name] = {"ms_per_frame": round(dt * 1e3, 3), "mpix_s": round(W * H / dt / 1e6, 2),
+                     "p99_ms": round(float(np.quantile(lat, 0.99)) * 1e3, 3), "active_pixels": active,
+                     "network_tflops": round(active * res["network_flop_per_pixel"] / dt / 1e12, 1)}
+        _log("rpnn %s: %.2f ms per frame, %d active pixels" % (name, dt * 1e3, active))
+    res["speedup_bf16_vs_fp32"] = round(res["fp32"]["ms_per_frame"] / res["bf16"]["ms_per_frame"], 2)
+    res["bf16_vs_fp32"] = _rel(frames["bf16"], frames["fp32"])
+    # MRPNN on the same view (batched BF16 path)
+    ctx.prepare_views([vol], [v])
+    ctx.render_batch([vol], [v], [out], M.BF16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        ctx.prepare_views([vol], [v])
+        ctx.render_batch([vol], [v], [out], M.BF16)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 10
+    res["mrpnn_bf16_same_view"] = {"ms_per_frame": round(dt * 1e3, 3)}
+    res["rpnn_bf16_over_mrpnn_bf16_time"] = round(res["bf16"]["ms_per_frame"] / (dt * 1e3), 2)
+    if args.no_cpu:
+        ctx.close()
+        return res
+    # the oracle on the host cores: a 960x540 frame of the same camera, and parity against it
+    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    sc = O.Scene.from_arrays(vol.get_data(), N, 1.0, vol.download_mips(), vol.download_tr(), ctx.download_hg_lut())
+    vs = view(960, 540)
+    t0 = time.perf_counter()
+    ref = O.render_rpnn(sc, O.View.from_buffer_copy(bytes(vs)))
+    cpu_s = time.perf_counter() - t0
+    act = ref["active"] > 0
+    got32 = vol.render_rpnn(vs, M.FP32).cpu().numpy()
+    got16 = vol.render_rpnn(vs, M.BF16).cpu().numpy()
+    rel = np.abs(got32[act, 0] - ref["out"][act, 0]) / np.maximum(ref["out"][act, 0], 1e-6)
+    res["cpu_baseline"] = {"pixels_s": round(960 * 540 / cpu_s, 1), "active_pixels_s": round(int(act.sum()) / cpu_s, 1),
+                           "seconds": round(cpu_s, 3), "cores": threads, "kind": "port",
+                           "sample": "one 960x540 frame of the same camera (descriptor and FP32 network)"}
+    res["parity_vs_oracle"] = {"fp32_median_rel": float(np.median(rel)), "fp32_max_rel": float(rel.max()),
+                               "sky_equal": bool(np.array_equal(got32[~act], ref["out"][~act])),
+                               "bf16": _rel(got16[act, 0], ref["out"][act, 0])}
+    ctx.close()
     return res
