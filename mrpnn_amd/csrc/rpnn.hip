@@ -362,13 +362,13 @@ __global__ __launch_bounds__(256) void k_rpnn_desc(DevVolume vol, const RpnnRec*
             const v3 a = eX * sp, b = eY * sp, c = LX * sp;
             const float u[3] = {u0.x, u0.y, u0.z}, A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, C[3] = {c.x, c.y, c.z};
             for (int i = 0; i < 3; ++i) {
-                P[i] = u[i];
+                // lowest coordinate of the lattice on this axis (x, y in [-2, 2], z in [-2, 6])
+                const float lo = floorf(u[i] - 2.0f * fabsf(A[i]) - 2.0f * fabsf(B[i]) + fminf(-2.0f * C[i], 6.0f * C[i]));
+                P[i] = u[i] - lo;  // lattice origin relative to the box
                 P[3 + i] = A[i];
                 P[6 + i] = B[i];
                 P[9 + i] = C[i];
-                // lowest coordinate of the lattice on this axis (x, y in [-2, 2], z in [-2, 6])
-                const float lo = u[i] - 2.0f * fabsf(A[i]) - 2.0f * fabsf(B[i]) + fminf(-2.0f * C[i], 6.0f * C[i]);
-                P[12 + i] = floorf(lo);
+                P[12 + i] = lo;
             }
             P[15] = acosf(dot3(X, LX));
         } else {
@@ -377,15 +377,21 @@ __global__ __launch_bounds__(256) void k_rpnn_desc(DevVolume vol, const RpnnRec*
         }
     }
     __syncthreads();
+    // the boxes: 16 pixels x 25 (x, y) rows of 5 texels along z, loads at clamped in-grid
+    // indices, zero outside the grid
     const float scale = alpha / 64.0f;
-    for (int i = threadIdx.x; i < 16 * kRBox; i += 256) {
-        const int px = i / kRBox, rr = i - px * kRBox;
-        const int ix = rr / 25, iy = (rr / 5) % 5, iz = rr % 5;
-        const int x = (int)sPix[px][12] + ix, y = (int)sPix[px][13] + iy, z = (int)sPix[px][14] + iz;
-        float v = 0.0f;
-        if (x >= 0 && y >= 0 && z >= 0 && x < res && y < res && z < res)
-            v = grid[((size_t)x * res + y) * res + z] * scale;
-        sBox[i] = v;
+    for (int i = threadIdx.x; i < 16 * 25; i += 256) {
+        const int px = i / 25, rr = i - px * 25;
+        const int x = (int)sPix[px][12] + rr / 5, y = (int)sPix[px][13] + rr % 5, z0 = (int)sPix[px][14];
+        const bool row_in = x >= 0 && y >= 0 && x < res && y < res;
+        const float* __restrict__ src = grid + ((size_t)(row_in ? x : 0) * res + (row_in ? y : 0)) * res;
+        float* dst = sBox + px * kRBox + rr * 5;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const int z = z0 + k;
+            const float v = src[min(max(z, 0), res - 1)] * scale;
+            dst[k] = row_in && z >= 0 && z < res ? v : 0.0f;
+        }
     }
     // slots 225 (gamma) .. 255 of the level input
     for (int i = threadIdx.x; i < 16 * 31; i += 256) {
@@ -393,17 +399,13 @@ __global__ __launch_bounds__(256) void k_rpnn_desc(DevVolume vol, const RpnnRec*
         sIn[px * kRDescStride + s] = s == kRS ? sPix[px][15] : 0.0f;
     }
     __syncthreads();
-    // 16 pixels x 45 lattice rows (y, z) of 5 samples along x
+    // 16 pixels x 45 lattice rows (y, z) of 5 samples along x; lanes take different pixels
     for (int i = threadIdx.x; i < 16 * 45; i += 256) {
-        const int px = i / 45, row = i - px * 45;
+        const int px = i & 15, row = i >> 4;
         const float* P = sPix[px];
         const float fy = (float)(row % 5 - 2), fz = (float)(row / 5 - 2);
-        float base[3], lo[3], hi[3];
-        for (int a = 0; a < 3; ++a) {
-            base[a] = fmaf(fz, P[9 + a], fmaf(fy, P[6 + a], P[a]));
-            lo[a] = P[12 + a];
-            hi[a] = P[12 + a] + 3.99f;
-        }
+        float base[3];
+        for (int a = 0; a < 3; ++a) base[a] = fmaf(fz, P[9 + a], fmaf(fy, P[6 + a], P[a]));
         const float* box = sBox + px * kRBox;
         float* dst = sIn + px * kRDescStride + row * 5;
 #pragma unroll
@@ -412,14 +414,13 @@ __global__ __launch_bounds__(256) void k_rpnn_desc(DevVolume vol, const RpnnRec*
             float w[3];
             int o[3];
             for (int a = 0; a < 3; ++a) {
-                // the clamp only absorbs rounding at the box's lower face (the lattice spans
-                // < 3.5 texels per axis from its lowest coordinate)
-                const float u = __builtin_amdgcn_fmed3f(fmaf(fx, P[3 + a], base[a]), lo[a], hi[a]);
-                const float f = floorf(u);
-                w[a] = u - f;
-                o[a] = (int)(f - lo[a]);
+                // box coordinate; the clamp only absorbs rounding at the box's lower face (the
+                // lattice spans < 3.5 texels per axis from its lowest coordinate)
+                const float d = __builtin_amdgcn_fmed3f(fmaf(fx, P[3 + a], base[a]), 0.0f, 3.99f);
+                w[a] = __builtin_amdgcn_fractf(d);
+                o[a] = (int)d;
             }
-            const float* b0 = box + o[0] * 25 + o[1] * 5 + o[2];
+            const float* b0 = box + (__mul24(o[0], 25) + __mul24(o[1], 5) + o[2]);  // o in [0, 3]
             const float c00 = fmaf(w[2], b0[1] - b0[0], b0[0]), c01 = fmaf(w[2], b0[6] - b0[5], b0[5]);
             const float c10 = fmaf(w[2], b0[26] - b0[25], b0[25]), c11 = fmaf(w[2], b0[31] - b0[30], b0[30]);
             const float c0 = fmaf(w[1], c01 - c00, c00), c1 = fmaf(w[1], c11 - c10, c10);
@@ -433,7 +434,9 @@ __global__ __launch_bounds__(256) void k_rpnn_desc(DevVolume vol, const RpnnRec*
         const float* src = sIn + c * kRDescStride + 32 * k + 4 * q;
         const float4 lo4 = *(const float4*)src, hi4 = *(const float4*)(src + 16);
         const float v[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
-        desc[(size_t)blockIdx.x * (kRKV * 64) + j] = 16 * t + c < m ? rpack8(v) : ru32x4_t{0u, 0u, 0u, 0u};
+        // streamed once: kept out of the caches the network kernel's weights live in
+        __builtin_nontemporal_store(16 * t + c < m ? rpack8(v) : ru32x4_t{0u, 0u, 0u, 0u},
+                                    desc + (size_t)blockIdx.x * (kRKV * 64) + j);
     }
 }
 
@@ -476,10 +479,12 @@ __global__ __launch_bounds__(512) void k_rpnn_mfma(const RpnnRec* __restrict__ r
         int si = 0;
         for (int l = 0; l < kRL; ++l) {
             // B = relu(V_l [desc_l, gamma] (+ W_l C + b)); the level's descriptor fragments are
-            // requested before the stage load so that their HBM latency overlaps it
+            // requested before the stage load so that their HBM latency overlaps it (read once:
+            // non-temporal, so they do not evict the weight fragments from L2)
             ru32x4_t dfr[kRKV];
             for (int k = 0; k < kRKV; ++k)
-                dfr[k] = (16 * t16 < m) ? dsc[(l * kRKV + k) * 64] : ru32x4_t{0u, 0u, 0u, 0u};
+                dfr[k] = (16 * t16 < m) ? __builtin_nontemporal_load(dsc + (l * kRKV + k) * 64)
+                                        : ru32x4_t{0u, 0u, 0u, 0u};
             rstage_load(sW, img, stages[si]);
             for (int t = 0; t < kRNM; ++t) acc[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
             for (int k = 0; k < kRKV; ++k) {
