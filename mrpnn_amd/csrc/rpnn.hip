@@ -302,18 +302,37 @@ __device__ inline void rstate_frags(const rf32x4_t (&d)[kRNM], ru32x4_t (&f)[kRK
                         rpack_bf16x2(b[2], b[3])};
     }
 }
+// Stage loads: the whole layer image (<= 104 KB) into LDS between two barriers
+template <int THREADS>
 __device__ inline void rstage_load(ru32x4_t* sW, const ru32x4_t* __restrict__ img, const RpnnStage& st) {
     __syncthreads();  // the previous layer's readers are done
     const int n = st.nm * st.nk * 64;
-    for (int i = threadIdx.x; i < n; i += 512) sW[i] = img[st.frag + i];
+    for (int i = threadIdx.x; i < n; i += THREADS) sW[i] = img[st.frag + i];
     __syncthreads();
 }
-// acc[t] += W(stage) x in for the 13 M-tiles, K chunks from registers
-// (one K chunk's 13 fragments in flight at a time: a scheduling barrier per chunk keeps the
-// compiler from hoisting all 91 LDS reads of the layer)
-__device__ inline void rlayer_state(const ru32x4_t* sW, const ru32x4_t (&in)[kRKS], rf32x4_t (&acc)[kRNM], int lane) {
-    for (int k = 0; k < kRKS; ++k) {
-        for (int t = 0; t < kRNM; ++t) acc[t] = rmfma(sW[(t * kRKS + k) * 64 + lane], in[k], acc[t]);
+// acc[j][t] += W(stage)[t] x in[j] for the 13 M-tiles and the wave's NT pixel tiles, K chunks
+// from registers.  Each A fragment read from LDS feeds NT MFMAs; its register is refilled with
+// the next K chunk's fragment right after them (scheduling group barriers pin that interleave,
+// so one K chunk's 13 fragments are live and their LDS latency hides behind the MFMAs).
+template <int NT, int NK>
+__device__ inline void rlayer(const ru32x4_t* sW, const ru32x4_t (&in)[NT][NK], rf32x4_t (&acc)[NT][kRNM], int lane) {
+    ru32x4_t a[kRNM];
+#pragma unroll
+    for (int t = 0; t < kRNM; ++t) a[t] = sW[t * NK * 64 + lane];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+#pragma unroll
+        for (int t = 0; t < kRNM; ++t) {
+#pragma unroll
+            for (int j = 0; j < NT; ++j) acc[j][t] = rmfma(a[t], in[j][k], acc[j][t]);
+            if (k + 1 < NK) a[t] = sW[(t * NK + k + 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int t = 0; t < kRNM; ++t) {
+            __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);  // NT MFMAs
+            if (k + 1 < NK) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
 }
@@ -460,103 +479,117 @@ __global__ __launch_bounds__(256) void k_rpnn_desc_rows(const float* __restrict_
     desc[item] = rpack8(v);
 }
 
-__global__ __launch_bounds__(512) void k_rpnn_mfma(const RpnnRec* __restrict__ recs, int m,
-                                                   const ru32x4_t* __restrict__ desc,
-                                                   const ru32x4_t* __restrict__ img, const float* __restrict__ bias,
-                                                   const RpnnStage* __restrict__ stages, v3 lc, v3 sr,
-                                                   float4* __restrict__ out, float* __restrict__ y_out) {
-    __shared__ ru32x4_t sW[kRImgMax];
+// The fused network: a 128-pixel workgroup of 8 / NT wavefronts, each with NT 16-pixel tiles
+// (the kernel runs NT = 1: two tiles per wavefront would halve the LDS reads per MFMA but do
+// not fit 512 registers without spilling).
+template <int NT>
+__device__ inline void rpnn_network(const RpnnRec* __restrict__ recs, int m, const ru32x4_t* __restrict__ desc,
+                                    const ru32x4_t* __restrict__ img, const float* __restrict__ bias,
+                                    const RpnnStage* __restrict__ stages, v3 lc, v3 sr, float4* __restrict__ out,
+                                    float* __restrict__ y_out, ru32x4_t* sW) {
+    constexpr int kThreads = 512 / NT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, q = lane >> 4;
     const int tiles = (m + 127) / 128;
     for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const int t16 = tile * 8 + wave;  // this wave's 16 pixels
-        const int p = 16 * t16 + c;
-        const bool valid = p < m;
-        const ru32x4_t* dsc = desc + (size_t)t16 * kRL * kRKV * 64 + lane;
+        const int t0 = tile * 8 + wave * NT;  // this wave's first 16-pixel tile
         // live across the levels: the state C (D layout); fragments are made just before use
-        rf32x4_t C[kRNM], acc[kRNM];
+        rf32x4_t C[NT][kRNM], acc[NT][kRNM];
         int si = 0;
         for (int l = 0; l < kRL; ++l) {
             // B = relu(V_l [desc_l, gamma] (+ W_l C + b)); the level's descriptor fragments are
             // requested before the stage load so that their HBM latency overlaps it (read once:
             // non-temporal, so they do not evict the weight fragments from L2)
-            ru32x4_t dfr[kRKV];
-            for (int k = 0; k < kRKV; ++k)
-                dfr[k] = (16 * t16 < m) ? __builtin_nontemporal_load(dsc + (l * kRKV + k) * 64)
-                                        : ru32x4_t{0u, 0u, 0u, 0u};
-            rstage_load(sW, img, stages[si]);
-            for (int t = 0; t < kRNM; ++t) acc[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-            for (int k = 0; k < kRKV; ++k) {
-                for (int t = 0; t < kRNM; ++t) acc[t] = rmfma(sW[(t * kRKV + k) * 64 + lane], dfr[k], acc[t]);
-                __builtin_amdgcn_sched_barrier(0);
+            ru32x4_t dfr[NT][kRKV];
+            for (int j = 0; j < NT; ++j) {
+                const ru32x4_t* dsc = desc + (size_t)(t0 + j) * kRL * kRKV * 64 + lane;
+                for (int k = 0; k < kRKV; ++k)
+                    dfr[j][k] = (16 * (t0 + j) < m) ? __builtin_nontemporal_load(dsc + (l * kRKV + k) * 64)
+                                                    : ru32x4_t{0u, 0u, 0u, 0u};
             }
+            rstage_load<kThreads>(sW, img, stages[si]);
+            for (int j = 0; j < NT; ++j)
+                for (int t = 0; t < kRNM; ++t) acc[j][t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            rlayer<NT, kRKV>(sW, dfr, acc, lane);
             ++si;
             if (l > 0) {
-                rstage_load(sW, img, stages[si]);
+                rstage_load<kThreads>(sW, img, stages[si]);
                 {
-                    ru32x4_t Cf[kRKS];
-                    rstate_frags(C, Cf);
-                    rlayer_state(sW, Cf, acc, lane);
+                    ru32x4_t Cf[NT][kRKS];
+                    for (int j = 0; j < NT; ++j) rstate_frags(C[j], Cf[j]);
+                    rlayer<NT, kRKS>(sW, Cf, acc, lane);
                 }
-                rbias_relu(acc, bias + stages[si].bias, q, true);
+                for (int j = 0; j < NT; ++j) rbias_relu(acc[j], bias + stages[si].bias, q, true);
                 ++si;
             } else {
-                rbias_relu(acc, nullptr, q, true);
+                for (int j = 0; j < NT; ++j) rbias_relu(acc[j], nullptr, q, true);
             }
             // C = relu((C +) U_l B + b), accumulated in place
-            rstage_load(sW, img, stages[si]);
+            rstage_load<kThreads>(sW, img, stages[si]);
             {
-                ru32x4_t Bf[kRKS];
-                rstate_frags(acc, Bf);
+                ru32x4_t Bf[NT][kRKS];
+                for (int j = 0; j < NT; ++j) rstate_frags(acc[j], Bf[j]);
                 if (l == 0)
-                    for (int t = 0; t < kRNM; ++t) C[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-                rlayer_state(sW, Bf, C, lane);
+                    for (int j = 0; j < NT; ++j)
+                        for (int t = 0; t < kRNM; ++t) C[j][t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+                rlayer<NT, kRKS>(sW, Bf, C, lane);
             }
-            rbias_relu(C, bias + stages[si].bias, q, true);
+            for (int j = 0; j < NT; ++j) rbias_relu(C[j], bias + stages[si].bias, q, true);
             ++si;
         }
         // F0, F1 (ReLU), F2 (one output row), radiance max(exp(y) - 1, 0)
-        rstage_load(sW, img, stages[si]);
-        for (int t = 0; t < kRNM; ++t) acc[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        rstage_load<kThreads>(sW, img, stages[si]);
+        for (int j = 0; j < NT; ++j)
+            for (int t = 0; t < kRNM; ++t) acc[j][t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
         {
-            ru32x4_t Cf[kRKS];
-            rstate_frags(C, Cf);
-            rlayer_state(sW, Cf, acc, lane);
+            ru32x4_t Cf[NT][kRKS];
+            for (int j = 0; j < NT; ++j) rstate_frags(C[j], Cf[j]);
+            rlayer<NT, kRKS>(sW, Cf, acc, lane);
         }
-        rbias_relu(acc, bias + stages[si].bias, q, true);
+        for (int j = 0; j < NT; ++j) rbias_relu(acc[j], bias + stages[si].bias, q, true);
         ++si;
-        rstage_load(sW, img, stages[si]);
-        for (int t = 0; t < kRNM; ++t) C[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        rstage_load<kThreads>(sW, img, stages[si]);
+        for (int j = 0; j < NT; ++j)
+            for (int t = 0; t < kRNM; ++t) C[j][t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
         {
-            ru32x4_t Bf[kRKS];
-            rstate_frags(acc, Bf);
-            rlayer_state(sW, Bf, C, lane);
+            ru32x4_t Bf[NT][kRKS];
+            for (int j = 0; j < NT; ++j) rstate_frags(acc[j], Bf[j]);
+            rlayer<NT, kRKS>(sW, Bf, C, lane);
         }
-        rbias_relu(C, bias + stages[si].bias, q, true);
+        for (int j = 0; j < NT; ++j) rbias_relu(C[j], bias + stages[si].bias, q, true);
         ++si;
-        rstage_load(sW, img, stages[si]);
-        rf32x4_t y = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-        {
+        rstage_load<kThreads>(sW, img, stages[si]);
+        for (int j = 0; j < NT; ++j) {
+            rf32x4_t y = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
             ru32x4_t Cf[kRKS];
-            rstate_frags(C, Cf);
+            rstate_frags(C[j], Cf);
             for (int k = 0; k < kRKS; ++k) y = rmfma(sW[k * 64 + lane], Cf[k], y);
-        }
-        if (valid && q == 0) {
-            float v = y[0] + bias[stages[si].bias];
-            v = v >= 0.0f ? v : 0.0f;
-            float o = (float)exp((double)v) - 1.0f;
-            o = o > 0.0f ? o : 0.0f;
-            if (y_out) {
-                y_out[p] = o;
-            } else {
-                const v3 rr = (mk3(o, o, o) * lc) * sr;
-                const RpnnRec& r = recs[p];
-                out[r.pixel] =
-                    make_float4(fmaxf(0.0f, rr.x), fmaxf(0.0f, rr.y), fmaxf(0.0f, rr.z), fmaxf(0.0f, r.dist));
+            const int p = 16 * (t0 + j) + c;
+            if (p < m && q == 0) {
+                float v = y[0] + bias[stages[si].bias];
+                v = v >= 0.0f ? v : 0.0f;
+                float o = __expf(v) - 1.0f;  // bf16 network: hardware exp2 based, ~1 ulp-class error
+                o = o > 0.0f ? o : 0.0f;
+                if (y_out) {
+                    y_out[p] = o;
+                } else {
+                    const v3 rr = (mk3(o, o, o) * lc) * sr;
+                    const RpnnRec& r = recs[p];
+                    out[r.pixel] =
+                        make_float4(fmaxf(0.0f, rr.x), fmaxf(0.0f, rr.y), fmaxf(0.0f, rr.z), fmaxf(0.0f, r.dist));
+                }
             }
         }
     }
+}
+
+__global__ __launch_bounds__(512) void k_rpnn_mfma(const RpnnRec* __restrict__ recs, int m,
+                                                   const ru32x4_t* __restrict__ desc,
+                                                   const ru32x4_t* __restrict__ img, const float* __restrict__ bias,
+                                                   const RpnnStage* __restrict__ stages, v3 lc, v3 sr,
+                                                   float4* __restrict__ out, float* __restrict__ y_out) {
+    __shared__ ru32x4_t sW[kRNM * kRKV * 64];
+    rpnn_network<1>(recs, m, desc, img, bias, stages, lc, sr, out, y_out, sW);
 }
 
 }  // namespace mrp
