@@ -8,7 +8,8 @@
 //     assigns datas_dev there, core/volume.cu:948-957) and no 1024^3 scratch array
 //     (core/volume.cu:951-952);
 //   * Update_TR(..., CPU) always bakes on the GPU;
-//   * RenderType::RPNN (a different network, not on this path) throws;
+//   * Render(size, ..., RPNN, ...) runs the RPNN network in the reference's FP32 arithmetic
+//     (mrp_render_rpnn); the interactive Render(target, histo, ..., RPNN) throws;
 //   * Render(..., PT, ...) returns the mean radiance with w = 0 (the reference's w is a hit
 //     fraction used only by its GUI).
 #pragma once

@@ -136,7 +136,6 @@ void VolumeRender::SetFov(float degrees) {
 std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float3 right, float3 lightDir,
                                          RenderType rt, float g, float alpha, float3 lightColor, int multiScatter,
                                          int sampleNum) {
-    if (rt == RenderType::RPNN) throw Error(MRP_ERR_INVALID, "RPNN is not part of this library");
     mrp_view v;
     memset(&v, 0, sizeof v);
     const float3 o[6] = {ori, up, right, lightDir, lightColor, scatter_rate_};
@@ -156,13 +155,14 @@ std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float
     v.frame = frame_counter_;
     frame_counter_ += (uint32_t)(sampleNum > 0 ? sampleNum : 1);
     std::vector<float4> out((size_t)size.x * size.y);
-    if (rt == RenderType::MRPNN) {
-        if (!has_env_) {
+    if (rt == RenderType::MRPNN || rt == RenderType::RPNN) {
+        if (rt == RenderType::MRPNN && !has_env_) {
             check(mrp_render_plain(default_context(), vol_, &v, sampleNum, (float*)out.data()));
             return out;
         }
-        // with an environment map every frame draws its own light (core/volume.cu:494-509);
-        // Roberts2 uses the same counter through one Render call, as the reference
+        // frame by frame (RPNN, or an environment map: every frame draws its own light,
+        // core/volume.cu:494-509; Roberts2 uses the same counter through one Render call, as
+        // the reference), averaged progressively; RPNN in the reference's FP32 arithmetic
         const int frames = sampleNum > 0 ? sampleNum : 1;
         float* dev = nullptr;
         if (hipMalloc((void**)&dev, out.size() * sizeof(float4)) != hipSuccess)
@@ -180,7 +180,8 @@ std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float
             vf.light_color[0] = lc.x;
             vf.light_color[1] = lc.y;
             vf.light_color[2] = lc.z;
-            rc = mrp_render_plain_device(default_context(), vol_, &vf, dev);
+            rc = rt == RenderType::RPNN ? mrp_render_rpnn(default_context(), vol_, &vf, MRP_FP32, dev)
+                                        : mrp_render_plain_device(default_context(), vol_, &vf, dev);
             if (rc == MRP_OK && hipMemcpy(out.data(), dev, out.size() * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess)
                 rc = MRP_ERR_HIP;
             const float rate = 1.0f / (1.0f + (float)f);  // the progressive average (volume.cu:226-227)

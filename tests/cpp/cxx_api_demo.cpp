@@ -1,6 +1,6 @@
 // The reference's README flow (README.md:72-93 / main.cu) written against mrpnn::VolumeRender
-// and mrpnn::Camera: fill a grid with SetDatas, Update, render the predictor to a PNG and
-// the path tracer to another.  Usage: demo <out_dir> [resolution]
+// and mrpnn::Camera: fill a grid with SetDatas, Update, render the predictor, the RPNN
+// comparison network and the path tracer to PNGs.  Usage: demo <out_dir> [resolution]
 #include <math.h>
 #include <stdio.h>
 
@@ -38,6 +38,15 @@ int main(int argc, char** argv) {
             s += p.x;
             act += p.w > 0;
         }
+        // the comparison network of the paper, in the reference's FP32 arithmetic
+        auto rp = cam.RenderToFile(dir + "/demo_rpnn", sun, float3{1, 1, 1}, 32.0f, 1, 0.857f, 1, Camera::None,
+                                   VolumeRender::RPNN);
+        double srp = 0;
+        int act_rp = 0;
+        for (auto& p : rp) {
+            srp += p.x;
+            act_rp += p.w > 0;
+        }
         auto pt = cam.RenderToFile(dir + "/demo_pt", sun, float3{1, 1, 1}, 32.0f, 64, 0.857f, 2, Camera::None,
                                    VolumeRender::PT);
         double sp = 0;
@@ -68,8 +77,8 @@ int main(int argc, char** argv) {
         int lit_pred = 0;
         for (unsigned int p : px) lit_pred += (p & 0xffffffu) != 0 && (p >> 24) == 0xffu;
         if (hipFree(accum) != hipSuccess || hipFree(histo) != hipSuccess || hipFree(rgba) != hipSuccess) return 4;
-        printf("active %d mean_pred %.6f mean_pt %.6f lit_pt %d lit_pred %d\n", act, s / img.size(),
-               sp / pt.size(), lit_pt, lit_pred);
+        printf("active %d mean_pred %.6f mean_pt %.6f lit_pt %d lit_pred %d active_rpnn %d mean_rpnn %.6f\n", act,
+               s / img.size(), sp / pt.size(), lit_pt, lit_pred, act_rp, srp / rp.size());
         return act > 0 ? 0 : 3;
     } catch (const Error& e) {
         fprintf(stderr, "mrpnn error %d: %s\n", e.code, e.what());

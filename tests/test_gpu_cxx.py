@@ -24,6 +24,9 @@ def test_cxx_demo_renders_pngs(tmp_path):
     # interactive frames (PT with the filter, then MRPNN) light part of the 64x64 RGBA8 image
     # (the soft ball covers about 200 pixels of the 64x64 view)
     assert int(words[words.index("lit_pt") + 1]) > 50 and int(words[words.index("lit_pred") + 1]) > 50, r.stdout
-    for name in ("demo_mrpnn.png", "demo_pt.png"):
+    # RPNN frame of the same camera (its own sub-pixel jitter): about the same active pixels, lit
+    active_rpnn = int(words[words.index("active_rpnn") + 1])
+    assert abs(active_rpnn - active) <= 0.05 * active and float(words[words.index("mean_rpnn") + 1]) > 0
+    for name in ("demo_mrpnn.png", "demo_rpnn.png", "demo_pt.png"):
         data = open(tmp_path / name, "rb").read()
         assert data[:8] == b"\x89PNG\r\n\x1a\n" and len(data) > 96 * 96 * 4
