@@ -128,3 +128,40 @@ def test_rpnn_bf16_descriptor_matches_rows(ctx, orc, rscene):
         rel = np.abs(got[m] - want[m]) / want[m]
         print("rpnn bf16 render vs bf16(rows): median %.2e p99 %.2e max %.2e" % (np.median(rel), np.quantile(rel, 0.99), rel.max()))
         assert np.median(rel) < 1e-3 and np.quantile(rel, 0.99) < 1e-2
+
+
+def test_rpnn_bf16_passes_and_tails(ctx, orc, rscene):
+    """The BF16 path in many small passes (4,096 pixels: tail tiles of 16 and 128 pixels in
+    every pass) gives the same frame as one pass: pixels are independent of the chunking."""
+    import mrpnn_amd as M
+    vol, sc = rscene
+    v = orc.orbit_view(160, 128, alpha=32.0, light=(0.3, 0.9, 0.2), azimuth=0.7, elevation=0.25)
+    vm = M.View.from_buffer_copy(bytes(v))
+    one = vol.render_rpnn(vm, M.BF16).cpu().numpy()
+    active = int((one[:, 3] > 0).sum())
+    assert active > 2 * 4096 and active % 128 != 0
+    try:
+        ctx.set_option(M.OPT_RPNN_CHUNK, 4096)
+        many = vol.render_rpnn(vm, M.BF16).cpu().numpy()
+    finally:
+        ctx.set_option(M.OPT_RPNN_CHUNK, 32768)
+    np.testing.assert_array_equal(one, many)
+
+
+def test_rpnn_empty_volume_and_bad_arguments(ctx, orc):
+    """An empty volume has no active pixel: the frame is the sky (distance 0), the same in both
+    precisions; unknown precisions and chunk sizes are refused."""
+    import mrpnn_amd as M
+    vol = M.Volume(ctx, 32)
+    vol.update()
+    v = orc.orbit_view(24, 20, alpha=32.0)
+    vm = M.View.from_buffer_copy(bytes(v))
+    a = vol.render_rpnn(vm, M.FP32).cpu().numpy()
+    b = vol.render_rpnn(vm, M.BF16).cpu().numpy()
+    assert np.all(a[:, 3] == 0) and np.array_equal(a, b)
+    with pytest.raises(RuntimeError):
+        vol.render_rpnn(vm, 7)
+    for bad in (100, 4095, 2 ** 21):
+        with pytest.raises(RuntimeError):
+            ctx.set_option(M.OPT_RPNN_CHUNK, bad)
+    vol.close()
