@@ -327,8 +327,9 @@ int mrp_interactive_set_exposure(mrp_interactive* it, float exposure); /* SetExp
  * 0.01 by an 11x11 average weighted by histogram similarity.  Tone mapping: exposure, then
  * none / Gamma 1/2.2 / ACES (core/render.cuh:1144-1236), saturate, 8 bits per channel.
  * The view's light, alpha and g update the LUT and TR mips as the reference does; the
- * jitter stream is keyed by (view->seed, randseed).  Asynchronous.  MRP_RT_RPNN is not
- * supported (MRP_ERR_INVALID). */
+ * jitter stream is keyed by (view->seed, randseed).  Asynchronous, except MRP_RT_RPNN:
+ * the RPNN network (MRP_FP32, any other precision runs its BF16 network) in place of the
+ * predictor, which synchronises with its march as mrp_render_rpnn. */
 int mrp_render_interactive(mrp_interactive* it, mrp_volume* vol, const mrp_view* view, int render_type, int precision,
                            int max_bounce, int randseed, int tone_type, int denoise, float* dev_target,
                            mrp_histogram* dev_histo, uint32_t* dev_rgba);

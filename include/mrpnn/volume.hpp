@@ -9,7 +9,8 @@
 //     (core/volume.cu:951-952);
 //   * Update_TR(..., CPU) always bakes on the GPU;
 //   * Render(size, ..., RPNN, ...) runs the RPNN network in the reference's FP32 arithmetic
-//     (mrp_render_rpnn); the interactive Render(target, histo, ..., RPNN) throws;
+//     (mrp_render_rpnn); the interactive Render(target, histo, ..., RPNN) in the interactive
+//     precision (MRP_FP32, else the BF16 network);
 //   * Render(..., PT, ...) returns the mean radiance with w = 0 (the reference's w is a hit
 //     fraction used only by its GUI).
 #pragma once

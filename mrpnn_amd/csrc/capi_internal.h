@@ -114,6 +114,10 @@ int mrp_tr_slot_find(const mrp_ctx* c, const mrp_volume* v, const float L[3], fl
 
 // rpnn.hip
 void mrp_rpnn_free(mrp_ctx* c);
+// mrp_render_rpnn on one checkerboard half (checker as mrp_render_batch_checker: 0 = all
+// pixels, else the pixels with (row + col + checker - 1) even); the others are not written
+int mrp_render_rpnn_checker(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int precision, float* dev_out4,
+                            int checker);
 
 // fused.hip
 int mrp_fused_init(mrp_ctx* c);

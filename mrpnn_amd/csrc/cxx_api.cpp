@@ -231,7 +231,6 @@ void VolumeRender::SetInteractivePrecision(int precision) {
 void VolumeRender::Render(float3* target, Histogram* histo_buffer, unsigned int* target2, int2 size, float3 ori,
                           float3 up, float3 right, float3 lightDir, float3 lightColor, float alpha, int multiScatter,
                           float g, int randseed, RenderType rt, int toneType, bool denoise) {
-    if (rt == RenderType::RPNN) throw Error(MRP_ERR_INVALID, "RPNN is not part of this library");
     // the renderer keeps its frame state (flip counter, last mode) across calls of one size
     if (!interactive_ || size.x != iw_ || size.y != ih_) {
         if (interactive_) mrp_interactive_destroy(interactive_);
@@ -268,7 +267,7 @@ void VolumeRender::Render(float3* target, Histogram* histo_buffer, unsigned int*
     v.jitter = -1.0f;
     v.width = size.x;
     v.height = size.y;
-    check(mrp_render_interactive(interactive_, vol_, &v, rt == RenderType::PT ? MRP_RT_PT : MRP_RT_MRPNN,
+    check(mrp_render_interactive(interactive_, vol_, &v, rt == RenderType::PT ? MRP_RT_PT : rt == RenderType::RPNN ? MRP_RT_RPNN : MRP_RT_MRPNN,
                                  interactive_precision_, multiScatter, randseed, toneType, denoise ? 1 : 0,
                                  reinterpret_cast<float*>(target), histo_buffer, target2));
     if (hipDeviceSynchronize() != hipSuccess) throw Error(MRP_ERR_HIP, "interactive frame failed");

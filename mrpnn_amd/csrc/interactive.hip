@@ -383,8 +383,7 @@ extern "C" int mrp_render_interactive(mrp_interactive* it, mrp_volume* v, const 
     if (rc) return rc;
     if (vw->width != it->W || vw->height != it->H)
         return mrp_fail(MRP_ERR_INVALID, "view is %d x %d, the renderer %d x %d", vw->width, vw->height, it->W, it->H);
-    if (render_type == MRP_RT_RPNN) return mrp_fail(MRP_ERR_INVALID, "RPNN is not part of this library");
-    if (render_type != MRP_RT_PT && render_type != MRP_RT_MRPNN)
+    if (render_type != MRP_RT_PT && render_type != MRP_RT_MRPNN && render_type != MRP_RT_RPNN)
         return mrp_fail(MRP_ERR_INVALID, "unknown render type %d", render_type);
     if (tone_type < 0 || tone_type > 2) return mrp_fail(MRP_ERR_INVALID, "unknown tone type %d", tone_type);
     if (randseed < 0) return mrp_fail(MRP_ERR_INVALID, "randseed must be >= 0");
@@ -415,8 +414,12 @@ extern "C" int mrp_render_interactive(mrp_interactive* it, mrp_volume* v, const 
         float* pred = reinterpret_cast<float*>(it->d_pred);
         mrp_volume* vols[1] = {v};
         const int checker = it->checkerboard ? 1 + ((H - 1 + flip) & 1) : 0;
-        rc = mrp_prepare_views(c, vols, 1, &view, 1);
-        if (!rc) rc = mrp_render_batch_checker(c, vols, 1, &view, 1, precision, &pred, &checker);
+        if (render_type == MRP_RT_RPNN) {  // RenderCamera<true, RPNN>: FP32, or the BF16 network
+            rc = mrp_render_rpnn_checker(c, v, &view, precision == MRP_FP32 ? MRP_FP32 : MRP_BF16, pred, checker);
+        } else {
+            rc = mrp_prepare_views(c, vols, 1, &view, 1);
+            if (!rc) rc = mrp_render_batch_checker(c, vols, 1, &view, 1, precision, &pred, &checker);
+        }
         if (rc) return rc;
         hipLaunchKernelGGL(k_commit_predict, dim3(blocks), dim3(256), 0, c->stream, it->d_pred, W, H,
                            it->checkerboard, flip, dev_target, it->d_rad);

@@ -79,11 +79,14 @@ struct RpnnRec {
 
 // RenderCamera<true, RPNN> up to the network: the same ray, march and basis as the MRPNN path
 // (pixel_state); sky pixels are written at once, active ones listed (wave-aggregated).
-__global__ __launch_bounds__(64) void k_rpnn_march(DevVolume vol, ViewDev vw, float4* __restrict__ out,
+// checker > 0 takes only the pixels with (row + col + checker - 1) even (the interactive
+// renderer's checkerboard half); the others are left untouched.
+__global__ __launch_bounds__(64) void k_rpnn_march(DevVolume vol, ViewDev vw, int checker, float4* __restrict__ out,
                                                    RpnnRec* __restrict__ recs, unsigned* __restrict__ count) {
     const int col = blockIdx.x * 8 + (threadIdx.x & 7);
     const int row = blockIdx.y * 8 + (threadIdx.x >> 3);
-    const bool valid = row < vw.height && col < vw.width;
+    const bool valid =
+        row < vw.height && col < vw.width && (checker == 0 || ((row + col + checker - 1) & 1) == 0);
     PixelState st;
     st.active = 0;
     if (valid) st = pixel_state(vol, vw, row, col);
@@ -723,7 +726,7 @@ extern "C" int mrp_ctx_load_rpnn(mrp_ctx* c, const char* path) {
 }
 
 // active-pixel list of a view (march) and the sky pixels of out4; returns the count on the host
-static int rpnn_march(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, float* dev_out4, int* n_active) {
+static int rpnn_march(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int checker, float* dev_out4, int* n_active) {
     RpnnState* s = c->rpnn;
     const size_t n = (size_t)vw->width * vw->height;
     if (n > s->rec_cap) {
@@ -737,7 +740,7 @@ static int rpnn_march(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, float* dev_
     }
     HIP_TRY(hipMemsetAsync(s->d_count, 0, sizeof(unsigned), c->stream));
     dim3 grid((vw->width + 7) / 8, (vw->height + 7) / 8);
-    hipLaunchKernelGGL(k_rpnn_march, grid, dim3(64), 0, c->stream, mrp_dev_volume(v), mrp_to_dev(vw),
+    hipLaunchKernelGGL(k_rpnn_march, grid, dim3(64), 0, c->stream, mrp_dev_volume(v), mrp_to_dev(vw), checker,
                        (float4*)dev_out4, s->d_recs, s->d_count);
     HIP_TRY(hipGetLastError());
     unsigned cnt = 0;
@@ -869,7 +872,13 @@ static int rpnn_bf16(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int m, float
 }
 
 extern "C" int mrp_render_rpnn(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int precision, float* dev_out4) {
+    return mrp_render_rpnn_checker(c, v, vw, precision, dev_out4, 0);
+}
+
+int mrp_render_rpnn_checker(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int precision, float* dev_out4,
+                            int checker) {
     if (!c || !v || !vw || !dev_out4) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    if (checker < 0 || checker > 2) return mrp_fail(MRP_ERR_INVALID, "checker must be 0, 1 or 2");
     int rc = mrp_check_view(vw);
     if (rc) return rc;
     if (precision != MRP_FP32 && precision != MRP_BF16)
@@ -877,7 +886,7 @@ extern "C" int mrp_render_rpnn(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, in
     if (!c->rpnn && (rc = mrp_ctx_load_rpnn(c, nullptr))) return rc;
     if (!v->mips_valid) return mrp_fail(MRP_ERR_STATE, "mrp_volume_update must run before rendering");
     int m = 0;
-    rc = rpnn_march(c, v, vw, dev_out4, &m);
+    rc = rpnn_march(c, v, vw, checker, dev_out4, &m);
     if (rc || m == 0) return rc;
     if (precision == MRP_FP32) return rpnn_fp32(c, v, vw, m, dev_out4, nullptr, nullptr);
     return rpnn_bf16(c, v, vw, m, dev_out4, nullptr, nullptr);
@@ -903,7 +912,7 @@ extern "C" int mrp_rpnn_features(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, 
     const size_t n = (size_t)vw->width * vw->height;
     HIP_TRY(hipMemsetAsync(dev_rows, 0, n * (kRL * kRS + 1) * sizeof(float), c->stream));
     int m = 0;
-    rc = rpnn_march(c, v, vw, dev_out4, &m);
+    rc = rpnn_march(c, v, vw, 0, dev_out4, &m);
     if (rc || m == 0) return rc;
     const v3 LX = normalize3(mk3(vw->light_dir[0], vw->light_dir[1], vw->light_dir[2]));
     hipLaunchKernelGGL(k_rpnn_rows, dim3(m), dim3(256), 0, c->stream, mrp_dev_volume(v), c->rpnn->d_recs, m, LX,

@@ -132,6 +132,39 @@ def test_mrpnn_bf16_frame_is_batched_render(ctx, orc, iscene):
     it.close()
 
 
+@pytest.mark.parametrize("checker", [0, 1])
+def test_rpnn_frames(ctx, orc, iscene, checker):
+    """RenderCamera<true, RPNN> in the interactive renderer: the frame's radiance is the RPNN
+    frame of the same view (FP32: the reference's arithmetic; BF16: its MFMA network) on the
+    frame's checkerboard half, then the same commit, histograms and tone map as MRPNN."""
+    import mrpnn_amd as M
+    vol, sc, L = iscene
+    W, H = 36, 28
+    n = W * H
+    it = M.Interactive(ctx, W, H)
+    it.set_checkerboard(checker)
+    target, histo, rgba = it.buffers()
+    tgt = np.zeros((n, 3), np.float32)
+    r, c = np.divmod(np.arange(n), W)
+    for f, prec in enumerate((M.FP32, M.FP32, M.BF16, M.BF16)):
+        print("interactive RPNN checker %d frame %d" % (checker, f), flush=True)
+        flip = f % 2
+        v, vm = _views(orc, M, W, H, L, f)
+        it.render(vol, vm, target, histo, rgba, M.RT_RPNN, prec, randseed=f, tone=M.TONE_GAMMA)
+        rad = it.radiance().cpu().numpy()
+        s = (((r + flip) % 2) + c) % 2 == 0 if checker else np.ones(n, bool)
+        frame = _flip_rows(vol.render_rpnn(vm, prec).cpu().numpy(), W, H)
+        assert np.array_equal(rad[s, :3], np.maximum(frame[s, :3], 0.0))
+        assert np.array_equal(rad[s, 3], frame[s, 3]) and np.all(rad[~s, 3] == -2.0)
+        assert (rad[s, 3] > 0).sum() > 50
+        orc.predict_commit(tgt, rad, W, H, checker, flip)
+        hst = np.zeros((n, 33), np.float32)
+        want = orc.reprojection_denoise(tgt, hst, 1.0, 1)
+        assert np.array_equal(target.cpu().numpy(), tgt)
+        _rgba_close(rgba.cpu().numpy().view(np.uint32), want)
+    it.close()
+
+
 def test_switch_from_pt_clears_histograms(ctx, orc, iscene):
     import mrpnn_amd as M
     vol, sc, L = iscene
@@ -156,7 +189,7 @@ def test_interactive_rejects_bad_arguments(ctx, iscene, orc):
     target, histo, rgba = it.buffers()
     v, vm = _views(orc, M, 16, 16, L, 0)
     with pytest.raises(M.MrpError):
-        it.render(vol, vm, target, histo, rgba, M.RT_RPNN)
+        it.render(vol, vm, target, histo, rgba, 9)  # unknown render type
     v2, vm2 = _views(orc, M, 16, 8, L, 0)
     with pytest.raises(M.MrpError):
         it.render(vol, vm2, target, histo, rgba, M.RT_MRPNN)
