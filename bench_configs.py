@@ -368,9 +368,10 @@ def tile(args):
 def interactive(args):
     """Interactive progressive renderer (the GUI path, VolumeRender::Render into caller-owned
     target / histogram / RGBA8 buffers) at 1024x1024 over a 512^3 cloud: frames per second of
-    the MRPNN predictor (BF16 and FP32, with and without the checkerboard) and of the
+    the MRPNN predictor (BF16 and FP32, with and without the checkerboard), of the
     delta-tracking path (512 bounces, albedo 1, the GUI defaults) with and without the
-    histogram-similarity filter; the oracle's filter on the host cores over the same frame as
+    histogram-similarity filter and of the RPNN predictor (BF16, with and without the
+    checkerboard); the oracle's filter on the host cores over the same frame as
     baseline and check."""
     import torch
 
@@ -388,8 +389,9 @@ def interactive(args):
     res = {"config": "interactive", "workload": "1024x1024 frames over a 512^3 fBm cloud (6 octaves), GUI camera "
                                                 "and sun, alpha 8, ACES; PT 512 bounces, albedo 1"}
     cases = (("mrpnn_bf16", M.RT_MRPNN, M.BF16, 1, False, 32), ("mrpnn_bf16_checker", M.RT_MRPNN, M.BF16, 1, True, 32),
-             ("mrpnn_fp32", M.RT_MRPNN, M.FP32, 1, False, 8), ("pt_no_denoise", M.RT_PT, M.FP32, 512, False, 8),
-             ("pt_denoise", M.RT_PT, M.FP32, 512, True, 8))
+             ("mrpnn_fp32", M.RT_MRPNN, M.FP32, 1, False, 8), ("rpnn_bf16", M.RT_RPNN, M.BF16, 1, False, 8),
+             ("rpnn_bf16_checker", M.RT_RPNN, M.BF16, 1, True, 8), ("pt_no_denoise", M.RT_PT, M.FP32, 512, False, 8),
+             ("pt_denoise", M.RT_PT, M.FP32, 512, True, 8))  # the PT cases last: the filter check reads their state
     calls = 0  # the renderer's checkerboard phase flips on every call
     for name, rt, prec, mb, chk, frames in cases:
         it.set_checkerboard(chk)

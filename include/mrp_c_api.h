@@ -58,7 +58,7 @@ extern "C" {
                                        persistent waves per CU */
 #define MRP_OPT_RPNN_CHUNK 6        /* pixels per pass of the BF16 RPNN path (descriptor fragments of
                                        5 KB per pixel, then the network): 4096..1048576, a multiple
-                                       of 128 (default 32768) */
+                                       of 128 (default 65536) */
 
 /* Number of features per pixel: [X_Val 0:192 | X_Val_Hg 192:384 | X_Val_Sub 384:576 | gamma]
  * (core/radiancePredict.cu:427-485, row order of PopulateInputData :562-565). */

@@ -76,7 +76,7 @@ struct mrp_ctx {
     float env_exp = 1.0f;
     // RPNN comparison network: weights and its working buffers, loaded on first use
     struct RpnnState* rpnn = nullptr;
-    int rpnn_chunk = 32768;  // pixels per RPNN descriptor buffer (BF16 path)
+    int rpnn_chunk = 65536;  // pixels per RPNN descriptor buffer (BF16 path)
     // LoadData's DataCache: host grids of the four named datasets, read once per context
     std::vector<float> data_cache[4];
     int data_cache_n[4] = {0, 0, 0, 0};

@@ -305,36 +305,48 @@ __device__ inline void rstate_frags(const rf32x4_t (&d)[kRNM], ru32x4_t (&f)[kRK
                         rpack_bf16x2(b[2], b[3])};
     }
 }
-// Stage loads: the whole layer image (<= 104 KB) into LDS between two barriers
-template <int THREADS>
-__device__ inline void rstage_load(ru32x4_t* sW, const ru32x4_t* __restrict__ img, const RpnnStage& st) {
-    __syncthreads();  // the previous layer's readers are done
-    const int n = st.nm * st.nk * 64;
-    for (int i = threadIdx.x; i < n; i += THREADS) sW[i] = img[st.frag + i];
+// Weight staging: each layer is split in K halves ([0, 4) and [4, nk) chunks) and a half's
+// fragments go from HBM / L2 straight into one of two LDS slots (global_load_lds, no registers),
+// issued one half ahead: while the workgroup multiplies with the half in one slot, the next
+// half streams into the other.  A slot holds 13 M-tiles x 4 K chunks (52 KB); fragment
+// (t, k) of a half sits at (t * nkh + k - K0) * 64 + lane.
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+__device__ inline void rdma_half(ru32x4_t* slot, const ru32x4_t* __restrict__ img, const RpnnStage& st, int K0,
+                                 int K1, int wave) {
+    const int nkh = K1 - K0, lane = threadIdx.x & 63;
+    for (int b = wave; b < st.nm * nkh; b += 8) {  // one 1 KB block per wavefront instruction
+        const int t = b / nkh, kk = b - t * nkh;
+        __builtin_amdgcn_global_load_lds((const void*)(img + st.frag + (t * st.nk + K0 + kk) * 64 + lane),
+                                         (lds_void_ptr)(slot + b * 64), 16, 0, 0);
+    }
+}
+// the end of a half: this wavefront's copies have landed, every wavefront is done with the slot
+// it read, and the other slot's fragments are visible to all
+__device__ inline void rhalf_end() {
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 }
-// acc[j][t] += W(stage)[t] x in[j] for the 13 M-tiles and the wave's NT pixel tiles, K chunks
-// from registers.  Each A fragment read from LDS feeds NT MFMAs; its register is refilled with
-// the next K chunk's fragment right after them (scheduling group barriers pin that interleave,
-// so one K chunk's 13 fragments are live and their LDS latency hides behind the MFMAs).
-template <int NT, int NK>
-__device__ inline void rlayer(const ru32x4_t* sW, const ru32x4_t (&in)[NT][NK], rf32x4_t (&acc)[NT][kRNM], int lane) {
+// acc[t] += W[t][K0..K1) x in[K0..K1) for the 13 M-tiles from a slot.  Each A fragment
+// register is refilled with the next K chunk's fragment right after its MFMA (scheduling
+// group barriers pin that interleave), so the LDS latency hides behind the MFMAs.
+template <int K0, int K1, int NK>
+__device__ inline void rcompute(const ru32x4_t* slot, const ru32x4_t (&in)[NK], rf32x4_t (&acc)[kRNM], int lane) {
+    constexpr int nkh = K1 - K0;
     ru32x4_t a[kRNM];
 #pragma unroll
-    for (int t = 0; t < kRNM; ++t) a[t] = sW[t * NK * 64 + lane];
+    for (int t = 0; t < kRNM; ++t) a[t] = slot[t * nkh * 64 + lane];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int k = 0; k < NK; ++k) {
+    for (int k = K0; k < K1; ++k) {
 #pragma unroll
         for (int t = 0; t < kRNM; ++t) {
-#pragma unroll
-            for (int j = 0; j < NT; ++j) acc[j][t] = rmfma(a[t], in[j][k], acc[j][t]);
-            if (k + 1 < NK) a[t] = sW[(t * NK + k + 1) * 64 + lane];
+            acc[t] = rmfma(a[t], in[k], acc[t]);
+            if (k + 1 < K1) a[t] = slot[(t * nkh + k + 1 - K0) * 64 + lane];
         }
 #pragma unroll
         for (int t = 0; t < kRNM; ++t) {
-            __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);  // NT MFMAs
-            if (k + 1 < NK) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // one MFMA
+            if (k + 1 < K1) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -482,117 +494,128 @@ __global__ __launch_bounds__(256) void k_rpnn_desc_rows(const float* __restrict_
     desc[item] = rpack8(v);
 }
 
-// The fused network: a 128-pixel workgroup of 8 / NT wavefronts, each with NT 16-pixel tiles
-// (the kernel runs NT = 1: two tiles per wavefront would halve the LDS reads per MFMA but do
-// not fit 512 registers without spilling).
-template <int NT>
-__device__ inline void rpnn_network(const RpnnRec* __restrict__ recs, int m, const ru32x4_t* __restrict__ desc,
-                                    const ru32x4_t* __restrict__ img, const float* __restrict__ bias,
-                                    const RpnnStage* __restrict__ stages, v3 lc, v3 sr, float4* __restrict__ out,
-                                    float* __restrict__ y_out, ru32x4_t* sW) {
-    constexpr int kThreads = 512 / NT;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int c = lane & 15, q = lane >> 4;
-    const int tiles = (m + 127) / 128;
-    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const int t0 = tile * 8 + wave * NT;  // this wave's first 16-pixel tile
-        // live across the levels: the state C (D layout); fragments are made just before use
-        rf32x4_t C[NT][kRNM], acc[NT][kRNM];
-        int si = 0;
-        for (int l = 0; l < kRL; ++l) {
-            // B = relu(V_l [desc_l, gamma] (+ W_l C + b)); the level's descriptor fragments are
-            // requested before the stage load so that their HBM latency overlaps it (read once:
-            // non-temporal, so they do not evict the weight fragments from L2)
-            ru32x4_t dfr[NT][kRKV];
-            for (int j = 0; j < NT; ++j) {
-                const ru32x4_t* dsc = desc + (size_t)(t0 + j) * kRL * kRKV * 64 + lane;
-                for (int k = 0; k < kRKV; ++k)
-                    dfr[j][k] = (16 * (t0 + j) < m) ? __builtin_nontemporal_load(dsc + (l * kRKV + k) * 64)
-                                                    : ru32x4_t{0u, 0u, 0u, 0u};
-            }
-            rstage_load<kThreads>(sW, img, stages[si]);
-            for (int j = 0; j < NT; ++j)
-                for (int t = 0; t < kRNM; ++t) acc[j][t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-            rlayer<NT, kRKV>(sW, dfr, acc, lane);
-            ++si;
-            if (l > 0) {
-                rstage_load<kThreads>(sW, img, stages[si]);
-                {
-                    ru32x4_t Cf[NT][kRKS];
-                    for (int j = 0; j < NT; ++j) rstate_frags(C[j], Cf[j]);
-                    rlayer<NT, kRKS>(sW, Cf, acc, lane);
-                }
-                for (int j = 0; j < NT; ++j) rbias_relu(acc[j], bias + stages[si].bias, q, true);
-                ++si;
-            } else {
-                for (int j = 0; j < NT; ++j) rbias_relu(acc[j], nullptr, q, true);
-            }
-            // C = relu((C +) U_l B + b), accumulated in place
-            rstage_load<kThreads>(sW, img, stages[si]);
-            {
-                ru32x4_t Bf[NT][kRKS];
-                for (int j = 0; j < NT; ++j) rstate_frags(acc[j], Bf[j]);
-                if (l == 0)
-                    for (int j = 0; j < NT; ++j)
-                        for (int t = 0; t < kRNM; ++t) C[j][t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-                rlayer<NT, kRKS>(sW, Bf, C, lane);
-            }
-            for (int j = 0; j < NT; ++j) rbias_relu(C[j], bias + stages[si].bias, q, true);
-            ++si;
-        }
-        // F0, F1 (ReLU), F2 (one output row), radiance max(exp(y) - 1, 0)
-        rstage_load<kThreads>(sW, img, stages[si]);
-        for (int j = 0; j < NT; ++j)
-            for (int t = 0; t < kRNM; ++t) acc[j][t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-        {
-            ru32x4_t Cf[NT][kRKS];
-            for (int j = 0; j < NT; ++j) rstate_frags(C[j], Cf[j]);
-            rlayer<NT, kRKS>(sW, Cf, acc, lane);
-        }
-        for (int j = 0; j < NT; ++j) rbias_relu(acc[j], bias + stages[si].bias, q, true);
-        ++si;
-        rstage_load<kThreads>(sW, img, stages[si]);
-        for (int j = 0; j < NT; ++j)
-            for (int t = 0; t < kRNM; ++t) C[j][t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-        {
-            ru32x4_t Bf[NT][kRKS];
-            for (int j = 0; j < NT; ++j) rstate_frags(acc[j], Bf[j]);
-            rlayer<NT, kRKS>(sW, Bf, C, lane);
-        }
-        for (int j = 0; j < NT; ++j) rbias_relu(C[j], bias + stages[si].bias, q, true);
-        ++si;
-        rstage_load<kThreads>(sW, img, stages[si]);
-        for (int j = 0; j < NT; ++j) {
-            rf32x4_t y = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-            ru32x4_t Cf[kRKS];
-            rstate_frags(C[j], Cf);
-            for (int k = 0; k < kRKS; ++k) y = rmfma(sW[k * 64 + lane], Cf[k], y);
-            const int p = 16 * (t0 + j) + c;
-            if (p < m && q == 0) {
-                float v = y[0] + bias[stages[si].bias];
-                v = v >= 0.0f ? v : 0.0f;
-                float o = __expf(v) - 1.0f;  // bf16 network: hardware exp2 based, ~1 ulp-class error
-                o = o > 0.0f ? o : 0.0f;
-                if (y_out) {
-                    y_out[p] = o;
-                } else {
-                    const v3 rr = (mk3(o, o, o) * lc) * sr;
-                    const RpnnRec& r = recs[p];
-                    out[r.pixel] =
-                        make_float4(fmaxf(0.0f, rr.x), fmaxf(0.0f, rr.y), fmaxf(0.0f, rr.z), fmaxf(0.0f, r.dist));
-                }
-            }
-        }
-    }
-}
+// Stage indices in the image (V0, U0, then V_l, W_l, U_l per level, F0, F1, F2)
+__device__ constexpr int rs_v(int l) { return l == 0 ? 0 : 3 * l - 1; }
+__device__ constexpr int rs_w(int l) { return 3 * l; }
+__device__ constexpr int rs_u(int l) { return l == 0 ? 1 : 3 * l + 1; }
+constexpr int kRsF0 = 3 * kRL - 1;
 
+// The fused network: a 128-pixel workgroup of 8 wavefronts with 16 pixels each, the 200-wide
+// states in registers (D layout), 64 weight halves per tile streamed through two LDS slots.
 __global__ __launch_bounds__(512) void k_rpnn_mfma(const RpnnRec* __restrict__ recs, int m,
                                                    const ru32x4_t* __restrict__ desc,
                                                    const ru32x4_t* __restrict__ img, const float* __restrict__ bias,
                                                    const RpnnStage* __restrict__ stages, v3 lc, v3 sr,
                                                    float4* __restrict__ out, float* __restrict__ y_out) {
-    __shared__ ru32x4_t sW[kRNM * kRKV * 64];
-    rpnn_network<1>(recs, m, desc, img, bias, stages, lc, sr, out, y_out, sW);
+    __shared__ ru32x4_t sA[kRNM * 4 * 64];
+    __shared__ ru32x4_t sB[kRNM * 4 * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, q = lane >> 4;
+    const int tiles = (m + 127) / 128;
+    if ((int)blockIdx.x >= tiles) return;
+    rdma_half(sA, img, stages[rs_v(0)], 0, 4, wave);
+    rhalf_end();
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int t16 = tile * 8 + wave;  // this wave's 16 pixels
+        const ru32x4_t* dsc = desc + (size_t)t16 * kRL * kRKV * 64 + lane;
+        rf32x4_t C[kRNM], acc[kRNM];
+        for (int l = 0; l < kRL; ++l) {
+            // B = relu(V_l [desc_l, gamma] (+ W_l C + b)); the level's descriptor fragments are
+            // requested first (read once: non-temporal, so they do not evict weights from L2)
+            ru32x4_t dfr[kRKV];
+            for (int k = 0; k < kRKV; ++k)
+                dfr[k] = (16 * t16 < m) ? __builtin_nontemporal_load(dsc + (l * kRKV + k) * 64)
+                                        : ru32x4_t{0u, 0u, 0u, 0u};
+            const RpnnStage& sv = stages[rs_v(l)];
+            rdma_half(sB, img, sv, 4, kRKV, wave);
+            for (int t = 0; t < kRNM; ++t) acc[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            rcompute<0, 4, kRKV>(sA, dfr, acc, lane);
+            rhalf_end();
+            if (l > 0) {
+                const RpnnStage& sw = stages[rs_w(l)];
+                rdma_half(sA, img, sw, 0, 4, wave);
+                rcompute<4, kRKV, kRKV>(sB, dfr, acc, lane);
+                rhalf_end();
+                ru32x4_t Cf[kRKS];
+                rstate_frags(C, Cf);
+                rdma_half(sB, img, sw, 4, kRKS, wave);
+                rcompute<0, 4, kRKS>(sA, Cf, acc, lane);
+                rhalf_end();
+                rdma_half(sA, img, stages[rs_u(l)], 0, 4, wave);
+                rcompute<4, kRKS, kRKS>(sB, Cf, acc, lane);
+                rbias_relu(acc, bias + sw.bias, q, true);
+            } else {
+                rdma_half(sA, img, stages[rs_u(l)], 0, 4, wave);
+                rcompute<4, kRKV, kRKV>(sB, dfr, acc, lane);
+                rbias_relu(acc, nullptr, q, true);
+            }
+            rhalf_end();
+            // C = relu((C +) U_l B + b), accumulated in place
+            const RpnnStage& su = stages[rs_u(l)];
+            ru32x4_t Bf[kRKS];
+            rstate_frags(acc, Bf);
+            if (l == 0)
+                for (int t = 0; t < kRNM; ++t) C[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            rdma_half(sB, img, su, 4, kRKS, wave);
+            rcompute<0, 4, kRKS>(sA, Bf, C, lane);
+            rhalf_end();
+            rdma_half(sA, img, stages[l + 1 < kRL ? rs_v(l + 1) : kRsF0], 0, 4, wave);
+            rcompute<4, kRKS, kRKS>(sB, Bf, C, lane);
+            rbias_relu(C, bias + su.bias, q, true);
+            rhalf_end();
+        }
+        // F0, F1 (ReLU), F2 (one output row), radiance max(exp(y) - 1, 0)
+        {
+            const RpnnStage& s0 = stages[kRsF0];
+            ru32x4_t Cf[kRKS];
+            rstate_frags(C, Cf);
+            for (int t = 0; t < kRNM; ++t) acc[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            rdma_half(sB, img, s0, 4, kRKS, wave);
+            rcompute<0, 4, kRKS>(sA, Cf, acc, lane);
+            rhalf_end();
+            rdma_half(sA, img, stages[kRsF0 + 1], 0, 4, wave);
+            rcompute<4, kRKS, kRKS>(sB, Cf, acc, lane);
+            rbias_relu(acc, bias + s0.bias, q, true);
+            rhalf_end();
+        }
+        {
+            const RpnnStage& s1 = stages[kRsF0 + 1];
+            ru32x4_t Bf[kRKS];
+            rstate_frags(acc, Bf);
+            for (int t = 0; t < kRNM; ++t) C[t] = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            rdma_half(sB, img, s1, 4, kRKS, wave);
+            rcompute<0, 4, kRKS>(sA, Bf, C, lane);
+            rhalf_end();
+            rdma_half(sA, img, stages[kRsF0 + 2], 0, 4, wave);
+            rcompute<4, kRKS, kRKS>(sB, Bf, C, lane);
+            rbias_relu(C, bias + s1.bias, q, true);
+            rhalf_end();
+        }
+        const RpnnStage& s2 = stages[kRsF0 + 2];
+        ru32x4_t Cf[kRKS];
+        rstate_frags(C, Cf);
+        rdma_half(sB, img, s2, 4, kRKS, wave);
+        rf32x4_t y = rf32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < 4; ++k) y = rmfma(sA[k * 64 + lane], Cf[k], y);
+        rhalf_end();
+        if (tile + (int)gridDim.x < tiles) rdma_half(sA, img, stages[rs_v(0)], 0, 4, wave);  // the next tile
+        for (int k = 4; k < kRKS; ++k) y = rmfma(sB[(k - 4) * 64 + lane], Cf[k], y);
+        const int p = 16 * t16 + c;
+        if (p < m && q == 0) {
+            float v = y[0] + bias[s2.bias];
+            v = v >= 0.0f ? v : 0.0f;
+            float o = __expf(v) - 1.0f;
+            o = o > 0.0f ? o : 0.0f;
+            if (y_out) {
+                y_out[p] = o;
+            } else {
+                const v3 rr = (mk3(o, o, o) * lc) * sr;
+                const RpnnRec& r = recs[p];
+                out[r.pixel] = make_float4(fmaxf(0.0f, rr.x), fmaxf(0.0f, rr.y), fmaxf(0.0f, rr.z), fmaxf(0.0f, r.dist));
+            }
+        }
+        rhalf_end();
+    }
 }
 
 }  // namespace mrp

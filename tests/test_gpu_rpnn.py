@@ -140,11 +140,12 @@ def test_rpnn_bf16_passes_and_tails(ctx, orc, rscene):
     one = vol.render_rpnn(vm, M.BF16).cpu().numpy()
     active = int((one[:, 3] > 0).sum())
     assert active > 2 * 4096 and active % 128 != 0
+    default = ctx.get_option(M.OPT_RPNN_CHUNK)
     try:
         ctx.set_option(M.OPT_RPNN_CHUNK, 4096)
         many = vol.render_rpnn(vm, M.BF16).cpu().numpy()
     finally:
-        ctx.set_option(M.OPT_RPNN_CHUNK, 32768)
+        ctx.set_option(M.OPT_RPNN_CHUNK, default)
     np.testing.assert_array_equal(one, many)
 
 
