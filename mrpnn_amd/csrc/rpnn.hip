@@ -11,7 +11,8 @@
 // FP32 path (the reference's arithmetic): the march lists the active pixels; per chunk of them
 // each level's 226 inputs are written to HBM and each layer is a GEMM over (pixels x 200
 // outputs) whose every output sums its products from input 0 upwards, as Linear does, with the
-// residual adds and ReLUs in the epilogues.  BF16 path: one fused MFMA kernel (below).
+// residual adds and ReLUs in the epilogues.  BF16 path: the descriptor as bf16 MFMA fragments
+// (k_rpnn_desc), then one fused kernel runs the 31 layers on 16x16x32 MFMA (k_rpnn_mfma).
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
@@ -261,8 +262,8 @@ __global__ __launch_bounds__(256) void k_rpnn_compose(const RpnnRec* __restrict_
 // output rows per M-tile, 13 tiles for the 200-wide states), activations as B fragments in the
 // K-slot order where slot 8q+i <-> input 32k+4q+i and slot 8q+4+i <-> input 32k+16+4q+i, so a
 // layer's D registers are the next layer's B operand.  A 512-thread workgroup (128 pixels)
-// stages one layer's fragments (<= 104 KB) in LDS at a time; the descriptor samples are taken
-// by the lanes that feed them to the first layer of each level.
+// streams each layer's fragments through two LDS slots in K halves; the descriptor comes as
+// ready-made B fragments from HBM (k_rpnn_desc).
 // ------------------------------------------------------------------------------------------
 typedef __bf16 rbf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 rbf16x2_t __attribute__((ext_vector_type(2)));
@@ -272,7 +273,6 @@ typedef unsigned ru32x4_t __attribute__((ext_vector_type(4)));
 constexpr int kRNM = 13;                    // M-tiles of a 200-wide layer
 constexpr int kRKV = 8, kRKS = 7;           // 32-wide K chunks of the level input (226) and a state (200)
 constexpr int kRStages = kRL * 3 - 1 + 3;   // V, W (levels 1..9), U per level; F0, F1, F2
-constexpr int kRImgMax = kRNM * kRKV * 64;  // uint4 of the largest layer image (104 KB)
 
 struct RpnnStage {
     int frag;  // first uint4 of the layer image
