@@ -863,15 +863,17 @@ static int rpnn_bf16(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int m, float
         lc = mk3(vw->light_color[0], vw->light_color[1], vw->light_color[2]);
         sr = mk3(vw->scatter_rate[0], vw->scatter_rate[1], vw->scatter_rate[2]);
     }
-    // descriptor fragments of c->rpnn_chunk pixels at a time (5 KB per pixel, bf16)
+    // descriptor fragments of c->rpnn_chunk pixels at a time (5 KB per pixel, bf16); the buffer
+    // grows to what the largest pass so far needed
     const int chunk = c->rpnn_chunk;
-    if (s->desc_cap < chunk) {
+    const int need = m < chunk ? (m + 127) / 128 * 128 : chunk;
+    if (s->desc_cap < need) {
         (void)hipFree(s->d_desc);
         s->d_desc = nullptr;
         s->desc_cap = 0;
-        if (hipMalloc(&s->d_desc, (size_t)chunk / 16 * kRL * kRKV * 64 * sizeof(ru32x4_t)) != hipSuccess)
+        if (hipMalloc(&s->d_desc, (size_t)need / 16 * kRL * kRKV * 64 * sizeof(ru32x4_t)) != hipSuccess)
             return mrp_fail(MRP_ERR_NOMEM, "RPNN descriptor buffer");
-        s->desc_cap = chunk;
+        s->desc_cap = need;
     }
     const int cus = c->props.multiProcessorCount > 0 ? c->props.multiProcessorCount : 256;
     for (int off = 0; off < m; off += chunk) {
