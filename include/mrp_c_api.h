@@ -173,6 +173,23 @@ int mrp_render_plain_device(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view,
  * mrp_prepare_views).  Asynchronous. */
 int mrp_render_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views,
                      int precision, float* const* dev_out4);
+/* mrp_render_batch without device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same kernels, arguments and results as mrp_render_batch, but the
+ * per-call device buffers (counters, volume and view tables, the active-pixel list of 64-byte
+ * records, 64 per 8x8 tile) live in one static device array of MRP_NOALLOC_SCRATCH_BYTES that
+ * the library keeps on each GPU it runs on until the process ends (resolved by
+ * mrp_ctx_create).  Limits, refused with MRP_ERR_INVALID: n_views <= MRP_NOALLOC_MAX_VIEWS and
+ * sum over views of 64 * ceil(width/8) * ceil(height/8) <= MRP_NOALLOC_MAX_RECORDS (one 1920x1080
+ * view needs 2,073,600).  The views must be prepared (mrp_prepare_views) as for
+ * mrp_render_batch; MRP_OPT_GRAPH and timing are ignored.  The array is shared by every call on
+ * one GPU and keeps the last call's contents: calls that overlap on one GPU (other streams or
+ * contexts) race; calls on one stream are ordered.  Asynchronous. */
+#define MRP_NOALLOC_MAX_VIEWS 64
+#define MRP_NOALLOC_MAX_RECORDS 2097152
+#define MRP_NOALLOC_SCRATCH_BYTES (65536 + (size_t)MRP_NOALLOC_MAX_RECORDS * 64)
+int mrp_render_batch_noalloc(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views,
+                             int n_views, int precision, float* const* dev_out4);
 /* Launch-shape autotuner (no counterpart in the reference, whose launch shapes are fixed):
  * renders the prepared batch with the LDS-weight kernel (12 waves per CU) and with the
  * L1/L2-weight kernel at 4, 8, 12, 16, 24 and 32 persistent waves per CU, keeps the fastest

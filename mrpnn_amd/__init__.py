@@ -135,6 +135,8 @@ def lib():
         L.mrp_prepare_views.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View), ctypes.c_int]
         L.mrp_render_batch.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View), ctypes.c_int,
                                        ctypes.c_int, ctypes.POINTER(_vp)]
+        L.mrp_render_batch_noalloc.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View),
+                                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
         L.mrp_gather_features.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp, _vp, _vp, _vp]
         L.mrp_gather_tiles.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _vp]
         L.mrp_predict_features.argtypes = [_vp, _vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.c_int, _vp]
@@ -278,6 +280,13 @@ class Context:
         va = (View * len(views))(*views)
         oa = (_vp * len(outs))(*[o.data_ptr() for o in outs])
         check(lib().mrp_render_batch(self.h, vh, len(volumes), va, len(views), int(precision), oa))
+
+    def render_batch_noalloc(self, volumes, views, outs, precision=BF16):
+        """mrp_render_batch_noalloc: the same batch with no device allocation (static scratch)."""
+        vh = (_vp * len(volumes))(*[v.h for v in volumes])
+        va = (View * len(views))(*views)
+        oa = (_vp * len(outs))(*[o.data_ptr() for o in outs])
+        check(lib().mrp_render_batch_noalloc(self.h, vh, len(volumes), va, len(views), int(precision), oa))
 
     def autotune(self, volumes, views, outs, precision=BF16):
         """Times the prepared batch per candidate waves-per-CU; keeps and returns the best

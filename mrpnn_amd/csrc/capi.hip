@@ -156,6 +156,7 @@ extern "C" int mrp_ctx_create(const char* weights_path, mrp_ctx** out) {
         for (int j = 0; j < 4; ++j) HIP_TRY(hipEventCreate(&c->ring.ev[i][j]));
     int prc = mrp_fused_init(c);
     if (prc) return prc;
+    mrp_noalloc_setup(c->device);
     *out = c;
     return MRP_OK;
 }

@@ -127,5 +127,7 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
 void mrp_fused_free(mrp_ctx* c);
 int mrp_fused_collect_stats(mrp_ctx* c);
 int mrp_fused_join_aux(mrp_ctx* c);
+// resolves the no-alloc scratch array on the current device (once per device)
+void mrp_noalloc_setup(int device);
 int mrp_fused_predict_features(mrp_ctx* c, const float* dev_feats, int n, const float sr[3], float g, int precision,
                                float* dev_rgb);

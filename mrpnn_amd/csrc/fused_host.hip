@@ -3,6 +3,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "capi_internal.h"
@@ -24,6 +25,8 @@ struct FusedState {
     ViewSlot* h_slots = nullptr;
     hipEvent_t staged = nullptr;  // staging buffers free once this completes
     bool have_counts = false;
+    const unsigned* count_src = nullptr;  // counters of the last batch (context or no-alloc buffer)
+    const unsigned long long* taps_src = nullptr;
     MfmaNet net;  // packed BF16 / FP8 weights
     int mfma_waves_per_cu = 12;  // three 256-thread blocks per CU (three waves per SIMD)
     bool lds_weights = true;     // one 768-thread block per CU with the weights in LDS
@@ -113,8 +116,8 @@ int mrp_fused_collect_stats(mrp_ctx* c) {
     if (f->have_counts) {
         unsigned cnt = 0;
         unsigned long long taps = 0;
-        if (hipMemcpyAsync(&cnt, f->d_counter, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipMemcpyAsync(&taps, f->d_taps, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        if (hipMemcpyAsync(&cnt, f->count_src, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(&taps, f->taps_src, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             return mrp_fail(MRP_ERR_HIP, "reading counters failed");
         c->stats.active_pixels = cnt;
@@ -161,8 +164,10 @@ static int tr_slot_claim(mrp_ctx* c) {
     if ((int)c->tr_slots.size() < c->tr_slot_cap) {
         TrSlot t;
         if (hipMalloc(&t.d_tr, kTrFloats * sizeof(float)) != hipSuccess ||
-            hipMalloc(&t.d_ptr, kPadTrFloats * sizeof(float)) != hipSuccess) {
+            hipMalloc(&t.d_ptr, kPadTrFloats * sizeof(float)) != hipSuccess ||
+            hipEventCreateWithFlags(&t.last_read, hipEventDisableTiming) != hipSuccess) {
             hipFree(t.d_tr);
+            hipFree(t.d_ptr);
             return -1;
         }
         // the padded copy's border must read as zero; the bake writes the interior only
@@ -226,17 +231,17 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
     return mrp_render_batch_checker(c, vols, n_vols, views, n_views, precision, dev_out4, nullptr);
 }
 
-int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
-                             int precision, float* const* dev_out4, const int* checker) {
+// Checks a batch's arguments and finds the TR set of every view (view_slot[i]); counts its 8x8
+// tiles and pixels.
+static int batch_validate(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
+                          int precision, float* const* dev_out4, int* view_slot, int* n_tiles, size_t* pixels) {
     if (!c || !vols || !views || !dev_out4 || n_vols < 1 || n_views < 1)
         return mrp_fail(MRP_ERR_INVALID, "bad batch arguments");
     if (precision != MRP_FP32 && precision != MRP_BF16 && precision != MRP_FP8 && precision != MRP_BF16_1T)
         return mrp_fail(MRP_ERR_INVALID, "unknown precision %d", precision);
-    FusedState* f = c->fused;
     if (!c->lut_valid || c->lut_g != views[0].g) return mrp_fail(MRP_ERR_STATE, "HG LUT not prepared for g");
-    std::vector<int> view_slot(n_views, -1);
-    int n_tiles = 0;
-    size_t pixels = 0;
+    *n_tiles = 0;
+    *pixels = 0;
     for (int i = 0; i < n_views; ++i) {
         const mrp_view& v = views[i];
         int rc = mrp_check_view(&v);
@@ -250,9 +255,83 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
         const int slot = vol->mips_valid ? mrp_tr_slot_find(c, vol, v.light_dir, v.alpha) : -1;
         if (slot < 0) return mrp_fail(MRP_ERR_STATE, "view %d: TR mips not prepared (call mrp_prepare_views)", i);
         view_slot[i] = slot;
-        n_tiles += ((v.width + 7) / 8) * ((v.height + 7) / 8);
-        pixels += (size_t)v.width * v.height;
+        *n_tiles += ((v.width + 7) / 8) * ((v.height + 7) / 8);
+        *pixels += (size_t)v.width * v.height;
     }
+    return MRP_OK;
+}
+
+// Host images of a batch's tables: one DevVolume per distinct TR set of the views (entry_slot[e]
+// is the set of entry e), one ViewSlot per view.  Returns the number of entries.
+static int batch_tables(const mrp_ctx* c, mrp_volume* const* vols, const mrp_view* views, int n_views,
+                        const int* view_slot, const int* checker, float* const* dev_out4, DevVolume* hv, ViewSlot* hs,
+                        int* entry_slot) {
+    int n_entries = 0;
+    int t0 = 0;
+    for (int i = 0; i < n_views; ++i) {
+        const int sl = view_slot[i];
+        int e = 0;
+        while (e < n_entries && entry_slot[e] != sl) ++e;
+        if (e == n_entries) {
+            const TrSlot& t = c->tr_slots[sl];
+            DevVolume d = mrp_dev_volume(vols[views[i].volume_index]);
+            size_t o = 0;
+            for (int m = 0; m < 8; ++m) {
+                d.tr[m] = t.d_tr + o;
+                o += (size_t)(128 >> m) * (128 >> m) * (128 >> m);
+            }
+            d.ptr = t.d_ptr;
+            entry_slot[n_entries] = sl;
+            hv[n_entries++] = d;
+        }
+        ViewSlot& s = hs[i];
+        s.vw = mrp_to_dev(&views[i]);
+        view_const_init(s.vw, &s.vc);
+        s.out = (float4*)dev_out4[i];
+        s.vol = e;
+        s.tile0 = t0;
+        s.tiles_x = (views[i].width + 7) / 8;
+        s.checker = checker ? checker[i] : 0;
+        t0 += s.tiles_x * ((views[i].height + 7) / 8);
+    }
+    return n_entries;
+}
+
+// The TR sets a batch reads may be re-baked (by a later mrp_prepare_views on the second stream)
+// only after it: one event per set, created when the set was claimed.
+static int batch_mark_reads(mrp_ctx* c, const int* entry_slot, int n_entries, hipStream_t st) {
+    for (int e = 0; e < n_entries; ++e) {
+        TrSlot& t = c->tr_slots[entry_slot[e]];
+        if (!t.last_read || hipEventRecord(t.last_read, st) != hipSuccess)
+            return mrp_fail(MRP_ERR_HIP, "recording the TR set's read event failed");
+        t.read_pending = true;
+    }
+    return MRP_OK;
+}
+
+// Network launch shape: the LDS-weight kernel runs one 768-thread workgroup per CU, the
+// L1/L2-weight kernel mfma_waves_per_cu / 4 256-thread workgroups per CU.
+static hipError_t launch_network(const mrp_ctx* c, int precision, const DevVolume* d_vols, const ViewSlot* d_slots,
+                                 const ActiveRec* d_recs, const unsigned* d_counter, size_t capacity, hipStream_t q) {
+    const FusedState* f = c->fused;
+    if (precision == MRP_FP32)
+        return launch_predict_active_fp32(d_vols, d_slots, c->d_stencil, c->d_weights, d_recs, d_counter,
+                                          (int)capacity, q);
+    const bool lds_w = f->lds_weights;
+    const int grid = lds_w ? c->props.multiProcessorCount : c->props.multiProcessorCount * f->mfma_waves_per_cu / 4;
+    return launch_mlp_mfma_active(precision, !c->exact_gather, lds_w, f->net, d_vols, d_slots, c->d_stencil,
+                                  c->d_fast, c->d_lut_rows, c->d_weights, d_recs, d_counter, grid, q);
+}
+
+int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
+                             int precision, float* const* dev_out4, const int* checker) {
+    if (!c || n_views < 1) return mrp_fail(MRP_ERR_INVALID, "bad batch arguments");
+    FusedState* f = c->fused;
+    std::vector<int> view_slot(n_views, -1);
+    int n_tiles = 0;
+    size_t pixels = 0;
+    int vrc = batch_validate(c, vols, n_vols, views, n_views, precision, dev_out4, view_slot.data(), &n_tiles, &pixels);
+    if (vrc) return vrc;
     // staging buffers are reused: wait for the previous batch's uploads
     if (hipEventSynchronize(f->staged) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "event sync failed");
     // one DevVolume entry per distinct (volume, TR set) of the batch
@@ -286,34 +365,9 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
         if (rc) return rc;
         f->cap = need;
     }
-    std::vector<int> entry_of_slot(c->tr_slots.size(), -1);
-    int n_entries = 0;
-    for (int i = 0; i < n_views; ++i) {
-        const int sl = view_slot[i];
-        if (entry_of_slot[sl] >= 0) continue;
-        const TrSlot& t = c->tr_slots[sl];
-        DevVolume d = mrp_dev_volume(vols[views[i].volume_index]);
-        size_t o = 0;
-        for (int m = 0; m < 8; ++m) {
-            d.tr[m] = t.d_tr + o;
-            o += (size_t)(128 >> m) * (128 >> m) * (128 >> m);
-        }
-        d.ptr = t.d_ptr;
-        entry_of_slot[sl] = n_entries;
-        f->h_vols[n_entries++] = d;
-    }
-    int t0 = 0;
-    for (int i = 0; i < n_views; ++i) {
-        ViewSlot& s = f->h_slots[i];
-        s.vw = mrp_to_dev(&views[i]);
-        view_const_init(s.vw, &s.vc);
-        s.out = (float4*)dev_out4[i];
-        s.vol = entry_of_slot[view_slot[i]];
-        s.tile0 = t0;
-        s.tiles_x = (views[i].width + 7) / 8;
-        s.checker = checker ? checker[i] : 0;
-        t0 += s.tiles_x * ((views[i].height + 7) / 8);
-    }
+    std::vector<int> entry_slot(n_views);
+    const int n_entries =
+        batch_tables(c, vols, views, n_views, view_slot.data(), checker, dev_out4, f->h_vols, f->h_slots, entry_slot.data());
     hipStream_t st = c->stream;
 #define HT(x)                                                                               \
     do {                                                                                    \
@@ -339,13 +393,7 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
         if (ev && (e = hipEventRecord(ev[1], q))) return e;
         if (wait_tr && (e = hipStreamWaitEvent(q, f->tr_done, 0))) return e;
         if (ev && (e = hipEventRecord(ev[3], q))) return e;
-        if (precision == MRP_FP32)
-            e = launch_predict_active_fp32(f->d_vols, f->d_slots, c->d_stencil, c->d_weights, f->d_recs, f->d_counter,
-                                           (int)need, q);
-        else
-            e = launch_mlp_mfma_active(precision, fast, lds_w, f->net, f->d_vols, f->d_slots, c->d_stencil, c->d_fast,
-                                       c->d_lut_rows, c->d_weights, f->d_recs, f->d_counter, grid, q);
-        if (e) return e;
+        if ((e = launch_network(c, precision, f->d_vols, f->d_slots, f->d_recs, f->d_counter, need, q))) return e;
         if (ev && (e = hipEventRecord(ev[2], q))) return e;
         return hipSuccess;
     };
@@ -389,14 +437,10 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
         if (ev) HT(hipEventRecord(ev[2], st));
         HT(hipEventRecord(f->staged, st));
     }
-    // the TR sets this batch reads may be re-baked only after it
-    for (size_t sl = 0; sl < entry_of_slot.size(); ++sl) {
-        if (entry_of_slot[sl] < 0) continue;
-        TrSlot& t = c->tr_slots[sl];
-        if (!t.last_read) HT(hipEventCreateWithFlags(&t.last_read, hipEventDisableTiming));
-        HT(hipEventRecord(t.last_read, st));
-        t.read_pending = true;
-    }
+    int mrc = batch_mark_reads(c, entry_slot.data(), n_entries, st);
+    if (mrc) return mrc;
+    f->count_src = f->d_counter;
+    f->taps_src = f->d_taps;
     f->have_counts = true;
     c->stats.pixels = (int64_t)pixels;
     c->stats.active_pixels = -1;
@@ -511,5 +555,135 @@ extern "C" int mrp_autotune_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vol
     if (rc) return rc;
     if (best_waves) *best_waves = best_lds ? 12 : best;
     if (best_ms) *best_ms = best_t;
+    return MRP_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// mrp_render_batch_noalloc: mrp_render_batch with its per-call device buffers in one static
+// device array, for callers that cannot allocate.  Layout of the array (bytes):
+//   [0, 4)            record counter of the march (unsigned)
+//   [128, 136)        march taps (unsigned long long)
+//   [256, ...)        DevVolume table, MRP_NOALLOC_MAX_VIEWS entries
+//   [kNaSlots, ...)   ViewSlot table, MRP_NOALLOC_MAX_VIEWS entries
+//   [65536, end)      active-pixel records, MRP_NOALLOC_MAX_RECORDS x 64 bytes
+// ------------------------------------------------------------------------------------------
+namespace {
+constexpr size_t kNaCounter = 0;
+constexpr size_t kNaTaps = 128;
+constexpr size_t kNaVols = 256;
+constexpr size_t kNaSlots = kNaVols + ((MRP_NOALLOC_MAX_VIEWS * sizeof(DevVolume) + 255) & ~(size_t)255);
+constexpr size_t kNaRecs = 65536;
+static_assert(kNaSlots + MRP_NOALLOC_MAX_VIEWS * sizeof(ViewSlot) <= kNaRecs, "tables overrun the records");
+static_assert(kNaRecs + (size_t)MRP_NOALLOC_MAX_RECORDS * sizeof(ActiveRec) == MRP_NOALLOC_SCRATCH_BYTES,
+              "MRP_NOALLOC_SCRATCH_BYTES does not match the layout");
+constexpr int kNaMaxDevices = 64;
+std::mutex na_mu;
+unsigned char* na_ptr[kNaMaxDevices];  // device address of `scratch` on each device, once resolved
+hipError_t na_err[kNaMaxDevices];
+bool na_resolved[kNaMaxDevices];
+
+// Tables travel as kernel arguments (no host staging buffer, no copy from pageable memory):
+// up to four volume entries and four view slots per launch.
+constexpr int kNaChunk = 4;
+struct TableChunk {
+    int n_vols, vol0, n_slots, slot0;
+    DevVolume v[kNaChunk];
+    ViewSlot s[kNaChunk];
+};
+static_assert(sizeof(TableChunk) <= 2048, "kernel argument block too large");
+}  // namespace
+
+// The scratch array.  In host code its name is a host-side stand-in and never the device
+// memory: host code uses the address hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) unsigned char scratch[MRP_NOALLOC_SCRATCH_BYTES];
+
+__global__ void k_stage_tables(TableChunk t, DevVolume* vols, ViewSlot* slots) {
+    const int i = threadIdx.x;
+    if (i < t.n_vols) vols[t.vol0 + i] = t.v[i];
+    if (i < t.n_slots) slots[t.slot0 + i] = t.s[i];
+}
+
+// Once per device, with that device current (called by mrp_ctx_create): resolves the device
+// address of the scratch array.  A failure is kept and reported by mrp_render_batch_noalloc.
+void mrp_noalloc_setup(int device) {
+    if (device < 0 || device >= kNaMaxDevices) return;
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (na_resolved[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) (void)hipGetLastError();  // do not leave the error for the next launch check
+    na_err[device] = e;
+    na_ptr[device] = e == hipSuccess ? (unsigned char*)p : nullptr;
+    na_resolved[device] = true;
+}
+
+extern "C" int mrp_render_batch_noalloc(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views,
+                                        int n_views, int precision, float* const* dev_out4) {
+    if (!c) return mrp_fail(MRP_ERR_INVALID, "ctx is null");
+    if (n_views < 1 || n_views > MRP_NOALLOC_MAX_VIEWS)
+        return mrp_fail(MRP_ERR_INVALID, "n_views %d outside [1, %d]", n_views, MRP_NOALLOC_MAX_VIEWS);
+    int view_slot[MRP_NOALLOC_MAX_VIEWS];
+    int n_tiles = 0;
+    size_t pixels = 0;
+    int rc = batch_validate(c, vols, n_vols, views, n_views, precision, dev_out4, view_slot, &n_tiles, &pixels);
+    if (rc) return rc;
+    const size_t need = (size_t)n_tiles * 64;
+    if (need > (size_t)MRP_NOALLOC_MAX_RECORDS)
+        return mrp_fail(MRP_ERR_INVALID, "the batch needs %zu active-pixel records (64 per 8x8 tile), the limit is %d",
+                        need, MRP_NOALLOC_MAX_RECORDS);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != c->device)
+        return mrp_fail(MRP_ERR_STATE, "the current device (%d) is not the context's (%d)", cur, c->device);
+    unsigned char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(na_mu);
+        if (c->device >= 0 && c->device < kNaMaxDevices && na_resolved[c->device]) {
+            base = na_ptr[c->device];
+            if (!base)
+                return mrp_fail(MRP_ERR_HIP, "scratch array of device %d unavailable: %s", c->device,
+                                hipGetErrorString(na_err[c->device]));
+        }
+    }
+    if (!base) return mrp_fail(MRP_ERR_STATE, "scratch array of device %d not resolved", c->device);
+    unsigned* counter = (unsigned*)(base + kNaCounter);
+    unsigned long long* taps = (unsigned long long*)(base + kNaTaps);
+    DevVolume* d_vols = (DevVolume*)(base + kNaVols);
+    ViewSlot* d_slots = (ViewSlot*)(base + kNaSlots);
+    ActiveRec* d_recs = (ActiveRec*)(base + kNaRecs);
+
+    DevVolume hv[MRP_NOALLOC_MAX_VIEWS];
+    ViewSlot hs[MRP_NOALLOC_MAX_VIEWS];
+    int entry_slot[MRP_NOALLOC_MAX_VIEWS];
+    const int n_entries = batch_tables(c, vols, views, n_views, view_slot, nullptr, dev_out4, hv, hs, entry_slot);
+    FusedState* f = c->fused;
+    hipStream_t st = c->stream;
+#define HT(x)                                                                                    \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) return mrp_fail(MRP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+    for (int k = 0; k < n_views || k < n_entries; k += kNaChunk) {
+        TableChunk t;
+        t.vol0 = t.slot0 = k;
+        t.n_vols = n_entries - k < 0 ? 0 : (n_entries - k < kNaChunk ? n_entries - k : kNaChunk);
+        t.n_slots = n_views - k < kNaChunk ? n_views - k : kNaChunk;
+        for (int i = 0; i < t.n_vols; ++i) t.v[i] = hv[k + i];
+        for (int i = 0; i < t.n_slots; ++i) t.s[i] = hs[k + i];
+        hipLaunchKernelGGL(k_stage_tables, dim3(1), dim3(64), 0, st, t, d_vols, d_slots);
+        HT(hipGetLastError());
+    }
+    HT(launch_reset_counters(counter, taps, st));
+    HT(launch_march_compact(d_vols, d_slots, n_views, n_tiles, d_recs, counter, taps, st));
+    if (f->tr_recorded) HT(hipStreamWaitEvent(st, f->tr_done, 0));
+    HT(launch_network(c, precision, d_vols, d_slots, d_recs, counter, need, st));
+#undef HT
+    rc = batch_mark_reads(c, entry_slot, n_entries, st);
+    if (rc) return rc;
+    f->count_src = counter;
+    f->taps_src = taps;
+    f->have_counts = true;
+    c->stats.pixels = (int64_t)pixels;
+    c->stats.active_pixels = -1;
+    c->stats.march_taps = -1;
     return MRP_OK;
 }

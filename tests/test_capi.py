@@ -46,6 +46,7 @@ def test_argument_errors_are_status_codes(lib):
     assert lib.mrp_volume_create(None, 64, ctypes.byref(h)) == 1
     assert lib.mrp_render_plain(None, None, None, 1, None) == 1
     assert lib.mrp_render_batch(None, None, 0, None, 0, M.BF16, None) == 1
+    assert lib.mrp_render_batch_noalloc(None, None, 0, None, 0, M.BF16, None) == 1
     assert lib.mrp_pt_render(None, None, None, 0, 1, 1, 0, 1.0, None, None) == 1
     assert lib.mrp_predict_features(None, None, 1, None, 0.5, 0, None) == 1
     # grid + density mips 0..8 + TR mips 0..7 + zero-padded copies ((res+3)^3) of density
