@@ -241,7 +241,8 @@ def cpu_baseline(vol, ctx, v, seconds, W, H):
     import torch
 
     import oracle as O
-    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    from bench_configs import cpu_info, cpu_threads
+    threads = cpu_threads()
     O.set_threads(threads)
     ctx.prepare_views([vol], [v])
     vol.update_tr(np.asarray(v.light_dir[:], np.float32), v.alpha)  # the volume's own TR mips for this sun
@@ -260,8 +261,9 @@ def cpu_baseline(vol, ctx, v, seconds, W, H):
     sc.render_predict_list(ov, pix)
     dt = time.perf_counter() - t0
     log("cpu baseline: %d pixels in %.2fs on %d threads" % (n, dt, threads))
-    return {"value": round(n / dt / 1e6, 5), "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": "%d pixels drawn uniformly from the benchmark 1080p frame, same pyramids (%.1f s)" % (n, dt)}
+    return dict({"value": round(n / dt / 1e6, 5), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                 "sample": "%d pixels drawn uniformly from the benchmark 1080p frame, same pyramids (%.1f s)" % (n, dt)},
+                **cpu_info())
 
 
 if __name__ == "__main__":

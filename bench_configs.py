@@ -32,6 +32,30 @@ import numpy as np
 G = 0.857
 
 
+def cpu_threads():
+    """Threads of the CPU baselines: the host CPU share this GPU is given (OMP_NUM_THREADS, which
+    the GPU box sets to 16 per GPU; MRP_CPU_THREADS overrides it), never more than the CPUs this
+    process may run on; all of those CPUs when neither variable is set."""
+    aff = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("MRP_CPU_THREADS", "0") or 0) or int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(env, aff) if env > 0 else aff
+
+
+def cpu_info():
+    """Host CPU description for the baseline records: model name, logical CPUs, affinity."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "host_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "thread_rule": "OMP_NUM_THREADS (the host CPU share of this GPU), capped by the affinity mask"}
+
+
 def _log(*a):
     import sys
     print("[bench]", *a, file=sys.stderr, flush=True)
@@ -85,7 +109,7 @@ def c1(args):
         torch.cuda.synchronize()
         res[name] = dict(img=out.cpu().numpy().copy(), mpix=256 * 256 / dt / 1e6)
     # CPU baseline: the oracle's NNPredict<MRPNN> restatement on all host cores, full frame
-    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     O.set_threads(threads)
     vol.update_tr(np.asarray(v.light_dir[:], np.float32), v.alpha)
     torch.cuda.synchronize()
@@ -104,7 +128,7 @@ def c1(args):
     return {
         "config": "c1", "workload": "256x256 view of a 128^3 fBm cloud (5 octaves, seed 1), sun 30 deg, alpha 64",
         "fp32_plain_mpix_s": round(res["fp32_plain"]["mpix"], 2), "bf16_mpix_s": round(res["bf16"]["mpix"], 2),
-        "cpu_predictor": {"mpix_s": round(256 * 256 / cpu_s / 1e6, 5), "cores": threads, "kind": "port",
+        "cpu_predictor": {"mpix_s": round(256 * 256 / cpu_s / 1e6, 5), "cores": threads, "cpu": cpu_info(), "kind": "port",
                           "seconds": round(cpu_s, 3)},
         "gpu_fp32_vs_cpu_oracle": _rel(res["fp32_plain"]["img"][:, 0], cpu[:, 0]),
         "bf16_vs_fp32": _rel(res["bf16"]["img"][:, 0], res["fp32_plain"]["img"][:, 0]),
@@ -121,7 +145,7 @@ def c2(args):
     import bench
     import mrpnn_amd as M
     import oracle as O
-    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     O.set_threads(threads)
     ctx = M.Context()
     suns = bench.sun_dirs(16)
@@ -346,7 +370,7 @@ def tile(args):
     # CPU baseline and parity: the oracle's GatherData restatement on the host cores, one frame
     alpha = 4.0
     got = data.fetch_input(alpha=alpha, pixelOffset=(3, 5)).reshape(rows, -1).cpu().numpy()
-    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     O.set_threads(threads)
     sc = O.Scene.from_arrays(vol.get_data(), N, 1.0, vol.download_mips(), vol.download_tr(), ctx.download_hg_lut())
     v = M.camera_view(data.w, data.h, (0.67085, -0.03808, -0.04856), (0.34281, 0.70711, 0.61845), alpha=alpha, g=G)
@@ -356,7 +380,7 @@ def tile(args):
     act = active > 0
     rel = np.abs(got[act, :581] - want[act, :581]) / np.maximum(np.abs(want[act, :581]), 1e-3)
     rel[:, 577:579] = np.abs(got[act, 577:579] - want[act, 577:579]) * (2e-4 / 1e-6)
-    res["cpu_baseline"] = {"rows_s": round(rows / cpu_s, 1), "seconds": round(cpu_s, 3), "cores": threads,
+    res["cpu_baseline"] = {"rows_s": round(rows / cpu_s, 1), "seconds": round(cpu_s, 3), "cores": threads, "cpu": cpu_info(),
                            "kind": "port", "sample": "one full 240x135 frame, alpha 4, offset (3, 5)"}
     res["parity_vs_oracle"] = {"active_equal": bool(np.array_equal(got[:, 0] != -1000.0, act)),
                                "t_grid_max_abs": float(np.abs(got[:, 581:] - want[:, 581:]).max()),
@@ -417,14 +441,14 @@ def interactive(args):
     if args.no_cpu:
         ctx.close()
         return res
-    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     O.set_threads(threads)
     got = rgba.cpu().numpy().view(np.uint32)
     t0 = time.perf_counter()
     want = O.denoise(t, h, W, W, last_frame, last_flip, 1, 1.0, M.TONE_ACES)
     cpu_s = time.perf_counter() - t0
     d = np.abs(got.view(np.uint8).reshape(-1, 4).astype(int) - want.view(np.uint8).reshape(-1, 4).astype(int))
-    res["cpu_baseline"] = {"filter_ms_per_frame": round(cpu_s * 1e3, 1), "cores": threads, "kind": "port",
+    res["cpu_baseline"] = {"filter_ms_per_frame": round(cpu_s * 1e3, 1), "cores": threads, "cpu": cpu_info(), "kind": "port",
                            "sample": "the histogram filter and ACES over one 1024x1024 PT frame"}
     res["parity_vs_oracle"] = {"rgba_equal_frac": round(float(np.mean(d.max(axis=1) == 0)), 5),
                                "rgba_max_step": int(d.max())}
@@ -606,7 +630,7 @@ def rpnn(args):
         ctx.close()
         return res
     # the oracle on the host cores: a 960x540 frame of the same camera, and parity against it
-    threads = int(os.environ.get("MRP_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     O.set_threads(threads)
     sc = O.Scene.from_arrays(vol.get_data(), N, 1.0, vol.download_mips(), vol.download_tr(), ctx.download_hg_lut())
     vs = view(960, 540)
@@ -618,7 +642,7 @@ def rpnn(args):
     got16 = vol.render_rpnn(vs, M.BF16).cpu().numpy()
     rel = np.abs(got32[act, 0] - ref["out"][act, 0]) / np.maximum(ref["out"][act, 0], 1e-6)
     res["cpu_baseline"] = {"pixels_s": round(960 * 540 / cpu_s, 1), "active_pixels_s": round(int(act.sum()) / cpu_s, 1),
-                           "seconds": round(cpu_s, 3), "cores": threads, "kind": "port",
+                           "seconds": round(cpu_s, 3), "cores": threads, "cpu": cpu_info(), "kind": "port",
                            "sample": "one 960x540 frame of the same camera (descriptor and FP32 network)"}
     res["parity_vs_oracle"] = {"fp32_median_rel": float(np.median(rel)), "fp32_max_rel": float(rel.max()),
                                "sky_equal": bool(np.array_equal(got32[~act], ref["out"][~act])),

@@ -345,8 +345,9 @@ int mrp_interactive_set_exposure(mrp_interactive* it, float exposure); /* SetExp
  * none / Gamma 1/2.2 / ACES (core/render.cuh:1144-1236), saturate, 8 bits per channel.
  * The view's light, alpha and g update the LUT and TR mips as the reference does; the
  * jitter stream is keyed by (view->seed, randseed).  Asynchronous, except MRP_RT_RPNN:
- * the RPNN network (MRP_FP32, any other precision runs its BF16 network) in place of the
- * predictor, which synchronises with its march as mrp_render_rpnn. */
+ * the RPNN network (MRP_FP32 or MRP_BF16; other precisions are refused with MRP_ERR_INVALID,
+ * as by mrp_render_rpnn) in place of the predictor, which synchronises with its march as
+ * mrp_render_rpnn. */
 int mrp_render_interactive(mrp_interactive* it, mrp_volume* vol, const mrp_view* view, int render_type, int precision,
                            int max_bounce, int randseed, int tone_type, int denoise, float* dev_target,
                            mrp_histogram* dev_histo, uint32_t* dev_rgba);

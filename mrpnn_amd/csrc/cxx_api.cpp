@@ -267,8 +267,10 @@ void VolumeRender::Render(float3* target, Histogram* histo_buffer, unsigned int*
     v.jitter = -1.0f;
     v.width = size.x;
     v.height = size.y;
+    // RPNN frames run its FP32 network, or its BF16 network for any MFMA precision
+    const int prec = rt == RenderType::RPNN && interactive_precision_ != MRP_FP32 ? MRP_BF16 : interactive_precision_;
     check(mrp_render_interactive(interactive_, vol_, &v, rt == RenderType::PT ? MRP_RT_PT : rt == RenderType::RPNN ? MRP_RT_RPNN : MRP_RT_MRPNN,
-                                 interactive_precision_, multiScatter, randseed, toneType, denoise ? 1 : 0,
+                                 prec, multiScatter, randseed, toneType, denoise ? 1 : 0,
                                  reinterpret_cast<float*>(target), histo_buffer, target2));
     if (hipDeviceSynchronize() != hipSuccess) throw Error(MRP_ERR_HIP, "interactive frame failed");
 }

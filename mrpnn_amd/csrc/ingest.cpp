@@ -15,6 +15,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -60,7 +62,15 @@ int parse_bin(const std::string& path, std::vector<float>& grid, int& N) {
         fclose(f);
         return mrp_fail(MRP_ERR_IO, "bad volume header in %s", path.c_str());
     }
-    grid.resize((size_t)n * n * n);
+    // the header may not size an allocation the file cannot fill
+    const size_t want = (size_t)n * n * n;
+    long size = -1;
+    if (fseek(f, 0, SEEK_END) == 0) size = ftell(f);
+    if (size < 0 || (size_t)size < 4 + want * sizeof(float) || fseek(f, 4, SEEK_SET) != 0) {
+        fclose(f);
+        return mrp_fail(MRP_ERR_IO, "truncated volume file %s (%d^3 floats declared)", path.c_str(), (int)n);
+    }
+    grid.resize(want);
     size_t got = fread(grid.data(), sizeof(float), grid.size(), f);
     fclose(f);
     if (got != grid.size()) return mrp_fail(MRP_ERR_IO, "truncated volume file %s", path.c_str());
@@ -76,6 +86,9 @@ int parse_txt(const std::string& path, std::vector<float>& grid, int& N) {
     char* e = nullptr;
     long n = strtol(p, &e, 10);
     if (e == p || n < 2 || n > 2048) return mrp_fail(MRP_ERR_IO, "bad resolution in %s", path.c_str());
+    // every value takes at least one character and one separator
+    if ((size_t)n * n * n > buf.size() / 2 + 1)
+        return mrp_fail(MRP_ERR_IO, "%s: %ld^3 values declared in %zu bytes", path.c_str(), n, buf.size() - 1);
     p = e;
     grid.resize((size_t)n * n * n);
     for (size_t i = 0; i < grid.size(); ++i) {
@@ -220,6 +233,8 @@ int read_hdr(const char* path, std::vector<float>& rgba, int& w, int& h) {
             break;
         }
     }
+    if ((uint64_t)rx * ry > ((uint64_t)1 << 26))  // the limit of mrp_ctx_set_env_map
+        return mrp_fail(MRP_ERR_IO, "%s: %u x %u pixels exceed 2^26", path, rx, ry);
     const unsigned char* p = (const unsigned char*)line;
     const unsigned char* pend = (const unsigned char*)end;
     rgba.assign((size_t)rx * ry * 4, 0.0f);
@@ -261,109 +276,131 @@ F3 image_sample(const float* rgba, int sx, int sy, float u, float v) {
     return f3lerp(f3lerp(px(x, y), px(x + 1, y), wx), f3lerp(px(x, y + 1), px(x + 1, y + 1), wx), wy);
 }
 
+// Host allocations of the readers may fail (std::bad_alloc); nothing may unwind through the C ABI.
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return mrp_fail(MRP_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return mrp_fail(MRP_ERR_IO, "%s", e.what());
+    }
+}
+
 }  // namespace
 
 extern "C" int mrp_volume_read(const char* path, float** host_grid, int* resolution, float* max_density) {
-    if (!path || !host_grid || !resolution || !max_density) return mrp_fail(MRP_ERR_INVALID, "null argument");
-    std::vector<float> grid;
-    int N = 0;
-    float md = 0.0f;
-    int rc = read_volume(path, true, grid, N, md);
-    if (rc) return rc;
-    float* out = (float*)malloc(grid.size() * sizeof(float));
-    if (!out) return mrp_fail(MRP_ERR_NOMEM, "host allocation failed");
-    memcpy(out, grid.data(), grid.size() * sizeof(float));
-    *host_grid = out;
-    *resolution = N;
-    *max_density = md;
-    return MRP_OK;
+    return guarded([&]() -> int {
+        if (!path || !host_grid || !resolution || !max_density) return mrp_fail(MRP_ERR_INVALID, "null argument");
+        std::vector<float> grid;
+        int N = 0;
+        float md = 0.0f;
+        int rc = read_volume(path, true, grid, N, md);
+        if (rc) return rc;
+        float* out = (float*)malloc(grid.size() * sizeof(float));
+        if (!out) return mrp_fail(MRP_ERR_NOMEM, "host allocation failed");
+        memcpy(out, grid.data(), grid.size() * sizeof(float));
+        *host_grid = out;
+        *resolution = N;
+        *max_density = md;
+        return MRP_OK;
+    });
 }
 
 extern "C" void mrp_host_free(void* p) { free(p); }
 
 extern "C" int mrp_volume_load(mrp_ctx* c, const char* path, mrp_volume** out, float* max_density) {
-    if (!c || !path || !out) return mrp_fail(MRP_ERR_INVALID, "null argument");
-    std::vector<float> local;
-    const std::vector<float>* grid = &local;
-    int N = 0;
-    float md = 0.0f;
-    const int slot = named_slot(path);
-    if (slot >= 0 && c->data_cache_n[slot] > 0) {  // DataCache hit (core/volume.cu:1008-1053)
-        grid = &c->data_cache[slot];
-        N = c->data_cache_n[slot];
-        md = kNamed[slot].max_density;
-    } else {
-        int rc = read_volume(path, true, local, N, md);
-        if (rc) return rc;
-        if (slot >= 0) {
-            c->data_cache[slot].swap(local);
-            c->data_cache_n[slot] = N;
+    return guarded([&]() -> int {
+        if (!c || !path || !out) return mrp_fail(MRP_ERR_INVALID, "null argument");
+        std::vector<float> local;
+        const std::vector<float>* grid = &local;
+        int N = 0;
+        float md = 0.0f;
+        const int slot = named_slot(path);
+        if (slot >= 0 && c->data_cache_n[slot] > 0) {  // DataCache hit (core/volume.cu:1008-1053)
             grid = &c->data_cache[slot];
+            N = c->data_cache_n[slot];
+            md = kNamed[slot].max_density;
+        } else {
+            int rc = read_volume(path, true, local, N, md);
+            if (rc) return rc;
+            if (slot >= 0) {
+                c->data_cache[slot].swap(local);
+                c->data_cache_n[slot] = N;
+                grid = &c->data_cache[slot];
+            }
         }
-    }
-    mrp_volume* v = nullptr;
-    int rc = mrp_volume_create(c, N, &v);
-    if (!rc) rc = mrp_volume_set_data(v, grid->data(), md);
-    if (!rc) rc = mrp_volume_update(v);
-    if (rc) {
-        if (v) mrp_volume_destroy(v);
-        return rc;
-    }
-    *out = v;
-    if (max_density) *max_density = md;
-    return MRP_OK;
+        mrp_volume* v = nullptr;
+        int rc = mrp_volume_create(c, N, &v);
+        if (!rc) rc = mrp_volume_set_data(v, grid->data(), md);
+        if (!rc) rc = mrp_volume_update(v);
+        if (rc) {
+            if (v) mrp_volume_destroy(v);
+            return rc;
+        }
+        *out = v;
+        if (max_density) *max_density = md;
+        return MRP_OK;
+    });
 }
 
 extern "C" int mrp_hdr_read(const char* path, float** host_rgba, int* width, int* height) {
-    if (!path || !host_rgba || !width || !height) return mrp_fail(MRP_ERR_INVALID, "null argument");
-    std::vector<float> px;
-    int w = 0, h = 0;
-    int rc = read_hdr(path, px, w, h);
-    if (rc) return rc;
-    float* o = (float*)malloc(px.size() * sizeof(float));
-    if (!o) return mrp_fail(MRP_ERR_NOMEM, "host allocation failed");
-    memcpy(o, px.data(), px.size() * sizeof(float));
-    *host_rgba = o;
-    *width = w;
-    *height = h;
-    return MRP_OK;
+    return guarded([&]() -> int {
+        if (!path || !host_rgba || !width || !height) return mrp_fail(MRP_ERR_INVALID, "null argument");
+        std::vector<float> px;
+        int w = 0, h = 0;
+        int rc = read_hdr(path, px, w, h);
+        if (rc) return rc;
+        float* o = (float*)malloc(px.size() * sizeof(float));
+        if (!o) return mrp_fail(MRP_ERR_NOMEM, "host allocation failed");
+        memcpy(o, px.data(), px.size() * sizeof(float));
+        *host_rgba = o;
+        *width = w;
+        *height = h;
+        return MRP_OK;
+    });
 }
 
 extern "C" int mrp_ctx_set_env_map(mrp_ctx* c, const float* host_rgba, int width, int height) {
-    if (!c) return mrp_fail(MRP_ERR_INVALID, "null argument");
-    if (c->d_env) {
-        if (hipStreamSynchronize(c->stream) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "stream sync failed");
-        (void)hipFree(c->d_env);
-        c->d_env = nullptr;
-    }
-    c->env_host.clear();
-    c->env_w = c->env_h = 0;
-    if (!host_rgba) return MRP_OK;  // no environment: sky radiance 0
-    if (width < 1 || height < 1 || (int64_t)width * height > (1 << 26))
-        return mrp_fail(MRP_ERR_INVALID, "environment map %d x %d out of range", width, height);
-    const size_t n = (size_t)width * height * 4;
-    if (hipMalloc(&c->d_env, n * sizeof(float)) != hipSuccess) {
-        c->d_env = nullptr;
-        return mrp_fail(MRP_ERR_NOMEM, "environment map allocation failed");
-    }
-    if (hipMemcpy(c->d_env, host_rgba, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(c->d_env);
-        c->d_env = nullptr;
-        return mrp_fail(MRP_ERR_HIP, "environment map upload failed");
-    }
-    c->env_host.assign(host_rgba, host_rgba + n);
-    c->env_w = width;
-    c->env_h = height;
-    return MRP_OK;
+    return guarded([&]() -> int {
+        if (!c) return mrp_fail(MRP_ERR_INVALID, "null argument");
+        if (c->d_env) {
+            if (hipStreamSynchronize(c->stream) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "stream sync failed");
+            (void)hipFree(c->d_env);
+            c->d_env = nullptr;
+        }
+        c->env_host.clear();
+        c->env_w = c->env_h = 0;
+        if (!host_rgba) return MRP_OK;  // no environment: sky radiance 0
+        if (width < 1 || height < 1 || (int64_t)width * height > (1 << 26))
+            return mrp_fail(MRP_ERR_INVALID, "environment map %d x %d out of range", width, height);
+        const size_t n = (size_t)width * height * 4;
+        if (hipMalloc(&c->d_env, n * sizeof(float)) != hipSuccess) {
+            c->d_env = nullptr;
+            return mrp_fail(MRP_ERR_NOMEM, "environment map allocation failed");
+        }
+        if (hipMemcpy(c->d_env, host_rgba, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(c->d_env);
+            c->d_env = nullptr;
+            return mrp_fail(MRP_ERR_HIP, "environment map upload failed");
+        }
+        c->env_host.assign(host_rgba, host_rgba + n);
+        c->env_w = width;
+        c->env_h = height;
+        return MRP_OK;
+    });
 }
 
 extern "C" int mrp_ctx_load_hdri(mrp_ctx* c, const char* path) {
-    if (!c || !path) return mrp_fail(MRP_ERR_INVALID, "null argument");
-    std::vector<float> px;
-    int w = 0, h = 0;
-    int rc = read_hdr(path, px, w, h);
-    if (rc) return rc;
-    return mrp_ctx_set_env_map(c, px.data(), w, h);
+    return guarded([&]() -> int {
+        if (!c || !path) return mrp_fail(MRP_ERR_INVALID, "null argument");
+        std::vector<float> px;
+        int w = 0, h = 0;
+        int rc = read_hdr(path, px, w, h);
+        if (rc) return rc;
+        return mrp_ctx_set_env_map(c, px.data(), w, h);
+    });
 }
 
 extern "C" int mrp_ctx_set_env_exp(mrp_ctx* c, float exposure) {

@@ -391,6 +391,8 @@ extern "C" int mrp_render_interactive(mrp_interactive* it, mrp_volume* v, const 
     if (!predict && max_bounce < 1) return mrp_fail(MRP_ERR_INVALID, "max_bounce >= 1 required");
     if (predict && (precision < MRP_FP32 || precision > MRP_BF16_1T))
         return mrp_fail(MRP_ERR_INVALID, "unknown precision %d", precision);
+    if (render_type == MRP_RT_RPNN && precision != MRP_FP32 && precision != MRP_BF16)  // as mrp_render_rpnn
+        return mrp_fail(MRP_ERR_INVALID, "RPNN runs in MRP_FP32 or MRP_BF16, not precision %d", precision);
     if (!v->mips_valid) return mrp_fail(MRP_ERR_STATE, "mrp_volume_update must run before rendering");
 
     mrp_view view = *vw;
@@ -415,7 +417,7 @@ extern "C" int mrp_render_interactive(mrp_interactive* it, mrp_volume* v, const 
         mrp_volume* vols[1] = {v};
         const int checker = it->checkerboard ? 1 + ((H - 1 + flip) & 1) : 0;
         if (render_type == MRP_RT_RPNN) {  // RenderCamera<true, RPNN>: FP32, or the BF16 network
-            rc = mrp_render_rpnn_checker(c, v, &view, precision == MRP_FP32 ? MRP_FP32 : MRP_BF16, pred, checker);
+            rc = mrp_render_rpnn_checker(c, v, &view, precision, pred, checker);
         } else {
             rc = mrp_prepare_views(c, vols, 1, &view, 1);
             if (!rc) rc = mrp_render_batch_checker(c, vols, 1, &view, 1, precision, &pred, &checker);

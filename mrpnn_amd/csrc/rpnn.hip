@@ -13,6 +13,7 @@
 // outputs) whose every output sums its products from input 0 upwards, as Linear does, with the
 // residual adds and ReLUs in the epilogues.  BF16 path: the descriptor as bf16 MFMA fragments
 // (k_rpnn_desc), then one fused kernel runs the 31 layers on 16x16x32 MFMA (k_rpnn_mfma).
+#include <algorithm>
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
@@ -864,10 +865,12 @@ static int rpnn_bf16(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, int m, float
         sr = mk3(vw->scatter_rate[0], vw->scatter_rate[1], vw->scatter_rate[2]);
     }
     // descriptor fragments of c->rpnn_chunk pixels at a time (5 KB per pixel, bf16); the buffer
-    // grows to what the largest pass so far needed
+    // grows geometrically (at least doubling, at most one pass) so that active counts creeping
+    // up frame after frame reallocate a few times only
     const int chunk = c->rpnn_chunk;
-    const int need = m < chunk ? (m + 127) / 128 * 128 : chunk;
-    if (s->desc_cap < need) {
+    const int fit = m < chunk ? (m + 127) / 128 * 128 : chunk;
+    if (s->desc_cap < fit) {
+        const int need = std::min(chunk, std::max(fit, 2 * s->desc_cap));
         (void)hipFree(s->d_desc);
         s->d_desc = nullptr;
         s->desc_cap = 0;

@@ -165,6 +165,25 @@ def test_rpnn_frames(ctx, orc, iscene, checker):
     it.close()
 
 
+def test_rpnn_frames_refuse_other_precisions(ctx, orc, iscene):
+    """RPNN runs in FP32 or BF16 only: the interactive renderer refuses FP8 and single-term BF16
+    as mrp_render_rpnn does, before touching the caller's buffers."""
+    import mrpnn_amd as M
+    vol, sc, L = iscene
+    W, H = 16, 12
+    it = M.Interactive(ctx, W, H)
+    target, histo, rgba = it.buffers()
+    target.fill_(3.0)
+    v, vm = _views(orc, M, W, H, L, 0)
+    for prec in (M.FP8, M.BF16_1T):
+        with pytest.raises(M.MrpError, match="RPNN"):
+            it.render(vol, vm, target, histo, rgba, M.RT_RPNN, prec, randseed=0, tone=M.TONE_GAMMA)
+        with pytest.raises(M.MrpError):
+            vol.render_rpnn(vm, prec)
+    assert bool((target == 3.0).all())
+    it.close()
+
+
 def test_switch_from_pt_clears_histograms(ctx, orc, iscene):
     import mrpnn_amd as M
     vol, sc, L = iscene

@@ -148,6 +148,29 @@ def test_hdr_reader_rejects_bad_files(M, tmp_path):
         M.read_hdr(str(p))
 
 
+def test_oversized_headers_fail_without_allocating(M, tmp_path):
+    """Headers that declare more data than the file holds are refused with a status code
+    before anything of that size is allocated (a 2048^3 grid is 32 GiB, a 65536^2 image 64 GiB)."""
+    p = tmp_path / "big.bin"
+    p.write_bytes(struct.pack("<i", 2048) + bytes(64))
+    with pytest.raises(M.MrpError, match="truncated"):
+        M.read_volume(str(p))
+    p = tmp_path / "big.txt"
+    p.write_text("2048\n1 2 3\n")
+    with pytest.raises(M.MrpError, match="declared"):
+        M.read_volume(str(p))
+    p = tmp_path / "big.hdr"
+    p.write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 65536 +X 65536\n" + bytes(16))
+    with pytest.raises(M.MrpError, match="exceed"):
+        M.read_hdr(str(p))
+    # a .bin whose size matches its header still loads
+    g = np.arange(8, dtype=np.float32)
+    p = tmp_path / "ok.bin"
+    p.write_bytes(struct.pack("<i", 2) + g.tobytes())
+    grid, n, _ = M.read_volume(str(p))
+    assert n == 2 and np.array_equal(grid, g)
+
+
 def test_env_light_choice(M):
     c = np.float32(0.37)
     img = np.full((8, 16, 4), c, np.float32)
