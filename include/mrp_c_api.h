@@ -214,6 +214,12 @@ int mrp_gather_features(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, flo
                         float* dev_pos3, int32_t* dev_taps);
 int mrp_predict_features(mrp_ctx* ctx, const float* dev_feats, int n, const float scatterrate[3], float g, int precision,
                          float* dev_rgb);
+/* Tooling (no counterpart in the reference): the 577 features of each pixel as the fused MFMA
+ * kernels gather them by default (fast gather: zero-padded pyramid copies, FMA lerps, hardware
+ * log), for feature-by-feature checks against mrp_gather_features.  Prepares the view's LUT and
+ * TR set (mrp_prepare_views) and runs one BF16 batch for its active-pixel list; rows of pixels
+ * without a first-scattering point are not written.  Blocking. */
+int mrp_gather_features_fast(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, float* dev_feats);
 
 /* Tile feature gather, the training-data path: VolumeRender::GatherData (core/volume.cu:630-708)
  * -> GatherSamples (core/volume.cu:257-283) -> NNGatherSamples / PrepareRay / GetSampleT

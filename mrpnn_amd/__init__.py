@@ -139,6 +139,7 @@ def lib():
                                                ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
         L.mrp_gather_features.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp, _vp, _vp, _vp]
         L.mrp_gather_tiles.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _vp]
+        L.mrp_gather_features_fast.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp]
         L.mrp_predict_features.argtypes = [_vp, _vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.c_int, _vp]
         L.mrp_pt_render.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_float, _vp, _vp]
@@ -406,6 +407,14 @@ class Volume:
         check(lib().mrp_gather_features(self.ctx.h, self.h, ctypes.byref(view), _ptr(feats), _ptr(active), _ptr(pos),
                                         _ptr(taps)))
         return feats, active, pos, taps
+
+    def gather_features_fast(self, view):
+        """mrp_gather_features_fast: (height*width, 577) float32 features of the fused kernels'
+        default gather on the GPU (zero rows where no first-scattering point exists)."""
+        import torch
+        feats = torch.zeros((view.width * view.height, NFEAT), dtype=torch.float32, device="cuda")
+        check(lib().mrp_gather_features_fast(self.ctx.h, self.h, ctypes.byref(view), _ptr(feats)))
+        return feats
 
     def gather_tiles(self, view, off_x=0, off_y=0, out=None):
         """Tile feature gather (mrp_gather_tiles): (height*width, 773) float32 rows on the GPU."""

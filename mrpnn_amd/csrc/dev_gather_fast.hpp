@@ -90,20 +90,24 @@ __device__ inline float clampf(float x, float lo, float hi) { return __builtin_a
 // ln(x) for x >= 1 (no denormal inputs): hardware log2 times ln 2.
 __device__ inline float log_ge1(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
 
-// Corner and weights of a trilinear blend on a padded level at the non-negative continuous
-// coordinate (cx, cy, cz) (already inside the padding): corner = floor, weights = fract.  The
-// linear index (x*R + y)*R + z stays below 2^24, so it is formed exactly with two float FMAs;
-// o4 = 4 * index (bytes, without the level base).  Levels of equal resolution share it.
+// Corner and weights of a trilinear blend on a padded level of R^3 texels at the non-negative
+// continuous coordinate (cx, cy, cz) (already inside the padding): corner = floor (a truncating
+// conversion, the coordinate being non-negative), weights = fract; o4 = 4 * ((x*R + y)*R + z)
+// (bytes, without the level base), formed with 24-bit integer multiply-adds: the padded 256^3
+// level has 259^3 > 2^24 texels, beyond the integers a float holds exactly.  Levels of equal
+// resolution share it.
 struct TapGeo {
     unsigned o4;
     float wx, wy, wz;
 };
-__device__ inline TapGeo tap_geo(float R, float cx, float cy, float cz) {
+__device__ inline TapGeo tap_geo(unsigned R, float cx, float cy, float cz) {
     TapGeo g;
     g.wx = __builtin_amdgcn_fractf(cx);
     g.wy = __builtin_amdgcn_fractf(cy);
     g.wz = __builtin_amdgcn_fractf(cz);
-    g.o4 = (unsigned)fmaf(fmaf(floorf(cx), R, floorf(cy)), R, floorf(cz)) * 4u;
+    // through int: defined for the negative coordinates of lanes whose result is discarded
+    const unsigned x = (unsigned)(int)cx, y = (unsigned)(int)cy, z = (unsigned)(int)cz;
+    g.o4 = (__umul24(__umul24(x, R) + y, R) + z) * 4u;
     return g;
 }
 __device__ inline float tri_geo(BufRsrc r, unsigned base, unsigned R4, unsigned RR4, const TapGeo& g) {
@@ -115,7 +119,7 @@ __device__ inline float tri_geo(BufRsrc r, unsigned base, unsigned R4, unsigned 
 }
 __device__ inline float tri_taps(BufRsrc r, unsigned base, float R, unsigned R4, unsigned RR4, float cx, float cy,
                                  float cz) {
-    return tri_geo(r, base, R4, RR4, tap_geo(R, cx, cy, cz));
+    return tri_geo(r, base, R4, RR4, tap_geo(R4 >> 2, cx, cy, cz));
 }
 // Padded continuous coordinate of p on a level of resolution res: data coordinate
 // (p + 0.5)*res - 0.5, plus one for the padding.
@@ -201,7 +205,7 @@ __device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3
             q = mk3(fmaf(fv.LX.x, off, p.x), fmaf(fv.LX.y, off, p.y), fmaf(fv.LX.z, off, p.z));
         }
         const v3 c = padded_coord(q, s.tres);
-        g = tap_geo(s.tR, clampf(c.x, 1.0f, s.tres), clampf(c.y, 1.0f, s.tres), clampf(c.z, 1.0f, s.tres));
+        g = tap_geo(s.tR4 >> 2, clampf(c.x, 1.0f, s.tres), clampf(c.y, 1.0f, s.tres), clampf(c.z, 1.0f, s.tres));
     }
     const float T = tri_geo(fv.ptr, s.tbase, s.tR4, s.tRR4, g);
     return miss ? 1.0f : T;
@@ -232,10 +236,10 @@ __device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const
     const bool outside = outside_box(p);
     // inside the box the padded coordinate is in [0.5, res + 0.5]: no clamp needed; lanes
     // outside clamp to the zero border [0, res + 1] (skipped when no lane of the wave is outside)
-    TapGeo g = tap_geo(s.dR, c.x, c.y, c.z);
+    TapGeo g = tap_geo(s.dR4 >> 2, c.x, c.y, c.z);
     if (__builtin_amdgcn_ballot_w64(outside) != 0 && outside) {
         const float hi = s.dres + 1.0f;
-        g = tap_geo(s.dR, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi), clampf(c.z, 0.0f, hi));
+        g = tap_geo(s.dR4 >> 2, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi), clampf(c.z, 0.0f, hi));
     }
     const float rho = tri_geo(fv.pmips, s.dbase, s.dR4, s.dRR4, g);
     const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));

@@ -449,6 +449,34 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
     return MRP_OK;
 }
 
+// Tooling: the fast gather's features of one view.  The batch (march, compaction, network) runs
+// once into a temporary image; the features kernel then reads the same active records.
+extern "C" int mrp_gather_features_fast(mrp_ctx* c, mrp_volume* v, const mrp_view* vw, float* dev_feats) {
+    if (!c || !v || !vw || !dev_feats) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    int rc = mrp_check_view(vw);
+    if (rc) return rc;
+    mrp_view view = *vw;
+    view.volume_index = 0;
+    mrp_volume* vols[1] = {v};
+    if ((rc = mrp_prepare_views(c, vols, 1, &view, 1))) return rc;
+    float* tmp = nullptr;
+    if (hipMalloc(&tmp, (size_t)view.width * view.height * 4 * sizeof(float)) != hipSuccess)
+        return mrp_fail(MRP_ERR_NOMEM, "temporary image allocation failed");
+    const bool graph = c->fused->use_graph;
+    c->fused->use_graph = false;  // the features kernel reads the context's own batch buffers
+    rc = mrp_render_batch(c, vols, 1, &view, 1, MRP_BF16, &tmp);
+    c->fused->use_graph = graph;
+    if (!rc) {
+        const FusedState* f = c->fused;
+        const hipError_t e = launch_fast_features(f->d_vols, f->d_slots, c->d_fast, c->d_lut_rows, f->d_recs,
+                                                  f->d_counter, c->props.multiProcessorCount * 4, dev_feats, c->stream);
+        if (e != hipSuccess) rc = mrp_fail(MRP_ERR_HIP, "fast features: %s", hipGetErrorString(e));
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = mrp_fail(MRP_ERR_HIP, "stream sync failed");
+    (void)hipFree(tmp);
+    return rc;
+}
+
 int mrp_fused_predict_features(mrp_ctx* c, const float* dev_feats, int n, const float sr[3], float g, int precision,
                                float* dev_rgb) {
     if (precision != MRP_BF16 && precision != MRP_FP8 && precision != MRP_BF16_1T)

@@ -131,6 +131,10 @@ hipError_t launch_mlp_mfma_active(int precision, bool fast, bool lds_w, const Mf
                                   const ViewSlot* slots, const StencilEntry* stencil, const void* fast_stencil,
                                   const float* lut_rows, const float* W, const ActiveRec* recs,
                                   const unsigned* counter, int grid, hipStream_t s);
+// The fast gather's 577 features of every active record into out (row = record pixel).
+hipError_t launch_fast_features(const DevVolume* vols, const ViewSlot* slots, const void* fast_stencil,
+                                const float* lut_rows, const ActiveRec* recs, const unsigned* counter, int grid,
+                                float* out, hipStream_t s);
 hipError_t launch_mlp_mfma_features(int precision, const MfmaNet& net, const float* W, const float* feats, int n,
                                     const float sr[3], float g, float* rgb, hipStream_t s);
 

@@ -598,6 +598,31 @@ struct FastGatherSrc {
             for (int i = 4; i < 8; ++i) F[0][i] = F[1][i] = F[2][i] = 0.0f;
     }
 };
+// The fast gather of one pixel (record state ps) of a view (constants vc, anisotropy g).
+__device__ inline FastGatherSrc fast_src(const DevVolume& vol, const ViewConst& vc, float g, const PixelState& ps,
+                                         const FastStencil* sfs, const float* lut_rows) {
+    FastGatherSrc src;
+    src.st = sfs;
+    src.fv.pmips = make_rsrc(vol.pmips, kPadMipFloats);
+    src.fv.ptr = make_rsrc(vol.ptr, kPadTrFloats);
+    src.fv.rows = make_rsrc(lut_rows, (long long)kPC * kLutRow);
+    src.fv.L = ps.L;
+    src.fv.LX = ps.LX;
+    src.fv.invLX = vc.invLX;
+    src.fv.alpha64 = vc.alpha64;
+    src.px.pos = ps.pos;
+    src.px.X = ps.X;
+    src.px.Y = ps.Y;
+    src.px.Z = ps.Z;
+    src.px.ax = dot3(ps.LX, ps.X);
+    src.px.ay = dot3(ps.LX, ps.Y);
+    src.px.az = dot3(ps.LX, ps.Z);
+    src.px.uc1 = fmaf(dot3(ps.X, ps.LX), 0.5f, 0.5f);
+    src.px.uc2 = fmaf(dot3(ps.LX, ps.LX), 0.5f, 0.5f);
+    src.px.xh_centre = log_ge1(hg_fast(dot3(ps.X, ps.L), g) + 1.0f);
+    return src;
+}
+
 struct RowSrc {
     const float* row;  // 577 floats [X_Val | X_Val_Hg | X_Val_Sub | gamma]
     __device__ void get(const LaneCtx& lc, int p0, bool wide, float (&F)[3][8]) const {
@@ -1118,26 +1143,7 @@ __global__ __launch_bounds__(lds_weights(PREC) ? 768 : 256) __attribute__((amdgp
         nt.lds = opaque_v(sWf);
         v3 rgb;
         if (FAST) {
-            const DevVolume& vol = vols[sl.vol];
-            FastGatherSrc src;
-            src.st = opaque_v(sfs);
-            src.fv.pmips = make_rsrc(vol.pmips, kPadMipFloats);
-            src.fv.ptr = make_rsrc(vol.ptr, kPadTrFloats);
-            src.fv.rows = make_rsrc(lut_rows, (long long)kPC * kLutRow);
-            src.fv.L = ps.L;
-            src.fv.LX = ps.LX;
-            src.fv.invLX = vc.invLX;
-            src.fv.alpha64 = vc.alpha64;
-            src.px.pos = ps.pos;
-            src.px.X = ps.X;
-            src.px.Y = ps.Y;
-            src.px.Z = ps.Z;
-            src.px.ax = dot3(ps.LX, ps.X);
-            src.px.ay = dot3(ps.LX, ps.Y);
-            src.px.az = dot3(ps.LX, ps.Z);
-            src.px.uc1 = fmaf(dot3(ps.X, ps.LX), 0.5f, 0.5f);
-            src.px.uc2 = fmaf(dot3(ps.LX, ps.LX), 0.5f, 0.5f);
-            src.px.xh_centre = log_ge1(hg_fast(dot3(ps.X, ps.L), vw.g) + 1.0f);
+            const FastGatherSrc src = fast_src(vols[sl.vol], vc, vw.g, ps, opaque_v(sfs), lut_rows);
             rgb = run_tile<PREC>(nt, opaque(W), opaque_v(sG), opaque_v(sW1), lc, src);
         } else {
             GatherSrc src;
@@ -1153,6 +1159,69 @@ __global__ __launch_bounds__(lds_weights(PREC) ? 768 : 256) __attribute__((amdgp
             sl.out[r.pixel] = make_float4(fmaxf(0.0f, o.x), fmaxf(0.0f, o.y), fmaxf(0.0f, o.z), fmaxf(0.0f, r.dist));
         }
     }
+}
+
+// The fast gather alone: for every active record its 577 features [X_Val | X_Val_Hg | X_Val_Sub |
+// gamma] into out[pixel] of the record's view (parity tooling for the fused kernels' gather).
+__global__ __launch_bounds__(256) void k_fast_features(const DevVolume* __restrict__ vols,
+                                                       const ViewSlot* __restrict__ slots,
+                                                       const FastStencil* __restrict__ fast,
+                                                       const float* __restrict__ lut_rows,
+                                                       const ActiveRec* __restrict__ recs,
+                                                       const unsigned* __restrict__ counter, float* __restrict__ out) {
+    __shared__ FastStencil sfs[kPC];
+    for (int i = threadIdx.x; i < kPC; i += blockDim.x) sfs[i] = fast[i];
+    __syncthreads();
+    const unsigned n = *counter;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    for (unsigned tile = wave; tile * 16u < n; tile += nwaves) {
+        LaneCtx lc;
+        lc.lane = lane;
+        lc.c = lane & 15;
+        lc.q = lane >> 4;
+        const ActiveRec r = recs[tile * 16u + lc.c];
+        lc.ok = r.pixel >= 0;
+        const ViewSlot& sl = slots[r.view];
+        PixelState ps;
+        ps.pos = mk3(r.pos[0], r.pos[1], r.pos[2]);
+        ps.X = mk3(r.X[0], r.X[1], r.X[2]);
+        ps.Y = mk3(r.Y[0], r.Y[1], r.Y[2]);
+        ps.Z = mk3(r.Z[0], r.Z[1], r.Z[2]);
+        ps.L = sl.vc.L;
+        ps.LX = sl.vc.LX;
+        const FastGatherSrc src = fast_src(vols[sl.vol], sl.vc, sl.vw.g, ps, sfs, lut_rows);
+        float* row = out + (size_t)(lc.ok ? r.pixel : 0) * kNFEAT;
+        constexpr int p0s[8] = {0, 16, 32, 48, 64, 96, 128, 160};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool wide = p0s[j] >= 64;
+            float F[3][8];
+            src.get(lc, p0s[j], wide, F);
+            if (!lc.ok) continue;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (h == 1 && !wide) break;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int k = p0s[j] + 16 * h + 4 * lc.q + i;
+                    row[k] = F[0][4 * h + i];
+                    row[kPC + k] = F[2][4 * h + i];
+                    row[2 * kPC + k] = F[1][4 * h + i];
+                }
+            }
+        }
+        if (lc.ok && lc.q == 0) row[3 * kPC] = gamma_of(ps);
+    }
+}
+
+hipError_t launch_fast_features(const DevVolume* vols, const ViewSlot* slots, const void* fast_stencil,
+                                const float* lut_rows, const ActiveRec* recs, const unsigned* counter, int grid,
+                                float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fast_features, dim3(grid), dim3(256), 0, s, vols, slots, (const FastStencil*)fast_stencil,
+                       lut_rows, recs, counter, out);
+    return hipGetLastError();
 }
 
 template <int PREC>

@@ -174,6 +174,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                     wx[k] = __builtin_amdgcn_fractf(c.x);
                     wy[k] = __builtin_amdgcn_fractf(c.y);
                     wz[k] = __builtin_amdgcn_fractf(c.z);
+                    // source levels 1..7 (at most 131^3 padded texels < 2^24): exact in float
                     const float lin = fmaf(fmaf(floorf(c.x), R, floorf(c.y)), R, floorf(c.z));
                     o[k] = (int)((unsigned)lin * 4u + base);
                 }

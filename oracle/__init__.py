@@ -70,12 +70,14 @@ def lib():
         vol = [_f, ctypes.c_int, ctypes.c_float, _f, _f, _f]
         L.orc_render_predict.argtypes = vol + [ctypes.POINTER(View), _f, _f, _i, _i, _f]
         L.orc_render_predict_rows.argtypes = vol + [ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _f]
-        L.orc_render_predict_list.argtypes = vol + [ctypes.POINTER(View), _i, ctypes.c_int, _f]
+        L.orc_render_predict_list.argtypes = vol + [ctypes.POINTER(View), _i, ctypes.c_int, _f, _f]
         L.orc_march.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, _f, ctypes.c_float, _f, _i, _i]
         L.orc_tile_gather.argtypes = vol + [ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _f, _i]
         L.orc_sample_t.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, _f, ctypes.c_float, _f]
         L.orc_gather_state.argtypes = vol + [ctypes.c_int, _f, _f, _f, _f, _f, ctypes.c_float, ctypes.c_float, _f]
         L.orc_basis.argtypes = [_f, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _f]
+        L.orc_pt_render_list.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, ctypes.POINTER(View), _i, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _d, _d]
         L.orc_pt_render.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, ctypes.POINTER(View), ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_float, _d, _d,
                                     ctypes.POINTER(ctypes.c_int64)]
@@ -250,11 +252,15 @@ class Scene:
         lib().orc_render_predict_rows(*self._vol(), ctypes.byref(view), int(p0), int(n), _p(out))
         return out
 
-    def render_predict_list(self, view, pix):
+    def render_predict_list(self, view, pix, want_feats=False):
+        """NNPredict<MRPNN> on the listed pixels: (n, 4) outputs, and with want_feats also their
+        (n, 577) features."""
         pix = np.ascontiguousarray(pix, np.int32)
         out = np.zeros((pix.size, 4), np.float32)
-        lib().orc_render_predict_list(*self._vol(), ctypes.byref(view), _p(pix, _i), int(pix.size), _p(out))
-        return out
+        feats = np.zeros((pix.size, NFEAT), np.float32) if want_feats else None
+        lib().orc_render_predict_list(*self._vol(), ctypes.byref(view), _p(pix, _i), int(pix.size), _p(out),
+                                      _p(feats))
+        return (out, feats) if want_feats else out
 
     @classmethod
     def from_arrays(cls, grid, N, max_density, mips, tr, lut):
@@ -290,6 +296,19 @@ class Scene:
         lib().orc_pt_render(_p(self.grid), self.N, self.max_density, _p(self.tr), ctypes.byref(view), spp,
                             max_bounce, nee_baked, tr_scale, _p(mean, _d), _p(m2, _d), ctypes.byref(taps))
         return mean, m2, taps.value
+
+
+def _pt_list(scene, view, pix, spp, max_bounce, nee_baked=0, tr_scale=1.0):
+    pix = np.ascontiguousarray(pix, np.int32)
+    mean = np.zeros((pix.size, 3), np.float64)
+    m2 = np.zeros((pix.size, 3), np.float64)
+    lib().orc_pt_render_list(_p(scene.grid), scene.N, scene.max_density, _p(scene.tr), ctypes.byref(view),
+                             _p(pix, _i), int(pix.size), spp, max_bounce, nee_baked, tr_scale, _p(mean, _d),
+                             _p(m2, _d))
+    return mean, m2
+
+
+Scene.pt_render_list = _pt_list
 
 
 def sample_t(grid, N, max_density, ori, d, alpha):

@@ -273,15 +273,19 @@ int orc_render_predict_rows(const float* grid, int N, float max_density, const f
     return 0;
 }
 
-// CPU baseline sample: the pixels listed in pix (row-major indices), OpenMP dynamic schedule.
+// CPU baseline sample: the pixels listed in pix (row-major indices), OpenMP dynamic schedule;
+// feats (optional): their 577 features.
 int orc_render_predict_list(const float* grid, int N, float max_density, const float* mips, const float* tr,
-                            const float* lut, const orc_view* ov, const int32_t* pix, int n, float* out4) {
+                            const float* lut, const orc_view* ov, const int32_t* pix, int n, float* out4,
+                            float* feats) {
     if (!g_net_ok) return 1;
     Volume v = to_volume(grid, N, max_density, mips, tr, lut);
     View view = to_view(ov);
     int W = view.width;
 #pragma omp parallel for schedule(dynamic, 4)
-    for (int q = 0; q < n; ++q) predict_pixel(v, g_net, view, pix[q] / W, pix[q] % W, out4 + 4 * (size_t)q);
+    for (int q = 0; q < n; ++q)
+        predict_pixel(v, g_net, view, pix[q] / W, pix[q] % W, out4 + 4 * (size_t)q,
+                      feats ? feats + (size_t)q * NFEAT : nullptr);
     return 0;
 }
 
@@ -327,6 +331,23 @@ int orc_pt_render(const float* grid, int N, float max_density, const float* tr, 
         taps += t;
     }
     if (taps_out) *taps_out = taps;
+    return 0;
+}
+
+// The path tracer on the listed pixels only (row-major indices): mean3 / m2_3 hold 3 doubles per
+// listed pixel.
+int orc_pt_render_list(const float* grid, int N, float max_density, const float* tr, const orc_view* ov,
+                       const int32_t* pix, int n, int spp, int max_bounce, int nee_baked, float tr_scale,
+                       double* mean3, double* m2_3) {
+    Volume v = to_volume(grid, N, max_density, nullptr, tr, nullptr);
+    View view = to_view(ov);
+    PTParams pp{spp, max_bounce, nee_baked, tr_scale};
+    const int W = view.width;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < n; ++q) {
+        long long t = 0;
+        pt_pixel(v, view, pp, pix[q] / W, pix[q] % W, mean3 + 3 * (size_t)q, m2_3 + 3 * (size_t)q, &t);
+    }
     return 0;
 }
 
