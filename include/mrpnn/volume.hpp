@@ -83,6 +83,14 @@ public:
     void Render(float3* target, Histogram* histo_buffer, unsigned int* target2, int2 size, float3 ori, float3 up,
                 float3 right, float3 lightDir, float3 lightColor = {1, 1, 1}, float alpha = 1, int multiScatter = 1,
                 float g = 0, int randseed = 0, RenderType rt = RenderType::PT, int toneType = 2, bool denoise = false);
+    // Training-data gather (core/volume.cu:630-708): size.x * size.y rows of MRP_TILE_ROW (773)
+    // floats into the caller's device buffer out_buffer, one per pixel of a size.x x size.y
+    // camera, each summarising an 8 x 8 grid of sub-rays (mrp_gather_tiles); pixelOffset shifts
+    // the grid by eighths of a pixel.  Updates the HG LUT and the TR mips for (g, lightDir, alpha)
+    // first.  Asynchronous on the library's stream, as the reference's kernel launch; randseed is
+    // unused, as in the reference.
+    void GatherData(float* out_buffer, int2 size, float3 ori, float3 up, float3 right, float3 lightDir,
+                    float alpha = 1, float g = 0, int2 pixelOffset = {0, 0}, int randseed = 0);
     void SetCheckboard(bool checkboard);
     void SetExposure(float exp);
     // Extension: network arithmetic of the interactive MRPNN frames (MRP_FP32 = the reference's,

@@ -12,6 +12,7 @@
 
 #include "../../include/mrpnn/camera.hpp"
 #include "../../include/mrpnn/volume.hpp"
+#include "tonemap.hpp"
 
 namespace mrpnn {
 
@@ -275,7 +276,36 @@ void VolumeRender::Render(float3* target, Histogram* histo_buffer, unsigned int*
     if (hipDeviceSynchronize() != hipSuccess) throw Error(MRP_ERR_HIP, "interactive frame failed");
 }
 
+void VolumeRender::GatherData(float* out_buffer, int2 size, float3 ori, float3 up, float3 right, float3 lightDir,
+                              float alpha, float g, int2 pixelOffset, int /*randseed*/) {
+    if (!out_buffer) throw Error(MRP_ERR_INVALID, "GatherData: null output buffer");
+    mrp_view v;
+    memset(&v, 0, sizeof v);
+    const float3 o[4] = {ori, up, right, lightDir};
+    float* dst[4] = {v.ori, v.up, v.right, v.light_dir};
+    for (int i = 0; i < 4; ++i) {
+        dst[i][0] = o[i].x;
+        dst[i][1] = o[i].y;
+        dst[i][2] = o[i].z;
+    }
+    v.light_color[0] = v.light_color[1] = v.light_color[2] = 1.0f;
+    v.scatter_rate[0] = scatter_rate_.x;
+    v.scatter_rate[1] = scatter_rate_.y;
+    v.scatter_rate[2] = scatter_rate_.z;
+    v.alpha = alpha;
+    v.g = g;
+    v.tan_half_fov = tan_half_fov_;
+    v.jitter = 0.5f;
+    v.width = size.x;
+    v.height = size.y;
+    check(mrp_gather_tiles(default_context(), vol_, &v, pixelOffset.x, pixelOffset.y, out_buffer));
+}
+
 // ------------------------------------------------------------------------------------------
+static float3 from_v3(mrp::v3 a) { return float3{a.x, a.y, a.z}; }
+float3 Camera::Gamma(float3 c) { return from_v3(mrp::gamma22(mrp::mk3(c.x, c.y, c.z))); }
+float3 Camera::ACES(float3 c) { return from_v3(mrp::aces(mrp::mk3(c.x, c.y, c.z))); }
+
 Camera::Camera(VolumeRender& vol, std::string nm, float3 position, int res) {
     volume = &vol;
     name = nm;
@@ -309,6 +339,29 @@ void Camera::Render(float3* target, Histogram* histo_buffer, unsigned int* targe
                     VolumeRender::RenderType rt, bool denoise) {  // camera.cu:259-261
     volume->Render(target, histo_buffer, target2, size, ori, up, right, lightDir, lightColor, alpha, multiScatter, g,
                    frame, rt, tone, denoise);
+}
+
+void Camera::GatherData(float* out_buffer, int2 size, float3 lightDir, float alpha, float g, int2 pixelOffset) {
+    volume->GatherData(out_buffer, size, ori, up, right, lightDir, alpha, g, pixelOffset);  // camera.cu:264-266
+}
+
+std::vector<float4> Camera::Render(float3 lightDir, float3 lightColor, float alpha, int multiScatter, float g,
+                                   int sampleNum, ToneMapFunction tone, VolumeRender::RenderType rt, float exp) {
+    std::vector<float4> img = volume->Render(int2{resolution, resolution}, ori, up, right, lightDir, rt, g, alpha,
+                                             lightColor, multiScatter, sampleNum);
+    std::vector<unsigned char> px((size_t)resolution * resolution * 4);
+    auto sat = [](float v) { return (unsigned char)((v < 0 ? 0 : (v > 1 ? 1 : v)) * 255); };
+    for (size_t i = 0; i < img.size(); ++i) {
+        const float3 t = tone(float3{img[i].x * exp, img[i].y * exp, img[i].z * exp});
+        px[4 * i] = sat(t.x);
+        px[4 * i + 1] = sat(t.y);
+        px[4 * i + 2] = sat(t.z);
+        px[4 * i + 3] = 255;
+    }
+    const char* affix = rt == VolumeRender::RenderType::PT ? "_PT" : (rt == VolumeRender::RenderType::RPNN ? "_RPNN" : "MRPNN");
+    if (!write_png_rgba(name + affix + ".png", resolution, resolution, px.data()))
+        throw Error(MRP_ERR_IO, "cannot write " + name + affix + ".png");
+    return img;
 }
 
 std::vector<float4> Camera::RenderToFile(std::string path, float3 lightDir, float3 lightColor, float alpha,

@@ -1,8 +1,12 @@
 // The reference's README flow (README.md:72-93 / main.cu) written against mrpnn::VolumeRender
 // and mrpnn::Camera: fill a grid with SetDatas, Update, render the predictor, the RPNN
-// comparison network and the path tracer to PNGs.  Usage: demo <out_dir> [resolution]
+// comparison network and the path tracer to PNGs, gather training rows into a device buffer as
+// main.cu:30-38 does, and check the class results against the C ABI calls they wrap.
+// Usage: demo <out_dir> [resolution]
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
 
 #include <string>
 #include <vector>
@@ -12,6 +16,37 @@
 #include "mrpnn/camera.hpp"
 
 using namespace mrpnn;
+
+// The camera's basis (protected in the reference's Camera, kept so here) for the C ABI views.
+struct BasisCamera : Camera {
+    using Camera::Camera;
+    void basis(float o[3], float u[3], float r[3]) const {
+        const float3 a[3] = {ori, up, right};
+        float* d[3] = {o, u, r};
+        for (int i = 0; i < 3; ++i) {
+            d[i][0] = a[i].x;
+            d[i][1] = a[i].y;
+            d[i][2] = a[i].z;
+        }
+    }
+};
+
+static mrp_view capi_view(const BasisCamera& cam, int w, int h, float3 sun, float alpha, float g, float sr) {
+    mrp_view v;
+    memset(&v, 0, sizeof v);
+    cam.basis(v.ori, v.up, v.right);
+    v.light_dir[0] = sun.x;
+    v.light_dir[1] = sun.y;
+    v.light_dir[2] = sun.z;
+    v.light_color[0] = v.light_color[1] = v.light_color[2] = 1.0f;
+    v.scatter_rate[0] = v.scatter_rate[1] = v.scatter_rate[2] = sr;
+    v.alpha = alpha;
+    v.g = g;
+    v.tan_half_fov = 1.0f;
+    v.width = w;
+    v.height = h;
+    return v;
+}
 
 int main(int argc, char** argv) {
     std::string dir = argc > 1 ? argv[1] : ".";
@@ -26,7 +61,7 @@ int main(int argc, char** argv) {
         v.max_density = 1.0f;
         v.Update();
         v.SetScatterRate(0.99f);
-        Camera cam(v, "demo", float3{1.2f, 0.6f, 1.2f}, 96);
+        BasisCamera cam(v, "demo", float3{1.2f, 0.6f, 1.2f}, 96);
         float3 sun = {0.5f, 0.8f, 0.33f};
         float l = sqrtf(sun.x * sun.x + sun.y * sun.y + sun.z * sun.z);
         sun = float3{sun.x / l, sun.y / l, sun.z / l};
@@ -77,9 +112,52 @@ int main(int argc, char** argv) {
         int lit_pred = 0;
         for (unsigned int p : px) lit_pred += (p & 0xffffffu) != 0 && (p >> 24) == 0xffu;
         if (hipFree(accum) != hipSuccess || hipFree(histo) != hipSuccess || hipFree(rgba) != hipSuccess) return 4;
-        printf("active %d mean_pred %.6f mean_pt %.6f lit_pt %d lit_pred %d active_rpnn %d mean_rpnn %.6f\n", act,
-               s / img.size(), sp / pt.size(), lit_pt, lit_pred, act_rp, srp / rp.size());
-        return act > 0 ? 0 : 3;
+        // the same MRPNN frame through the C ABI the class wraps (first Render call: frame 0,
+        // fresh jitter stream, seed 0): RenderToFile must return it bit for bit
+        mrp_view pv = capi_view(cam, 96, 96, sun, 32.0f, 0.857f, 0.99f);
+        pv.jitter = -1.0f;
+        std::vector<float4> capi(img.size());
+        if (mrp_render_plain(default_context(), v.handle(), &pv, 1, (float*)capi.data()) != MRP_OK) return 5;
+        const int render_equal = memcmp(capi.data(), img.data(), img.size() * sizeof(float4)) == 0;
+        // main.cu:30-38: training rows of a resolution/8 camera into a device buffer
+        const int T = cam.resolution / 8;
+        const size_t row_bytes = (192 * 3 + 5 + 64 * 3) * sizeof(float);
+        float* d_buffer = nullptr;
+        float* d_ref = nullptr;
+        if (hipMalloc((void**)&d_buffer, T * T * row_bytes) != hipSuccess ||
+            hipMalloc((void**)&d_ref, T * T * row_bytes) != hipSuccess)
+            return 4;
+        int gather_equal = 1, gather_active = 0;
+        const int offsets[2][2] = {{0, 0}, {3, 5}};
+        for (const auto& o : offsets) {
+            cam.GatherData(d_buffer, int2{T, T}, sun, 32.0f, 0.857f, int2{o[0], o[1]});
+            mrp_view gv = capi_view(cam, T, T, sun, 32.0f, 0.857f, 0.99f);
+            gv.jitter = 0.5f;
+            if (mrp_gather_tiles(default_context(), v.handle(), &gv, o[0], o[1], d_ref) != MRP_OK) return 5;
+            std::vector<float> a(T * T * row_bytes / 4), b(a.size());
+            if (hipDeviceSynchronize() != hipSuccess ||
+                hipMemcpy(a.data(), d_buffer, T * T * row_bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(b.data(), d_ref, T * T * row_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+                return 4;
+            gather_equal &= memcmp(a.data(), b.data(), a.size() * 4) == 0;
+            for (int r = 0; r < T * T; ++r) gather_active += a[(size_t)r * 773] != -1000.0f;
+        }
+        if (hipFree(d_buffer) != hipSuccess || hipFree(d_ref) != hipSuccess) return 4;
+        // the tone maps on a few colours, for comparison with the display path
+        const float3 cs[3] = {{0.5f, 0.2f, 0.1f}, {2.0f, 1.0f, 0.25f}, {0.01f, 0.03f, 0.02f}};
+        printf("tones");
+        for (const float3& c : cs) {
+            const float3 ga = Camera::Gamma(c), ac = Camera::ACES(c);
+            printf(" %.9g %.9g %.9g %.9g %.9g %.9g", ga.x, ga.y, ga.z, ac.x, ac.y, ac.z);
+        }
+        printf("\n");
+        // the per-frame Render of the camera: name + "MRPNN.png" with the ACES tone map
+        cam.name = dir + "/demo_cam";
+        cam.Render(sun, float3{1, 1, 1}, 32.0f, 512, 0.857f, 1, Camera::ACES, VolumeRender::MRPNN);
+        printf("active %d mean_pred %.6f mean_pt %.6f lit_pt %d lit_pred %d active_rpnn %d mean_rpnn %.6f "
+               "render_equal %d gather_equal %d gather_active %d\n", act, s / img.size(), sp / pt.size(), lit_pt,
+               lit_pred, act_rp, srp / rp.size(), render_equal, gather_equal, gather_active);
+        return act > 0 && render_equal && gather_equal ? 0 : 3;
     } catch (const Error& e) {
         fprintf(stderr, "mrpnn error %d: %s\n", e.code, e.what());
         return 2;

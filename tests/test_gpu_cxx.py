@@ -1,5 +1,8 @@
 """The C++ drop-in API (mrpnn::VolumeRender / mrpnn::Camera) end to end on the GPU: the
-reference's README flow renders predictor and path-traced PNGs."""
+reference's README flow renders predictor and path-traced PNGs; Camera::RenderToFile(..., MRPNN)
+equals mrp_render_plain of the same view bit for bit; Camera::GatherData into a device buffer
+(main.cu:30-38) equals mrp_gather_tiles bit for bit (two pixel offsets); Camera::Gamma /
+Camera::ACES agree with the oracle's display-path tone maps."""
 import os
 import subprocess
 
@@ -27,6 +30,16 @@ def test_cxx_demo_renders_pngs(tmp_path):
     # RPNN frame of the same camera (its own sub-pixel jitter): about the same active pixels, lit
     active_rpnn = int(words[words.index("active_rpnn") + 1])
     assert abs(active_rpnn - active) <= 0.05 * active and float(words[words.index("mean_rpnn") + 1]) > 0
-    for name in ("demo_mrpnn.png", "demo_rpnn.png", "demo_pt.png"):
+    assert int(words[words.index("render_equal") + 1]) == 1 and int(words[words.index("gather_equal") + 1]) == 1
+    assert int(words[words.index("gather_active") + 1]) > 20  # the 12x12 camera sees the ball
+    import numpy as np
+    import oracle as O
+    tones = np.array([float(x) for x in r.stdout.split("tones")[1].split("\n")[0].split()]).reshape(3, 2, 3)
+    cols = np.array([[0.5, 0.2, 0.1], [2.0, 1.0, 0.25], [0.01, 0.03, 0.02]], np.float32)
+    _, ga = O.tonemap(cols, 1.0, 1)
+    _, ac = O.tonemap(cols, 1.0, 2)
+    np.testing.assert_allclose(tones[:, 0], ga, rtol=2e-6)
+    np.testing.assert_allclose(tones[:, 1], ac, rtol=2e-6)
+    for name in ("demo_mrpnn.png", "demo_rpnn.png", "demo_pt.png", "demo_camMRPNN.png"):
         data = open(tmp_path / name, "rb").read()
         assert data[:8] == b"\x89PNG\r\n\x1a\n" and len(data) > 96 * 96 * 4
