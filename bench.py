@@ -58,7 +58,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=4)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device-to-device (scale reference for HBM counters)")
-    ap.add_argument("--config", default="headline", choices=["headline", "c1", "c2", "c3", "c4", "c5", "tile", "interactive", "ingest", "rpnn"],
+    ap.add_argument("--config", default="headline", choices=["headline", "c1", "c2", "c3", "c4", "c5", "c5sweep", "tile", "interactive", "ingest",
+                             "rpnn"],
                     help="headline = the metric line; c1..c5, tile, interactive, ingest, rpnn = the other workloads (one JSON line each)")
     ap.add_argument("--pt-res", type=int, default=1024, help="c3: image side")
     ap.add_argument("--pt-spp", type=int, default=1024, help="c3: samples per pixel")
@@ -78,10 +79,6 @@ def main():
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda")
-    if args.config != "headline":
-        import bench_configs
-        print(json.dumps(getattr(bench_configs, args.config)(args)))
-        return
     if args.copy_probe:
         src = torch.ones(256 * 1024 * 1024, device=dev)
         dst = torch.empty_like(src)
@@ -89,6 +86,10 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log("copy probe: 1 GiB read + 1 GiB written")
+    if args.config != "headline":
+        import bench_configs
+        print(json.dumps(getattr(bench_configs, args.config)(args)))
+        return
     ctx = M.Context()
     ctx.set_timing(True)
     # one cached TR set, as the reference keeps (volume.cu:1592): the sun moves every frame,

@@ -72,6 +72,43 @@ def _band(pred, mean, m2, n):
     return float(np.mean(np.abs(pred[:, 0] - mean[:, 0]) <= 3.0 * se))
 
 
+def _judge(pred, mean, m2, n, W, H, taps=None):
+    """The per-pixel 3-sigma judge of the predictor against the delta-tracking image, with where
+    the failing pixels sit: (a) pixels the predictor leaves as sky (no first-scattering point:
+    survival never drops below 0.5 along the ray) while the path tracer sees light; (b) among
+    predicted pixels, by radiance decile of the path-traced mean, by distance (pixels) to the
+    predictor's silhouette, and by march steps to the hit; (c) the signed relative bias."""
+    from scipy import ndimage
+    p, mu = np.asarray(pred, np.float64)[:, 0], np.asarray(mean, np.float64)[:, 0]
+    se = np.sqrt(np.maximum(np.asarray(m2, np.float64)[:, 0], 0.0) / (n * (n - 1)))
+    act = np.asarray(pred)[:, 3] > 0
+    ok = np.abs(p - mu) <= 3.0 * se
+    lit = mu > 1e-4 * mu.max()
+    res = {"pixels": int(p.size), "pass_all": round(float(ok.mean()), 4),
+           "sky_in_predictor_but_lit": int((~act & lit & ~ok).sum()),
+           "sky_in_predictor_fail_share": round(float((~act & ~ok).sum() / max((~ok).sum(), 1)), 4),
+           "pass_predicted": round(float(ok[act].mean()), 4) if act.any() else None}
+    a = act & lit
+    rel = (p[a] - mu[a]) / mu[a]
+    res["predicted_bias"] = {"mean_signed_rel": round(float(rel.mean()), 4),
+                             "mean_abs_rel": round(float(np.abs(rel).mean()), 4),
+                             "median_se_rel": round(float(np.median(se[a] / mu[a])), 4)}
+    dec = np.quantile(mu[a], np.linspace(0, 1, 11))
+    idx = np.clip(np.searchsorted(dec, mu[a], side="right") - 1, 0, 9)
+    res["fail_by_radiance_decile"] = [round(float(1 - ok[a][idx == k].mean()), 3) for k in range(10)]
+    res["signed_rel_by_radiance_decile"] = [round(float(rel[idx == k].mean()), 3) for k in range(10)]
+    dist = ndimage.distance_transform_edt(act.reshape(H, W)).ravel()[a]
+    bands = [(1, 2), (2, 4), (4, 8), (8, 16), (16, 1e9)]
+    res["fail_by_silhouette_distance_px"] = {"%g-%g" % b: round(float(1 - ok[a][(dist >= b[0]) & (dist < b[1])].mean()), 3)
+                                            for b in bands if ((dist >= b[0]) & (dist < b[1])).any()}
+    if taps is not None:
+        tp = np.asarray(taps)[a]
+        q = np.quantile(tp, [0, 0.25, 0.5, 0.75, 1.0])
+        res["fail_by_march_steps_quartile"] = [round(float(1 - ok[a][(tp >= q[k]) & (tp <= q[k + 1])].mean()), 3)
+                                               for k in range(4)]
+    return res
+
+
 def _rel(a, b, floor_frac=1e-3):
     """Mean and p99 relative error of a against b over pixels brighter than floor_frac * max."""
     a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
@@ -135,6 +172,7 @@ def c1(args):
         "pt": {"spp": 256, "max_bounce": 64, "seconds": round(pt_s, 3),
                "fp32_in_3sigma": _band(res["fp32_plain"]["img"], mean, m2, 256),
                "bf16_in_3sigma": _band(res["bf16"]["img"], mean, m2, 256),
+               "judge_fp32": _judge(res["fp32_plain"]["img"], mean, m2, 256, 256, 256),
                "fp32_vs_pt": _rel(res["fp32_plain"]["img"][:, 0], mean[:, 0])},
     }
 
@@ -226,6 +264,8 @@ def c3(args):
             "samples_per_s": round(R * R * spp / dt, 1),
             "fp32_in_3sigma": _band(preds["fp32"], mean, m2, spp),
             "bf16_in_3sigma": _band(preds["bf16"], mean, m2, spp),
+            "judge_fp32": _judge(preds["fp32"], mean, m2, spp, R, R),
+            "judge_bf16": _judge(preds["bf16"], mean, m2, spp, R, R),
             "fp32_vs_pt": _rel(preds["fp32"][:, 0], mean[:, 0]),
             "bf16_vs_pt": _rel(preds["bf16"][:, 0], mean[:, 0])}
 
@@ -330,6 +370,67 @@ def c5(args):
         srv.close()
     ctx.close()
     return res
+
+
+def c5sweep(args):
+    """C5's Poisson load at offered rates from 100 to 2000 requests/s (BF16; windows long enough
+    for about 400 requests): Mpix/s, p50 and p99 latency per rate, the highest rate whose p99
+    stays within 16 ms, and the per-request device cost that bounds the rate."""
+    import torch
+
+    import mrpnn_amd as M
+    ctx = M.Context()
+    ctx.set_option(M.OPT_TR_SLOTS, 128)
+    sizes = [(512, 512), (1280, 720), (1920, 1080)]
+    rng = np.random.default_rng(5)
+    specs = [(int(rng.choice([128, 256, 384, 512])), 100 + k) for k in range(32)]
+    srv = M.Server(ctx, M.BF16, max_batch_views=32, max_batch_pixels=8 << 20, max_inflight=2, g=G)
+    ids = [srv.add_fbm_volume(N, 6, 4.0, seed) for N, seed in specs]
+    tiny = torch.zeros((1, 4), device="cuda")
+    for vid in ids:
+        srv.submit(vid, M.orbit_view(1, 1, g=G), tiny)
+    srv.drain()
+    srv.poll(64)
+    rows = []
+    for rate in (100.0, 200.0, 300.0, 400.0, 500.0, 600.0, 700.0, 800.0, 1000.0, 1500.0, 2000.0):
+        window = max(0.5, 400.0 / rate)
+        rep = srv.run_poisson(rate, window, sizes, seed=int(rate), buffers_per_size=48)
+        row = {"offered_rps": rate, "window_s": round(window, 2), "requests": rep["requests"],
+               "served_rps": round(rep["completed"] / rep["wall_s"], 1), "mpix_s": round(rep["mpix_s"], 1),
+               "p50_ms": round(rep["p50_ms"], 2), "p99_ms": round(rep["p99_ms"], 2), "max_ms": round(rep["max_ms"], 2)}
+        _log("c5sweep: %s" % row)
+        rows.append(row)
+    srv.close()
+    ok = [r["offered_rps"] for r in rows if r["p99_ms"] <= 16.0]
+    # per-request device cost of the mix: one TR bake (a new sun) plus the network over its pixels
+    vol = M.Volume(ctx, 512)
+    vol.fill_fbm(6, 4.0, 1)
+    vol.update()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    cost = {}
+    for w, h in sizes:
+        outs = torch.zeros((w * h, 4), device="cuda")
+        ts = []
+        for i in range(8):
+            sun = (0.3 + 0.05 * i, 0.8, 0.2)
+            v = M.orbit_view(w, h, dist=2.0, azimuth=0.5, elevation=0.2, fov_deg=40.0, light=sun, alpha=64.0, g=G,
+                             albedo=0.99, seed=1)
+            torch.cuda.synchronize()
+            ev[0].record()
+            ctx.prepare_views([vol], [v])
+            ctx.render_batch([vol], [v], [outs], M.BF16)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        cost["%dx%d" % (w, h)] = round(float(np.median(ts[2:])), 3)
+    vol.close()
+    ctx.close()
+    mean_ms = float(np.mean(list(cost.values())))
+    return {"config": "c5sweep", "workload": "C5 Poisson load over 32 fBm volumes (128^3..512^3), sizes {512^2, "
+                                             "1280x720, 1920x1080}, BF16, one sun per request",
+            "rates": rows, "max_rate_p99_le_16ms": max(ok) if ok else None,
+            "request_ms_with_bake": cost, "mix_mean_request_ms": round(mean_ms, 3),
+            "device_bound_rps": round(1000.0 / mean_ms, 1)}
 
 
 def tile(args):
