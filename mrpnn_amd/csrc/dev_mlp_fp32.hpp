@@ -4,13 +4,15 @@
 //   * untemplated LinearReLU / *_Unbias / LinearSm_Unbias: zero start, bias last (:41-86);
 //   * every multiply-accumulate is one fmaf (nvcc contracts `Out += In * W`);
 //   * AVGMAX: sequential fp32 sum divided by n, max from -10000 (:21-33);
-//   * sigmoid as 1.0/(1.0+exp(-x)) in double (:84); exp and pow rounded once from FP64.
+//   * sigmoid as 1/(1+exp(-x)) (:84), exp(P) of the output and scatterrate^4 in single precision,
+//     exp from fexp.h (no double-precision arithmetic).
 // Weights are the fp32 asset in header order, addressed by the offsets in net_layout.h;
 // their addresses are wavefront-uniform, so loads go through the scalar cache.
 // This is the drop-in baseline and the FP32 parity target; the fast path lives in the
 // fused MFMA kernels.
 #pragma once
 #include "dev_common.hpp"
+#include "fexp.h"
 #include "net_layout.h"
 
 namespace mrp {
@@ -24,10 +26,10 @@ struct Fp32Mlp {
     float pool[75], aw[16], awp[6], tap[8], comb[128];
 
     __device__ static float relu(float x) { return x >= 0.0f ? x : 0.0f; }
-    // exp / pow are evaluated in FP64 and rounded once (correctly rounded): device and host libm
-    // differ by an ulp in fp32, which exp(P)-1 amplifies past 1e-5 for small outputs.
-    __device__ static float sigm(float x) { return (float)(1.0 / (1.0 + exp(-(double)x))); }
-    __device__ static float expr(float x) { return (float)exp((double)x); }
+    // exp is exp_f32 (fexp.h), the same float operations as the CPU restatement: device and host
+    // libm expf differ by an ulp, which exp(P)-1 amplifies past 1e-5 for small outputs.
+    __device__ static float sigm(float x) { return 1.0f / (1.0f + exp_f32(-x)); }
+    __device__ static float expr(float x) { return exp_f32(x); }
 
     __device__ void lin_relu_bf(const float* in, float* out, int from, int to, int w, int b) {
         for (int i = 0; i < to; ++i) {
@@ -133,7 +135,7 @@ struct Fp32Mlp {
         const int DI = 160, P_DI = 24;
         tap[6] = g;
         tap[7] = gamma;
-        float srp[3] = {(float)pow((double)sr.x, 4.0), (float)pow((double)sr.y, 4.0), (float)pow((double)sr.z, 4.0)};
+        float srp[3] = {(sr.x * sr.x) * (sr.x * sr.x), (sr.y * sr.y) * (sr.y * sr.y), (sr.z * sr.z) * (sr.z * sr.z)};
         pool[72] = g;
         pool[73] = gamma;
         block(-1, 0, 0, 8, 0, LSE01W, LSE02W, L01W, L01B, L_Tr01W, L_Tr01B, L_Hg01W, L_Hg01B);

@@ -23,6 +23,7 @@
 #include "../../include/mrp_c_api.h"
 #include "capi_internal.h"
 #include "dev_common.hpp"
+#include "fexp.h"
 
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
@@ -182,7 +183,7 @@ enum RpnnEpi {
     kEpiAddRelu = 2,     // relu(acc + add)                    V_l A + (W_l C + b)
     kEpiBiasRelu = 3,    // relu(acc + b)                      U_0 B + b, F0, F1
     kEpiResBiasRelu = 4, // relu(add + (acc + b)), in place    C + (U_l B + b)
-    kEpiOut = 5          // max(exp(relu(acc + b)) - 1, 0)     F2 (exp in double, rounded once)
+    kEpiOut = 5          // max(exp(relu(acc + b)) - 1, 0)     F2 (exp_f32 of fexp.h)
 };
 constexpr int kGT = 64, kGK = 32;
 
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(256) void k_rpnn_gemm(const float* __restrict__ A, 
             } else {
                 v = v + bias[o];
                 v = v >= 0.0f ? v : 0.0f;
-                const float e = (float)exp((double)v) - 1.0f;
+                const float e = exp_f32(v) - 1.0f;
                 v = e > 0.0f ? e : 0.0f;
             }
             out[idx] = v;

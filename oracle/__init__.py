@@ -53,6 +53,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.orc_load_weights.argtypes = [_f, ctypes.c_int]
         L.orc_philox.argtypes = [_u, ctypes.c_uint32, ctypes.c_uint32, _u]
+        L.orc_exp_f32.argtypes = [_f, ctypes.c_int, _f]
         L.orc_u01.argtypes = [ctypes.c_uint32]
         L.orc_u01.restype = ctypes.c_float
         L.orc_halton.argtypes = [ctypes.c_uint, ctypes.c_uint]
@@ -133,6 +134,14 @@ def philox(ctr, key):
 
 def u01(x):
     return lib().orc_u01(int(x))
+
+
+def exp_f32(x):
+    """The FP32 networks' exp (mrpnn_amd/csrc/fexp.h) on an array."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib().orc_exp_f32(_p(x), x.size, _p(out))
+    return out
 
 
 def halton(dim, index):

@@ -77,13 +77,14 @@ const float* Net::get(const char* name) const {
 // T = double is the FP64 evaluation of the same network.
 // ---------------------------------------------------------------------------------------
 template <class T> static inline T sigm(T x);
-// 1.0/(1.0+exp(-x)) in double (radiancePredict.cu:84).  exp and pow of the fp32 network are
-// evaluated in FP64 and rounded once, i.e. correctly rounded, so host libm and device math
-// agree exactly (the reference's -use_fast_math versions are not reproducible anyway).
-template <> inline float sigm<float>(float x) { return (float)(1.0 / (1.0 + exp(-(double)x))); }
+// 1/(1+exp(-x)) (radiancePredict.cu:84).  exp of the fp32 network is exp_f32 of fexp.h, the same
+// single-precision operations as the device, so host and device agree exactly without double
+// arithmetic (the reference's -use_fast_math versions are not reproducible anyway); the FP64
+// network uses libm exp.
+template <> inline float sigm<float>(float x) { return 1.0f / (1.0f + mrp::exp_f32(-x)); }
 template <> inline double sigm<double>(double x) { return 1.0 / (1.0 + exp(-x)); }
 template <class T> static inline T expT(T x);
-template <> inline float expT<float>(float x) { return (float)exp((double)x); }
+template <> inline float expT<float>(float x) { return mrp::exp_f32(x); }
 template <> inline double expT<double>(double x) { return exp(x); }
 template <class T> static inline T relu(T x) { return x >= (T)0 ? x : (T)0; }
 
@@ -210,7 +211,9 @@ struct Mlp {
 };
 
 template <class T> static inline T powT(T a, T b);
-template <> inline float powT<float>(float a, float b) { return (float)pow((double)a, (double)b); }
+template <> inline float powT<float>(float a, float b) {  // the only use: scatterrate^4
+    return b == 4.0f ? (a * a) * (a * a) : powf(a, b);
+}
 template <> inline double powT<double>(double a, double b) { return pow(a, b); }
 
 template <class T>

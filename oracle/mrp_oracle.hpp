@@ -38,6 +38,7 @@
 
 #include <vector>
 
+#include "../mrpnn_amd/csrc/fexp.h"
 #include "../mrpnn_amd/csrc/philox.h"
 #include "../mrpnn_amd/csrc/ref_tables.h"
 

@@ -115,6 +115,11 @@ void orc_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]
 
 float orc_u01(uint32_t x) { return mrp::u01(x); }
 
+// exp_f32 (fexp.h), the single-precision exp of the FP32 networks, on n arguments.
+void orc_exp_f32(const float* x, int n, float* out) {
+    for (int i = 0; i < n; ++i) out[i] = mrp::exp_f32(x[i]);
+}
+
 float orc_halton(unsigned dim, unsigned index) { return halton_scrambled(dim, index); }
 
 void orc_stencil(int index, float* offset, int* mip, int* type, int* localindex) {

@@ -19,6 +19,11 @@
 using namespace oracle;
 
 namespace {
+// exp of the FP32 network: exp_f32 (fexp.h, the device's operations); of the FP64 network: libm
+template <class T> inline T exp_of(T x);
+template <> inline float exp_of<float>(float x) { return mrp::exp_f32(x); }
+template <> inline double exp_of<double>(double x) { return exp(x); }
+
 
 constexpr int L = 10, S = 225, NIN = 226, B = 200;
 std::vector<float> g_rw;  // the asset in header order
@@ -103,7 +108,7 @@ T rpnn_net(const Layout& w, const float* desc /* 2250 */, float gamma) {
     T y;
     linear<T>(C, &y, B, 1, w.F2, w.F2b);
     relu<T>(&y, 1);  // LinearReLU for F2 as well (radiancePredict.cu:716)
-    T o = (T)exp((double)y) - (T)1;
+    T o = exp_of<T>(y) - (T)1;
     return o > (T)0 ? o : (T)0;
 }
 
