@@ -36,8 +36,8 @@ extern "C" {
 #define MRP_FP32 0    /* scalar fp32, the reference's arithmetic */
 #define MRP_BF16 1    /* bf16 MFMA, fp32 accumulation; the tail layers after LC0 (LC1, LC2, LX, LX0..LX12)
                          with weights as two bf16 terms (hi + lo) */
-#define MRP_FP8 2     /* fp8 e4m3 MFMA, per-layer max-abs weight scales, per-pixel activation
-                         scales, fp32 accumulation */
+#define MRP_FP8 2     /* fp8 e4m3 MFMA, per-layer max-abs scales of weights and of activations
+                         (calibrated, see mrp_fp8_calibrate), fp32 accumulation */
 #define MRP_BF16_1T 3 /* bf16 MFMA with single-term bf16 weights (fastest, least accurate) */
 
 /* Context options (mrp_ctx_set_option).  No counterpart in the reference, whose launch
@@ -214,6 +214,27 @@ int mrp_gather_features(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, flo
                         float* dev_pos3, int32_t* dev_taps);
 int mrp_predict_features(mrp_ctx* ctx, const float* dev_feats, int n, const float scatterrate[3], float g, int precision,
                          float* dev_rgb);
+/* FP8 activation scales (no counterpart in the reference, whose networks run in FP32).  The
+ * MRP_FP8 network stores the inputs of each of its MRP_FP8_LAYERS MFMA layers (the 27 trunk
+ * layers L01..L81 of the three branches, the 12 DI layers, LD41 / LD_Tr41 / LD_Hg41, LSEFin2,
+ * LC0, LC1, LC2, LX, LX0..LX11, LX12, in that order) as e4m3 values of clamp(x, +-448*2^-e) * 2^e
+ * with one exponent e per layer: max-abs scaling, where the largest |input| the layer saw in
+ * calibration maps to at most 448, e4m3's largest finite value.  A context starts with the
+ * exponents calibrated on seeded fBm clouds that ship with the library.
+ * mrp_fp8_calibrate: runs the BF16 network over n feature rows (577 floats each, the
+ *   mrp_predict_features format) recording per layer the largest |input|; reset != 0 clears the
+ *   recorded maxima first.  Installs the exponents of all maxima recorded since the last reset
+ *   (the largest e with max * 2^e <= 448) and, when maxima is not null, writes the
+ *   MRP_FP8_LAYERS maxima there.  Blocking; it orders itself after earlier work on the context's
+ *   stream, and work issued on other streams that reads the exponents must not overlap it.
+ * mrp_fp8_get_exponents / mrp_fp8_set_exponents: the MRP_FP8_LAYERS exponents (n must be
+ *   MRP_FP8_LAYERS; each in [-100, 100]).  Setting is blocking like calibration. */
+#define MRP_FP8_LAYERS 60
+int mrp_fp8_calibrate(mrp_ctx* ctx, const float* dev_feats, int n, const float scatterrate[3], float g, int reset,
+                      float* maxima);
+int mrp_fp8_get_exponents(mrp_ctx* ctx, int* exps, int n);
+int mrp_fp8_set_exponents(mrp_ctx* ctx, const int* exps, int n);
+
 /* Tooling (no counterpart in the reference): the 577 features of each pixel as the fused MFMA
  * kernels gather them by default (fast gather: zero-padded pyramid copies, FMA lerps, hardware
  * log), for feature-by-feature checks against mrp_gather_features.  Prepares the view's LUT and

@@ -16,6 +16,7 @@ FP32, BF16, FP8, BF16_1T = 0, 1, 2, 3
 PRECISIONS = {"fp32": FP32, "bf16": BF16, "fp8": FP8, "bf16_1t": BF16_1T}
 OPT_MFMA_WAVES_PER_CU, OPT_EXACT_GATHER, OPT_TR_SLOTS, OPT_GRAPH, OPT_LDS_WEIGHTS, OPT_RPNN_CHUNK = 1, 2, 3, 4, 5, 6
 NFEAT = 577
+FP8_LAYERS = 60  # MFMA layers with an FP8 activation exponent (MRP_FP8_LAYERS)
 TILE_ROW = 773  # tile feature gather row (mrp_gather_tiles)
 MIP_TOTAL = sum((256 >> m) ** 3 for m in range(9))
 TR_TOTAL = sum((128 >> m) ** 3 for m in range(8))
@@ -141,6 +142,9 @@ def lib():
         L.mrp_gather_tiles.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, _vp]
         L.mrp_gather_features_fast.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp]
         L.mrp_predict_features.argtypes = [_vp, _vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.c_int, _vp]
+        L.mrp_fp8_calibrate.argtypes = [_vp, _vp, ctypes.c_int, _fp, ctypes.c_float, ctypes.c_int, _fp]
+        L.mrp_fp8_get_exponents.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.mrp_fp8_set_exponents.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.mrp_pt_render.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_float, _vp, _vp]
         L.mrp_volume_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(_fp), ctypes.POINTER(ctypes.c_int),
@@ -275,6 +279,24 @@ class Context:
         srv = (ctypes.c_float * 3)(*[float(x) for x in sr])
         check(lib().mrp_predict_features(self.h, _ptr(feats), n, srv, float(g), int(precision), _ptr(out)))
         return out
+
+    def fp8_calibrate(self, feats, sr, g, reset=True):
+        """mrp_fp8_calibrate over feature rows (torch float32 [n, 577] on the GPU, or None to
+        only re-derive the exponents); returns the per-layer input maxima (numpy, 60)."""
+        srv = (ctypes.c_float * 3)(*[float(x) for x in sr])
+        mx = (ctypes.c_float * FP8_LAYERS)()
+        n = 0 if feats is None else feats.shape[0]
+        check(lib().mrp_fp8_calibrate(self.h, None if feats is None else _ptr(feats), n, srv, float(g), int(reset), mx))
+        return np.frombuffer(mx, np.float32).copy()
+
+    def fp8_exponents(self):
+        e = (ctypes.c_int * FP8_LAYERS)()
+        check(lib().mrp_fp8_get_exponents(self.h, e, FP8_LAYERS))
+        return np.frombuffer(e, np.int32).copy()
+
+    def set_fp8_exponents(self, exps):
+        e = (ctypes.c_int * FP8_LAYERS)(*[int(x) for x in exps])
+        check(lib().mrp_fp8_set_exponents(self.h, e, len(exps)))
 
     def render_batch(self, volumes, views, outs, precision=BF16):
         vh = (_vp * len(volumes))(*[v.h for v in volumes])

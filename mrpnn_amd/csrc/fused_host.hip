@@ -9,6 +9,7 @@
 #include "capi_internal.h"
 #include "fused.h"
 #include "mlp_mfma.h"
+#include "fp8_scales.h"
 #include "kernels.h"
 
 using namespace mrp;
@@ -28,6 +29,8 @@ struct FusedState {
     const unsigned* count_src = nullptr;  // counters of the last batch (context or no-alloc buffer)
     const unsigned long long* taps_src = nullptr;
     MfmaNet net;  // packed BF16 / FP8 weights
+    int fp8_wexp[ML_COUNT] = {};  // FP8 weight and activation exponents of the layers
+    int fp8_aexp[ML_COUNT] = {};
     int mfma_waves_per_cu = 12;  // three 256-thread blocks per CU (three waves per SIMD)
     bool lds_weights = true;     // one 768-thread block per CU with the weights in LDS
     // graph mode: the batch (staging copies, counter reset, march, network) is captured once
@@ -80,8 +83,9 @@ int mrp_fused_init(mrp_ctx* c) {
         return mrp_fail(MRP_ERR_HIP, "event creation failed");
     if (hipStreamCreateWithFlags(&f->aux, hipStreamNonBlocking) != hipSuccess)
         return mrp_fail(MRP_ERR_HIP, "stream creation failed");
-    int rc = mfma_net_create(c->weights_host.data(), &f->net);
+    int rc = mfma_net_create(c->weights_host.data(), &f->net, f->fp8_wexp);
     if (rc) return rc;
+    memcpy(f->fp8_aexp, kFp8ActExp, sizeof f->fp8_aexp);
     return MRP_OK;
 }
 
@@ -484,6 +488,62 @@ int mrp_fused_predict_features(mrp_ctx* c, const float* dev_feats, int n, const 
     hipError_t e = launch_mlp_mfma_features(precision, c->fused->net, c->d_weights, dev_feats, n, sr, g, dev_rgb,
                                             c->stream);
     if (e != hipSuccess) return mrp_fail(MRP_ERR_HIP, "mfma predictor: %s", hipGetErrorString(e));
+    return MRP_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP8 activation scales
+// ------------------------------------------------------------------------------------------
+static_assert(MRP_FP8_LAYERS == ML_COUNT, "FP8 layer count of the C ABI");
+
+extern "C" int mrp_fp8_calibrate(mrp_ctx* c, const float* dev_feats, int n, const float sr[3], float g, int reset,
+                                 float* maxima) {
+    if (!c || !sr || (n > 0 && !dev_feats)) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    if (n < 0) return mrp_fail(MRP_ERR_INVALID, "n must be >= 0");
+    FusedState* f = c->fused;
+    const MfmaNet& net = f->net;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "stream sync failed");
+    if (reset && hipMemsetAsync(mfma_cal_words(net), 0, ML_COUNT * 4, c->stream) != hipSuccess)
+        return mrp_fail(MRP_ERR_HIP, "clearing the calibration maxima failed");
+    if (n > 0) {
+        hipError_t e = launch_mlp_mfma_calibrate(net, c->d_weights, dev_feats, n, sr, g, c->stream);
+        if (e != hipSuccess) return mrp_fail(MRP_ERR_HIP, "calibration kernel: %s", hipGetErrorString(e));
+    }
+    unsigned bits[ML_COUNT];
+    if (hipMemcpyAsync(bits, mfma_cal_words(net), sizeof bits, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return mrp_fail(MRP_ERR_HIP, "reading the calibration maxima failed");
+    int exps[ML_COUNT];
+    for (int id = 0; id < ML_COUNT; ++id) {
+        float m;
+        memcpy(&m, &bits[id], 4);
+        if (!(m >= 0.0f) || !isfinite(m)) return mrp_fail(MRP_ERR_STATE, "layer %d: non-finite activation maximum", id);
+        exps[id] = fp8_exp_for_max(m);
+        if (maxima) maxima[id] = m;
+    }
+    int rc = mfma_net_set_fp8_exps(net, f->fp8_wexp, exps);
+    if (rc) return rc;
+    memcpy(f->fp8_aexp, exps, sizeof exps);
+    return MRP_OK;
+}
+
+extern "C" int mrp_fp8_get_exponents(mrp_ctx* c, int* exps, int n) {
+    if (!c || !exps) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    if (n != ML_COUNT) return mrp_fail(MRP_ERR_INVALID, "n must be MRP_FP8_LAYERS (%d)", ML_COUNT);
+    memcpy(exps, c->fused->fp8_aexp, sizeof(int) * ML_COUNT);
+    return MRP_OK;
+}
+
+extern "C" int mrp_fp8_set_exponents(mrp_ctx* c, const int* exps, int n) {
+    if (!c || !exps) return mrp_fail(MRP_ERR_INVALID, "null argument");
+    if (n != ML_COUNT) return mrp_fail(MRP_ERR_INVALID, "n must be MRP_FP8_LAYERS (%d)", ML_COUNT);
+    for (int id = 0; id < ML_COUNT; ++id)
+        if (exps[id] < -100 || exps[id] > 100) return mrp_fail(MRP_ERR_INVALID, "exponent of layer %d out of range", id);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "stream sync failed");
+    FusedState* f = c->fused;
+    int rc = mfma_net_set_fp8_exps(f->net, f->fp8_wexp, exps);
+    if (rc) return rc;
+    memcpy(f->fp8_aexp, exps, sizeof(int) * ML_COUNT);
     return MRP_OK;
 }
 

@@ -36,8 +36,12 @@ enum MfmaLayerId {
 // fragments are staged in LDS (one 768-thread workgroup per CU shares one copy) instead of
 // being read through the L1/L2 path for every tile.
 constexpr int kLdsW = 16;
+// kCal: the BF16 network that also records, per layer, the largest |input| it quantises
+// (the calibration of the FP8 activation scales).
+constexpr int kCal = 32;
 constexpr int base_prec(int v) { return v & (kLdsW - 1); }
 constexpr bool lds_weights(int v) { return (v & kLdsW) != 0; }
+constexpr bool calibrating(int v) { return (v & kCal) != 0; }
 
 // Compile-time shape of every layer: rows, inputs, first destination row inside the padded
 // 16-row M-tiles, and whether K comes in 32-wide chunks (else K <= 16, one chunk whose
@@ -117,11 +121,24 @@ struct MfmaNet {
     const uint4* bf16_lo = nullptr;  // bf16(W - bf16(W)): second term of the two-term weights
     const uint2* fp8 = nullptr;   // fragments, 8 bytes per lane
     const float* bias = nullptr;
-    const int* wexp = nullptr;    // FP8: layer weights stored as fp8(W * 2^wexp[id])
+    // FP8 per layer: weights stored as fp8(W * 2^wexp), inputs as fp8(clamp(x) * 2^aexp);
+    // f8[id] = {2^-aexp (the divisor of the scaled conversion), 448 * 2^-aexp (input clamp),
+    // 2^-(aexp + wexp) (product scale), 0}.  The same allocation continues with ML_COUNT
+    // unsigned words: the per-layer max |input| (float bits) that the kCal kernels record.
+    // (The struct is a kernel argument copied per tile: host-side state lives elsewhere.)
+    const float4* f8 = nullptr;
 };
+__host__ __device__ inline unsigned* mfma_cal_words(const MfmaNet& n) { return (unsigned*)(n.f8 + ML_COUNT); }
 
-int mfma_net_create(const float* weights_fp32, MfmaNet* out);
+// Packs the weights; wexp receives the per-layer FP8 weight exponents.
+int mfma_net_create(const float* weights_fp32, MfmaNet* out, int* wexp);
 void mfma_net_destroy(MfmaNet* net);
+// Installs FP8 activation exponents (aexp[id]: layer id's inputs are stored as x * 2^aexp[id]).
+// Blocking; the caller orders it after work that reads the table.
+int mfma_net_set_fp8_exps(const MfmaNet& net, const int* wexp, const int* aexp);
+// The exponent max-abs scaling gives a layer whose inputs reach amax: the largest e with
+// amax * 2^e <= 448 (e4m3's largest finite value); 0 for amax = 0.
+int fp8_exp_for_max(float amax);
 
 // Host encoders used by the packer (exposed for tests through the C ABI of the library).
 uint16_t host_f32_to_bf16(float f);
@@ -137,5 +154,9 @@ hipError_t launch_fast_features(const DevVolume* vols, const ViewSlot* slots, co
                                 float* out, hipStream_t s);
 hipError_t launch_mlp_mfma_features(int precision, const MfmaNet& net, const float* W, const float* feats, int n,
                                     const float sr[3], float g, float* rgb, hipStream_t s);
+// The BF16 network on n feature rows recording per layer the largest |input| into net.cal
+// (no output is written).
+hipError_t launch_mlp_mfma_calibrate(const MfmaNet& net, const float* W, const float* feats, int n, const float sr[3],
+                                     float g, hipStream_t s);
 
 }  // namespace mrp

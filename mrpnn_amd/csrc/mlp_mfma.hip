@@ -21,6 +21,7 @@
 #include "mlp_mfma.h"
 #include "dev_gather_fast.hpp"
 #include "net_layout.h"
+#include "fp8_scales.h"
 
 namespace mrp {
 
@@ -65,6 +66,7 @@ __host__ __device__ inline int fp8_exp_for(float amax) {
     float f = frexpf(amax, &x);  // amax = f * 2^x, f in [0.5,1); 448 = 0.875 * 2^9
     return f <= 0.875f ? 9 - x : 8 - x;
 }
+int fp8_exp_for_max(float amax) { return fp8_exp_for(amax); }
 
 namespace {
 struct Spec {
@@ -105,13 +107,12 @@ void layer_specs(Spec* s) {
 }
 }  // namespace
 
-int mfma_net_create(const float* W, MfmaNet* out) {
+int mfma_net_create(const float* W, MfmaNet* out, int* wexp) {
     Spec spec[ML_COUNT];
     layer_specs(spec);
     std::vector<uint16_t> bf, bflo;  // 8 per lane per fragment
     std::vector<uint8_t> f8;
     std::vector<float> bias;
-    std::vector<int> wexp(ML_COUNT, 0);
     int frag = 0;
     for (int id = 0; id < ML_COUNT; ++id) {
         const Spec& s = spec[id];
@@ -150,22 +151,25 @@ int mfma_net_create(const float* W, MfmaNet* out) {
                 bias.push_back(src >= 0 && src < s.M ? W[s.b + src] : 0.0f);
             }
     }
-    void *dbf = nullptr, *dbl = nullptr, *df8 = nullptr, *db = nullptr, *dw = nullptr;
+    void *dbf = nullptr, *dbl = nullptr, *df8 = nullptr, *db = nullptr, *ds = nullptr;
+    const size_t sbytes = ML_COUNT * sizeof(float4) + ML_COUNT * 4;  // scales, then calibration words
     if (hipMalloc(&dbf, bf.size() * 2) != hipSuccess || hipMalloc(&dbl, bflo.size() * 2) != hipSuccess ||
-        hipMalloc(&df8, f8.size()) != hipSuccess ||
-        hipMalloc(&db, bias.size() * 4) != hipSuccess || hipMalloc(&dw, wexp.size() * 4) != hipSuccess)
+        hipMalloc(&df8, f8.size()) != hipSuccess || hipMalloc(&db, bias.size() * 4) != hipSuccess ||
+        hipMalloc(&ds, sbytes) != hipSuccess)
         return mrp_fail(MRP_ERR_NOMEM, "MFMA weight allocation failed");
     if (hipMemcpy(dbf, bf.data(), bf.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dbl, bflo.data(), bflo.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(df8, f8.data(), f8.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(db, bias.data(), bias.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dw, wexp.data(), wexp.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemset(ds, 0, sbytes) != hipSuccess)
         return mrp_fail(MRP_ERR_HIP, "MFMA weight upload failed");
     out->bf16 = (const uint4*)dbf;
     out->bf16_lo = (const uint4*)dbl;
     out->fp8 = (const uint2*)df8;
     out->bias = (const float*)db;
-    out->wexp = (const int*)dw;
+    out->f8 = (const float4*)ds;
+    int rc = mfma_net_set_fp8_exps(*out, wexp, kFp8ActExp);
+    if (rc) return rc;
     // LDS images of the LDS-weight kernels: hi fragments of every layer, lo fragments of the
     // two-term layers, biases (offsets lds_hi_off / lds_lo_off / lds_frag_bytes)
     const int precs[3] = {MRP_BF16, MRP_FP8, MRP_BF16_1T};
@@ -196,17 +200,29 @@ int mfma_net_create(const float* W, MfmaNet* out) {
     return MRP_OK;
 }
 
+int mfma_net_set_fp8_exps(const MfmaNet& net, const int* wexp, const int* aexp) {
+    float4 t[ML_COUNT];
+    for (int id = 0; id < ML_COUNT; ++id) {
+        if (aexp[id] < -100 || aexp[id] > 100) return mrp_fail(MRP_ERR_INVALID, "FP8 exponent of layer %d out of range", id);
+        t[id] = make_float4(ldexpf(1.0f, -aexp[id]), ldexpf(448.0f, -aexp[id]), ldexpf(1.0f, -(aexp[id] + wexp[id])),
+                            0.0f);
+    }
+    if (hipMemcpy((void*)net.f8, t, sizeof t, hipMemcpyHostToDevice) != hipSuccess)
+        return mrp_fail(MRP_ERR_HIP, "FP8 scale upload failed");
+    return MRP_OK;
+}
+
 void mfma_net_destroy(MfmaNet* net) {
     hipFree((void*)net->bf16);
     hipFree((void*)net->bf16_lo);
     hipFree((void*)net->fp8);
     hipFree((void*)net->bias);
-    hipFree((void*)net->wexp);
+    hipFree((void*)net->f8);
     for (int i = 0; i < 4; ++i) {
         hipFree((void*)net->img[i]);
         net->img[i] = nullptr;
     }
-    net->wexp = nullptr;
+    net->f8 = nullptr;
     net->bf16 = nullptr;
     net->fp8 = nullptr;
     net->bias = nullptr;
@@ -224,11 +240,16 @@ __device__ inline uint32_t pack_bf16x2(float a, float b) {  // v_cvt_pk_bf16_f32
     const bf16x2_t v = {(__bf16)a, (__bf16)b};
     return __builtin_bit_cast(uint32_t, v);
 }
-__device__ inline uint32_t pack_fp8x4(float a, float b, float c, float d) {
-    int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
-    w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
-    return (uint32_t)w;
+typedef short s16x2_t __attribute__((ext_vector_type(2)));
+// Four values to fp8 e4m3 as x / 2^exponent(scale) (v_cvt_scalef32_pk_fp8_f32: round to nearest
+// even, only the scale's exponent is used; a result beyond 448 becomes NaN, so callers clamp).
+__device__ inline uint32_t pack_fp8x4_scaled(float a, float b, float c, float d, float scale) {
+    s16x2_t w = {0, 0};
+    w = __builtin_amdgcn_cvt_scalef32_pk_fp8_f32(w, a, b, scale, false);
+    w = __builtin_amdgcn_cvt_scalef32_pk_fp8_f32(w, c, d, scale, true);
+    return __builtin_bit_cast(uint32_t, w);
 }
+__device__ inline float clamp_f8(float x, float lim) { return __builtin_amdgcn_fmed3f(x, -lim, lim); }
 
 // Cross-row lane exchanges of the tile layout (lane = c + 16q) with the gfx950 permlane
 // swaps (plain VALU, no LDS round trip).  permlane16_swap(x, x) returns rows (0,0,2,2) and
@@ -257,8 +278,10 @@ struct NetRs {
     BufRsrc w;    // bf16 hi (16 bytes per lane per fragment) or fp8 (8 bytes) fragments
     BufRsrc wlo;  // bf16 lo fragments
     BufRsrc bias;
-    const int* wexp;
-    const char* lw;  // LDS copy of the weight image (variants with kLdsW)
+    const float4* f8;  // FP8 per-layer scales (MfmaNet::f8)
+    unsigned* cal;     // kCal: per-layer max |input|
+    bool cal_lane;     // kCal: the lane's pixel is a real row (padding lanes are not recorded)
+    const char* lw;    // LDS copy of the weight image (variants with kLdsW)
 };
 // The pointers are uniform (kernel arguments); reading them through readfirstlane keeps the
 // descriptors in scalar registers also where they arrive in vector registers (call arguments).
@@ -268,13 +291,15 @@ __device__ inline void* uniform_ptr(const void* p) {
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
     return (void*)(((uint64_t)hi << 32) | lo);
 }
-__device__ inline NetRs make_net_rs(const MfmaNet& n, bool fp8) {
+__device__ inline NetRs make_net_rs(const MfmaNet& n, bool fp8, bool lane_ok) {
     NetRs r;
     r.w = fp8 ? __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.fp8), (short)0, kMfmaFrags * 512, 0x00020000)
               : __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.bf16), (short)0, kMfmaFrags * 1024, 0x00020000);
     r.wlo = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.bf16_lo), (short)0, kMfmaFrags * 1024, 0x00020000);
     r.bias = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(n.bias), (short)0, kMfmaBias * 4, 0x00020000);
-    r.wexp = (const int*)uniform_ptr(n.wexp);
+    r.f8 = (const float4*)uniform_ptr(n.f8);
+    r.cal = (unsigned*)uniform_ptr(mfma_cal_words(n));
+    r.cal_lane = lane_ok;
     r.lw = (const char*)n.lds;
     return r;
 }
@@ -289,6 +314,18 @@ __device__ inline u32x4_t lds_frag(const char* p) {
     } else {
         return *(const u32x4_t*)p;
     }
+}
+
+// Calibration: the largest |x| over one layer call's inputs (the real rows of the 16-pixel tile),
+// atomically folded into net.cal[id] (non-negative floats order as their bit patterns).
+template <int N>
+__device__ inline void cal_note(const NetRs& net, int id, const float* x) {
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) m = fmaxf(m, fabsf(x[i]));
+    if (!net.cal_lane) m = 0.0f;
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(net.cal + id, __float_as_uint(m));
 }
 
 // out[NM][4] = act(W_ID * in + b): in[NK][8] in the K-slot order of mlp_mfma.h.  One M-tile
@@ -311,6 +348,7 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
     // Fragments of M-tile mt+1 are requested before the MFMAs of M-tile mt (two buffers), so
     // the weight latency of a multi-tile layer is paid once.
     if constexpr (P == MRP_BF16 || P == MRP_BF16_1T) {
+        if constexpr (calibrating(PREC)) cal_note<NK * 8>(net, ID, &in[0][0]);
         u32x4_t b[NK];
 #pragma unroll
         for (int kc = 0; kc < NK; ++kc)
@@ -368,22 +406,19 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
             if constexpr (SB) __builtin_amdgcn_sched_barrier(0);
         }
     } else {
-        float m = 0.0f;
-#pragma unroll
-        for (int kc = 0; kc < NK; ++kc)
-#pragma unroll
-            for (int s = 0; s < 8; ++s) m = fmaxf(m, fabsf(in[kc][s]));
-        m = q_max(m);
-        const int ea = fp8_exp_for(m);
-        const float sc = ldexpf(1.0f, ea);
+        // per-layer max-abs scales: inputs clamped to +-448 * 2^-aexp, stored as x * 2^aexp
+        const float4 s8 = net.f8[ID];
         long b[NK];
 #pragma unroll
         for (int kc = 0; kc < NK; ++kc) {
-            uint32_t lo = pack_fp8x4(in[kc][0] * sc, in[kc][1] * sc, in[kc][2] * sc, in[kc][3] * sc);
-            uint32_t hi = pack_fp8x4(in[kc][4] * sc, in[kc][5] * sc, in[kc][6] * sc, in[kc][7] * sc);
+            float v[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) v[s] = clamp_f8(in[kc][s], s8.y);
+            const uint32_t lo = pack_fp8x4_scaled(v[0], v[1], v[2], v[3], s8.x);
+            const uint32_t hi = pack_fp8x4_scaled(v[4], v[5], v[6], v[7], s8.x);
             b[kc] = (long)(((uint64_t)hi << 32) | lo);
         }
-        const float ds = ldexpf(1.0f, -(ea + net.wexp[ID]));
+        const float ds = s8.z;
         long wa[2][NK];
         u32x4_t bb[2];
         auto fetch = [&](int mt, int buf) {
@@ -756,6 +791,7 @@ __device__ inline void frag1_apply(const NetRs& net, const Frag1& f, const float
     constexpr int P = base_prec(PREC);
     constexpr bool TWO = two_term(PREC, ID);
     if constexpr (P == MRP_BF16 || P == MRP_BF16_1T) {
+        if constexpr (calibrating(PREC)) cal_note<4>(net, ID, in);
         const u32x4_t b = u32x4_t{pack_bf16x2(in[0], in[1]), pack_bf16x2(in[2], in[3]), 0u, 0u};
         f32x4_t acc = f32x4_t{__uint_as_float(f.b[0]), __uint_as_float(f.b[1]), __uint_as_float(f.b[2]),
                               __uint_as_float(f.b[3])};
@@ -771,15 +807,11 @@ __device__ inline void frag1_apply(const NetRs& net, const Frag1& f, const float
             out[i] = relu ? fmaxf(v, 0.0f) : v;
         }
     } else {
-        float m = 0.0f;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) m = fmaxf(m, fabsf(in[s]));
-        m = q_max(m);
-        const int ea = fp8_exp_for(m);
-        const float sc = ldexpf(1.0f, ea);
-        const long b = (long)(uint64_t)pack_fp8x4(in[0] * sc, in[1] * sc, in[2] * sc, in[3] * sc);
+        const float4 s8 = net.f8[ID];
+        const long b = (long)(uint64_t)pack_fp8x4_scaled(clamp_f8(in[0], s8.y), clamp_f8(in[1], s8.y),
+                                                         clamp_f8(in[2], s8.y), clamp_f8(in[3], s8.y), s8.x);
         const long w = (long)(((uint64_t)f.w[1] << 32) | f.w[0]);
-        const float ds = ldexpf(1.0f, -(ea + net.wexp[ID]));
+        const float ds = s8.z;
         const f32x4_t acc =
             __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(w, b, f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 #pragma unroll
@@ -925,7 +957,7 @@ __device__ __attribute__((noinline)) float tail_call(MfmaNet nt, const float* __
                                                      const float* __restrict__ sW1, const LaneCtx& lc, float srp,
                                                      const float (&hacc)[4], const float (&comb)[3][8],
                                                      const float (&tA)[4], const float (&tB)[4]) {
-    return tail<PREC>(make_net_rs(nt, base_prec(PREC) == MRP_FP8), W, sW1, lc, srp, hacc, comb, tA, tB);
+    return tail<PREC>(make_net_rs(nt, base_prec(PREC) == MRP_FP8, lc.ok), W, sW1, lc, srp, hacc, comb, tA, tB);
 }
 
 // Full network on one 16-pixel tile; returns, in lanes q == 0, max(e^P - 1, 0) * srp per
@@ -935,7 +967,7 @@ __device__ inline v3 run_tile(const MfmaNet& nt, const float* __restrict__ W, co
                               const float* __restrict__ sW1,
                               const LaneCtx& lc, const Src& src) {
     using namespace net;
-    const NetRs net = make_net_rs(nt, base_prec(PREC) == MRP_FP8);
+    const NetRs net = make_net_rs(nt, base_prec(PREC) == MRP_FP8, lc.ok);
     const bool lo = lc.q < 2;
     float F[3][8], G[3][8], R[3][8], O[3][8], s[3];
     float hacc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -1256,7 +1288,7 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const fl
         nt.fp8 = opaque(net.fp8);
         nt.bias = opaque(net.bias);
         v3 o = run_tile<PREC>(nt, opaque(W), opaque_v(sG), opaque_v(sW1), lc, src);
-        if (lc.ok && lc.q == 0) {
+        if (!calibrating(PREC) && lc.ok && lc.q == 0) {
             rgb[3 * (size_t)idx] = o.x;
             rgb[3 * (size_t)idx + 1] = o.y;
             rgb[3 * (size_t)idx + 2] = o.z;
@@ -1314,6 +1346,16 @@ hipError_t launch_mlp_mfma_features(int precision, const MfmaNet& net, const flo
     else
         hipLaunchKernelGGL(k_mlp_mfma_features<MRP_FP8>, dim3(grid), dim3(256), 0, s, net, W, feats, n,
                            mk3(sr[0], sr[1], sr[2]), g, rgb);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_mfma_calibrate(const MfmaNet& net, const float* W, const float* feats, int n, const float sr[3],
+                                     float g, hipStream_t s) {
+    int tiles = (n + 15) / 16;
+    int grid = (tiles + 3) / 4;
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(k_mlp_mfma_features<MRP_BF16 | kCal>, dim3(grid), dim3(256), 0, s, net, W, feats, n,
+                       mk3(sr[0], sr[1], sr[2]), g, (float*)nullptr);
     return hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ def lib():
         L.orc_stencil.argtypes = [ctypes.c_int, _f, _i, _i, _i]
         L.orc_predict_f32.argtypes = [_f, ctypes.c_int, _f, ctypes.c_float, _f]
         L.orc_predict_f64.argtypes = [_d, ctypes.c_int, _d, ctypes.c_double, _d]
+        L.orc_predict_amax_f64.argtypes = [_d, ctypes.c_int, _d, ctypes.c_double, _d]
         L.orc_build_pyramid.argtypes = [_f, ctypes.c_int, _f]
         L.orc_bake_tr.argtypes = [_f, _f, ctypes.c_float, _f]
         L.orc_hg_lut_qmc.argtypes = [ctypes.c_float, ctypes.c_uint32, _f]
@@ -168,6 +169,16 @@ def predict_f64(feats, sr, g):
     sr = np.asarray(sr, np.float64)
     out = np.zeros((feats.shape[0], 3), np.float64)
     lib().orc_predict_f64(_p(feats, _d), feats.shape[0], _p(sr, _d), float(g), _p(out, _d))
+    return out
+
+
+def predict_amax_f64(feats, sr, g):
+    """Per MFMA layer (the library's 60 FP8 layers, in order) the largest |input| of the FP64
+    network over the rows."""
+    feats = np.ascontiguousarray(feats, np.float64).reshape(-1, NFEAT)
+    sr = np.asarray(sr, np.float64)
+    out = np.zeros(60, np.float64)
+    lib().orc_predict_amax_f64(_p(feats, _d), feats.shape[0], _p(sr, _d), float(g), _p(out, _d))
     return out
 
 

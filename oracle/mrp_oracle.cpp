@@ -88,6 +88,46 @@ template <> inline float expT<float>(float x) { return mrp::exp_f32(x); }
 template <> inline double expT<double>(double x) { return exp(x); }
 template <class T> static inline T relu(T x) { return x >= (T)0 ? x : (T)0; }
 
+// Largest |input| of each weight matrix that the MFMA form runs as a matrix product (FP8
+// calibration checks): per-thread, switched on by set_amax_recorder.
+static thread_local double* t_amax = nullptr;
+static thread_local const float* t_amax_w[kAmaxLayers];
+void set_amax_recorder(const Net* net, double* amax) {
+    t_amax = amax;
+    if (!net) return;
+    static const char* const tb[3] = {"L", "L_Tr", "L_Hg"}, *db[3] = {"LD", "LD_Tr", "LD_Hg"};
+    char nm[32];
+    int id = 0;
+    for (int b = 0; b < 3; ++b)
+        for (int k = 0; k < 9; ++k) {
+            snprintf(nm, sizeof nm, "%s%d1W", tb[b], k);
+            t_amax_w[id++] = net->get(nm);
+        }
+    for (int b = 0; b < 3; ++b)
+        for (int k = 0; k < 4; ++k) {
+            snprintf(nm, sizeof nm, "%s%d1W", db[b], k);
+            t_amax_w[id++] = net->get(nm);
+        }
+    const char* rest[6] = {"LD41W", "LD_Tr41W", "LD_Hg41W", "LSEFin2W", "LC0W", "LC1W"};
+    for (const char* r : rest) t_amax_w[id++] = net->get(r);
+    t_amax_w[id++] = net->get("LC2W");
+    t_amax_w[id++] = net->get("LXW");
+    for (int k = 0; k <= 12; ++k) {
+        snprintf(nm, sizeof nm, "LX%dW", k);
+        t_amax_w[id++] = net->get(nm);
+    }
+    if (id != kAmaxLayers) abort();
+}
+template <class T>
+static inline void note_input(const float* W, const T* in, int from) {
+    if (!t_amax) return;
+    for (int id = 0; id < kAmaxLayers; ++id)
+        if (t_amax_w[id] == W) {
+            for (int j = 0; j < from; ++j) t_amax[id] = std::max(t_amax[id], fabs((double)in[j]));
+            return;
+        }
+}
+
 // AVGMAX (radiancePredict.cu:21-33)
 template <class T>
 static void avgmax(const T* in, T* out, int n, int read_at, int w1, int w2) {
@@ -102,6 +142,7 @@ static void avgmax(const T* in, T* out, int n, int read_at, int w1, int w2) {
 // LinearReLU_SFL / LinearReLU<Size>: bias first, j ascending (radiancePredict.cu:129-224)
 template <class T>
 static void lin_relu_bf(const T* in, T* out, int from, int to, const float* W, const float* B) {
+    note_input(W, in, from);
     for (int i = 0; i < to; ++i) {
         T acc = (T)B[i];
         for (int j = 0; j < from; ++j) acc = mac<T>(acc, in[j], (T)W[j + i * from]);
@@ -111,6 +152,7 @@ static void lin_relu_bf(const T* in, T* out, int from, int to, const float* W, c
 // untemplated LinearReLU: zero start, bias last (radiancePredict.cu:41-56)
 template <class T>
 static void lin_relu_bl(const T* in, T* out, int from, int to, const float* W, const float* B) {
+    note_input(W, in, from);
     for (int i = 0; i < to; ++i) {
         T acc = (T)0;
         for (int j = 0; j < from; ++j) acc = mac<T>(acc, in[j], (T)W[j + i * from]);
@@ -129,6 +171,7 @@ static void lin_relu_nb(const T* in, T* out, int from, int to, const float* W) {
 }
 template <class T>
 static void lin_sigm_nb(const T* in, T* out, int from, int to, const float* W) {
+    note_input(W, in, from);
     for (int i = 0; i < to; ++i) {
         T acc = (T)0;
         for (int j = 0; j < from; ++j) acc = mac<T>(acc, in[j], (T)W[j + i * from]);

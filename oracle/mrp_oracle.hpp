@@ -427,6 +427,12 @@ template <> inline double mac<double>(double acc, double a, double b) { return a
 template <class T>
 void predict_internal(const Net& net, const T* X_Val, const T* X_Val_Sub, const T* X_Val_Hg, T gamma, T g,
                       const T scatterrate[3], T out[3]);
+// While set (per thread; net = nullptr switches it off), predict_internal folds the largest
+// |input| of each of the 60 matrix-product layers into amax[id]: L01..L81 of the three branches
+// (L, L_Tr, L_Hg), LD01..LD31 of the three, LD41, LD_Tr41, LD_Hg41, LSEFin2, LC0, LC1, LC2, LX,
+// LX0..LX12 -- the FP8 layers of the library's C ABI in its order.
+constexpr int kAmaxLayers = 60;
+void set_amax_recorder(const Net* net, double* amax);
 
 // One pixel of the plain per-frame predictor: RenderCamera<true,MRPNN> (volume.cu:198-228)
 // -> NNPredict<MRPNN> (render.cuh:808-875).  out = (rgb, distance) after max(0,.).

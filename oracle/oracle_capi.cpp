@@ -150,6 +150,27 @@ int orc_predict_f64(const double* feats, int n, const double sr[3], double g, do
     return 0;
 }
 
+// Per layer the largest |input| of the FP64 network over n rows (the FP8 calibration statistic).
+int orc_predict_amax_f64(const double* feats, int n, const double sr[3], double g, double* amax) {
+    if (!g_net_ok) return 1;
+    for (int id = 0; id < kAmaxLayers; ++id) amax[id] = 0.0;
+#pragma omp parallel
+    {
+        double loc[kAmaxLayers] = {};
+        set_amax_recorder(&g_net, loc);
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < n; ++i) {
+            const double* f = feats + (size_t)i * NFEAT;
+            double rgb[3];
+            predict_internal<double>(g_net, f, f + 2 * PC, f + PC, f[3 * PC], g, sr, rgb);
+        }
+        set_amax_recorder(nullptr, nullptr);
+#pragma omp critical
+        for (int id = 0; id < kAmaxLayers; ++id) amax[id] = std::max(amax[id], loc[id]);
+    }
+    return 0;
+}
+
 void orc_build_pyramid(const float* grid, int N, float* mips_out) {
     float* m[9];
     for (int i = 0; i < 9; ++i) m[i] = mips_out + mip_offset(i);

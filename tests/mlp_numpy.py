@@ -18,7 +18,14 @@ def load_weights():
     return {a["name"]: blob[a["offset"]:a["offset"] + a["count"]] for a in man["arrays"]}
 
 
+_AMAX = None   # while predict(..., amax=d) runs: weight name -> largest |input| of that product
+_NAMES = None  # id(weight array) -> weight name
+
+
 def _lin(W, x, n_out):  # W[out][in] row-major; x: [P, in]
+    if _AMAX is not None and x.size:
+        nm = _NAMES[id(W)]
+        _AMAX[nm] = max(_AMAX.get(nm, 0.0), float(np.abs(x).max()))
     return x @ W.reshape(n_out, -1).T
 
 
@@ -30,8 +37,20 @@ def _sig(x):
     return 1.0 / (1.0 + np.exp(-x))
 
 
-def predict(w, feats, g, scatterrate):
-    """feats: [P, 577] rows [X_Val | X_Val_Hg | X_Val_Sub | gamma] -> [P, 3]."""
+def predict(w, feats, g, scatterrate, amax=None):
+    """feats: [P, 577] rows [X_Val | X_Val_Hg | X_Val_Sub | gamma] -> [P, 3].  amax (a dict), when
+    given, receives per weight matrix name the largest |input| of its product."""
+    global _AMAX, _NAMES
+    if amax is None:
+        return _predict(w, feats, g, scatterrate)
+    _AMAX, _NAMES = amax, {id(v): k for k, v in w.items()}
+    try:
+        return _predict(w, feats, g, scatterrate)
+    finally:
+        _AMAX = _NAMES = None
+
+
+def _predict(w, feats, g, scatterrate):
     feats = np.asarray(feats, np.float64)
     P = feats.shape[0]
     X, H, S = feats[:, 0:192], feats[:, 192:384], feats[:, 384:576]

@@ -2,7 +2,7 @@
 image sizes, the TR cache, HIP-graph replay and the launch-shape autotuner.
 
 Bar: bit-exact.  Every 16-record network tile holds pixels of one 8x8 screen tile of one view
-(padded compaction) and the FP8 activation scales are per pixel, so a view's pixels are
+(padded compaction) and the FP8 activation scales are fixed per layer, so a view's pixels are
 computed by the same instructions on the same data whether it is rendered alone or in a
 mix, eagerly or from a graph, with any persistent grid size.
 """

@@ -71,3 +71,26 @@ def test_inactive_rows_constant(orc):
     z[:, 576] = 1.1
     out = orc.predict_f32(z, np.array([0.99 / 1.001] * 3, np.float32), G)
     assert np.all(out == out[0])
+
+
+def fp8_layer_names():
+    """The FP8 layers of the library's C ABI in its order (MRP_FP8_LAYERS, mrp_c_api.h)."""
+    names = ["%s%d1W" % (b, k) for b in ("L", "L_Tr", "L_Hg") for k in range(9)]
+    names += ["%s%d1W" % (b, k) for b in ("LD", "LD_Tr", "LD_Hg") for k in range(4)]
+    names += ["LD41W", "LD_Tr41W", "LD_Hg41W", "LSEFin2W", "LC0W", "LC1W", "LC2W", "LXW"]
+    return names + ["LX%dW" % k for k in range(13)]
+
+
+@pytest.mark.parametrize("sr", [[0.99 / 1.001] * 3, [0.97 / 1.001, 0.93 / 1.001, 0.99 / 1.001]])
+def test_layer_input_maxima_match_numpy(orc, feats, w, sr):
+    """The FP8 calibration statistic (largest |input| of every matrix-product layer) of the
+    oracle's FP64 network equals the numpy restatement's, layer by layer."""
+    sr = np.asarray(sr, np.float64)
+    a = orc.predict_amax_f64(feats.astype(np.float64), sr, G)
+    d = {}
+    mlp_numpy.predict(w, feats, G, sr, amax=d)
+    names = fp8_layer_names()
+    assert len(names) == 60 and len(set(names)) == 60
+    b = np.array([d[n] for n in names])
+    np.testing.assert_allclose(a, b, rtol=1e-12)
+    assert (a > 0).all()
