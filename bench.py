@@ -56,6 +56,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--plain-steps", type=int, default=4)
+    ap.add_argument("--tile-w", type=int, default=8, help="march tile width of the batched path (4..64)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy 1 GiB device-to-device (scale reference for HBM counters)")
     ap.add_argument("--config", default="headline", choices=["headline", "c1", "c2", "c3", "c4", "c5", "c5sweep", "tile", "interactive", "ingest",
@@ -98,6 +99,7 @@ def main():
     # sets it could also overlap the previous frame's network, which measured slower (both are
     # VALU-bound and share the CUs)
     ctx.set_option(M.OPT_TR_SLOTS, 1)
+    ctx.set_option(M.OPT_TILE_W, args.tile_w)
     W, H = args.width, args.height
     vol = M.Volume(ctx, args.res)
     t0 = time.time()

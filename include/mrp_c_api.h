@@ -56,6 +56,8 @@ extern "C" {
                                        per CU holding all weight fragments and biases in LDS;
                                        0: fragments are read through L1/L2 by MRP_OPT_MFMA_WAVES_PER_CU
                                        persistent waves per CU */
+#define MRP_OPT_TILE_W 7            /* width in pixels of the batched path's march tiles, one wavefront of
+                                       64 pixels each: 4, 8 (default), 16, 32 or 64 (height 64 / width) */
 #define MRP_OPT_RPNN_CHUNK 6        /* pixels per pass of the BF16 RPNN path (descriptor fragments of
                                        5 KB per pixel, then the network): 4096..1048576, a multiple
                                        of 128 (default 65536) */
@@ -177,11 +179,11 @@ int mrp_render_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, co
  * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc).  Same kernels, arguments and results as mrp_render_batch, but the
  * per-call device buffers (counters, volume and view tables, the active-pixel list of 64-byte
- * records, 64 per 8x8 tile) live in one static device array of MRP_NOALLOC_SCRATCH_BYTES that
+ * records, 64 per march tile) live in one static device array of MRP_NOALLOC_SCRATCH_BYTES that
  * the library keeps on each GPU it runs on until the process ends (resolved by
  * mrp_ctx_create).  Limits, refused with MRP_ERR_INVALID: n_views <= MRP_NOALLOC_MAX_VIEWS and
- * sum over views of 64 * ceil(width/8) * ceil(height/8) <= MRP_NOALLOC_MAX_RECORDS (one 1920x1080
- * view needs 2,073,600).  The views must be prepared (mrp_prepare_views) as for
+ * sum over views of 64 * (march tiles of the view) <= MRP_NOALLOC_MAX_RECORDS (one 1920x1080
+ * view needs 2,073,600 with 8x8 tiles, 2,088,960 with 4x16 tiles).  The views must be prepared (mrp_prepare_views) as for
  * mrp_render_batch; MRP_OPT_GRAPH and timing are ignored.  The array is shared by every call on
  * one GPU and keeps the last call's contents: calls that overlap on one GPU (other streams or
  * contexts) race; calls on one stream are ordered.  Asynchronous. */
@@ -191,10 +193,11 @@ int mrp_render_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, co
 int mrp_render_batch_noalloc(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views,
                              int n_views, int precision, float* const* dev_out4);
 /* Launch-shape autotuner (no counterpart in the reference, whose launch shapes are fixed):
- * renders the prepared batch with the LDS-weight kernel (12 waves per CU) and with the
- * L1/L2-weight kernel at 4, 8, 12, 16, 24 and 32 persistent waves per CU, keeps the fastest
- * (MRP_OPT_LDS_WEIGHTS, MRP_OPT_MFMA_WAVES_PER_CU) and reports its waves per CU and time per
- * batch.  Blocking. */
+ * renders the prepared batch first with march tiles 8x8, 16x4, 4x16, 32x2 and 64x1 and keeps the
+ * fastest (MRP_OPT_TILE_W), then with the LDS-weight network kernel (12 waves per CU) and the
+ * L1/L2-weight kernel at 4, 8, 12, 16, 24 and 32 persistent waves per CU and keeps the fastest
+ * (MRP_OPT_LDS_WEIGHTS, MRP_OPT_MFMA_WAVES_PER_CU); reports the network's waves per CU and the
+ * time per batch of the chosen shape.  Images do not depend on the shape.  Blocking. */
 int mrp_autotune_batch(mrp_ctx* ctx, mrp_volume* const* volumes, int n_volumes, const mrp_view* views, int n_views,
                        int precision, float* const* dev_out4, int* best_waves_per_cu, float* best_ms);
 /* Brings the HG LUT (all views share g) and the batch TR cache up to date: one set of TR mips

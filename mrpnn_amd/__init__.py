@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmrpnn_hip.so")
 FP32, BF16, FP8, BF16_1T = 0, 1, 2, 3
 PRECISIONS = {"fp32": FP32, "bf16": BF16, "fp8": FP8, "bf16_1t": BF16_1T}
 OPT_MFMA_WAVES_PER_CU, OPT_EXACT_GATHER, OPT_TR_SLOTS, OPT_GRAPH, OPT_LDS_WEIGHTS, OPT_RPNN_CHUNK = 1, 2, 3, 4, 5, 6
+OPT_TILE_W = 7
 NFEAT = 577
 FP8_LAYERS = 60  # MFMA layers with an FP8 activation exponent (MRP_FP8_LAYERS)
 TILE_ROW = 773  # tile feature gather row (mrp_gather_tiles)

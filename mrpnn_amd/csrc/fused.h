@@ -20,15 +20,17 @@ struct ViewConst {
 };
 void view_const_init(const ViewDev& vw, ViewConst* c);
 
-// One view of a batch as the device sees it.
+// One view of a batch as the device sees it.  Its pixels are marched in tiles of 64 (one
+// wavefront each) whose shape, the same for every view of a batch, the autotuner chooses.
 struct ViewSlot {
     ViewDev vw;
     ViewConst vc;
     float4* out;  // width*height (rgb, distance)
     int vol;      // index into the batch's DevVolume table
-    int tile0;    // first 8x8 tile of this view in the flattened tile grid
+    int tile0;    // first 64-pixel tile of this view in the flattened tile grid
     int tiles_x;  // tiles per row
     int checker;  // 0: every pixel; 1 + p: only pixels with (row + col + p) even, the others untouched
+    int tile_lw;  // log2 of the tile width (2..6): tiles of (1 << tile_lw) x (64 >> tile_lw) pixels
 };
 
 // One active pixel after the march: 64 bytes.

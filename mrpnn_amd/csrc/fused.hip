@@ -1,9 +1,10 @@
 // Batched predictor: many views of several resident volumes per call.
 //
-// Stage 1, k_march_compact: one 64-lane wavefront per 8x8 pixel tile (tiles of all views
-//   flattened into one grid): camera ray, first-hit march, basis; inactive pixels are
-//   written (0,0,0,0) at once, active ones are appended to a compact list with one atomic
-//   per wavefront, so the expensive stages see dense work.
+// Stage 1, k_march_compact: one 64-lane wavefront per 64-pixel tile (8x8 by default; 4x16,
+//   16x4, 32x2 or 64x1 as chosen), the tiles of all views flattened into one grid: camera
+//   ray, first-hit march, basis; inactive pixels are written (0,0,0,0) at once, active ones
+//   are appended to a compact list with one atomic per wavefront, so the expensive stages see
+//   dense work.
 // Stage 2, gather + MLP over the active list:
 //   FP32: one thread per active pixel, the reference's scalar network (parity path);
 //   BF16 / FP8: the MFMA kernels of mlp_mfma.hip (16 pixels per wavefront tile).
@@ -26,8 +27,9 @@ __global__ __launch_bounds__(64) void k_march_compact(const DevVolume* __restric
     const ViewDev vw = sl.vw;
     const int lt = tile - sl.tile0;
     const int tx = lt % sl.tiles_x, ty = lt / sl.tiles_x;
-    const int col = tx * 8 + (threadIdx.x & 7);
-    const int row = ty * 8 + (threadIdx.x >> 3);
+    const int lw = sl.tile_lw;
+    const int col = (tx << lw) + (threadIdx.x & ((1 << lw) - 1));
+    const int row = ty * (64 >> lw) + (threadIdx.x >> lw);
     const bool valid = row < vw.height && col < vw.width && (sl.checker == 0 || ((row + col + sl.checker - 1) & 1) == 0);
     PixelState st;
     st.active = 0;

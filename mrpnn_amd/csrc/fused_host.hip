@@ -32,13 +32,14 @@ struct FusedState {
     int fp8_wexp[ML_COUNT] = {};  // FP8 weight and activation exponents of the layers
     int fp8_aexp[ML_COUNT] = {};
     int mfma_waves_per_cu = 12;  // three 256-thread blocks per CU (three waves per SIMD)
+    int tile_lw = 3;             // march tiles of (1 << tile_lw) x (64 >> tile_lw) pixels (8x8)
     bool lds_weights = true;     // one 768-thread block per CU with the weights in LDS
     // graph mode: the batch (staging copies, counter reset, march, network) is captured once
     // per launch shape on a private stream and replayed on the context's stream
     bool use_graph = false;
     hipStream_t cap_stream = nullptr;
     struct Graph {
-        int n_views, n_entries, n_tiles, precision, fast, grid, lds_w;
+        int n_views, n_entries, n_tiles, precision, fast, grid, lds_w, tile_lw;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -235,10 +236,15 @@ extern "C" int mrp_render_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols,
     return mrp_render_batch_checker(c, vols, n_vols, views, n_views, precision, dev_out4, nullptr);
 }
 
-// Checks a batch's arguments and finds the TR set of every view (view_slot[i]); counts its 8x8
-// tiles and pixels.
+// Tiles of a view with march tiles (1 << lw) pixels wide and (64 >> lw) high.
+static inline int tiles_x_of(int width, int lw) { return (width + (1 << lw) - 1) >> lw; }
+static inline int tiles_y_of(int height, int lw) { return (height + (64 >> lw) - 1) / (64 >> lw); }
+
+// Checks a batch's arguments and finds the TR set of every view (view_slot[i]); counts its
+// 64-pixel march tiles and pixels.
 static int batch_validate(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
                           int precision, float* const* dev_out4, int* view_slot, int* n_tiles, size_t* pixels) {
+    const int lw = c->fused->tile_lw;
     if (!c || !vols || !views || !dev_out4 || n_vols < 1 || n_views < 1)
         return mrp_fail(MRP_ERR_INVALID, "bad batch arguments");
     if (precision != MRP_FP32 && precision != MRP_BF16 && precision != MRP_FP8 && precision != MRP_BF16_1T)
@@ -259,7 +265,7 @@ static int batch_validate(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const
         const int slot = vol->mips_valid ? mrp_tr_slot_find(c, vol, v.light_dir, v.alpha) : -1;
         if (slot < 0) return mrp_fail(MRP_ERR_STATE, "view %d: TR mips not prepared (call mrp_prepare_views)", i);
         view_slot[i] = slot;
-        *n_tiles += ((v.width + 7) / 8) * ((v.height + 7) / 8);
+        *n_tiles += tiles_x_of(v.width, lw) * tiles_y_of(v.height, lw);
         *pixels += (size_t)v.width * v.height;
     }
     return MRP_OK;
@@ -272,6 +278,7 @@ static int batch_tables(const mrp_ctx* c, mrp_volume* const* vols, const mrp_vie
                         int* entry_slot) {
     int n_entries = 0;
     int t0 = 0;
+    const int lw = c->fused->tile_lw;
     for (int i = 0; i < n_views; ++i) {
         const int sl = view_slot[i];
         int e = 0;
@@ -294,9 +301,10 @@ static int batch_tables(const mrp_ctx* c, mrp_volume* const* vols, const mrp_vie
         s.out = (float4*)dev_out4[i];
         s.vol = e;
         s.tile0 = t0;
-        s.tiles_x = (views[i].width + 7) / 8;
+        s.tiles_x = tiles_x_of(views[i].width, lw);
         s.checker = checker ? checker[i] : 0;
-        t0 += s.tiles_x * ((views[i].height + 7) / 8);
+        s.tile_lw = lw;
+        t0 += s.tiles_x * tiles_y_of(views[i].height, lw);
     }
     return n_entries;
 }
@@ -361,7 +369,7 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
         if (rc) return rc;
         f->slots_cap = n_views;
     }
-    // each 8x8 tile takes its active pixels rounded up to 16 records: at most 64 per tile
+    // each 64-pixel tile takes its active pixels rounded up to 16 records: at most 64 per tile
     const size_t need = (size_t)n_tiles * 64;
     if (need > f->cap) {
         clear_graphs(f);
@@ -412,7 +420,7 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
         FusedState::Graph* g = nullptr;
         for (auto& x : f->graphs)
             if (x.n_views == n_views && x.n_entries == n_entries && x.n_tiles == n_tiles && x.precision == precision &&
-                x.fast == (int)fast && x.grid == grid && x.lds_w == (int)lds_w)
+                x.fast == (int)fast && x.grid == grid && x.lds_w == (int)lds_w && x.tile_lw == f->tile_lw)
                 g = &x;
         if (!g) {
             if (!f->cap_stream) HT(hipStreamCreateWithFlags(&f->cap_stream, hipStreamNonBlocking));
@@ -427,7 +435,7 @@ int mrp_render_batch_checker(mrp_ctx* c, mrp_volume* const* vols, int n_vols, co
             hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             hipGraphDestroy(graph);
             HT(ie);
-            f->graphs.push_back({n_views, n_entries, n_tiles, precision, (int)fast, grid, (int)lds_w, exec});
+            f->graphs.push_back({n_views, n_entries, n_tiles, precision, (int)fast, grid, (int)lds_w, f->tile_lw, exec});
             g = &f->graphs.back();
         }
         // graph mode times the whole replay as the network time (march time reads 0)
@@ -563,6 +571,13 @@ extern "C" int mrp_ctx_set_option(mrp_ctx* c, int option, int value) {
         case MRP_OPT_LDS_WEIGHTS:
             c->fused->lds_weights = value != 0;
             return MRP_OK;
+        case MRP_OPT_TILE_W: {
+            int lw = 2;
+            while (lw < 6 && (1 << lw) != value) ++lw;
+            if ((1 << lw) != value) return mrp_fail(MRP_ERR_INVALID, "MRP_OPT_TILE_W is 4, 8, 16, 32 or 64");
+            c->fused->tile_lw = lw;
+            return MRP_OK;
+        }
         case MRP_OPT_RPNN_CHUNK:
             if (value < 4096 || value > 1048576 || value % 128)
                 return mrp_fail(MRP_ERR_INVALID, "MRP_OPT_RPNN_CHUNK in [4096, 1048576], a multiple of 128");
@@ -595,6 +610,7 @@ extern "C" int mrp_ctx_get_option(mrp_ctx* c, int option, int* value) {
         case MRP_OPT_TR_SLOTS: *value = c->tr_slot_cap; return MRP_OK;
         case MRP_OPT_GRAPH: *value = c->fused->use_graph ? 1 : 0; return MRP_OK;
         case MRP_OPT_LDS_WEIGHTS: *value = c->fused->lds_weights ? 1 : 0; return MRP_OK;
+        case MRP_OPT_TILE_W: *value = 1 << c->fused->tile_lw; return MRP_OK;
         case MRP_OPT_RPNN_CHUNK: *value = c->rpnn_chunk; return MRP_OK;
         default: return mrp_fail(MRP_ERR_INVALID, "unknown option %d", option);
     }
@@ -605,42 +621,67 @@ extern "C" int mrp_ctx_get_option(mrp_ctx* c, int option, int* value) {
 extern "C" int mrp_autotune_batch(mrp_ctx* c, mrp_volume* const* vols, int n_vols, const mrp_view* views, int n_views,
                                   int precision, float* const* dev_out4, int* best_waves, float* best_ms) {
     if (!c) return mrp_fail(MRP_ERR_INVALID, "ctx is null");
-    // candidate 0: LDS-resident weights (one 768-thread workgroup per CU); then L1/L2 weights
-    // at each waves-per-CU count
-    static const int cand[] = {0, 4, 8, 12, 16, 24, 32};
+    FusedState* f = c->fused;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return mrp_fail(MRP_ERR_HIP, "event creation failed");
-    const int keep = c->fused->mfma_waves_per_cu;
-    const bool keep_lds = c->fused->lds_weights;
+    // milliseconds per batch of the current launch shape: one warm-up, then three timed batches
+    auto time_batch = [&](float* ms) -> int {
+        int rc = mrp_render_batch(c, vols, n_vols, views, n_views, precision, dev_out4);
+        if (rc) return rc;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return mrp_fail(MRP_ERR_HIP, "event record failed");
+        for (int r = 0; r < 3 && !rc; ++r) rc = mrp_render_batch(c, vols, n_vols, views, n_views, precision, dev_out4);
+        if (rc) return rc;
+        float t = 0.0f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+            return mrp_fail(MRP_ERR_HIP, "event timing failed");
+        *ms = t / 3.0f;
+        return MRP_OK;
+    };
+    const int keep = f->mfma_waves_per_cu, keep_lw = f->tile_lw;
+    const bool keep_lds = f->lds_weights;
+    int rc = MRP_OK;
+    float best_t = 1e30f;
+    // 1. the march tile shape (widths 8, 16, 4, 32, 64), with the network's current launch
+    int best_lw = keep_lw;
+    static const int lws[] = {3, 4, 2, 5, 6};
+    for (int lw : lws) {
+        f->tile_lw = lw;
+        float ms = 0.0f;
+        if ((rc = time_batch(&ms))) break;
+        if (ms < best_t) {
+            best_t = ms;
+            best_lw = lw;
+        }
+    }
+    f->tile_lw = rc ? keep_lw : best_lw;
+    // 2. the network's launch: LDS-resident weights (one 768-thread workgroup per CU), then
+    // L1/L2 weights at each waves-per-CU count
+    static const int cand[] = {0, 4, 8, 12, 16, 24, 32};
     int best = keep;
     bool best_lds = keep_lds;
-    float best_t = 1e30f;
-    int rc = MRP_OK;
+    best_t = 1e30f;
     for (int w : cand) {
-        c->fused->lds_weights = w == 0;
-        c->fused->mfma_waves_per_cu = w == 0 ? keep : w;
-        if ((rc = mrp_render_batch(c, vols, n_vols, views, n_views, precision, dev_out4))) break;  // warm-up
-        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = mrp_fail(MRP_ERR_HIP, "event record failed"); break; }
-        for (int r = 0; r < 3 && !rc; ++r) rc = mrp_render_batch(c, vols, n_vols, views, n_views, precision, dev_out4);
         if (rc) break;
+        f->lds_weights = w == 0;
+        f->mfma_waves_per_cu = w == 0 ? keep : w;
         float ms = 0.0f;
-        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = mrp_fail(MRP_ERR_HIP, "event timing failed");
-            break;
-        }
-        if (ms / 3.0f < best_t) {
-            best_t = ms / 3.0f;
+        if ((rc = time_batch(&ms))) break;
+        if (ms < best_t) {
+            best_t = ms;
             best = w == 0 ? keep : w;
             best_lds = w == 0;
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    c->fused->mfma_waves_per_cu = rc ? keep : best;
-    c->fused->lds_weights = rc ? keep_lds : best_lds;
-    if (rc) return rc;
+    f->mfma_waves_per_cu = rc ? keep : best;
+    f->lds_weights = rc ? keep_lds : best_lds;
+    if (rc) {
+        f->tile_lw = keep_lw;
+        return rc;
+    }
     if (best_waves) *best_waves = best_lds ? 12 : best;
     if (best_ms) *best_ms = best_t;
     return MRP_OK;
@@ -717,7 +758,7 @@ extern "C" int mrp_render_batch_noalloc(mrp_ctx* c, mrp_volume* const* vols, int
     if (rc) return rc;
     const size_t need = (size_t)n_tiles * 64;
     if (need > (size_t)MRP_NOALLOC_MAX_RECORDS)
-        return mrp_fail(MRP_ERR_INVALID, "the batch needs %zu active-pixel records (64 per 8x8 tile), the limit is %d",
+        return mrp_fail(MRP_ERR_INVALID, "the batch needs %zu active-pixel records (64 per march tile), the limit is %d",
                         need, MRP_NOALLOC_MAX_RECORDS);
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess || cur != c->device)

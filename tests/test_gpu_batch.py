@@ -149,11 +149,49 @@ def test_autotune_keeps_results(bctx, volumes):
     assert lds in (0, 1)
     if not lds:
         assert bctx.get_option(M.OPT_MFMA_WAVES_PER_CU) == w
+    assert bctx.get_option(M.OPT_TILE_W) in (4, 8, 16, 32, 64)
     got = _render(bctx, volumes, views, M.BF16)
     for i in range(len(views)):
         assert np.array_equal(got[i], ref[i]), i
     bctx.set_option(M.OPT_MFMA_WAVES_PER_CU, 12)
     bctx.set_option(M.OPT_LDS_WEIGHTS, 1)
+    bctx.set_option(M.OPT_TILE_W, 8)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_tile_shapes_give_identical_images(bctx, volumes, prec):
+    """March tiles of 4x16, 16x4, 32x2 and 64x1 pixels give the 8x8 tiles' images bit for bit
+    (each pixel's march, gather and network do not depend on its tile): eager, graph replay and
+    the no-allocation entry point."""
+    import torch
+    import mrpnn_amd as M
+    views = _views()
+    p = M.PRECISIONS[prec]
+    assert bctx.get_option(M.OPT_TILE_W) == 8
+    ref = _render(bctx, volumes, views, p)
+    try:
+        for tw in (4, 16, 32, 64):
+            bctx.set_option(M.OPT_TILE_W, tw)
+            assert bctx.get_option(M.OPT_TILE_W) == tw
+            got = _render(bctx, volumes, views, p)
+            for i in range(len(views)):
+                assert np.array_equal(got[i], ref[i]), (tw, i)
+            bctx.set_option(M.OPT_GRAPH, 1)
+            got = _render(bctx, volumes, views, p)
+            bctx.set_option(M.OPT_GRAPH, 0)
+            for i in range(len(views)):
+                assert np.array_equal(got[i], ref[i]), ("graph", tw, i)
+            outs = [torch.full((v.height * v.width, 4), -7.0, device="cuda") for v in views]
+            bctx.render_batch_noalloc(volumes, views, outs, p)
+            torch.cuda.synchronize()
+            for i in range(len(views)):
+                assert np.array_equal(outs[i].cpu().numpy(), ref[i]), ("noalloc", tw, i)
+        for bad in (0, 2, 12, 128):
+            with pytest.raises(M.MrpError):
+                bctx.set_option(M.OPT_TILE_W, bad)
+    finally:
+        bctx.set_option(M.OPT_GRAPH, 0)
+        bctx.set_option(M.OPT_TILE_W, 8)
 
 
 @pytest.mark.gpu
