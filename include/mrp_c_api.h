@@ -158,6 +158,22 @@ int mrp_volume_download_mips(mrp_volume* vol, float* host_mips);
 int mrp_volume_download_tr(mrp_volume* vol, float* host_tr);
 int mrp_volume_resolution(mrp_volume* vol);
 const float* mrp_volume_grid_device(mrp_volume* vol);
+/* State of a volume's device pyramids (no counterpart: the reference keeps host copies of its
+ * mips / TR mips next to the device ones and reads them directly).  mips_generation counts
+ * successful mrp_volume_update calls, so a host copy can tell when it went stale. */
+typedef struct mrp_volume_info {
+    int32_t resolution;
+    float max_density;
+    int32_t mips_valid;
+    uint32_t mips_generation;
+    int32_t tr_valid;           /* TR mips of mrp_volume_update_tr present for ... */
+    float tr_light_dir[3];      /* ... this light_dir and */
+    float tr_alpha;             /* ... this alpha */
+    int32_t cached_tr_res;      /* mrp_volume_bake_cached_tr grid, 0 when none */
+} mrp_volume_info;
+int mrp_volume_get_info(const mrp_volume* vol, mrp_volume_info* out);
+/* g of the context's current HG LUT; MRP_ERR_STATE when none was built. */
+int mrp_ctx_get_hg_lut_g(mrp_ctx* ctx, float* g);
 
 /* Plain per-frame FP32 predictor: VolumeRender::Render(size, ..., MRPNN, sampleNum)
  * (core/volume.cu:487-548): for each frame, rebuild LUT/TR if (g, light, alpha) changed,
@@ -324,10 +340,49 @@ int mrp_ctx_env_light(mrp_ctx* ctx, const float light_dir[3], const float light_
 /* Delta-tracking path tracer (judge): RenderCamera<false> / CalculateRadiance
  * (core/volume.cu:198-228, core/render.cuh:535-597) with Philox streams keyed
  * (seed, frame) / (pixel, sample) and FP64 per-pixel Welford accumulators.
- * nee_baked = 1 uses TR mip 0 for next-event estimation as the reference
- * (core/render.cuh:589), 0 uses the unbiased stochastic Tr (core/render.cuh:60-103).
+ * Next-event transmittance (core/render.cuh:589 reads whatever TR mip 0 holds):
+ *   nee_baked = 0: the unbiased stochastic Tr (core/render.cuh:60-103) at every vertex;
+ *   nee_baked = 1: TR mip 0 of mrp_volume_update_tr (Update_TR, 128^3), the view's light/alpha;
+ *   nee_baked = 2: the cached stochastic bake of mrp_volume_bake_cached_tr (Update_CachedTR,
+ *     768^3 by default), which the reference's Render(..., PT, ...) uses; it must have been
+ *     baked for the view's light_dir, alpha and this tr_scale.
  * dev_mean3 / dev_m2_3: width*height*3 doubles; samples [spp0, spp0+spp) are added to
  * the running (n = spp0 on entry) accumulators. */
+/* Update_CachedTR -> Fill_BakeTR (core/volume.cu:1669-1696, 1562-1587): a res^3 grid over the
+ * unit box (res = 0: the reference's 768; 2..1024), texel (x, y, z) at ((x, y, z) + 0.5) / res -
+ * 0.5 holding the mean of 5 stochastic Tr estimates towards normalize(light_dir) where the
+ * density at the texel centre is positive, else of 2.  Each estimate draws from its own Philox
+ * stream keyed (seed, 1) / (texel, estimate); the reference re-seeds one cuRAND state inside
+ * its loop from the x/y thread and block indices only, so its estimates of a texel repeat and
+ * texels differing in z share draws -- same expectation, more variance.  Stored with the
+ * volume (res^3 floats, 1.8 GB at 768) until the grid changes or the volume is destroyed;
+ * cached on exact (light_dir, alpha, tr_scale, seed, res) like the reference's
+ * (lightDir, alpha), so a repeated call is free.  Asynchronous on the context's stream. */
+int mrp_volume_bake_cached_tr(mrp_volume* vol, const float light_dir[3], float alpha, float tr_scale, uint32_t seed,
+                              int res);
+int mrp_volume_download_cached_tr(mrp_volume* vol, float* host); /* res^3 floats; blocking */
+/* GetSamples -> GetSampleMulti -> GetSample (core/volume.cu:710-780, 36-47;
+ * core/render.cuh:615-654): the radiance arriving at point ori_i (inside the unit box) that
+ * dataGen.cu uses as the predictor's training target.  Each path starts in a direction drawn
+ * from the HG(g_i) lobe around dir_i, then delta tracking with next-event estimation through the
+ * stochastic Tr towards light_i (unit vectors), throughput times 1.001 * scatter_i per vertex,
+ * at most multi_scatter vertices, escapes add the environment map.  Path s of task i draws from
+ * the Philox stream keyed (seed, 0) / (i, s).  dev_out3: n*3 floats, the mean over spp paths;
+ * dev_m2_3 (may be NULL): n*3 doubles, the FP64 sum of squared deviations (Welford), so
+ * sqrt(m2 / (spp (spp - 1))) is the mean's standard error.  All arrays are device arrays of n
+ * entries (float3 as 3 floats).  Asynchronous. */
+int mrp_point_radiance(mrp_ctx* ctx, mrp_volume* vol, int n, const float* dev_ori3, const float* dev_dir3,
+                       const float* dev_light3, const float* dev_alpha, const float* dev_g, const float* dev_scatter,
+                       const float light_color[3], int multi_scatter, int spp, uint32_t seed, float tr_scale,
+                       float* dev_out3, double* dev_m2_3);
+/* ShadowTerm_TR / ShadowTerm_TRs (core/render.cuh:656-680, behind VolumeRender::GetTr / GetTrs,
+ * core/volume.cu:381-403, 795-830): per task the mean over spp of Tr(ori_i -> light) *
+ * HG(dot(dir_i, light), g) with the stochastic Tr towards normalize(light_dir); sample s of task
+ * i draws from the Philox stream keyed (seed, 2) / (i, s).  dev_out: n floats.  (GetTrs scales
+ * this by lightColor * pi.)  Asynchronous. */
+int mrp_tr_samples(mrp_ctx* ctx, mrp_volume* vol, int n, const float* dev_ori3, const float* dev_dir3,
+                   const float light_dir[3], float alpha, float g, int spp, uint32_t seed, float tr_scale,
+                   float* dev_out);
 int mrp_pt_render(mrp_ctx* ctx, mrp_volume* vol, const mrp_view* view, int spp0, int spp, int max_bounce,
                   int nee_baked, float tr_scale, double* dev_mean3, double* dev_m2_3);
 

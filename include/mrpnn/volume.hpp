@@ -12,7 +12,13 @@
 //     (mrp_render_rpnn); the interactive Render(target, histo, ..., RPNN) in the interactive
 //     precision (MRP_FP32, else the BF16 network);
 //   * Render(..., PT, ...) returns the mean radiance with w = 0 (the reference's w is a hit
-//     fraction used only by its GUI).
+//     fraction used only by its GUI); its next-event transmittance is the cached stochastic
+//     bake of Update_CachedTR (768^3) as the reference's, whether or not the volume was loaded
+//     with cacheTR (the reference needs cacheTR = true there, or it writes outside its array);
+//   * the random draws of GetSamples / GetTr / GetTrs / Render(PT) come from Philox streams
+//     (fixed seeds, so repeated calls repeat, as the reference's per-thread cuRAND seeds do);
+//   * GetHGLut reproduces the reference's host lookup, including its exchanged interpolation
+//     weights (the v fraction blends along u and the u fraction along v, core/volume.cu:1438-1440).
 #pragma once
 #include <hip/hip_vector_types.h>
 
@@ -97,11 +103,47 @@ public:
     // MRP_BF16 / MRP_FP8 / MRP_BF16_1T the MFMA forms).
     void SetInteractivePrecision(int precision);
 
+    // Host probes of the pyramids (core/volume.cu:1767-1803, 1424-1442): trilinear lookups at
+    // cell centres of density mip `mip` (zero border, max(0, .)) at a position in the unit box
+    // (pos) or its texture coordinate (uv); fractional mips blend levels int(mip) and +1;
+    // TR mip `mip` (clamped) of the last Update_TR, points outside the box looking along the
+    // light to its entry (1 when the light misses it); the HG LUT of the last UpdateHGLut at
+    // (cos, cone angle in radians).  The first probe after Update / Update_TR / UpdateHGLut (or
+    // a render that rebuilt them) copies the pyramid to the host.
+    float DensityAtPosition(int mip, float3 pos);
+    float TrAtPosition(int mip, float3 pos, float3 lightDir);
+    float DensityAtPosition(float mip, float3 pos);
+    float DensityAtUV(int mip, float3 uv);
+    float DensityAtUV(float mip, float3 uv);
+    float GetHGLut(float cos, float angle);
+    // Stochastic transmittance probes (core/volume.cu:381-403, 795-830): GetTr averages
+    // Tr(ori -> lightDir) * HG(dot(dir, lightDir), g) over 32 * ceil(sampleNum / 32) draws;
+    // GetTrs returns lightColor * pi * the mean of sampleNum draws per (ori, dir).
+    float3 GetTr(float3 ori, float3 dir, float3 lightDir, float alpha = 1, float g = 0, int sampleNum = 1) const;
+    std::vector<float3> GetTrs(float alpha, std::vector<float3> ori, std::vector<float3> dir, float3 lightDir,
+                               float3 lightColor, float g = 0, int sampleNum = 1) const;
+    // Point-radiance training targets (core/volume.cu:710-780, core/render.cuh:615-654; the
+    // caller is dataGen.cu:328): per task the radiance arriving at point ori[i] averaged over
+    // sampleNum paths that start in the HG(g[i]) lobe around dir[i] (mrp_point_radiance).
+    std::vector<float3> GetSamples(std::vector<float> alpha, std::vector<float3> ori, std::vector<float3> dir,
+                                   std::vector<float3> lightDir, std::vector<float> g, std::vector<float> scatter,
+                                   float3 lightColor = {1, 1, 1}, int multiScatter = 1, int sampleNum = 1) const;
+
     int resolution() const { return resolution_; }
     mrp_volume* handle() const { return vol_; }
     const std::vector<float>& datas() const { return datas_; }
 
 private:
+    // core/volume.cu:1669-1696: the path tracer's cached transmittance for (lightDir, alpha)
+    void Update_CachedTR(float3 lightDir, float alpha);
+    // host copies of the pyramids for the probes, with the device state they were taken from
+    std::vector<float> mips_host_, tr_host_, lut_host_;
+    uint32_t mips_host_gen_ = 0, tr_host_gen_ = 0;
+    float tr_host_key_[4] = {0, 0, 0, -1};
+    float lut_host_g_ = -100.0f;
+    const float* host_mips();
+    const float* host_tr();
+    const float* host_lut();
     int resolution_ = 0;
     std::vector<float> datas_;
     mrp_volume* vol_ = nullptr;

@@ -132,6 +132,12 @@ def lib():
         L.mrp_volume_update_tr.argtypes = [_vp, _fp, ctypes.c_float]
         L.mrp_volume_download_mips.argtypes = [_vp, _fp]
         L.mrp_volume_download_tr.argtypes = [_vp, _fp]
+        L.mrp_volume_bake_cached_tr.argtypes = [_vp, _fp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_int]
+        L.mrp_volume_download_cached_tr.argtypes = [_vp, _fp]
+        L.mrp_point_radiance.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _fp, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_uint32, ctypes.c_float, _vp, _vp]
+        L.mrp_tr_samples.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp, _fp, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_int, ctypes.c_uint32, ctypes.c_float, _vp]
         L.mrp_render_plain.argtypes = [_vp, _vp, ctypes.POINTER(View), ctypes.c_int, _fp]
         L.mrp_render_plain_device.argtypes = [_vp, _vp, ctypes.POINTER(View), _vp]
         L.mrp_prepare_views.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(View), ctypes.c_int]
@@ -448,6 +454,46 @@ class Volume:
         if out.numel() < n * TILE_ROW or not out.is_contiguous() or out.dtype != torch.float32:
             raise MrpError("gather_tiles: out must be a contiguous float32 tensor of %d x %d" % (n, TILE_ROW))
         check(lib().mrp_gather_tiles(self.ctx.h, self.h, ctypes.byref(view), int(off_x), int(off_y), _ptr(out)))
+        return out
+
+    def bake_cached_tr(self, light, alpha, tr_scale=1.0, seed=1, res=0):
+        """mrp_volume_bake_cached_tr: the path tracer's cached transmittance (res^3, 0 = 768)."""
+        L = (ctypes.c_float * 3)(*[float(x) for x in light])
+        check(lib().mrp_volume_bake_cached_tr(self.h, L, float(alpha), float(tr_scale), int(seed), int(res)))
+
+    def download_cached_tr(self, res):
+        out = np.zeros(res ** 3, np.float32)
+        check(lib().mrp_volume_download_cached_tr(self.h, out.ctypes.data_as(_fp)))
+        return out
+
+    def point_radiance(self, ori, dirs, lights, alpha, g, scatter, light_color=(1, 1, 1), multi_scatter=512,
+                       spp=1024, seed=1, tr_scale=1.0):
+        """mrp_point_radiance (GetSamples): per task the mean radiance (torch float32 [n, 3]) and
+        its FP64 M2 (torch float64 [n, 3]) on the GPU.  Inputs: array-likes of n rows."""
+        import torch
+        dev = torch.device("cuda")
+        t = lambda a, k: torch.as_tensor(np.asarray(a, np.float32).reshape(-1, k), device=dev).contiguous()
+        ori, dirs, lights = t(ori, 3), t(dirs, 3), t(lights, 3)
+        alpha, g, scatter = t(alpha, 1), t(g, 1), t(scatter, 1)
+        n = ori.shape[0]
+        out = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+        m2 = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        lc = (ctypes.c_float * 3)(*[float(x) for x in light_color])
+        check(lib().mrp_point_radiance(self.ctx.h, self.h, n, _ptr(ori), _ptr(dirs), _ptr(lights), _ptr(alpha), _ptr(g),
+                                       _ptr(scatter), lc, int(multi_scatter), int(spp), int(seed), float(tr_scale),
+                                       _ptr(out), _ptr(m2)))
+        return out, m2
+
+    def tr_samples(self, ori, dirs, light, alpha, g, spp, seed=1, tr_scale=1.0):
+        """mrp_tr_samples: per task the mean of Tr(ori -> light) * HG(dot(dir, light), g)."""
+        import torch
+        dev = torch.device("cuda")
+        ori = torch.as_tensor(np.asarray(ori, np.float32).reshape(-1, 3), device=dev).contiguous()
+        dirs = torch.as_tensor(np.asarray(dirs, np.float32).reshape(-1, 3), device=dev).contiguous()
+        out = torch.zeros(ori.shape[0], dtype=torch.float32, device=dev)
+        L = (ctypes.c_float * 3)(*[float(x) for x in light])
+        check(lib().mrp_tr_samples(self.ctx.h, self.h, ori.shape[0], _ptr(ori), _ptr(dirs), L, float(alpha), float(g),
+                                   int(spp), int(seed), float(tr_scale), _ptr(out)))
         return out
 
     def pt_render(self, view, spp, max_bounce, nee_baked=0, tr_scale=1.0, chunk=None, mean=None, m2=None, spp0=0):

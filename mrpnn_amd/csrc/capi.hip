@@ -354,6 +354,7 @@ extern "C" int mrp_volume_destroy(mrp_volume* v) {
     hipFree(v->d_pmips);
     hipFree(v->d_ptr);
     hipFree(v->d_occ);
+    hipFree(v->d_ctr);
     if (v->mips_ready) hipEventDestroy(v->mips_ready);
     delete v;
     return MRP_OK;
@@ -362,6 +363,7 @@ extern "C" int mrp_volume_destroy(mrp_volume* v) {
 static void invalidate(mrp_volume* v) {
     v->mips_valid = false;
     v->tr_valid = false;
+    v->ctr_valid = false;
 }
 
 extern "C" int mrp_volume_set_data(mrp_volume* v, const float* host, float max_density) {
@@ -456,8 +458,31 @@ extern "C" int mrp_volume_update(mrp_volume* v) {
     if (!v->mips_ready) HIP_TRY(hipEventCreateWithFlags(&v->mips_ready, hipEventDisableTiming));
     HIP_TRY(hipEventRecord(v->mips_ready, s));
     v->mips_valid = true;
+    ++v->mips_gen;
     v->tr_valid = false;
+    v->ctr_valid = false;
     mrp_tr_slots_drop(v->ctx, v);
+    return MRP_OK;
+}
+
+extern "C" int mrp_volume_get_info(const mrp_volume* v, mrp_volume_info* out) {
+    CHECK_ARG(v && out, "null argument");
+    memset(out, 0, sizeof *out);
+    out->resolution = v->N;
+    out->max_density = v->max_density;
+    out->mips_valid = v->mips_valid ? 1 : 0;
+    out->mips_generation = v->mips_gen;
+    out->tr_valid = v->tr_valid ? 1 : 0;
+    memcpy(out->tr_light_dir, v->tr_light, sizeof out->tr_light_dir);
+    out->tr_alpha = v->tr_alpha;
+    out->cached_tr_res = v->ctr_valid ? v->ctr_res : 0;
+    return MRP_OK;
+}
+
+extern "C" int mrp_ctx_get_hg_lut_g(mrp_ctx* c, float* g) {
+    CHECK_ARG(c && g, "null argument");
+    if (!c->lut_valid) return mrp_fail(MRP_ERR_STATE, "no HG LUT built");
+    *g = c->lut_g;
     return MRP_OK;
 }
 

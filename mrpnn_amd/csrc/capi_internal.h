@@ -96,10 +96,18 @@ struct mrp_volume {
     float* mip[9]{};
     float* trm[8]{};
     bool mips_valid = false;
+    uint32_t mips_gen = 0;            // successful pyramid rebuilds
     hipEvent_t mips_ready = nullptr;  // recorded after the last pyramid rebuild
     bool tr_valid = false;
     float tr_light[3]{};
     float tr_alpha = 0.0f;
+    // cached stochastic transmittance bake of the path tracer's NEE (Update_CachedTR)
+    float* d_ctr = nullptr;
+    int ctr_res = 0;
+    bool ctr_valid = false;
+    float ctr_light[3]{};
+    float ctr_alpha = 0.0f, ctr_scale = 0.0f;
+    uint32_t ctr_seed = 0;
 };
 
 int mrp_fail(int code, const char* fmt, ...);

@@ -198,9 +198,9 @@ std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float
         check(rc);
         return acc;
     }
-    // PT: sampleNum frames of 32 paths (core/volume.cu:222), NEE through the TR mips
-    check(mrp_ctx_update_hg_lut(default_context(), g, 1u));
-    check(mrp_volume_update_tr(vol_, v.light_dir, alpha));
+    // PT: sampleNum frames of 32 paths (core/volume.cu:222), NEE through the cached
+    // stochastic transmittance bake (core/volume.cu:515-518, core/render.cuh:589)
+    Update_CachedTR(lightDir, alpha);
     const size_t n = (size_t)size.x * size.y;
     double *mean = nullptr, *m2 = nullptr;
     if (hipMalloc((void**)&mean, n * 3 * sizeof(double)) != hipSuccess ||
@@ -211,7 +211,7 @@ std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float
     const int spp = 32 * (sampleNum > 0 ? sampleNum : 1);
     int rc = MRP_OK;
     for (int s0 = 0; s0 < spp && rc == MRP_OK; s0 += 32)
-        rc = mrp_pt_render(default_context(), vol_, &v, s0, 32, multiScatter, 1, tr_scale_, mean, m2);
+        rc = mrp_pt_render(default_context(), vol_, &v, s0, 32, multiScatter, 2, tr_scale_, mean, m2);
     std::vector<double> h(n * 3);
     if (rc == MRP_OK && hipMemcpy(h.data(), mean, n * 3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         rc = MRP_ERR_HIP;
@@ -220,6 +220,207 @@ std::vector<float4> VolumeRender::Render(int2 size, float3 ori, float3 up, float
     check(rc);
     for (size_t i = 0; i < n; ++i) out[i] = float4{(float)h[3 * i], (float)h[3 * i + 1], (float)h[3 * i + 2], 0.0f};
     return out;
+}
+
+void VolumeRender::Update_CachedTR(float3 lightDir, float alpha) {
+    const float L[3] = {lightDir.x, lightDir.y, lightDir.z};
+    check(mrp_volume_bake_cached_tr(vol_, L, alpha, tr_scale_, 0u, 0));  // cached on the exact key
+}
+
+// ------------------------------------------------------------------------------------------
+// Host probes (core/volume.cu:835-880, 1424-1442, 1767-1803) on host copies of the pyramids
+// ------------------------------------------------------------------------------------------
+namespace {
+using mrp::v3;
+float fetch_border_h(const float* d, int res, int x, int y, int z) {  // Sample(data, res, x, y, z)
+    if (x < 0 || y < 0 || z < 0 || x >= res || y >= res || z >= res) return 0.0f;
+    return fmaxf(0.0f, d[((size_t)x * res + y) * res + z]);
+}
+float fetch_clamp_h(const float* d, int res, int x, int y, int z) {  // SampleClamp(data, res, x, y, z)
+    x = x < 0 ? 0 : (x > res - 1 ? res - 1 : x);
+    y = y < 0 ? 0 : (y > res - 1 ? res - 1 : y);
+    z = z < 0 ? 0 : (z > res - 1 ? res - 1 : z);
+    return fmaxf(0.0f, d[((size_t)x * res + y) * res + z]);
+}
+// Sample / SampleClamp (data, res, uv): trilinear at cell centres, x the slowest axis
+template <float (*F)(const float*, int, int, int, int)>
+float trilinear_h(const float* d, int res, v3 uv) {
+    const v3 p = uv * (float)res - 0.5f;
+    const int x = (int)floorf(p.x), y = (int)floorf(p.y), z = (int)floorf(p.z);
+    const v3 w = p - mrp::mk3((float)x, (float)y, (float)z);
+    using mrp::lerp_ref;
+    return lerp_ref(lerp_ref(lerp_ref(F(d, res, x, y, z), F(d, res, x, y, z + 1), w.z),
+                             lerp_ref(F(d, res, x, y + 1, z), F(d, res, x, y + 1, z + 1), w.z), w.y),
+                    lerp_ref(lerp_ref(F(d, res, x + 1, y, z), F(d, res, x + 1, y, z + 1), w.z),
+                             lerp_ref(F(d, res, x + 1, y + 1, z), F(d, res, x + 1, y + 1, z + 1), w.z), w.y),
+                    w.x);
+}
+size_t level_offset(int top, int m) {  // levels top^3, (top/2)^3, ... concatenated
+    size_t o = 0;
+    for (int k = 0; k < m; ++k) o += (size_t)(top >> k) * (top >> k) * (top >> k);
+    return o;
+}
+v3 v3_of(float3 a) { return mrp::mk3(a.x, a.y, a.z); }
+
+// Device buffers of the probes, freed on scope exit.
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) {
+        if (hipMalloc(&p, bytes ? bytes : 4) != hipSuccess) throw Error(MRP_ERR_NOMEM, "probe allocation failed");
+    }
+    ~DevBuf() { (void)hipFree(p); }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+void upload(void* dst, const void* src, size_t bytes) {
+    if (hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) throw Error(MRP_ERR_HIP, "probe upload failed");
+}
+}  // namespace
+
+const float* VolumeRender::host_mips() {
+    mrp_volume_info in;
+    check(mrp_volume_get_info(vol_, &in));
+    if (!in.mips_valid) throw Error(MRP_ERR_STATE, "Update() must run before the density probes");
+    if (mips_host_.empty() || mips_host_gen_ != in.mips_generation) {
+        mips_host_.resize(level_offset(256, 9));
+        check(mrp_volume_download_mips(vol_, mips_host_.data()));
+        mips_host_gen_ = in.mips_generation;
+    }
+    return mips_host_.data();
+}
+const float* VolumeRender::host_tr() {
+    mrp_volume_info in;
+    check(mrp_volume_get_info(vol_, &in));
+    if (!in.tr_valid) throw Error(MRP_ERR_STATE, "Update_TR() must run before the transmittance probes");
+    const float key[4] = {in.tr_light_dir[0], in.tr_light_dir[1], in.tr_light_dir[2], in.tr_alpha};
+    if (tr_host_.empty() || memcmp(key, tr_host_key_, sizeof key) != 0 || tr_host_gen_ != in.mips_generation) {
+        tr_host_.resize(level_offset(128, 8));
+        check(mrp_volume_download_tr(vol_, tr_host_.data()));
+        memcpy(tr_host_key_, key, sizeof key);
+        tr_host_gen_ = in.mips_generation;
+    }
+    return tr_host_.data();
+}
+const float* VolumeRender::host_lut() {
+    float g = 0.0f;
+    check(mrp_ctx_get_hg_lut_g(default_context(), &g));
+    if (lut_host_.empty() || g != lut_host_g_) {
+        lut_host_.resize(512 * 512);
+        check(mrp_ctx_download_hg_lut(default_context(), lut_host_.data()));
+        lut_host_g_ = g;
+    }
+    return lut_host_.data();
+}
+
+float VolumeRender::DensityAtPosition(int mip, float3 pos) {  // core/volume.cu:1767-1769
+    if (mip < 0 || mip > 8) throw Error(MRP_ERR_INVALID, "density mip in [0, 8]");
+    return trilinear_h<fetch_border_h>(host_mips() + level_offset(256, mip), 256 >> mip, v3_of(pos) + 0.5f);
+}
+float VolumeRender::DensityAtUV(int mip, float3 uv) {  // core/volume.cu:1792-1794
+    if (mip < 0 || mip > 8) throw Error(MRP_ERR_INVALID, "density mip in [0, 8]");
+    return trilinear_h<fetch_border_h>(host_mips() + level_offset(256, mip), 256 >> mip, v3_of(uv));
+}
+float VolumeRender::DensityAtPosition(float mip, float3 pos) {  // core/volume.cu:1786-1790
+    if (!(mip >= 0.0f && mip <= 8.0f)) throw Error(MRP_ERR_INVALID, "density mip in [0, 8]");
+    const int a = (int)mip;
+    const float w = mip - (float)a;
+    return mrp::lerp_ref(DensityAtPosition(a, pos), a < 8 ? DensityAtPosition(a + 1, pos) : 0.0f, w);
+}
+float VolumeRender::DensityAtUV(float mip, float3 uv) {  // core/volume.cu:1796-1800
+    if (!(mip >= 0.0f && mip <= 8.0f)) throw Error(MRP_ERR_INVALID, "density mip in [0, 8]");
+    const int a = (int)mip;
+    const float w = mip - (float)a;
+    return mrp::lerp_ref(DensityAtUV(a, uv), a < 8 ? DensityAtUV(a + 1, uv) : 0.0f, w);
+}
+float VolumeRender::TrAtPosition(int mip, float3 pos, float3 lightDir) {  // core/volume.cu:1770-1784
+    if (mip < 0 || mip > 7) throw Error(MRP_ERR_INVALID, "TR mip in [0, 7]");
+    const float* tr = host_tr() + level_offset(128, mip);
+    const int res = 128 >> mip;
+    const v3 p = v3_of(pos), L = v3_of(lightDir);
+    if (p.x < -0.5f || p.y < -0.5f || p.z < -0.5f || p.x > 0.5f || p.y > 0.5f || p.z > 0.5f) {
+        const float off = mrp::ray_box_offset(p, L);
+        if (off >= 0.0f) return trilinear_h<fetch_clamp_h>(tr, res, p + 0.5f + L * off);
+        return 1.0f;  // TR_MUL
+    }
+    return trilinear_h<fetch_clamp_h>(tr, res, p + 0.5f);
+}
+float VolumeRender::GetHGLut(float cs, float angle) {  // core/volume.cu:1424-1442, weights as there
+    const float* lut = host_lut();
+    const float u = (float)(cs * 0.5 + 0.5);
+    const float v = (float)(angle / (3.1415926535 * 60.0 / 180.0));
+    auto cl = [](int i) { return i < 0 ? 0 : (i > 511 ? 511 : i); };
+    const int u0 = cl((int)(u * 512)), v0 = cl((int)(v * 512));
+    const int u1 = cl((int)(u * 512) + 1), v1 = cl((int)(v * 512) + 1);
+    const float a00 = lut[u0 + v0 * 512], a10 = lut[u1 + v0 * 512];
+    const float a01 = lut[u0 + v1 * 512], a11 = lut[u1 + v1 * 512];
+    const float fv = v * 512 - floorf(v * 512), fu = u * 512 - floorf(u * 512);
+    const float ax0 = mrp::lerp_ref(a00, a10, fv), ax1 = mrp::lerp_ref(a01, a11, fv);
+    return mrp::lerp_ref(ax0, ax1, fu);
+}
+
+// ------------------------------------------------------------------------------------------
+// Stochastic probes and point-radiance targets on the GPU
+// ------------------------------------------------------------------------------------------
+float3 VolumeRender::GetTr(float3 ori, float3 dir, float3 lightDir, float alpha, float g, int sampleNum) const {
+    const int draws = 32 * (((sampleNum > 0 ? sampleNum : 1) + 31) / 32);  // whole 32-thread groups
+    const float od[6] = {ori.x, ori.y, ori.z, dir.x, dir.y, dir.z};
+    DevBuf d_od(sizeof od), d_out(sizeof(float));
+    upload(d_od.p, od, sizeof od);
+    const float L[3] = {lightDir.x, lightDir.y, lightDir.z};
+    check(mrp_tr_samples(default_context(), vol_, 1, d_od.as<float>(), d_od.as<float>() + 3, L, alpha, g, draws, 0u,
+                         tr_scale_, d_out.as<float>()));
+    float r = 0.0f;
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&r, d_out.p, sizeof r, hipMemcpyDeviceToHost) != hipSuccess)
+        throw Error(MRP_ERR_HIP, "GetTr download failed");
+    return float3{r, r, r};
+}
+
+std::vector<float3> VolumeRender::GetTrs(float alpha, std::vector<float3> ori, std::vector<float3> dir, float3 lightDir,
+                                         float3 lightColor, float g, int sampleNum) const {
+    if (ori.size() != dir.size()) throw Error(MRP_ERR_INVALID, "GetTrs: ori and dir differ in length");
+    const int n = (int)ori.size();
+    std::vector<float3> res(n);
+    if (n == 0) return res;
+    DevBuf d_o(n * sizeof(float3)), d_d(n * sizeof(float3)), d_out(n * sizeof(float));
+    upload(d_o.p, ori.data(), n * sizeof(float3));
+    upload(d_d.p, dir.data(), n * sizeof(float3));
+    const float L[3] = {lightDir.x, lightDir.y, lightDir.z};
+    check(mrp_tr_samples(default_context(), vol_, n, d_o.as<float>(), d_d.as<float>(), L, alpha, g,
+                         sampleNum > 0 ? sampleNum : 1, 0u, tr_scale_, d_out.as<float>()));
+    std::vector<float> t(n);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(t.data(), d_out.p, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        throw Error(MRP_ERR_HIP, "GetTrs download failed");
+    const float k = 3.1415926535f;  // ShadowTerm_TRs: lightColor * pi * mean (core/render.cuh:679)
+    for (int i = 0; i < n; ++i) res[i] = float3{lightColor.x * k * t[i], lightColor.y * k * t[i], lightColor.z * k * t[i]};
+    return res;
+}
+
+std::vector<float3> VolumeRender::GetSamples(std::vector<float> alpha, std::vector<float3> ori, std::vector<float3> dir,
+                                             std::vector<float3> lightDir, std::vector<float> g,
+                                             std::vector<float> scatter, float3 lightColor, int multiScatter,
+                                             int sampleNum) const {
+    const size_t n = ori.size();
+    if (dir.size() != n || lightDir.size() != n || alpha.size() != n || g.size() != n || scatter.size() != n)
+        throw Error(MRP_ERR_INVALID, "GetSamples: task arrays differ in length");
+    std::vector<float3> res(n);
+    if (n == 0) return res;
+    DevBuf d_o(n * sizeof(float3)), d_d(n * sizeof(float3)), d_l(n * sizeof(float3)), d_a(n * sizeof(float)),
+        d_g(n * sizeof(float)), d_s(n * sizeof(float)), d_out(n * sizeof(float3));
+    upload(d_o.p, ori.data(), n * sizeof(float3));
+    upload(d_d.p, dir.data(), n * sizeof(float3));
+    upload(d_l.p, lightDir.data(), n * sizeof(float3));
+    upload(d_a.p, alpha.data(), n * sizeof(float));
+    upload(d_g.p, g.data(), n * sizeof(float));
+    upload(d_s.p, scatter.data(), n * sizeof(float));
+    const float lc[3] = {lightColor.x, lightColor.y, lightColor.z};
+    check(mrp_point_radiance(default_context(), vol_, (int)n, d_o.as<float>(), d_d.as<float>(), d_l.as<float>(),
+                             d_a.as<float>(), d_g.as<float>(), d_s.as<float>(), lc, multiScatter > 0 ? multiScatter : 1,
+                             sampleNum > 0 ? sampleNum : 1, 0u, tr_scale_, d_out.as<float>(), nullptr));
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(res.data(), d_out.p, n * sizeof(float3), hipMemcpyDeviceToHost) != hipSuccess)
+        throw Error(MRP_ERR_HIP, "GetSamples download failed");
+    return res;
 }
 
 void VolumeRender::SetCheckboard(bool checkboard) { checkboard_ = checkboard; }

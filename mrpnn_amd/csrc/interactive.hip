@@ -64,7 +64,8 @@ __global__ __launch_bounds__(256) void k_pt_frame(DevVolume vol, ViewDev vw, int
     const v3 fwd = normalize3(-vw.ori);
     const float sy = vw.tan_half_fov, sx = vw.tan_half_fov * ((float)W / (float)H);
     unsigned taps = 0;
-    v3 res = pt_sample(vol, vw, light, fwd, sx, sy, row, j, (uint32_t)(row * W + j), 0, max_bounce, 1, 1.0f, taps);
+    const NeeSrc nee = {NEE_TR_MIP0, nullptr, 0};
+    v3 res = pt_sample(vol, vw, light, fwd, sx, sy, row, j, (uint32_t)(row * W + j), 0, max_bounce, nee, 1.0f, taps);
     res = mk3(fmaxf(0.0f, res.x), fmaxf(0.0f, res.y), fmaxf(0.0f, res.z));
     rad[id] = make_float4(res.x, res.y, res.z, 0.0f);
     mrp_histogram* h = histo + id;

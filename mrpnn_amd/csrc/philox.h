@@ -56,6 +56,9 @@ enum : uint32_t {
     RNG_BASIS_MAIN = 2u,  // ray basis attempts, a = view, b = attempt
     RNG_LUT = 3u,         // HG LUT per-texel jitter, a = 0, b = 0
     RNG_PT = 4u,          // path tracer: a = sample index, b = block of 4 draws
+    RNG_POINT = 5u,       // point-radiance paths: pixel word = task, a = sample index
+    RNG_BAKE = 6u,        // cached TR bake: pixel word = texel, a = estimate index
+    RNG_TR = 7u,          // transmittance probes: pixel word = task, a = sample index
 };
 
 // Sequential draws from one (key, {pixel, purpose, a, *}) stream.

@@ -83,6 +83,17 @@ def lib():
         L.orc_pt_render.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, ctypes.POINTER(View), ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_float, _d, _d,
                                     ctypes.POINTER(ctypes.c_int64)]
+        L.orc_pt_render_list_cached.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, ctypes.c_int, ctypes.POINTER(View),
+                                                _i, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _d, _d]
+        L.orc_bake_cached_tr.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.c_int, _f]
+        L.orc_bake_cached_tr.restype = None
+        L.orc_point_radiance.argtypes = [_f, ctypes.c_int, ctypes.c_float, ctypes.c_int, _f, _f, _f, _f, _f, _f, _f,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_float, _d, _d]
+        L.orc_point_radiance.restype = None
+        L.orc_tr_samples.argtypes = [_f, ctypes.c_int, ctypes.c_float, ctypes.c_int, _f, _f, _f, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_int, ctypes.c_uint32, ctypes.c_float, _d]
+        L.orc_tr_samples.restype = None
         L.orc_tr_stochastic.argtypes = [_f, ctypes.c_int, ctypes.c_float, _f, _f, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_float, ctypes.c_uint32, ctypes.c_int, _f]
         L.orc_fbm_volume.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, _f]
@@ -329,6 +340,61 @@ def _pt_list(scene, view, pix, spp, max_bounce, nee_baked=0, tr_scale=1.0):
 
 
 Scene.pt_render_list = _pt_list
+
+
+def _pt_list_cached(scene, view, pix, spp, max_bounce, ctr, ctr_res, tr_scale=1.0):
+    """The path tracer on listed pixels with NEE from a cached transmittance bake (ctr_res^3)."""
+    pix = np.ascontiguousarray(pix, np.int32)
+    ctr = np.ascontiguousarray(ctr, np.float32)
+    mean = np.zeros((pix.size, 3), np.float64)
+    m2 = np.zeros((pix.size, 3), np.float64)
+    lib().orc_pt_render_list_cached(_p(scene.grid), scene.N, scene.max_density, _p(ctr), int(ctr_res),
+                                    ctypes.byref(view), _p(pix, _i), int(pix.size), spp, max_bounce, tr_scale,
+                                    _p(mean, _d), _p(m2, _d))
+    return mean, m2
+
+
+def _bake_cached_tr(scene, light, alpha, tr_scale, seed, res, texels):
+    """Listed texels (flat indices) of the Fill_BakeTR grid of resolution res."""
+    t = np.ascontiguousarray(texels, np.int64)
+    out = np.zeros(t.size, np.float32)
+    L = np.asarray(light, np.float32)
+    lib().orc_bake_cached_tr(_p(scene.grid), scene.N, scene.max_density, _p(L), float(alpha), float(tr_scale),
+                             int(seed), int(res), t.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(t.size),
+                             _p(out))
+    return out
+
+
+def _point_radiance(scene, ori, dirs, lights, alpha, g, scatter, light_color, multi_scatter, spp, seed, tr_scale=1.0):
+    """GetSamples: per task (mean, m2) over spp point-radiance paths, float64 [n, 3] each."""
+    f = lambda a, k: np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, k))
+    ori, dirs, lights = f(ori, 3), f(dirs, 3), f(lights, 3)
+    alpha, g, scatter = f(alpha, 1), f(g, 1), f(scatter, 1)
+    n = ori.shape[0]
+    lc = np.asarray(light_color, np.float32)
+    mean = np.zeros((n, 3), np.float64)
+    m2 = np.zeros((n, 3), np.float64)
+    lib().orc_point_radiance(_p(scene.grid), scene.N, scene.max_density, n, _p(ori), _p(dirs), _p(lights), _p(alpha),
+                             _p(g), _p(scatter), _p(lc), int(multi_scatter), int(spp), int(seed), float(tr_scale),
+                             _p(mean, _d), _p(m2, _d))
+    return mean, m2
+
+
+def _tr_samples(scene, ori, dirs, light, alpha, g, spp, seed, tr_scale=1.0):
+    """ShadowTerm_TR(s): per task the mean over spp of Tr(ori -> light) * HG(dot(dir, light), g)."""
+    ori = np.ascontiguousarray(np.asarray(ori, np.float32).reshape(-1, 3))
+    dirs = np.ascontiguousarray(np.asarray(dirs, np.float32).reshape(-1, 3))
+    out = np.zeros(ori.shape[0], np.float64)
+    L = np.asarray(light, np.float32)
+    lib().orc_tr_samples(_p(scene.grid), scene.N, scene.max_density, ori.shape[0], _p(ori), _p(dirs), _p(L),
+                         float(alpha), float(g), int(spp), int(seed), float(tr_scale), _p(out, _d))
+    return out
+
+
+Scene.pt_render_list_cached = _pt_list_cached
+Scene.bake_cached_tr = _bake_cached_tr
+Scene.point_radiance = _point_radiance
+Scene.tr_samples = _tr_samples
 
 
 def sample_t(grid, N, max_density, ori, d, alpha):

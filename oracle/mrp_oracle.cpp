@@ -674,7 +674,9 @@ void pt_pixel(const Volume& v, const View& view, const PTParams& pp, int row, in
                 csr = csr * view.scatter_rate;
                 float ld = ray_box_distance(spos, light);
                 float phase = henyey_greenstein(dot(rdir, light), view.g);
-                float tr = pp.nee_baked ? mip_tr(v, 0, spos) : tr_stochastic(v, rs, spos, light, ld, view.alpha, pp.tr_scale);
+                float tr = pp.nee_baked == 1   ? mip_tr(v, 0, spos)
+                           : pp.nee_baked == 2 ? sample_clamp(pp.ctr, pp.ctr_res, spos + 0.5f)
+                                               : tr_stochastic(v, rs, spos, light, ld, view.alpha, pp.tr_scale);
                 res = res + view.light_color * csr * (tr * phase);
                 rdir = ndir;
             }
@@ -687,6 +689,43 @@ void pt_pixel(const Volume& v, const View& view, const PTParams& pp, int row, in
         }
     }
     for (int c = 0; c < 3; ++c) { mean3[c] = mean[c]; m2_3[c] = m2[c]; }
+}
+
+float cached_tr_texel(const Volume& v, int res, long long i, f3 light, float alpha, float tr_scale, uint32_t seed) {
+    const int z = (int)(i % res), y = (int)((i / res) % res), x = (int)(i / ((long long)res * res));
+    const f3 pos = (mk((float)x, (float)y, (float)z) + 0.5f) / (float)res - 0.5f;  // volume.cu:1570-1573
+    const int k = density(v, pos) > 0.0f ? 5 : 2;                                   // :1574
+    const float dis = ray_box_distance(pos, light);
+    float acc = 0.0f;
+    for (int e = 0; e < k; ++e) {
+        mrp::PhiloxStream rs(seed, 1u, (uint32_t)i, mrp::RNG_BAKE, (uint32_t)e);
+        acc += tr_stochastic(v, rs, pos, light, dis, alpha, tr_scale);
+    }
+    return acc / (float)k;  // TR_MUL = 1 (volume.cu:885)
+}
+
+f3 point_sample(const Volume& v, mrp::PhiloxStream& rs, f3 ori, f3 dir, f3 light, f3 light_color, float scatter,
+                float alpha, float g, int max_bounce, float tr_scale) {
+    f3 res = mk(0, 0, 0);
+    const float e0 = rs.next();
+    const float e1 = rs.next();
+    f3 spos = ori, rdir = sample_hg(e0, e1, dir, g), csr = mk(1, 1, 1);  // render.cuh:623-624
+    long long taps = 0;
+    for (int b = 0; b < max_bounce; ++b) {
+        f3 npos, ndir;
+        const float dis = ray_box_distance(spos, rdir);
+        if (!next_vertex(v, rs, alpha, g, spos, rdir, dis, &npos, &ndir, &taps)) {  // :631-636
+            res = res + sky_box(v, rdir) * csr;
+            break;
+        }
+        spos = npos;
+        csr = (csr * 1.001f) * scatter;  // :640
+        const float ld = ray_box_distance(spos, light);
+        const float phase = henyey_greenstein(dot(rdir, light), g);
+        res = res + light_color * csr * (tr_stochastic(v, rs, spos, light, ld, alpha, tr_scale) * phase);  // :642-644
+        rdir = ndir;
+    }
+    return res;
 }
 
 // ---------------------------------------------------------------------------------------

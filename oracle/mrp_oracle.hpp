@@ -487,11 +487,20 @@ float halton_scrambled(unsigned dim, unsigned index);                           
 // Path tracer (render.cuh:60-103, 399-434, 535-597) with Philox and FP64 Welford.
 struct PTParams {
     int spp, max_bounce;
-    int nee_baked;  // 1: NEE via TR mip 0 as the reference (render.cuh:589); 0: stochastic Tr
+    int nee_baked;  // 0: stochastic Tr; 1: TR mip 0 (render.cuh:589); 2: the cached bake below
     float tr_scale;
+    const float* ctr = nullptr;  // nee_baked 2: ctr_res^3 cached transmittance texels
+    int ctr_res = 0;
 };
 float tr_stochastic(const Volume& v, mrp::PhiloxStream& rs, f3 pos, f3 dir, float dis, float alpha, float tr_scale);
 void pt_pixel(const Volume& v, const View& view, const PTParams& pp, int row, int col, double* mean3, double* m2_3,
               long long* taps);
+// Fill_BakeTR (volume.cu:1562-1587) texel i of a res^3 grid: mean of 5 (density > 0 at the
+// centre) or 2 stochastic Tr estimates towards light (unit), estimate e from the Philox stream
+// (seed, 1) / (i, RNG_BAKE, e).
+float cached_tr_texel(const Volume& v, int res, long long i, f3 light, float alpha, float tr_scale, uint32_t seed);
+// GetSample (render.cuh:615-654), one path from stream rs.
+f3 point_sample(const Volume& v, mrp::PhiloxStream& rs, f3 ori, f3 dir, f3 light, f3 light_color, float scatter,
+                float alpha, float g, int max_bounce, float tr_scale);
 
 }  // namespace oracle

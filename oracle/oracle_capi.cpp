@@ -377,6 +377,81 @@ int orc_pt_render_list(const float* grid, int N, float max_density, const float*
     return 0;
 }
 
+// The path tracer with NEE from a cached transmittance bake (nee_baked = 2) on listed pixels.
+int orc_pt_render_list_cached(const float* grid, int N, float max_density, const float* ctr, int ctr_res,
+                              const orc_view* ov, const int32_t* pix, int n, int spp, int max_bounce, float tr_scale,
+                              double* mean3, double* m2_3) {
+    Volume v = to_volume(grid, N, max_density, nullptr, nullptr, nullptr);
+    View view = to_view(ov);
+    PTParams pp{spp, max_bounce, 2, tr_scale, ctr, ctr_res};
+    const int W = view.width;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < n; ++q) {
+        long long t = 0;
+        pt_pixel(v, view, pp, pix[q] / W, pix[q] % W, mean3 + 3 * (size_t)q, m2_3 + 3 * (size_t)q, &t);
+    }
+    return 0;
+}
+
+// Listed texels of the cached transmittance bake (Fill_BakeTR) of a res^3 grid.
+void orc_bake_cached_tr(const float* grid, int N, float max_density, const float* light, float alpha, float tr_scale,
+                        uint32_t seed, int res, const int64_t* texels, int n, float* out) {
+    Volume v = to_volume(grid, N, max_density, nullptr, nullptr, nullptr);
+    const f3 L = normalize(mk(light[0], light[1], light[2]));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int q = 0; q < n; ++q) out[q] = cached_tr_texel(v, res, texels[q], L, alpha, tr_scale, seed);
+}
+
+// GetSamples: per task the FP64 mean / M2 over spp point-radiance paths (streams (seed, 0) / (i, s)).
+void orc_point_radiance(const float* grid, int N, float max_density, int n, const float* ori3, const float* dir3,
+                        const float* light3, const float* alpha, const float* g, const float* scatter,
+                        const float* light_color, int multi_scatter, int spp, uint32_t seed, float tr_scale,
+                        double* mean3, double* m2_3) {
+    Volume v = to_volume(grid, N, max_density, nullptr, nullptr, nullptr);
+    const f3 lc = mk(light_color[0], light_color[1], light_color[2]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i) {
+        const f3 ori = mk(ori3[3 * i], ori3[3 * i + 1], ori3[3 * i + 2]);
+        const f3 dir = mk(dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]);
+        const f3 L = mk(light3[3 * i], light3[3 * i + 1], light3[3 * i + 2]);
+        double mean[3] = {0, 0, 0}, m2[3] = {0, 0, 0};
+        for (int s = 0; s < spp; ++s) {
+            mrp::PhiloxStream rs(seed, 0u, (uint32_t)i, mrp::RNG_POINT, (uint32_t)s);
+            const f3 r = point_sample(v, rs, ori, dir, L, lc, scatter[i], alpha[i], g[i], multi_scatter, tr_scale);
+            const double x[3] = {r.x, r.y, r.z};
+            for (int c = 0; c < 3; ++c) {
+                const double d = x[c] - mean[c];
+                mean[c] += d / (double)(s + 1);
+                m2[c] += d * (x[c] - mean[c]);
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            mean3[3 * (size_t)i + c] = mean[c];
+            m2_3[3 * (size_t)i + c] = m2[c];
+        }
+    }
+}
+
+// ShadowTerm_TR(s): per task the mean over spp of Tr(ori -> light) * HG(dot(dir, light), g).
+void orc_tr_samples(const float* grid, int N, float max_density, int n, const float* ori3, const float* dir3,
+                    const float* light, float alpha, float g, int spp, uint32_t seed, float tr_scale, double* out) {
+    Volume v = to_volume(grid, N, max_density, nullptr, nullptr, nullptr);
+    const f3 L = normalize(mk(light[0], light[1], light[2]));
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i) {
+        const f3 ori = mk(ori3[3 * i], ori3[3 * i + 1], ori3[3 * i + 2]);
+        const f3 dir = mk(dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]);
+        const float dis = ray_box_distance(ori, L);
+        const float phase = henyey_greenstein(dot(dir, L), g);
+        double acc = 0.0;
+        for (int s = 0; s < spp; ++s) {
+            mrp::PhiloxStream rs(seed, 2u, (uint32_t)i, mrp::RNG_TR, (uint32_t)s);
+            acc += (double)(tr_stochastic(v, rs, ori, L, dis, alpha, tr_scale) * phase);
+        }
+        out[i] = acc / (double)spp;
+    }
+}
+
 // Stochastic transmittance estimates (n independent streams) along one ray.
 void orc_tr_stochastic(const float* grid, int N, float max_density, const float* pos, const float* dir, float dis,
                        float alpha, float tr_scale, uint32_t seed, int n, float* out) {

@@ -154,6 +154,69 @@ int main(int argc, char** argv) {
         // the per-frame Render of the camera: name + "MRPNN.png" with the ACES tone map
         cam.name = dir + "/demo_cam";
         cam.Render(sun, float3{1, 1, 1}, 32.0f, 512, 0.857f, 1, Camera::ACES, VolumeRender::MRPNN);
+        // host probes (dataGen.cu:74,173) on the pyramids of the last MRPNN frame (TR for sun,
+        // alpha 32; LUT for g 0.857), with the arrays they read written out for the test
+        {
+            std::vector<float> mips(19173961), tr(2396745), lut(512 * 512);
+            if (mrp_volume_download_mips(v.handle(), mips.data()) != MRP_OK ||
+                mrp_volume_download_tr(v.handle(), tr.data()) != MRP_OK ||
+                mrp_ctx_download_hg_lut(default_context(), lut.data()) != MRP_OK)
+                return 5;
+            FILE* f = fopen((dir + "/probe_arrays.bin").c_str(), "wb");
+            if (!f) return 6;
+            fwrite(mips.data(), 4, mips.size(), f);
+            fwrite(tr.data(), 4, tr.size(), f);
+            fwrite(lut.data(), 4, lut.size(), f);
+            fclose(f);
+            const float3 pts[5] = {{0.1f, -0.05f, 0.2f}, {-0.3f, 0.12f, 0.07f}, {0.0f, 0.0f, 0.0f},
+                                   {0.45f, 0.3f, -0.2f}, {0.7f, -0.6f, 0.1f}};
+            printf("probes");
+            for (const float3& q : pts) {
+                printf(" %.9g %.9g %.9g %.9g %.9g %.9g", v.DensityAtPosition(0, q), v.DensityAtPosition(2, q),
+                       v.DensityAtPosition(1.25f, q), v.DensityAtUV(1, float3{q.x + 0.5f, q.y + 0.5f, q.z + 0.5f}),
+                       v.TrAtPosition(0, q, sun), v.TrAtPosition(3, q, sun));
+            }
+            const float hg[4][2] = {{0.3f, 0.2f}, {-0.7f, 0.9f}, {0.99f, 0.01f}, {-1.0f, 1.2f}};
+            for (const auto& c : hg) printf(" %.9g", v.GetHGLut(c[0], c[1]));
+            printf("\n");
+            // stochastic probes: GetSamples / GetTrs equal the C ABI calls they wrap (seed 0)
+            std::vector<float3> so = {{0.05f, 0.1f, -0.1f}, {-0.2f, 0.0f, 0.15f}, {0.2f, -0.25f, 0.0f}};
+            std::vector<float3> sd = {{1, 0, 0}, {0, 0, 1}, {0.6f, 0.8f, 0.0f}};
+            std::vector<float3> sl = {sun, sun, sun};
+            std::vector<float> sa = {32, 32, 16}, sg = {0.857f, 0.5f, 0.857f}, ss = {0.99f, 0.99f, 0.9f};
+            auto gs = v.GetSamples(sa, so, sd, sl, sg, ss, float3{1, 1, 1}, 64, 256);
+            auto gt = v.GetTrs(32.0f, so, sd, sun, float3{1, 0.5f, 0.25f}, 0.857f, 64);
+            float3 t1 = v.GetTr(so[0], sd[0], sun, 32.0f, 0.857f, 40);
+            float *d_in = nullptr, *d_out = nullptr, *d_t = nullptr;
+            if (hipMalloc((void**)&d_in, 3 * 15 * sizeof(float)) != hipSuccess ||
+                hipMalloc((void**)&d_out, 9 * sizeof(float)) != hipSuccess ||
+                hipMalloc((void**)&d_t, 3 * sizeof(float)) != hipSuccess)
+                return 4;
+            std::vector<float> hin;
+            for (auto* arr : {&so, &sd, &sl})
+                for (const float3& q : *arr) hin.insert(hin.end(), {q.x, q.y, q.z});
+            hin.insert(hin.end(), sa.begin(), sa.end());
+            hin.insert(hin.end(), sg.begin(), sg.end());
+            hin.insert(hin.end(), ss.begin(), ss.end());
+            if (hipMemcpy(d_in, hin.data(), hin.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return 4;
+            const float lc[3] = {1, 1, 1}, L[3] = {sun.x, sun.y, sun.z};
+            if (mrp_point_radiance(default_context(), v.handle(), 3, d_in, d_in + 9, d_in + 18, d_in + 27, d_in + 30,
+                                   d_in + 33, lc, 64, 256, 0u, 1.0f, d_out, nullptr) != MRP_OK ||
+                mrp_tr_samples(default_context(), v.handle(), 3, d_in, d_in + 9, L, 32.0f, 0.857f, 64, 0u, 1.0f, d_t) !=
+                    MRP_OK)
+                return 5;
+            float hout[9], ht[3];
+            if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(hout, d_out, sizeof hout, hipMemcpyDeviceToHost) ||
+                hipMemcpy(ht, d_t, sizeof ht, hipMemcpyDeviceToHost) != hipSuccess)
+                return 4;
+            int samples_equal = memcmp(hout, gs.data(), sizeof hout) == 0;
+            for (int i = 0; i < 3; ++i)
+                samples_equal &= gt[i].x == 1.0f * 3.1415926535f * ht[i] && gt[i].z == 0.25f * 3.1415926535f * ht[i];
+            (void)hipFree(d_in);
+            (void)hipFree(d_out);
+            (void)hipFree(d_t);
+            printf("stochastic %.9g %.9g %.9g %.9g samples_equal %d\n", gs[0].x, gs[1].y, gs[2].z, t1.x, samples_equal);
+        }
         printf("active %d mean_pred %.6f mean_pt %.6f lit_pt %d lit_pred %d active_rpnn %d mean_rpnn %.6f "
                "render_equal %d gather_equal %d gather_active %d\n", act, s / img.size(), sp / pt.size(), lit_pt,
                lit_pred, act_rp, srp / rp.size(), render_equal, gather_equal, gather_active);

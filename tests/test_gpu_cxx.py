@@ -6,6 +6,7 @@ Camera::ACES agree with the oracle's display-path tone maps."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -32,7 +33,6 @@ def test_cxx_demo_renders_pngs(tmp_path):
     assert abs(active_rpnn - active) <= 0.05 * active and float(words[words.index("mean_rpnn") + 1]) > 0
     assert int(words[words.index("render_equal") + 1]) == 1 and int(words[words.index("gather_equal") + 1]) == 1
     assert int(words[words.index("gather_active") + 1]) > 20  # the 12x12 camera sees the ball
-    import numpy as np
     import oracle as O
     tones = np.array([float(x) for x in r.stdout.split("tones")[1].split("\n")[0].split()]).reshape(3, 2, 3)
     cols = np.array([[0.5, 0.2, 0.1], [2.0, 1.0, 0.25], [0.01, 0.03, 0.02]], np.float32)
@@ -43,3 +43,91 @@ def test_cxx_demo_renders_pngs(tmp_path):
     for name in ("demo_mrpnn.png", "demo_rpnn.png", "demo_pt.png", "demo_camMRPNN.png"):
         data = open(tmp_path / name, "rb").read()
         assert data[:8] == b"\x89PNG\r\n\x1a\n" and len(data) > 96 * 96 * 4
+
+
+def _lerp(a, b, t):
+    f = np.float32
+    return f(a) * (f(1) - f(t)) + f(b) * f(t)
+
+
+def _tri(data, res, uv, clamp):
+    """Sample / SampleClamp (core/volume.cu:835-880) in float32."""
+    f = np.float32
+    pos = [f(u) * f(res) - f(0.5) for u in uv]
+    i0 = [int(np.floor(p)) for p in pos]
+    w = [f(p - f(i)) for p, i in zip(pos, i0)]
+
+    def at(x, y, z):
+        if clamp:
+            x, y, z = (min(max(c, 0), res - 1) for c in (x, y, z))
+        elif min(x, y, z) < 0 or max(x, y, z) >= res:
+            return f(0)
+        return max(f(0), data[(x * res + y) * res + z])
+    x, y, z = i0
+    return _lerp(_lerp(_lerp(at(x, y, z), at(x, y, z + 1), w[2]), _lerp(at(x, y + 1, z), at(x, y + 1, z + 1), w[2]), w[1]),
+                 _lerp(_lerp(at(x + 1, y, z), at(x + 1, y, z + 1), w[2]),
+                       _lerp(at(x + 1, y + 1, z), at(x + 1, y + 1, z + 1), w[2]), w[1]), w[0])
+
+
+def _ray_box_offset(p, d):
+    """RayBoxOffset (core/vector.cu:46-63): entry distance into [-0.4999, 0.4999]^3, -1 on a miss."""
+    f = np.float32
+    inv = [f(999999999.0) if c == 0 else f(1) / f(c) for c in d]
+    to = [f(-pc) * ic for pc, ic in zip(p, inv)]
+    a2 = [f(0.4999) * ic for ic in inv]
+    d0 = [a + b for a, b in zip(to, a2)]
+    d1 = [a - b for a, b in zip(to, a2)]
+    tmi = max(min(a, b) for a, b in zip(d0, d1))
+    tma = min(max(a, b) for a, b in zip(d0, d1))
+    return max(tmi, f(0)) if tma >= tmi else f(-1)
+
+
+def test_cxx_host_probes(tmp_path):
+    """VolumeRender::DensityAtPosition / DensityAtUV / TrAtPosition / GetHGLut (core/volume.cu:
+    1767-1803, 1424-1442) on the demo's pyramids equal a float32 restatement of the reference's
+    host lookups on the same arrays (written out by the demo); GetSamples / GetTrs equal the C ABI
+    calls they wrap bit for bit; GetTr and the point radiance are finite and in range."""
+    if not os.path.exists(DEMO):
+        pytest.skip("demo not built (run __graft_entry__.build())")
+    r = subprocess.run([DEMO, str(tmp_path), "48"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = [float(x) for x in r.stdout.split("probes")[1].split("\n")[0].split()]
+    raw = np.fromfile(tmp_path / "probe_arrays.bin", np.float32)
+    nm, nt = 19173961, 2396745
+    mips, tr, lut = raw[:nm], raw[nm:nm + nt], raw[nm + nt:]
+    moff = np.cumsum([0] + [(256 >> m) ** 3 for m in range(9)])
+    toff = np.cumsum([0] + [(128 >> m) ** 3 for m in range(8)])
+    mip = lambda m: mips[moff[m]:moff[m + 1]]
+    trm = lambda m: tr[toff[m]:toff[m + 1]]
+    sun = np.array([0.5, 0.8, 0.33], np.float32)
+    sun = (sun / np.float32(np.sqrt(np.float32(sun @ sun)))).astype(np.float32)
+    pts = np.array([[0.1, -0.05, 0.2], [-0.3, 0.12, 0.07], [0.0, 0.0, 0.0], [0.45, 0.3, -0.2], [0.7, -0.6, 0.1]],
+                   np.float32)
+    f = np.float32
+    want = []
+    for q in pts:
+        uv = q + f(0.5)
+        d = lambda m: _tri(mip(m), 256 >> m, uv, False)
+        outside = (q < -0.5).any() or (q > 0.5).any()
+
+        def trat(m):
+            if outside:
+                off = _ray_box_offset(q, sun)
+                return _tri(trm(m), 128 >> m, q + f(0.5) + sun * off, True) if off >= 0 else f(1)
+            return _tri(trm(m), 128 >> m, uv, True)
+        want += [d(0), d(2), _lerp(d(1), d(2), 0.25), d(1), trat(0), trat(3)]
+    for cs, ang in ((0.3, 0.2), (-0.7, 0.9), (0.99, 0.01), (-1.0, 1.2)):
+        u = f(f(cs) * 0.5 + 0.5)
+        v = f(f(ang) / (3.1415926535 * 60.0 / 180.0))
+        cl = lambda i: min(max(i, 0), 511)
+        u0, v0, u1, v1 = cl(int(u * f(512))), cl(int(v * f(512))), cl(int(u * f(512)) + 1), cl(int(v * f(512)) + 1)
+        fv, fu = v * f(512) - np.floor(v * f(512)), u * f(512) - np.floor(u * f(512))
+        ax0 = _lerp(lut[u0 + v0 * 512], lut[u1 + v0 * 512], fv)
+        ax1 = _lerp(lut[u0 + v1 * 512], lut[u1 + v1 * 512], fv)
+        want.append(_lerp(ax0, ax1, fu))
+    np.testing.assert_allclose(vals, np.array(want, np.float64), rtol=1e-5, atol=1e-9)  # float32 rounding order
+    assert any(x > 0 for x in vals[:6]) and vals[4 * 6 + 4] > 0  # the ball is dense at the centre; light reaches
+    st = r.stdout.split("stochastic")[1].split()
+    assert int(st[st.index("samples_equal") + 1]) == 1
+    s = [float(x) for x in st[:4]]
+    assert all(np.isfinite(s)) and all(x >= 0 for x in s) and s[3] <= 0.25 * (1 + 0.857) / (1 - 0.857) ** 2
