@@ -79,7 +79,7 @@ MRP_HD float sample_hg_cos(float s, float g) {  // vector.cu:216-223
 }
 MRP_HD v3 sample_hg(float e0, float e1, v3 v, float g) {  // vector.cu:225-242
     float ct = sample_hg_cos(e0, g);
-    float phi = (float)(2.0 * 3.14159265359 * (double)e1);
+    const float phi = 6.28318531f * e1;  // 2 pi e1 (vector.cu:232, double there)
     float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
     v3 v2;
     if (fabsf(v.z) > 0.70710678118654757f) v2 = normalize3(cross3(v, mk3(0, 1, 0)));
@@ -504,7 +504,8 @@ __device__ inline void basis_from_normal(v3 v1, const ViewDev& vw, uint32_t pix,
         float e0 = rs.next() * 0.1f;
         float e1 = rs.next() * 0.1f;
         v3 r = uniform_sample_sphere(e0, e1);
-        if ((double)fabsf(dot3(r, v1)) < 0.00001) continue;  // double compare as the reference
+        // the reference's double compare |d| < 1e-5: float(1e-5) is the largest float below 1e-5
+        if (fabsf(dot3(r, v1)) <= 0.00001f) continue;
         v2 = normalize3(cross3(v1, r));
         v3_ = cross3(v2, v1);
         break;

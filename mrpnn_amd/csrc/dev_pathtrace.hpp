@@ -24,7 +24,8 @@ __device__ inline float tr_stochastic(const DevVolume& v, PhiloxStream& rs, v3 p
         tr *= 1.0f - fmaxf(0.0f, S / smax);
         if (d < 0.0f) t -= d;
     }
-    while (loop++ < 10000 && (double)tr > 0.001) {
+    // the reference's tr > 0.001 in double: float(0.001) is the smallest float above 0.001
+    while (loop++ < 10000 && tr >= 0.001f) {
         float rk = rs.next();
         t -= logf(1.0f - rk) / smax * tr_scale;
         if (t > dis) break;

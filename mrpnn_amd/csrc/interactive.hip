@@ -220,7 +220,7 @@ __global__ __launch_bounds__(128) void k_denoise_pixels(const float* __restrict_
                 sum += used ? qv : 0.0f;
             }
             const float cmp = p == 0.0f ? 0.0f : sum / p;
-            s_w[tid] = (float)fmax(0.0, 1.0 - 1.2 * (double)cmp);
+            s_w[tid] = fmaxf(0.0f, 1.0f - 1.2f * cmp);
             s_t[0][tid] = target[3 * nid];
             s_t[1][tid] = target[3 * nid + 1];
             s_t[2][tid] = target[3 * nid + 2];

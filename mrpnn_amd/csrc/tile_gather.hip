@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_tile_gather(DevVolume vol, const Stenci
         // forever (|dot| < 1e-5) the basis keeps (0, 1, 0), (0, 0, 1)
         v3 X = normalize3(avg), Y = mk3(0.0f, 1.0f, 0.0f), Z = mk3(0.0f, 0.0f, 1.0f);
         const v3 r = uniform_sample_sphere(0.0f, 0.0f);
-        if (!((double)fabsf(dot3(r, X)) < 0.00001)) {
+        if (!(fabsf(dot3(r, X)) <= 0.00001f)) {  // |d| < 1e-5 in double: float(1e-5) is the largest float below
             Y = normalize3(cross3(X, r));
             Z = cross3(Y, X);
         }

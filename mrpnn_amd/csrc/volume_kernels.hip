@@ -248,9 +248,11 @@ __global__ __launch_bounds__(64) void k_hg_lut(float* __restrict__ lut, float g,
     const int i = blockIdx.x * 8 + (threadIdx.x & 7);
     const int j = blockIdx.y * 8 + (threadIdx.x >> 3);
     if (i >= 512 || j >= 512) return;
-    float costheta = (float)((((double)(float)i + 0.5) / 512) * 2.0 - 1.0);
-    float aa = (float)(((double)(float)j + 0.5) / 512);
-    float angle = (float)((double)aa * 3.1415926535 * 60.0 / 180.0);
+    // texel centres (i + 0.5) / 512 and their cos are exact in float; the cone angle is
+    // aa * pi/3 (volume.cu:1330-1332 in double; one float rounding here)
+    const float costheta = ((float)i + 0.5f) / 512.0f * 2.0f - 1.0f;
+    const float aa = ((float)j + 0.5f) / 512.0f;
+    const float angle = aa * 1.04719755f;
     PhiloxStream rs(seed, 0u, (uint32_t)(i + j * 512), RNG_LUT, 0u);
     float bn = rs.next();
     float hg = 0.0f;
@@ -287,8 +289,8 @@ __global__ __launch_bounds__(64) void k_hg_lut(float* __restrict__ lut, float g,
         } else {
             float rx = frac_ref(halton_scrambled(0, k) + bn), ry = frac_ref(halton_scrambled(1, k) + bn),
                   rz = frac_ref(halton_scrambled(2, k) + bn);
-            float theta = (float)((double)rx * 2.0 * 3.1415926535);
-            float phi = (float)acos(2.0 * (double)ry - 1.0);
+            const float theta = rx * 6.28318531f;
+            const float phi = acosf(2.0f * ry - 1.0f);
             float r = powf(rz, 1.0f / 3.0f);
             float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);
             float x = R * r * sp * ct, y = R * r * sp * st, z = R * r * cp;

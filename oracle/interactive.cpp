@@ -184,7 +184,7 @@ void orc_denoise(const float* target, const void* histo_v, uint32_t* rgba, int W
                 for (int i = -5; i <= 5; i++)
                     for (int j = -5; j <= 5; j++) {
                         int t = unroll(r + i, c + j, W, H);
-                        float w = (float)fmax(0.0, 1.0f - 1.2 * (double)compare(center, histo[t]));
+                        float w = fmaxf(0.0f, 1.0f - 1.2f * compare(center, histo[t]));  // single precision as the library
                         res = res + ld(target, t) * w;
                         ws += w;
                     }

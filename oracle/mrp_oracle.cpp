@@ -472,9 +472,10 @@ void hg_lut_qmc(float g, uint32_t seed, float* lut) {
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < LUT; ++j)
         for (int i = 0; i < LUT; ++i) {
-            float costheta = (float)((((double)(float)i + 0.5) / LUT) * 2.0 - 1.0);
-            float aa = (float)(((double)(float)j + 0.5) / LUT);
-            float angle = (float)((double)aa * 3.1415926535 * 60.0 / 180.0);
+            // volume.cu:1330-1332 (double there); the library's single-precision form
+            const float costheta = ((float)i + 0.5f) / (float)LUT * 2.0f - 1.0f;
+            const float aa = ((float)j + 0.5f) / (float)LUT;
+            const float angle = aa * 1.04719755f;
             mrp::PhiloxStream rs(seed, 0u, (uint32_t)(i + j * LUT), mrp::RNG_LUT, 0u);
             float bn = rs.next();
             float hg = 0.0f;
@@ -515,8 +516,8 @@ void hg_lut_qmc(float g, uint32_t seed, float* lut) {
                 } else {
                     float rx = frac_(halton_scrambled(0, k) + bn), ry = frac_(halton_scrambled(1, k) + bn),
                           rz = frac_(halton_scrambled(2, k) + bn);
-                    float theta = (float)((double)rx * 2.0 * 3.1415926535);
-                    float phi = (float)acos(2.0 * (double)ry - 1.0);
+                    const float theta = rx * 6.28318531f;
+                    const float phi = acosf(2.0f * ry - 1.0f);
                     float r = powf(rz, 1.0f / 3.0f);
                     float st = sinf(theta), ct = cosf(theta), sp = sinf(phi), cp = cosf(phi);
                     float x = R * r * sp * ct, y = R * r * sp * st, z = R * r * cp;

@@ -112,7 +112,7 @@ inline float sample_hg_cos(float s, float g) {
 }
 inline f3 sample_hg(float e0, float e1, f3 v, float g) {
     float ct = sample_hg_cos(e0, g);
-    float phi = (float)(2.0 * 3.14159265359 * (double)e1);
+    const float phi = 6.28318531f * e1;  // 2 pi e1 (vector.cu:232), single precision as the library
     float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
     f3 v2, v3;
     if ((double)fabsf(v.z) > sqrt(0.5)) v2 = normalize(cross(v, mk(0, 1, 0)));
