@@ -301,6 +301,7 @@ def c4(args):
     bake_ms = (time.perf_counter() - t0) * 1e3
     best_w, best_ms = ctx.autotune(vols, views, outs, M.BF16)
     best_lds = ctx.get_option(M.OPT_LDS_WEIGHTS)
+    best_tile = ctx.get_option(M.OPT_TILE_W)
     ctx.set_option(M.OPT_GRAPH, 1)
     res = {}
     for p in ("bf16", "fp8"):
@@ -330,8 +331,8 @@ def c4(args):
     ctx.close()
     return {"config": "c4", "workload": "64 views, 8 volumes (2 x 128^3, 256^3, 384^3, 512^3), mixed resolutions",
             "pixels_per_batch": pixels, "tr_bake_ms_64_suns": round(bake_ms, 1),
-            "autotune": {"waves_per_cu": best_w, "lds_weights": best_lds,
-                         "ms": round(best_ms, 3)}, "graph": True,
+            "autotune": {"march_tile": "%dx%d" % (best_tile, 64 // best_tile), "waves_per_cu": best_w,
+                         "lds_weights": best_lds, "ms": round(best_ms, 3)}, "graph": True,
             "bf16": res["bf16"], "fp8": res["fp8"],
             "fp32_plain_mpix_s_incl_tr_rebake": round(pixels / plain_s / 1e6, 1)}
 
