@@ -285,7 +285,8 @@ extern "C" int mrp_ctx_upload_hg_lut(mrp_ctx* c, const float* host, float g) {
 
 extern "C" int64_t mrp_volume_bytes(int N) {
     const int64_t on = (N + 8) >> 3;
-    return (int64_t)((size_t)N * N * N + kMipFloats + kTrFloats + kPadMipFloats + kPadTrFloats) * 4 + on * on * on;
+    return (int64_t)((size_t)N * N * N + kMipFloats + kTrFloats + kPadMipFloats + kPadTrFloats) * 4 + kQuadTexels * 16 +
+           on * on * on;
 }
 
 extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
@@ -304,11 +305,13 @@ extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
     if (hipMalloc(&v->d_mips, kMipFloats * sizeof(float)) != hipSuccess ||
         hipMalloc(&v->d_tr, kTrFloats * sizeof(float)) != hipSuccess ||
         hipMalloc(&v->d_pmips, kPadMipFloats * sizeof(float)) != hipSuccess ||
+        hipMalloc(&v->d_qmips, kQuadTexels * sizeof(float4)) != hipSuccess ||
         hipMalloc(&v->d_ptr, kPadTrFloats * sizeof(float)) != hipSuccess) {
         hipFree(v->d_grid);
         hipFree(v->d_mips);
         hipFree(v->d_tr);
         hipFree(v->d_pmips);
+        hipFree(v->d_qmips);
         delete v;
         return mrp_fail(MRP_ERR_NOMEM, "pyramid allocation failed");
     }
@@ -319,12 +322,14 @@ extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
         hipFree(v->d_mips);
         hipFree(v->d_tr);
         hipFree(v->d_pmips);
+        hipFree(v->d_qmips);
         hipFree(v->d_ptr);
         delete v;
         return mrp_fail(MRP_ERR_NOMEM, "occupancy allocation failed");
     }
     HIP_TRY(hipMemsetAsync(v->d_occ, 0, occ_bytes, c->stream));
     HIP_TRY(hipMemsetAsync(v->d_pmips, 0, kPadMipFloats * sizeof(float), c->stream));
+    HIP_TRY(hipMemsetAsync(v->d_qmips, 0, kQuadTexels * sizeof(float4), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_ptr, 0, kPadTrFloats * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_grid, 0, n * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_mips, 0, kMipFloats * sizeof(float), c->stream));
@@ -352,6 +357,7 @@ extern "C" int mrp_volume_destroy(mrp_volume* v) {
     hipFree(v->d_mips);
     hipFree(v->d_tr);
     hipFree(v->d_pmips);
+    hipFree(v->d_qmips);
     hipFree(v->d_ptr);
     hipFree(v->d_occ);
     hipFree(v->d_ctr);
@@ -448,6 +454,7 @@ extern "C" int mrp_volume_update(mrp_volume* v) {
         const float* mips[9];
         for (int m = 0; m < 9; ++m) mips[m] = v->mip[m];
         HIP_TRY(launch_pad_mips(mips, v->d_pmips, s));
+        HIP_TRY(launch_quad_mips(v->d_pmips, v->d_qmips, s));
     }
     HIP_TRY(launch_occupancy(v->d_grid, v->N, v->d_occ, v->occ_n, s));
     if (tmp[0] || tmp[1]) {
@@ -500,7 +507,7 @@ int mrp_tr_bake_into(mrp_volume* v, const float L[3], float alpha, float* d_tr, 
         tr[m] = d_tr + o;
         o += (size_t)(128 >> m) * (128 >> m) * (128 >> m);
     }
-    HIP_TRY(launch_tr_bake(mips, v->d_pmips, tr, d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), s));
+    HIP_TRY(launch_tr_bake(mips, v->d_pmips, v->d_qmips, tr, d_ptr, alpha_lvl, mk3(L[0], L[1], L[2]), s));
     return MRP_OK;
 }
 

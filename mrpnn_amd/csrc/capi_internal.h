@@ -90,6 +90,7 @@ struct mrp_volume {
     float* d_mips = nullptr;
     float* d_tr = nullptr;
     float* d_pmips = nullptr;  // zero-padded density mips 0..7
+    float4* d_qmips = nullptr; // (y, z) quads of padded density mips 1..7 (TR bake)
     float* d_ptr = nullptr;    // zero-padded TR mips 0..6
     uint8_t* d_occ = nullptr;  // brick emptiness for the first-hit march, occ_n^3
     int occ_n = 0;

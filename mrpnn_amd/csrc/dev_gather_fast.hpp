@@ -8,7 +8,10 @@
 //     range, so every tap is in bounds and no per-tap border test is needed; the clamp also
 //     reproduces the border (density) and clamp (TR) addressing exactly;
 //   * every tap pair (z, z+1) is one 64-bit buffer load at a 32-bit offset from a uniform
-//     buffer resource (hardware bounds-checked);
+//     buffer resource;
+//   * the 144 points of stencil chunks 16..159 share one density mip per 16-point chunk, so
+//     their level geometry is a compile-time constant (gather_chunk): no per-point table reads
+//     of it, the texel index is two 24-bit multiply-adds by constants;
 //   * the per-point constants (direction * offset, 1/|direction|, LUT v, mip geometry) come
 //     from a 192-entry table; for sphere points the first LUT lookup depends only on the
 //     point (the basis is orthonormal, dot(XMain, direction) = dx/|d|), so it is tabulated
@@ -35,6 +38,11 @@ MRP_HD constexpr long long ptr_offset(int m) {  // TR mips 0..6, base 128
 }
 constexpr long long kPadMipFloats = pmip_offset(8);
 constexpr long long kPadTrFloats = ptr_offset(7);
+// (y, z) quads of the padded density mips 1..7 (the TR bake's sources): texel (x, y, z) of a
+// padded level holds (p(x,y,z), p(x,y,z+1), p(x,y+1,z), p(x,y+1,z+1)), so a trilinear blend
+// is two 16-byte loads (slices x and x + 1).  Offsets in quads.
+MRP_HD constexpr long long qmip_offset(int m) { return pmip_offset(m) - pmip_offset(1); }
+constexpr long long kQuadTexels = qmip_offset(8);
 
 // One stencil point of the fast gather (80 bytes).
 struct FastStencil {
@@ -43,7 +51,7 @@ struct FastStencil {
     float off;            // offset (light points)
     float hg0;            // LUT(dx/|d| * 0.5 + 0.5, v) (sphere points)
     float dres, tres;     // density / TR mip resolution
-    float dR, tR;         // padded dims (exact small integers)
+    unsigned dRu, tRu;    // padded dims
     unsigned dbase, tbase;  // padded level offsets (bytes)
     unsigned dR4, dRR4;     // 4 * R, 4 * R * R (byte strides of the density level)
     unsigned tR4, tRR4;     // same for the TR level
@@ -64,6 +72,7 @@ constexpr int kLutRow = 516;
 // 64-bit load and no 64-bit address arithmetic.
 typedef __amdgpu_buffer_rsrc_t BufRsrc;
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 __device__ inline BufRsrc make_rsrc(const float* p, long long n_floats) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n_floats * 4), 0x00020000);
 }
@@ -71,10 +80,25 @@ __device__ inline f32x2_t ld_pair(BufRsrc r, int byte_off) {
     return __builtin_bit_cast(f32x2_t, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
 }
 
+// a * b + c for a, b < 2^24 (v_mad_u32_u24, full rate).  Written out: for the texel index the
+// compiler otherwise emits a 64-bit multiply-add or a 32-bit multiply (both quarter rate)
+// although every operand is a small non-negative integer.  _s: b wave-uniform (scalar).
+__device__ inline unsigned mad_u24_s(unsigned a, unsigned b, unsigned c) {
+    unsigned d;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+__device__ inline unsigned mad_u24_v(unsigned a, unsigned b, unsigned c) {
+    unsigned d;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
 // Per-tile (uniform) inputs.
 struct FastView {
     BufRsrc pmips, ptr, rows;  // padded density mips, padded TR mips, LUT rows
     v3 L, LX, invLX;  // light direction as the gather uses it; LX = normalize(L); inv3(LX)
+    v3 a2f;           // 0.4999 * invLX (the box half-width over the light direction)
     float alpha64;    // alpha / 64
 };
 // Per-pixel inputs.
@@ -106,9 +130,32 @@ __device__ inline TapGeo tap_geo(unsigned R, float cx, float cy, float cz) {
     g.wy = __builtin_amdgcn_fractf(cy);
     g.wz = __builtin_amdgcn_fractf(cz);
     // through int: defined for the negative coordinates of lanes whose result is discarded
+    // (their index is then any 32-bit value; such lanes only occur where the caller clamps)
     const unsigned x = (unsigned)(int)cx, y = (unsigned)(int)cy, z = (unsigned)(int)cz;
-    g.o4 = (__umul24(__umul24(x, R) + y, R) + z) * 4u;
+    g.o4 = mad_u24_v(mad_u24_v(x, R, y), R, z) * 4u;
     return g;
+}
+// The same with a compile-time R (scalar operand).
+template <unsigned R>
+__device__ inline TapGeo tap_geo_k(float cx, float cy, float cz) {
+    TapGeo g;
+    g.wx = __builtin_amdgcn_fractf(cx);
+    g.wy = __builtin_amdgcn_fractf(cy);
+    g.wz = __builtin_amdgcn_fractf(cz);
+    const unsigned x = (unsigned)(int)cx, y = (unsigned)(int)cy, z = (unsigned)(int)cz;
+    g.o4 = mad_u24_s(mad_u24_s(x, R, y), R, z) * 4u;
+    return g;
+}
+// Trilinear blend on a level at byte offset BASE with strides R4 = 4R, RR4 = 4R^2, all
+// compile-time.  (Addressing the four tap pairs through the instruction's scalar offset instead
+// of three vector adds was measured 5 % slower on the network kernel.)
+template <unsigned BASE, unsigned R4, unsigned RR4>
+__device__ inline float tri_k(BufRsrc r, const TapGeo& g) {
+    const int o = (int)(g.o4 + BASE);
+    const f32x2_t c00 = ld_pair(r, o), c01 = ld_pair(r, o + (int)R4);
+    const f32x2_t c10 = ld_pair(r, o + (int)RR4), c11 = ld_pair(r, o + (int)(RR4 + R4));
+    return flerp(flerp(flerp(c00.x, c00.y, g.wz), flerp(c01.x, c01.y, g.wz), g.wy),
+                 flerp(flerp(c10.x, c10.y, g.wz), flerp(c11.x, c11.y, g.wz), g.wy), g.wx);
 }
 __device__ inline float tri_geo(BufRsrc r, unsigned base, unsigned R4, unsigned RR4, const TapGeo& g) {
     const int o = (int)(g.o4 + base);
@@ -117,9 +164,9 @@ __device__ inline float tri_geo(BufRsrc r, unsigned base, unsigned R4, unsigned 
     return flerp(flerp(flerp(c00.x, c00.y, g.wz), flerp(c01.x, c01.y, g.wz), g.wy),
                  flerp(flerp(c10.x, c10.y, g.wz), flerp(c11.x, c11.y, g.wz), g.wy), g.wx);
 }
-__device__ inline float tri_taps(BufRsrc r, unsigned base, float R, unsigned R4, unsigned RR4, float cx, float cy,
+__device__ inline float tri_taps(BufRsrc r, unsigned base, unsigned R, unsigned R4, unsigned RR4, float cx, float cy,
                                  float cz) {
-    return tri_geo(r, base, R4, RR4, tap_geo(R4 >> 2, cx, cy, cz));
+    return tri_geo(r, base, R4, RR4, tap_geo(R, cx, cy, cz));
 }
 // Padded continuous coordinate of p on a level of resolution res: data coordinate
 // (p + 0.5)*res - 0.5, plus one for the padding.
@@ -131,14 +178,14 @@ __device__ inline v3 padded_coord(v3 p, float res) {
 // outside -> exact 0).
 __device__ inline float tri_border_c(BufRsrc r, const FastStencil& s, v3 c) {
     const float hi = s.dres + 1.0f;
-    return tri_taps(r, s.dbase, s.dR, s.dR4, s.dRR4, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi),
+    return tri_taps(r, s.dbase, s.dRu, s.dR4, s.dRR4, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi),
                     clampf(c.z, 0.0f, hi));
 }
 // Clamp-addressing trilinear (TR mips): data coordinate clamped to [0, res-1], i.e. padded
 // coordinate to [1, res].
 __device__ inline float tri_clamp_c(BufRsrc r, const FastStencil& s, v3 c) {
     const float hi = s.tres;
-    return tri_taps(r, s.tbase, s.tR, s.tR4, s.tRR4, clampf(c.x, 1.0f, hi), clampf(c.y, 1.0f, hi),
+    return tri_taps(r, s.tbase, s.tRu, s.tR4, s.tRR4, clampf(c.x, 1.0f, hi), clampf(c.y, 1.0f, hi),
                     clampf(c.z, 1.0f, hi));
 }
 // Point-level LUT lookup: linear interpolation along u of the point's LUT row.
@@ -157,11 +204,10 @@ __device__ inline float hg_fast(float cs, float g) {
 
 // Zero-border trilinear on padded level data of resolution res at byte offset base.
 __device__ inline float tri_border_lvl(BufRsrc r, unsigned base, float res, v3 p) {
-    const float R = res + 3.0f;
-    const unsigned Ru = (unsigned)R;
+    const unsigned Ru = (unsigned)(res + 3.0f);
     const v3 c = padded_coord(p, res);
     const float hi = res + 1.0f;
-    return tri_taps(r, base, R, Ru * 4u, Ru * Ru * 4u, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi),
+    return tri_taps(r, base, Ru, Ru * 4u, Ru * Ru * 4u, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi),
                     clampf(c.z, 0.0f, hi));
 }
 // 512x512 LUT, bilinear, clamp (used to build the rows).
@@ -195,8 +241,7 @@ __device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3
         if (outside) {
             // RayBoxOffset(p, LX) (vector.cu:46-63) with inv(LX) precomputed
             const v3 to_axil = (-p) * fv.invLX;
-            const v3 a2f = mk3(0.4999f, 0.4999f, 0.4999f) * fv.invLX;
-            const v3 d0 = to_axil + a2f, d1 = to_axil - a2f;
+            const v3 d0 = to_axil + fv.a2f, d1 = to_axil - fv.a2f;
             const v3 tmin = vmin3(d0, d1), tmax = vmax3(d0, d1);
             const float tmi = fmaxf(tmin.x, fmaxf(tmin.y, tmin.z));
             const float tma = fminf(tmax.x, fminf(tmax.y, tmax.z));
@@ -205,7 +250,7 @@ __device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3
             q = mk3(fmaf(fv.LX.x, off, p.x), fmaf(fv.LX.y, off, p.y), fmaf(fv.LX.z, off, p.z));
         }
         const v3 c = padded_coord(q, s.tres);
-        g = tap_geo(s.tR4 >> 2, clampf(c.x, 1.0f, s.tres), clampf(c.y, 1.0f, s.tres), clampf(c.z, 1.0f, s.tres));
+        g = tap_geo(s.tRu, clampf(c.x, 1.0f, s.tres), clampf(c.y, 1.0f, s.tres), clampf(c.z, 1.0f, s.tres));
     }
     const float T = tri_geo(fv.ptr, s.tbase, s.tR4, s.tRR4, g);
     return miss ? 1.0f : T;
@@ -236,15 +281,70 @@ __device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const
     const bool outside = outside_box(p);
     // inside the box the padded coordinate is in [0.5, res + 0.5]: no clamp needed; lanes
     // outside clamp to the zero border [0, res + 1] (skipped when no lane of the wave is outside)
-    TapGeo g = tap_geo(s.dR4 >> 2, c.x, c.y, c.z);
+    TapGeo g = tap_geo(s.dRu, c.x, c.y, c.z);
     if (__builtin_amdgcn_ballot_w64(outside) != 0 && outside) {
         const float hi = s.dres + 1.0f;
-        g = tap_geo(s.dR4 >> 2, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi), clampf(c.z, 0.0f, hi));
+        g = tap_geo(s.dRu, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi), clampf(c.z, 0.0f, hi));
     }
     const float rho = tri_geo(fv.pmips, s.dbase, s.dR4, s.dRR4, g);
     const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
     const float xs = shadow_fast(fv, s, p, g, same, outside);
     return mk3(xv, xh, xs);
+}
+
+// Light-ray entry of a point outside the box (RayBoxOffset, vector.cu:46-63, with inv(LX) and
+// 0.4999 * inv(LX) precomputed): the entry point q, miss = the ray passes the box.
+__device__ inline v3 light_entry(const FastView& fv, v3 p, bool& miss) {
+    const v3 to_axil = (-p) * fv.invLX;
+    const v3 d0 = to_axil + fv.a2f, d1 = to_axil - fv.a2f;
+    const v3 tmin = vmin3(d0, d1), tmax = vmax3(d0, d1);
+    const float tmi = fmaxf(tmin.x, fmaxf(tmin.y, tmin.z));
+    const float tma = fminf(tmax.x, fminf(tmax.y, tmax.z));
+    miss = !(tma >= tmi);
+    const float off = fmaxf(tmi, 0.0f);
+    return mk3(fmaf(fv.LX.x, off, p.x), fmaf(fv.LX.y, off, p.y), fmaf(fv.LX.z, off, p.z));
+}
+
+// (X_Val, X_Val_Hg, X_Val_Sub) of a sphere point of a chunk whose density mip M (2..7) is a
+// compile-time constant; TR level M - 1 has the same resolution.  The same arithmetic as
+// gather_fast<false>(..., centre = false, same = true); only the addressing differs.
+template <int M>
+__device__ inline v3 gather_chunk(const FastView& fv, const FastStencil& s, const FastPixel& px) {
+    static_assert(M >= 2 && M <= 7, "uniform chunks use density mips 2..7");
+    constexpr unsigned res = 256u >> M, R = res + 3u;
+    constexpr unsigned DB = (unsigned)pmip_offset(M) * 4u, TB = (unsigned)ptr_offset(M - 1) * 4u;
+    constexpr unsigned R4 = 4u * R, RR4 = 4u * R * R;
+    static_assert(res == (128u >> (M - 1)), "TR level M - 1 has the density level's resolution");
+    constexpr float fres = (float)res;
+    v3 p;
+    p.x = fmaf(px.X.x, s.ddx, fmaf(px.Y.x, s.ddy, fmaf(px.Z.x, s.ddz, px.pos.x)));
+    p.y = fmaf(px.X.y, s.ddx, fmaf(px.Y.y, s.ddy, fmaf(px.Z.y, s.ddz, px.pos.y)));
+    p.z = fmaf(px.X.z, s.ddx, fmaf(px.Y.z, s.ddy, fmaf(px.Z.z, s.ddz, px.pos.z)));
+    const float c2 = fmaf(px.ax, s.ddx, fmaf(px.ay, s.ddy, px.az * s.ddz)) * s.cscale;
+    const float h1 = lut_row(fv.rows, s.lbase, fmaf(c2, 0.5f, 0.5f));
+    const float xh = log_ge1(fmaf(s.hg0, h1, 1.0f));
+    v3 c = padded_coord(p, fres);
+    const bool outside = outside_box(p);
+    const bool any_out = __builtin_amdgcn_ballot_w64(outside) != 0;
+    // zero border: clamp to [0, res + 1] (inside the box the coordinate is in [0.5, res + 0.5],
+    // which the clamp leaves alone; the rings of mips >= 5 reach past the box in most
+    // wavefronts, so they clamp unconditionally)
+    if (M >= 5 || (any_out && outside))
+        c = mk3(clampf(c.x, 0.0f, fres + 1.0f), clampf(c.y, 0.0f, fres + 1.0f), clampf(c.z, 0.0f, fres + 1.0f));
+    const TapGeo g = tap_geo_k<R>(c.x, c.y, c.z);
+    const float rho = tri_k<DB, R4, RR4>(fv.pmips, g);
+    const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
+    // ShadowTerm_TRTex: inside the box the density blend's corner and weights (replicated TR
+    // edges); outside, the light ray's entry point, clamp addressing
+    TapGeo gt = g;
+    bool miss = false;
+    if (any_out && outside) {
+        const v3 q = light_entry(fv, p, miss);
+        const v3 ct = padded_coord(q, fres);
+        gt = tap_geo_k<R>(clampf(ct.x, 1.0f, fres), clampf(ct.y, 1.0f, fres), clampf(ct.z, 1.0f, fres));
+    }
+    const float T = tri_k<TB, R4, RR4>(fv.ptr, gt);
+    return mk3(xv, xh, miss ? 1.0f : T);
 }
 
 }  // namespace mrp

@@ -14,9 +14,11 @@ hipError_t launch_fbm(float* out, int N, int octaves, float base_freq, const int
                       hipStream_t s);
 hipError_t launch_box2(const float* src, float* dst, int res, hipStream_t s);
 hipError_t launch_resample(const float* src, int sres, float* dst, int res, hipStream_t s);
-hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, float* const tr[8], float* tr_pad,
+hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, const float4* qmips, float* const tr[8],
+                          float* tr_pad,
                           const float alpha_lvl[8], v3 light, hipStream_t s);
 hipError_t launch_pad_mips(const float* const mips[9], float* pmips, hipStream_t s);
+hipError_t launch_quad_mips(const float* pmips, float4* qmips, hipStream_t s);
 hipError_t launch_occupancy(const float* grid, int N, uint8_t* occ, int occ_n, hipStream_t s);
 hipError_t launch_fast_stencil(const StencilEntry* st, const float* lut, void* fast, float* lut_rows, hipStream_t s);
 hipError_t launch_hg_lut(float* lut, float g, uint32_t seed, hipStream_t s);

@@ -613,13 +613,22 @@ struct FastGatherSrc {
             for (int i = 0; i < 4; ++i) {
                 const int k = p0 + 16 * h + 4 * lc.q + i;
                 v3 f = mk3(0.0f, 0.0f, 0.0f);
-                // density and TR levels share the resolution for stencil mips >= 1: all points
+                // chunks 16..159: one density mip per chunk, known here (gather_chunk); chunk 0
+                // and the light points mix two mips across the lane rows (table geometry).
+                // Density and TR levels share the resolution for stencil mips >= 1: all points
                 // of chunks 16..159 and the second half of the light points
                 const bool same = (p0 >= 16 && p0 < 160) || (p0 == 160 && h == 1);
-                if (lc.ok) {
-                    if (p0 >= 160) f = gather_fast<true>(fv, st[k], px, false, same);
-                    else f = gather_fast<false>(fv, st[k], px, p0 == 0 && h == 0 && i == 0 && lc.q == 0, same);
-                }
+                // the lane test also keeps the compiler from interleaving consecutive points
+                // (without it the kernel spills)
+                if (!lc.ok) {
+                } else if (p0 == 16) f = gather_chunk<2>(fv, st[k], px);
+                else if (p0 == 32) f = gather_chunk<3>(fv, st[k], px);
+                else if (p0 == 48) f = gather_chunk<4>(fv, st[k], px);
+                else if (p0 == 64) f = gather_chunk<5>(fv, st[k], px);
+                else if (p0 == 96) f = gather_chunk<6>(fv, st[k], px);
+                else if (p0 == 128) f = gather_chunk<7>(fv, st[k], px);
+                else if (p0 >= 160) f = gather_fast<true>(fv, st[k], px, false, same);
+                else f = gather_fast<false>(fv, st[k], px, p0 == 0 && h == 0 && i == 0 && lc.q == 0, same);
                 F[0][4 * h + i] = f.x;
                 F[2][4 * h + i] = f.y;
                 F[1][4 * h + i] = f.z;
@@ -644,6 +653,7 @@ __device__ inline FastGatherSrc fast_src(const DevVolume& vol, const ViewConst& 
     src.fv.L = ps.L;
     src.fv.LX = ps.LX;
     src.fv.invLX = vc.invLX;
+    src.fv.a2f = mk3(0.4999f, 0.4999f, 0.4999f) * vc.invLX;
     src.fv.alpha64 = vc.alpha64;
     src.px.pos = ps.pos;
     src.px.X = ps.X;
@@ -689,8 +699,8 @@ __device__ inline void block_dense(const NetRs& net, const LaneCtx& lc, const fl
         float in[1][8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            float v = F[b][i] * s[b];
-            if (has_res) v = v + R[b][i];
+            // one rounding (the sum is rounded to bf16 / fp8 for the MFMA right after)
+            const float v = has_res ? fmaf(F[b][i], s[b], R[b][i]) : F[b][i] * s[b];
             in[0][i] = (lane_has && (NM == 2 || i < 4)) ? v : 0.0f;
         }
         float o[NM][4];

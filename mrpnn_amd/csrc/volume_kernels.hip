@@ -129,8 +129,10 @@ struct TrArgs {
     float* tr[8];
     float* tr_pad;       // zero-padded copy of levels 0..6 (fast gather), may be null
     const float* pmips;  // zero-padded density mips 0..7, may be null (exact border sampling)
+    const float4* qmips; // (y, z) quads of the padded mips 1..7 (k_tr_bake_lvl)
     float alpha_lvl[8];
     long long start[9];  // flattened texel offsets of each level
+    int bstart[9];       // k_tr_bake_lvl: first workgroup of each level
     v3 light;
 };
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_tr_bake(TrArgs a) {
@@ -217,6 +219,94 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 for (int py = lo[1]; py <= hi[1]; ++py)
                     for (int pz = lo[2]; pz <= hi[2]; ++pz) dst[((long long)px * R + py) * R + pz] = T;
         }
+    }
+}
+
+// The bake over the (y, z) quads of the zero-padded density mips, one level per workgroup: the
+// level, its resolution and strides are wave-uniform (scalar registers), texel coordinates are
+// shifts of the thread index, a sample's eight taps are two 16-byte loads (slices x and x + 1),
+// and each thread keeps MRP_TR_BAKE_ROUND samples' loads in flight.  The wavefront's 64 lanes
+// scatter over tens of cache lines per load (their rays leave the box at different distances, so
+// their samples drift apart), and the texture pipeline's time follows the cache lines it
+// touches: two quad loads per sample touch half the lines of four pair loads.  The arithmetic of
+// every texel is k_tr_bake's (same positions, taps, blends and summation order).
+#ifndef MRP_TR_BAKE_ROUND
+#define MRP_TR_BAKE_ROUND 4
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_tr_bake_lvl(TrArgs a) {
+    constexpr int K = MRP_TR_BAKE_ROUND;
+    static_assert(128 % K == 0, "the round size divides the 128 steps");
+    int m = 0;
+    while (m < 7 && (int)blockIdx.x >= a.bstart[m + 1]) ++m;
+    const int lres = 7 - m, res = 1 << lres;
+    const unsigned i_lvl = (unsigned)((int)blockIdx.x - a.bstart[m]) * 256u + threadIdx.x;
+    if (i_lvl >= (1u << (3 * lres))) return;
+    const int z = (int)(i_lvl & (unsigned)(res - 1));
+    const int y = (int)((i_lvl >> lres) & (unsigned)(res - 1));
+    const int x = (int)(i_lvl >> (2 * lres));
+    v3 uv = (mk3((float)x, (float)y, (float)z) + 0.5f) / (float)res;
+    v3 ori = uv - 0.5f;
+    float dis = ray_box_distance(ori, a.light);
+    float st = dis / 128.0f;
+    v3 lp = ori;
+    float sum = 0.0f;
+    const int sres = 256 >> (m + 1);
+    if (m + 1 >= 8) {  // level 7 samples the 1^3 mip 8, which has no padded copy
+        const float* src = a.mips[m + 1];
+        for (int i = 0; i < 128; ++i) {
+            lp = lp + a.light * st;
+            sum = sum + sample_border_exact(src, sres, lp + 0.5f);
+        }
+        a.tr[m][i_lvl] = expf(-sum * a.alpha_lvl[m] * st) * 1.0f;
+        return;
+    }
+    const unsigned base = (unsigned)qmip_offset(m + 1) * 16u;
+    const float fres = (float)sres;
+    const float R = fres + 3.0f;
+    const unsigned Ru = (unsigned)R;
+    const unsigned RR16 = Ru * Ru * 16u;
+    const BufRsrc rs = __builtin_amdgcn_make_buffer_rsrc((void*)a.qmips, (short)0, (int)(kQuadTexels * 16), 0x00020000);
+    for (int i = 0; i < 128; i += K) {
+        float wx[K], wy[K], wz[K];
+        int o[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            lp = lp + a.light * st;
+            const v3 c = padded_coord(lp, fres);
+            wx[k] = __builtin_amdgcn_fractf(c.x);
+            wy[k] = __builtin_amdgcn_fractf(c.y);
+            wz[k] = __builtin_amdgcn_fractf(c.z);
+            // source levels 1..7 (at most 131^3 padded texels < 2^24): exact in float
+            const float lin = fmaf(fmaf(floorf(c.x), R, floorf(c.y)), R, floorf(c.z));
+            o[k] = (int)((unsigned)lin * 16u + base);
+        }
+        f32x4_t q0[K], q1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            q0[k] = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(rs, o[k], 0, 0));
+            q1[k] = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(rs, o[k] + (int)RR16, 0, 0));
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            // quad (x, y) = (y, z), (y, z+1), (y+1, z), (y+1, z+1) of slice x
+            const float v = flerp(flerp(flerp(q0[k].x, q0[k].y, wz[k]), flerp(q0[k].z, q0[k].w, wz[k]), wy[k]),
+                                  flerp(flerp(q1[k].x, q1[k].y, wz[k]), flerp(q1[k].z, q1[k].w, wz[k]), wy[k]),
+                                  wx[k]);
+            sum = sum + v;
+        }
+    }
+    const float T = expf(-sum * a.alpha_lvl[m] * st) * 1.0f;
+    a.tr[m][i_lvl] = T;
+    if (a.tr_pad && m < 7) {
+        // padded copy with replicated edges (as k_tr_bake)
+        const int Rp = pad_dim(res);
+        const int lo[3] = {x == 0 ? 0 : x + 1, y == 0 ? 0 : y + 1, z == 0 ? 0 : z + 1};
+        const int hi[3] = {x == res - 1 ? res + 2 : x + 1, y == res - 1 ? res + 2 : y + 1,
+                           z == res - 1 ? res + 2 : z + 1};
+        float* dst = a.tr_pad + ptr_offset(m);
+        for (int px = lo[0]; px <= hi[0]; ++px)
+            for (int py = lo[1]; py <= hi[1]; ++py)
+                for (int pz = lo[2]; pz <= hi[2]; ++pz) dst[(px * Rp + py) * Rp + pz] = T;
     }
 }
 
@@ -322,6 +412,27 @@ __global__ __launch_bounds__(256) void k_pad_mips(PadArgs a, float* __restrict__
     }
 }
 
+// (y, z) quads of the padded density mips 1..7 (every padded texel, the border included; taps
+// beyond the padded level read 0).
+__global__ __launch_bounds__(256) void k_quad_mips(const float* __restrict__ pm, float4* __restrict__ q) {
+    const long long total = kQuadTexels;
+    for (long long g = (long long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long long)gridDim.x * 256) {
+        int m = 1;
+        while (m < 7 && g >= qmip_offset(m + 1)) ++m;
+        const int R = pad_dim(256 >> m);
+        const int l = (int)(g - qmip_offset(m));
+        const int z = l % R, y = (l / R) % R;
+        const float* p = pm + pmip_offset(m);
+        const bool zn = z + 1 < R, yn = y + 1 < R;
+        float4 o;
+        o.x = p[l];
+        o.y = zn ? p[l + 1] : 0.0f;
+        o.z = yn ? p[l + R] : 0.0f;
+        o.w = (yn && zn) ? p[l + R + 1] : 0.0f;
+        q[g] = o;
+    }
+}
+
 // Brick emptiness of the full-resolution grid for the first-hit march: brick b covers
 // texels [8b-1, 8b+7] per axis (the corners a sample in that brick can read); 1 = all zero.
 __global__ __launch_bounds__(256) void k_occupancy(const float* __restrict__ grid, int N, uint8_t* __restrict__ occ,
@@ -365,8 +476,8 @@ __global__ void k_fast_stencil(const StencilEntry* __restrict__ st, const float*
     const unsigned dR = (unsigned)pad_dim(256 >> e.mip), tR = (unsigned)pad_dim(128 >> tm);
     f.dres = (float)(256 >> e.mip);
     f.tres = (float)(128 >> tm);
-    f.dR = (float)dR;
-    f.tR = (float)tR;
+    f.dRu = dR;
+    f.tRu = tR;
     f.dbase = (unsigned)pmip_offset(e.mip) * 4u;
     f.tbase = (unsigned)ptr_offset(tm) * 4u;
     f.dR4 = dR * 4u;
@@ -417,7 +528,8 @@ hipError_t launch_resample(const float* src, int sres, float* dst, int res, hipS
                        res);
     return hipGetLastError();
 }
-hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, float* const tr[8], float* tr_pad,
+hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, const float4* qmips, float* const tr[8],
+                          float* tr_pad,
                           const float alpha_lvl[8], v3 light, hipStream_t s) {
     TrArgs a;
     a.tr_pad = tr_pad;
@@ -432,7 +544,18 @@ hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, float*
     }
     a.start[8] = acc;
     a.light = light;
-    hipLaunchKernelGGL(k_tr_bake, dim3(grid_for((size_t)acc, 256)), dim3(256), 0, s, a);
+    a.qmips = qmips;
+    if (pmips && qmips) {
+        int nb = 0;
+        for (int m = 0; m < 8; ++m) {
+            a.bstart[m] = nb;
+            nb += (int)(((128 >> m) * (128 >> m) * (128 >> m) + 255) / 256);
+        }
+        a.bstart[8] = nb;
+        hipLaunchKernelGGL(k_tr_bake_lvl, dim3(nb), dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_tr_bake, dim3(grid_for((size_t)acc, 256)), dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 hipError_t launch_pad_mips(const float* const mips[9], float* pmips, hipStream_t s) {
@@ -445,6 +568,10 @@ hipError_t launch_pad_mips(const float* const mips[9], float* pmips, hipStream_t
     }
     a.start[8] = acc;
     hipLaunchKernelGGL(k_pad_mips, dim3(grid_for((size_t)acc, 256)), dim3(256), 0, s, a, pmips);
+    return hipGetLastError();
+}
+hipError_t launch_quad_mips(const float* pmips, float4* qmips, hipStream_t s) {
+    hipLaunchKernelGGL(k_quad_mips, dim3(grid_for((size_t)kQuadTexels, 256)), dim3(256), 0, s, pmips, qmips);
     return hipGetLastError();
 }
 hipError_t launch_occupancy(const float* grid, int N, uint8_t* occ, int occ_n, hipStream_t s) {

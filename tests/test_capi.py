@@ -53,8 +53,10 @@ def test_argument_errors_are_status_codes(lib):
     # mips 0..7 and TR mips 0..6
     pad_d = sum(((256 >> m) + 3) ** 3 for m in range(8))
     pad_t = sum(((128 >> m) + 3) ** 3 for m in range(7))
+    # + (y, z) quads (16 bytes per padded texel) of density mips 1..7 for the TR bake
+    quads = sum(((256 >> m) + 3) ** 3 for m in range(1, 8))
     # + one emptiness byte per 8^3 brick of the march ((N + 8) // 8 per axis)
-    assert lib.mrp_volume_bytes(512) == (512 ** 3 + 19173961 + 2396745 + pad_d + pad_t) * 4 + 65 ** 3
+    assert lib.mrp_volume_bytes(512) == (512 ** 3 + 19173961 + 2396745 + pad_d + pad_t) * 4 + quads * 16 + 65 ** 3
 
 
 def test_version_string(lib):
