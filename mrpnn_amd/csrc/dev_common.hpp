@@ -199,6 +199,20 @@ __device__ inline float sample_lut_exact(const float* __restrict__ lut, float u,
 // 64-bit global loads at clamped in-grid indices, one 64-bit slab address per x tap and
 // 32-bit offsets inside a slab, then the border taps are zeroed by selects (no branches).
 // Needs N >= 2.
+// a * b + c for a, b < 2^24 (v_mad_u32_u24, full rate).  Written out: for the texel index the
+// compiler otherwise emits a 64-bit multiply-add or a 32-bit multiply (both quarter rate)
+// although every operand is a small non-negative integer.  _s: b wave-uniform (scalar).
+__device__ inline unsigned mad_u24_s(unsigned a, unsigned b, unsigned c) {
+    unsigned d;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+__device__ inline unsigned mad_u24_v(unsigned a, unsigned b, unsigned c) {
+    unsigned d;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
 typedef float f32x2u_t __attribute__((ext_vector_type(2), aligned(4)));
 typedef __attribute__((address_space(1))) const f32x2u_t* gpair_ptr;
 struct FullTap {  // integer corner and weights of one sample
@@ -209,13 +223,41 @@ __device__ inline FullTap full_tap(int N, v3 p) {
     const v3 uv = p + 0.5f;
     const v3 pos = uv * (float)N - 0.5f;
     FullTap t;
-    t.x = (int)floorf(pos.x);
-    t.y = (int)floorf(pos.y);
-    t.z = (int)floorf(pos.z);
-    t.w = pos - mk3((float)t.x, (float)t.y, (float)t.z);
+    const v3 fl = mk3(floorf(pos.x), floorf(pos.y), floorf(pos.z));
+    t.x = (int)fl.x;
+    t.y = (int)fl.y;
+    t.z = (int)fl.z;
+    t.w = pos - fl;
     return t;
 }
+// All eight taps inside the grid (corner in [0, N-2]^3): pair loads at 32-bit byte offsets
+// ((x*N + y)*N + z)*4 from a buffer resource, no border selects; the same taps and blend as
+// the border path below.  Grids up to 1023^3 (byte offsets below 2^32).
+__device__ inline bool full_tap_inner(int N, const FullTap& t) {
+    return max((unsigned)t.x, max((unsigned)t.y, (unsigned)t.z)) <= (unsigned)(N - 2);
+}
+__device__ inline float density_inner(const DevVolume& v, const FullTap& t) {
+    const unsigned N = (unsigned)v.N;
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void*)v.grid, (short)0, (int)(N * N * N * 4u), 0x00020000);
+    const unsigned o = mad_u24_v(mad_u24_v((unsigned)t.x, N, (unsigned)t.y), N, (unsigned)t.z) * 4u;
+    const unsigned s1 = 4u * N, s2 = 4u * N * N;
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 c00 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, 0, 0));
+    const f2 c01 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(o + s1), 0, 0));
+    const f2 c10 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(o + s2), 0, 0));
+    const f2 c11 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(o + s2 + s1), 0, 0));
+    const v3 w = t.w;
+    return lerp_ref(lerp_ref(lerp_ref(c00.x, c00.y, w.z), lerp_ref(c01.x, c01.y, w.z), w.y),
+                    lerp_ref(lerp_ref(c10.x, c10.y, w.z), lerp_ref(c11.x, c11.y, w.z), w.y), w.x);
+}
+__device__ inline float density_border_at(const DevVolume& v, const FullTap& t);
+// Interior fast path when no lane of the wavefront is near a face, else the border path.
 __device__ inline float density_full_at(const DevVolume& v, const FullTap& t) {
+    if (v.N <= 1023 && __builtin_amdgcn_ballot_w64(!full_tap_inner(v.N, t)) == 0) return density_inner(v, t);
+    return density_border_at(v, t);
+}
+__device__ inline float density_border_at(const DevVolume& v, const FullTap& t) {
     const int N = v.N;
     const int z0 = min(max(t.z, 0), N - 2);  // pair (z0, z0 + 1) lies inside the row
     const bool zlo = t.z >= 0, zhi = t.z + 1 < N;
@@ -281,7 +323,7 @@ __device__ inline bool tap_in_empty_brick(const DevVolume& v, const FullTap& t, 
     const int n = v.occ_n;
     const int bx = (t.x + 1) >> 3, by = (t.y + 1) >> 3, bz = (t.z + 1) >> 3;
     const bool in = (unsigned)bx < (unsigned)n && (unsigned)by < (unsigned)n && (unsigned)bz < (unsigned)n;
-    const int key = in ? (bx * n + by) * n + bz : -1;
+    const int key = in ? (int)mad_u24_v(mad_u24_v((unsigned)bx, (unsigned)n, (unsigned)by), (unsigned)n, (unsigned)bz) : -1;
     if (key != bc.key) {
         bc.key = key;
         bc.empty = in && *(const __attribute__((address_space(1))) uint8_t*)(v.occ + key) != 0;

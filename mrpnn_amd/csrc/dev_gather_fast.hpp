@@ -80,20 +80,6 @@ __device__ inline f32x2_t ld_pair(BufRsrc r, int byte_off) {
     return __builtin_bit_cast(f32x2_t, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
 }
 
-// a * b + c for a, b < 2^24 (v_mad_u32_u24, full rate).  Written out: for the texel index the
-// compiler otherwise emits a 64-bit multiply-add or a 32-bit multiply (both quarter rate)
-// although every operand is a small non-negative integer.  _s: b wave-uniform (scalar).
-__device__ inline unsigned mad_u24_s(unsigned a, unsigned b, unsigned c) {
-    unsigned d;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
-    return d;
-}
-__device__ inline unsigned mad_u24_v(unsigned a, unsigned b, unsigned c) {
-    unsigned d;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
-    return d;
-}
-
 // Per-tile (uniform) inputs.
 struct FastView {
     BufRsrc pmips, ptr, rows;  // padded density mips, padded TR mips, LUT rows
