@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD every 4 cycles at the
+# 2.4 GHz peak engine clock (wave64 instructions per second / 1e9)
+VALU_PEAK_GINSTR = 256 * 4 * 2.4 / 4
 FLOP_PER_PIXEL = 97072  # 2 x 48,536 MAC per pixel, one distinct colour channel
 GATHER_BYTES_PER_PIXEL = 18412  # logical fp32 texel bytes of the 192-point gather (+ LUT pairs, output)
 
@@ -187,9 +190,18 @@ def main():
     gather_gbs = GATHER_BYTES_PER_PIXEL * n_active / (pred_ms * 1e-3) / 1e9
     march_gbs = 32.0 * taps / (march_ms * 1e-3) / 1e9
     traffic = None
+    valu = None
     tpath = os.path.join(ROOT, "profiles", "traffic_mlp_mfma_active_bf16_1080p_512.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        prof = json.load(open(tpath))
+        traffic = prof.get("hbm_bytes_per_launch")
+        per_tile = prof.get("valu_instr_per_16px_tile")
+        if per_tile:
+            # the network kernel issues VALU instructions, not MFMAs, at its bound: rocprof's VALU
+            # instruction count per 16-pixel tile over this run's kernel time, against the issue peak
+            g = per_tile * (n_active / 16.0) / (pred_ms * 1e-3) / 1e9
+            valu = {"bound": "valu", "achieved": round(g, 1), "peak": VALU_PEAK_GINSTR, "unit": "Gwave-instr/s",
+                    "frac": round(g / VALU_PEAK_GINSTR, 4), "valu_instr_per_tile": per_tile}
 
     cpu = None
     if not args.no_cpu:
@@ -227,6 +239,7 @@ def main():
                       "predict_plain_fp32": round(kstats["plain"]["predict_ms"], 3)},
         "roofline": {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tf / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": traffic},
+        "roofline_valu": valu,
         "roofline_gather": {"bound": "hbm", "achieved": round(gather_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(gather_gbs / HBM_PEAK_GBS, 4),
                             "note": "logical fp32 texel bytes of the gather; > 1 means on-chip reuse"},
