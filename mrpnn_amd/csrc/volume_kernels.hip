@@ -233,14 +233,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 #ifndef MRP_TR_BAKE_ROUND
 #define MRP_TR_BAKE_ROUND 4
 #endif
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_tr_bake_lvl(TrArgs a) {
+// TR texel i_lvl of level m (1 << 3 (7 - m) texels) over the quads.
+__device__ inline void tr_quad_texel(const TrArgs& a, int m, unsigned i_lvl) {
     constexpr int K = MRP_TR_BAKE_ROUND;
     static_assert(128 % K == 0, "the round size divides the 128 steps");
-    int m = 0;
-    while (m < 7 && (int)blockIdx.x >= a.bstart[m + 1]) ++m;
     const int lres = 7 - m, res = 1 << lres;
-    const unsigned i_lvl = (unsigned)((int)blockIdx.x - a.bstart[m]) * 256u + threadIdx.x;
-    if (i_lvl >= (1u << (3 * lres))) return;
     const int z = (int)(i_lvl & (unsigned)(res - 1));
     const int y = (int)((i_lvl >> lres) & (unsigned)(res - 1));
     const int x = (int)(i_lvl >> (2 * lres));
@@ -307,6 +304,243 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         for (int px = lo[0]; px <= hi[0]; ++px)
             for (int py = lo[1]; py <= hi[1]; ++py)
                 for (int pz = lo[2]; pz <= hi[2]; ++pz) dst[(px * Rp + py) * Rp + pz] = T;
+    }
+}
+// Block blk of the levels a.bstart covers (256 texels per workgroup of 256 threads, or
+// nthreads per workgroup), level from a.bstart.
+__device__ inline void tr_quad_block(const TrArgs& a, int blk, int nthreads) {
+    int m = 0;
+    while (m < 7 && blk >= a.bstart[m + 1]) ++m;
+    const int lres = 7 - m;
+    const unsigned i_lvl = (unsigned)(blk - a.bstart[m]) * (unsigned)nthreads + threadIdx.x;
+    if (i_lvl >= (1u << (3 * lres))) return;
+    tr_quad_texel(a, m, i_lvl);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_tr_bake_lvl(TrArgs a) {
+    tr_quad_block(a, (int)blockIdx.x, 256);
+}
+
+// ------------------------------------------------------------------------------------------
+// TR level 0 (87 % of the bake) from LDS.  Every sample of a texel lies on the ray from the
+// texel towards the light, so in sheared coordinates along the light's principal axis A
+// (u' = c_U - r_U c_A, v' = c_V - r_V c_A with r = L / L_A, |r| <= 1, padded texel coordinates
+// c) a texel's whole ray stays at its own (u', v').  A workgroup takes the texels of one 8 x 8
+// tile of (u', v'), stages for each of the 130 padded slabs c_A = s the 14 x 14 window of density
+// mip 1 that the tile's rays cross there (window origin floor(u'_lo + r_U s - |r_U| - 0.01),
+// wide enough for the tile, the ray's slope over the slab pair a blend touches, the second tap
+// and float slack of the running position sum), and blends every sample from LDS: the texture
+// pipeline then carries no tap traffic and the concurrent march has it to itself.  Each texel's
+// arithmetic is k_tr_bake_lvl's (same positions, taps, blend and summation order), only the
+// tap loads come from LDS.
+// ------------------------------------------------------------------------------------------
+constexpr int kL0Tile = 4;              // tile side in (u', v'), texels
+constexpr int kL0Win = kL0Tile + 6;     // window side per slab
+constexpr int kL0Slabs = 130;           // padded slabs that samples touch (c_A in [0.5, 128.5])
+constexpr int kL0Cand = kL0Tile + 3;    // candidate texels per axis and layer
+// texels per tile: 8 per axis and layer in exact arithmetic, at most 9 with rounding at the
+// tile's edges
+constexpr int kL0List = 128 * (kL0Tile + 1) * (kL0Tile + 1);
+constexpr int kL0Threads = 512;
+struct BakeL0 {
+    float ru, rv;      // L_U / L_A, L_V / L_A
+    float umin, vmin;  // (u', v') origin of tile (0, 0)
+    int ntu;           // tiles along u'
+    int nlvl;          // the first nlvl workgroups take levels 1..7 (a.bstart, kL0Threads texels each)
+    int up;            // L_A > 0: rays run towards higher slabs
+};
+__device__ inline int l0_origin(float lo, float r, int s) { return (int)floorf(lo + r * (float)s - fabsf(r) - 0.01f); }
+
+template <int A, int U, int V>
+__global__ __launch_bounds__(kL0Threads) void k_tr_bake_l0(TrArgs a, BakeL0 b) {
+    static_assert(A != U && A != V && U != V, "axis permutation");
+    constexpr int WW = kL0Win * kL0Win;
+    __shared__ float win[kL0Slabs * WW];
+    __shared__ int base[kL0Slabs];
+    __shared__ unsigned list[kL0List];
+    __shared__ unsigned count;
+    __shared__ int lay_lo, lay_hi;
+    if ((int)blockIdx.x < b.nlvl) {  // levels 1..7 over the quads, dispatched first (short)
+        tr_quad_block(a, (int)blockIdx.x, kL0Threads);
+        return;
+    }
+    const int tile = (int)blockIdx.x - b.nlvl;
+    const int tu = tile % b.ntu, tv = tile / b.ntu;
+    const float ulo = b.umin + (float)(kL0Tile * tu), vlo = b.vmin + (float)(kL0Tile * tv);
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) {
+        count = 0u;
+        lay_lo = 1 << 30;
+        lay_hi = -1;
+    }
+    __syncthreads();
+    // the tile's texels: candidates around the tile per layer, kept where (u', v') falls in the
+    // tile (the same float test in every workgroup, so each texel is taken exactly once)
+    const int nc = 128 * kL0Cand * kL0Cand;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int ia = k / (kL0Cand * kL0Cand), r = k - ia * (kL0Cand * kL0Cand);
+        const int du = r / kL0Cand, dv = r - du * kL0Cand;
+        const float ca = (float)(ia + 1);
+        const int iu = (int)floorf(fmaf(b.ru, ca, ulo)) - 2 + du;
+        const int iv = (int)floorf(fmaf(b.rv, ca, vlo)) - 2 + dv;
+        bool take = (unsigned)iu < 128u && (unsigned)iv < 128u;
+        if (take) {
+            const float up = fmaf(-b.ru, ca, (float)(iu + 1)), vp = fmaf(-b.rv, ca, (float)(iv + 1));
+            take = (int)floorf((up - b.umin) * (1.0f / kL0Tile)) == tu && (int)floorf((vp - b.vmin) * (1.0f / kL0Tile)) == tv;
+        }
+        // active lanes are a prefix of the wavefront (k = tid + j * blockDim), so lane 0 is one
+        const unsigned long long m = __ballot(take);
+        unsigned at = 0u;
+        if (lane == 0 && m) at = atomicAdd(&count, (unsigned)__popcll(m));
+        at = __shfl(at, 0);
+        const unsigned slot = at + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+        if (take && slot < (unsigned)kL0List) list[slot] = ((unsigned)ia << 14) | ((unsigned)iu << 7) | (unsigned)iv;
+        if (m) {  // the tile's layer range (a wavefront's candidates span at most two layers)
+            const int l0 = __shfl(ia, __ffsll((unsigned long long)m) - 1), l1 = __shfl(ia, 63 - __clzll(m));
+            if (lane == 0) {
+                atomicMin(&lay_lo, l0);
+                atomicMax(&lay_hi, l1);
+            }
+        }
+    }
+    __syncthreads();
+    const unsigned n = min(count, (unsigned)kL0List);
+    if (n == 0u) return;  // uniform: every thread read the same count after the barrier
+    const int lay_lo_v = lay_lo, lay_hi_v = lay_hi;
+    __syncthreads();  // every thread has read the layer range before it is reused
+    // the slabs the tile's rays touch: a texel of layer l sits at c_A = l + 1 and its samples run
+    // to the exit face (towards higher slabs when L_A > 0), taps at floor(c_A) and floor(c_A) + 1
+    // and, of those, the slabs where the tile's window meets the padded level (elsewhere the
+    // column runs outside the box, where no sample lies)
+    constexpr int R = 131;
+    if (threadIdx.x == 0) {
+        lay_lo = 1 << 30;
+        lay_hi = -1;
+    }
+    __syncthreads();
+    {
+        const int s0 = b.up ? max(lay_lo_v - 1, 0) : 0;
+        const int s1 = b.up ? kL0Slabs - 1 : min(lay_hi_v + 3, kL0Slabs - 1);
+        for (int s = s0 + (int)threadIdx.x; s <= s1; s += kL0Threads) {
+            const int ou = l0_origin(ulo, b.ru, s), ov = l0_origin(vlo, b.rv, s);
+            base[s] = s * WW - ou * kL0Win - ov;
+            if (ou + kL0Win > 0 && ou < R && ov + kL0Win > 0 && ov < R) {
+                atomicMin(&lay_lo, s);
+                atomicMax(&lay_hi, s);
+            }
+        }
+    }
+    __syncthreads();
+    const int s_lo = lay_lo, s_hi = lay_hi;  // non-empty: the tile's texels lie in the level
+    // the windows (zero outside the padded level, whose own border is zero); for A = z the
+    // slab index runs fastest, along the level's contiguous z rows
+    const float* p1 = a.pmips + pmip_offset(1);
+    const int ns = s_hi - s_lo + 1, tot = ns * WW;
+    constexpr int NL = 8;  // loads in flight per thread before their LDS stores
+    for (int i0 = threadIdx.x; i0 < tot; i0 += NL * kL0Threads) {
+        float val[NL];
+        int dst[NL];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            const int i = i0 + j * kL0Threads;
+            val[j] = 0.0f;
+            dst[j] = -1;
+            if (i < tot) {
+                int s, r;
+                if (A == 2) {
+                    s = i % ns;
+                    r = i / ns;
+                } else {
+                    s = i / WW;
+                    r = i - s * WW;
+                }
+                s += s_lo;
+                const int wu = r / kL0Win, wv = r - wu * kL0Win;
+                int c3[3];
+                c3[A] = s;
+                c3[U] = l0_origin(ulo, b.ru, s) + wu;
+                c3[V] = l0_origin(vlo, b.rv, s) + wv;
+                const bool in = (unsigned)c3[0] < (unsigned)R && (unsigned)c3[1] < (unsigned)R && (unsigned)c3[2] < (unsigned)R;
+                if (in) val[j] = p1[(c3[0] * R + c3[1]) * R + c3[2]];
+                dst[j] = s * WW + r;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NL; ++j)
+            if (dst[j] >= 0) win[dst[j]] = val[j];
+    }
+    __syncthreads();
+    constexpr int K = 4;
+    for (unsigned e = threadIdx.x; e < n; e += blockDim.x) {
+        const unsigned wrd = list[e];
+        int c3[3];
+        c3[A] = (int)(wrd >> 14);
+        c3[U] = (int)((wrd >> 7) & 127u);
+        c3[V] = (int)(wrd & 127u);
+        const int x = c3[0], y = c3[1], z = c3[2];
+        v3 uv = (mk3((float)x, (float)y, (float)z) + 0.5f) / 128.0f;
+        v3 ori = uv - 0.5f;
+        float dis = ray_box_distance(ori, a.light);
+        float st = dis / 128.0f;
+        v3 lp = ori;
+        float sum = 0.0f;
+        for (int i = 0; i < 128; i += K) {
+            float wx[K], wy[K], wz[K];
+            int a0[K], a1[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                lp = lp + a.light * st;
+                const v3 c = padded_coord(lp, 128.0f);
+                wx[k] = __builtin_amdgcn_fractf(c.x);
+                wy[k] = __builtin_amdgcn_fractf(c.y);
+                wz[k] = __builtin_amdgcn_fractf(c.z);
+                // non-negative coordinates: truncation is the floor
+                int f3[3] = {(int)c.x, (int)c.y, (int)c.z};
+                const int off = (int)mad_u24_v((unsigned)f3[U], (unsigned)kL0Win, (unsigned)f3[V]);
+                a0[k] = base[f3[A]] + off;
+                a1[k] = base[f3[A] + 1] + off;
+            }
+            float t[K][2][2][2];  // [sample][slab +0/+1][u +0/+1][v +0/+1]
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                t[k][0][0][0] = win[a0[k]];
+                t[k][0][0][1] = win[a0[k] + 1];
+                t[k][0][1][0] = win[a0[k] + kL0Win];
+                t[k][0][1][1] = win[a0[k] + kL0Win + 1];
+                t[k][1][0][0] = win[a1[k]];
+                t[k][1][0][1] = win[a1[k] + 1];
+                t[k][1][1][0] = win[a1[k] + kL0Win];
+                t[k][1][1][1] = win[a1[k] + kL0Win + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                // corner (dx, dy, dz) of the blend is window tap [d_A][d_U][d_V]
+                float cc[2][2][2];
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+                    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                        for (int dz = 0; dz < 2; ++dz) {
+                            const int d[3] = {dx, dy, dz};
+                            cc[dx][dy][dz] = t[k][d[A]][d[U]][d[V]];
+                        }
+                const float v = flerp(flerp(flerp(cc[0][0][0], cc[0][0][1], wz[k]), flerp(cc[0][1][0], cc[0][1][1], wz[k]), wy[k]),
+                                      flerp(flerp(cc[1][0][0], cc[1][0][1], wz[k]), flerp(cc[1][1][0], cc[1][1][1], wz[k]), wy[k]),
+                                      wx[k]);
+                sum = sum + v;
+            }
+        }
+        const float T = expf(-sum * a.alpha_lvl[0] * st) * 1.0f;
+        a.tr[0][(x * 128 + y) * 128 + z] = T;
+        if (a.tr_pad) {  // padded copy with replicated edges (as k_tr_bake)
+            const int Rp = pad_dim(128);
+            const int lo[3] = {x == 0 ? 0 : x + 1, y == 0 ? 0 : y + 1, z == 0 ? 0 : z + 1};
+            const int hi[3] = {x == 127 ? 130 : x + 1, y == 127 ? 130 : y + 1, z == 127 ? 130 : z + 1};
+            float* dst = a.tr_pad;
+            for (int px = lo[0]; px <= hi[0]; ++px)
+                for (int py = lo[1]; py <= hi[1]; ++py)
+                    for (int pz = lo[2]; pz <= hi[2]; ++pz) dst[(px * Rp + py) * Rp + pz] = T;
+        }
     }
 }
 
@@ -546,13 +780,43 @@ hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, const 
     a.light = light;
     a.qmips = qmips;
     if (pmips && qmips) {
+        // level 0 from LDS along the light's principal axis
+        const float Lc[3] = {light.x, light.y, light.z};
+        int A = 0;
+        for (int i = 1; i < 3; ++i)
+            if (fabsf(Lc[i]) > fabsf(Lc[A])) A = i;
+        const int U = A == 0 ? 1 : 0, Vx = A == 2 ? 1 : 2;
+        BakeL0 b;
+        b.ru = Lc[U] / Lc[A];
+        b.rv = Lc[Vx] / Lc[A];
+        float lo[2] = {1e30f, 1e30f}, hi[2] = {-1e30f, -1e30f};
+        for (float ca : {1.0f, 128.0f})
+            for (float cw : {1.0f, 128.0f}) {
+                const float r2[2] = {b.ru, b.rv};
+                for (int j = 0; j < 2; ++j) {
+                    const float w = cw - r2[j] * ca;
+                    lo[j] = fminf(lo[j], w);
+                    hi[j] = fmaxf(hi[j], w);
+                }
+            }
+        b.umin = floorf(lo[0]) - 1.0f;
+        b.vmin = floorf(lo[1]) - 1.0f;
+        b.ntu = (int)((hi[0] + 1.0f - b.umin) / kL0Tile) + 1;
+        const int ntv = (int)((hi[1] + 1.0f - b.vmin) / kL0Tile) + 1;
+        b.up = Lc[A] > 0.0f ? 1 : 0;
+        // levels 1..7 by the quad path in workgroups after the tiles (level 0 gets none)
         int nb = 0;
-        for (int m = 0; m < 8; ++m) {
+        a.bstart[0] = 0;
+        for (int m = 1; m < 8; ++m) {
             a.bstart[m] = nb;
-            nb += (int)(((128 >> m) * (128 >> m) * (128 >> m) + 255) / 256);
+            nb += (int)(((128 >> m) * (128 >> m) * (128 >> m) + kL0Threads - 1) / kL0Threads);
         }
         a.bstart[8] = nb;
-        hipLaunchKernelGGL(k_tr_bake_lvl, dim3(nb), dim3(256), 0, s, a);
+        b.nlvl = nb;
+        const dim3 g0(nb + b.ntu * ntv);
+        if (A == 0) hipLaunchKernelGGL((k_tr_bake_l0<0, 1, 2>), g0, dim3(kL0Threads), 0, s, a, b);
+        else if (A == 1) hipLaunchKernelGGL((k_tr_bake_l0<1, 0, 2>), g0, dim3(kL0Threads), 0, s, a, b);
+        else hipLaunchKernelGGL((k_tr_bake_l0<2, 0, 1>), g0, dim3(kL0Threads), 0, s, a, b);
     } else {
         hipLaunchKernelGGL(k_tr_bake, dim3(grid_for((size_t)acc, 256)), dim3(256), 0, s, a);
     }
