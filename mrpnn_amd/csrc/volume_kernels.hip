@@ -8,6 +8,7 @@
 //     eight levels in one launch;
 //   * HG LUT, Fill_Hg (core/volume.cu:1322-1400).
 // All are HBM-streaming or ALU kernels with coalesced z-fastest access.
+#include "bake_order.h"
 #include "dev_common.hpp"
 #include "fbm.h"
 #include "kernels.h"
@@ -344,8 +345,9 @@ constexpr int kL0Threads = 512;
 struct BakeL0 {
     float ru, rv;      // L_U / L_A, L_V / L_A
     float umin, vmin;  // (u', v') origin of tile (0, 0)
-    int ntu;           // tiles along u'
-    int nlvl;          // the first nlvl workgroups take levels 1..7 (a.bstart, kL0Threads texels each)
+    int ntu, ntv;      // tiles along u', v'
+    int ciu, civ;      // tile of the cube centre's (u', v'): the heaviest tiles surround it
+    int nlvl;          // the last nlvl workgroups take levels 1..7 (a.bstart, kL0Threads texels each)
     int up;            // L_A > 0: rays run towards higher slabs
 };
 __device__ inline int l0_origin(float lo, float r, int s) { return (int)floorf(lo + r * (float)s - fabsf(r) - 0.01f); }
@@ -359,11 +361,15 @@ __global__ __launch_bounds__(kL0Threads) void k_tr_bake_l0(TrArgs a, BakeL0 b) {
     __shared__ unsigned list[kL0List];
     __shared__ unsigned count;
     __shared__ int lay_lo, lay_hi;
-    if ((int)blockIdx.x < b.nlvl) {  // levels 1..7 over the quads, dispatched first (short)
-        tr_quad_block(a, (int)blockIdx.x, kL0Threads);
+    const int ntiles = b.ntu * b.ntv;
+    if ((int)blockIdx.x >= ntiles) {  // levels 1..7 over the quads, after the tiles: dispatched
+        // while the concurrent march is running, their texture loads would compete with its taps
+        tr_quad_block(a, (int)blockIdx.x - ntiles, kL0Threads);
         return;
     }
-    const int tile = (int)blockIdx.x - b.nlvl;
+    // tiles heaviest first: rings around the centre tile
+    const int tile = ring_tile((int)blockIdx.x, b.ntu, b.ntv, b.ciu, b.civ);
+    if (tile < 0) return;  // not reached: ranks below ntiles map onto the grid
     const int tu = tile % b.ntu, tv = tile / b.ntu;
     const float ulo = b.umin + (float)(kL0Tile * tu), vlo = b.vmin + (float)(kL0Tile * tv);
     const int lane = threadIdx.x & 63;
@@ -802,7 +808,11 @@ hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, const 
         b.umin = floorf(lo[0]) - 1.0f;
         b.vmin = floorf(lo[1]) - 1.0f;
         b.ntu = (int)((hi[0] + 1.0f - b.umin) / kL0Tile) + 1;
-        const int ntv = (int)((hi[1] + 1.0f - b.vmin) / kL0Tile) + 1;
+        b.ntv = (int)((hi[1] + 1.0f - b.vmin) / kL0Tile) + 1;
+        // the cube centre (padded coordinate 64.5 on every axis) in (u', v') tiles
+        const float cu = (64.5f - b.ru * 64.5f - b.umin) / kL0Tile, cv = (64.5f - b.rv * 64.5f - b.vmin) / kL0Tile;
+        b.ciu = std::min(std::max((int)floorf(cu), 0), b.ntu - 1);
+        b.civ = std::min(std::max((int)floorf(cv), 0), b.ntv - 1);
         b.up = Lc[A] > 0.0f ? 1 : 0;
         // levels 1..7 by the quad path in workgroups after the tiles (level 0 gets none)
         int nb = 0;
@@ -813,7 +823,7 @@ hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, const 
         }
         a.bstart[8] = nb;
         b.nlvl = nb;
-        const dim3 g0(nb + b.ntu * ntv);
+        const dim3 g0(b.ntu * b.ntv + nb);
         if (A == 0) hipLaunchKernelGGL((k_tr_bake_l0<0, 1, 2>), g0, dim3(kL0Threads), 0, s, a, b);
         else if (A == 1) hipLaunchKernelGGL((k_tr_bake_l0<1, 0, 2>), g0, dim3(kL0Threads), 0, s, a, b);
         else hipLaunchKernelGGL((k_tr_bake_l0<2, 0, 1>), g0, dim3(kL0Threads), 0, s, a, b);
