@@ -1147,8 +1147,21 @@ __global__ __launch_bounds__(lds_weights(PREC) ? 768 : 256) __attribute__((amdgp
     const unsigned n = *counter;
     const int lane = threadIdx.x & 63;
     const int wave = xcd_swizzle(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * (blockDim.x >> 6);
-    for (unsigned tile = wave; tile * 16u < n; tile += nwaves) {
+    const unsigned nwaves = gridDim.x * (blockDim.x >> 6);
+    // full rounds: every wave one tile, an XCD's waves on a contiguous range of tiles (L2
+    // locality).  The last, partial round takes the leftover tiles in an order that spreads them
+    // over the workgroups first (one per CU, then a second ...), so it neither lands on the first
+    // XCD alone nor queues behind busy waves of the same CU
+    const unsigned ntiles = (n + 15u) / 16u, kfull = ntiles / nwaves;
+    const unsigned tail = (threadIdx.x >> 6) * gridDim.x + blockIdx.x;
+    for (unsigned k = 0;; ++k) {
+        unsigned tile;
+        if (k < kfull) {
+            tile = k * nwaves + (unsigned)wave;
+        } else {
+            tile = kfull * nwaves + tail;
+            if (k > kfull || tile >= ntiles) break;
+        }
         LaneCtx lc;
         lc.lane = lane;
         lc.c = lane & 15;
