@@ -1,4 +1,4 @@
-// Dispatch order of the level-0 TR bake's tiles.  Host and device.
+// Dispatch order of tiles by rings around a centre tile.  Host and device.
 //
 // The bake's tiles are light-sheared texel columns of very different sizes: a tile near the
 // centre of the sheared (u', v') domain holds up to 16 texels in each of the 128 layers, a tile
@@ -31,8 +31,15 @@ MRP_HD int ring_count_within(int d, int a0, int a1, int b0, int b1) {
 MRP_HD int ring_tile(int k, int ntu, int ntv, int ciu, int civ) {
     if (k < 0 || k >= ntu * ntv) return -1;
     const int a0 = -ciu, a1 = ntu - 1 - ciu, b0 = -civ, b1 = ntv - 1 - civ;
-    int d = 0;
-    while (ring_count_within(d, a0, a1, b0, b1) <= k) ++d;
+    // the ring of rank k: the smallest d with more than k tiles within distance d (binary search;
+    // every tile lies within distance dmax)
+    int lo = 0, hi = ring_imax(ring_imax(-a0, a1), ring_imax(-b0, b1));
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ring_count_within(mid, a0, a1, b0, b1) > k) hi = mid;
+        else lo = mid + 1;
+    }
+    const int d = lo;
     int j = k - ring_count_within(d - 1, a0, a1, b0, b1);
     const int rlo = ring_imax(-d, a0), rhi = ring_imin(d, a1);  // clipped extent of a full row
     const int nrow = rhi - rlo + 1;

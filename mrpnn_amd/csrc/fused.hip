@@ -11,6 +11,7 @@
 #include "capi_internal.h"
 #include "dev_common.hpp"
 #include "dev_mlp_fp32.hpp"
+#include "bake_order.h"
 #include "fused.h"
 
 namespace mrp {
@@ -25,9 +26,13 @@ __global__ __launch_bounds__(64) void k_march_compact(const DevVolume* __restric
     while (v + 1 < n_views && tile >= slots[v + 1].tile0) ++v;
     const ViewSlot& sl = slots[v];
     const ViewDev vw = sl.vw;
-    const int lt = tile - sl.tile0;
-    const int tx = lt % sl.tiles_x, ty = lt / sl.tiles_x;
     const int lw = sl.tile_lw;
+    // tiles in rings around the image centre, where the camera looks at the cloud's centre and
+    // rays cross the most volume: the longest tiles (hundreds of dependent steps, as long as the
+    // whole march) start first instead of wherever the row-major order reaches them
+    const int tiles_y = (vw.height + (64 >> lw) - 1) / (64 >> lw);
+    const int lt = ring_tile(tile - sl.tile0, sl.tiles_x, tiles_y, (vw.width >> 1) >> lw, (vw.height >> 1) / (64 >> lw));
+    const int tx = lt % sl.tiles_x, ty = lt / sl.tiles_x;
     const int col = (tx << lw) + (threadIdx.x & ((1 << lw) - 1));
     const int row = ty * (64 >> lw) + (threadIdx.x >> lw);
     const bool valid = row < vw.height && col < vw.width && (sl.checker == 0 || ((row + col + sl.checker - 1) & 1) == 0);

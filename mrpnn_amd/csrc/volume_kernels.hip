@@ -347,7 +347,8 @@ struct BakeL0 {
     float umin, vmin;  // (u', v') origin of tile (0, 0)
     int ntu, ntv;      // tiles along u', v'
     int ciu, civ;      // tile of the cube centre's (u', v'): the heaviest tiles surround it
-    int nlvl;          // the last nlvl workgroups take levels 1..7 (a.bstart, kL0Threads texels each)
+    int nlvl;          // nlvl workgroups take levels 1..7 (a.bstart, kL0Threads texels each) ...
+    int lvl_at;        // ... from rank lvl_at on, between the heavier and the lighter tiles
     int up;            // L_A > 0: rays run towards higher slabs
 };
 __device__ inline int l0_origin(float lo, float r, int s) { return (int)floorf(lo + r * (float)s - fabsf(r) - 0.01f); }
@@ -361,14 +362,17 @@ __global__ __launch_bounds__(kL0Threads) void k_tr_bake_l0(TrArgs a, BakeL0 b) {
     __shared__ unsigned list[kL0List];
     __shared__ unsigned count;
     __shared__ int lay_lo, lay_hi;
-    const int ntiles = b.ntu * b.ntv;
-    if ((int)blockIdx.x >= ntiles) {  // levels 1..7 over the quads, after the tiles: dispatched
-        // while the concurrent march is running, their texture loads would compete with its taps
-        tr_quad_block(a, (int)blockIdx.x - ntiles, kL0Threads);
+    // levels 1..7 over the quads, in the middle of the dispatch order: at its front their
+    // texture loads competed with the concurrent march's taps, at its end they were the bake's
+    // last, longest workgroups
+    int rank = (int)blockIdx.x;
+    if (rank >= b.lvl_at && rank < b.lvl_at + b.nlvl) {
+        tr_quad_block(a, rank - b.lvl_at, kL0Threads);
         return;
     }
+    if (rank >= b.lvl_at + b.nlvl) rank -= b.nlvl;
     // tiles heaviest first: rings around the centre tile
-    const int tile = ring_tile((int)blockIdx.x, b.ntu, b.ntv, b.ciu, b.civ);
+    const int tile = ring_tile(rank, b.ntu, b.ntv, b.ciu, b.civ);
     if (tile < 0) return;  // not reached: ranks below ntiles map onto the grid
     const int tu = tile % b.ntu, tv = tile / b.ntu;
     const float ulo = b.umin + (float)(kL0Tile * tu), vlo = b.vmin + (float)(kL0Tile * tv);
@@ -814,7 +818,7 @@ hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, const 
         b.ciu = std::min(std::max((int)floorf(cu), 0), b.ntu - 1);
         b.civ = std::min(std::max((int)floorf(cv), 0), b.ntv - 1);
         b.up = Lc[A] > 0.0f ? 1 : 0;
-        // levels 1..7 by the quad path in workgroups after the tiles (level 0 gets none)
+        // levels 1..7 by the quad path in workgroups of their own (level 0 gets none)
         int nb = 0;
         a.bstart[0] = 0;
         for (int m = 1; m < 8; ++m) {
@@ -823,6 +827,7 @@ hipError_t launch_tr_bake(const float* const mips[9], const float* pmips, const 
         }
         a.bstart[8] = nb;
         b.nlvl = nb;
+        b.lvl_at = b.ntu * b.ntv / 2;
         const dim3 g0(b.ntu * b.ntv + nb);
         if (A == 0) hipLaunchKernelGGL((k_tr_bake_l0<0, 1, 2>), g0, dim3(kL0Threads), 0, s, a, b);
         else if (A == 1) hipLaunchKernelGGL((k_tr_bake_l0<1, 0, 2>), g0, dim3(kL0Threads), 0, s, a, b);

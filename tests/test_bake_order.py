@@ -1,6 +1,6 @@
-"""Dispatch order of the level-0 TR bake's tiles (mrpnn_amd/csrc/bake_order.h), on the host: a
-bijection from workgroup ranks onto the tile grid in rings of growing distance from the centre
-tile, for every grid shape and centre the bake can see."""
+"""Dispatch order of the level-0 TR bake's and the march's tiles (mrpnn_amd/csrc/bake_order.h), on
+the host: a bijection from workgroup ranks onto the tile grid in rings of growing distance from the
+centre tile, for every grid shape and centre the bake can see and for large march grids."""
 import os
 import shutil
 import subprocess
