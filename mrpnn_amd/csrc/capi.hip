@@ -285,7 +285,7 @@ extern "C" int mrp_ctx_upload_hg_lut(mrp_ctx* c, const float* host, float g) {
 
 extern "C" int64_t mrp_volume_bytes(int N) {
     const int64_t on = (N + 8) >> 3;
-    return (int64_t)((size_t)N * N * N + kMipFloats + kTrFloats + kPadMipFloats + kPadTrFloats) * 4 + kQuadTexels * 16 +
+    return (int64_t)((size_t)N * N * N + kMipFloats + kTrFloats + kPadMipFloats + kPadTrAlloc) * 4 + kQuadTexels * 16 +
            on * on * on;
 }
 
@@ -306,7 +306,7 @@ extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
         hipMalloc(&v->d_tr, kTrFloats * sizeof(float)) != hipSuccess ||
         hipMalloc(&v->d_pmips, kPadMipFloats * sizeof(float)) != hipSuccess ||
         hipMalloc(&v->d_qmips, kQuadTexels * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&v->d_ptr, kPadTrFloats * sizeof(float)) != hipSuccess) {
+        hipMalloc(&v->d_ptr, kPadTrAlloc * sizeof(float)) != hipSuccess) {
         hipFree(v->d_grid);
         hipFree(v->d_mips);
         hipFree(v->d_tr);
@@ -330,7 +330,7 @@ extern "C" int mrp_volume_create(mrp_ctx* c, int N, mrp_volume** out) {
     HIP_TRY(hipMemsetAsync(v->d_occ, 0, occ_bytes, c->stream));
     HIP_TRY(hipMemsetAsync(v->d_pmips, 0, kPadMipFloats * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_qmips, 0, kQuadTexels * sizeof(float4), c->stream));
-    HIP_TRY(hipMemsetAsync(v->d_ptr, 0, kPadTrFloats * sizeof(float), c->stream));
+    HIP_TRY(hipMemsetAsync(v->d_ptr, 0, kPadTrAlloc * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_grid, 0, n * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_mips, 0, kMipFloats * sizeof(float), c->stream));
     HIP_TRY(hipMemsetAsync(v->d_tr, 0, kTrFloats * sizeof(float), c->stream));
@@ -584,6 +584,7 @@ DevVolume mrp_dev_volume(const mrp_volume* v) {
     d.lut = v->ctx->d_lut;
     d.pmips = v->d_pmips;
     d.ptr = v->d_ptr;
+    d.qmips = v->d_qmips;
     d.occ = v->d_occ;
     d.occ_n = v->occ_n;
     d.N = v->N;

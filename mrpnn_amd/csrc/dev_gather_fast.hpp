@@ -43,6 +43,13 @@ constexpr long long kPadTrFloats = ptr_offset(7);
 // is two 16-byte loads (slices x and x + 1).  Offsets in quads.
 MRP_HD constexpr long long qmip_offset(int m) { return pmip_offset(m) - pmip_offset(1); }
 constexpr long long kQuadTexels = qmip_offset(8);
+// (y, z) quads of the padded TR mips 1..6 (the TR blends of stencil chunks 16..159), stored after
+// the padded TR mips in the same allocation (from float kPadTrQuadBase, 16-byte aligned); offsets
+// in quads.  The bake writes them together with the padded copy.
+MRP_HD constexpr long long qtr_offset(int m) { return ptr_offset(m) - ptr_offset(1); }
+constexpr long long kQuadTrTexels = qtr_offset(7);
+constexpr long long kPadTrQuadBase = (kPadTrFloats + 3) & ~3LL;
+constexpr long long kPadTrAlloc = kPadTrQuadBase + 4 * kQuadTrTexels;  // floats of the allocation
 
 // One stencil point of the fast gather (80 bytes).
 struct FastStencil {
@@ -83,6 +90,7 @@ __device__ inline f32x2_t ld_pair(BufRsrc r, int byte_off) {
 // Per-tile (uniform) inputs.
 struct FastView {
     BufRsrc pmips, ptr, rows;  // padded density mips, padded TR mips, LUT rows
+    BufRsrc qm, qt;            // (y, z) quads of the padded density mips 1..7 and TR mips 1..6
     v3 L, LX, invLX;  // light direction as the gather uses it; LX = normalize(L); inv3(LX)
     v3 a2f;           // 0.4999 * invLX (the box half-width over the light direction)
     float alpha64;    // alpha / 64
@@ -142,6 +150,17 @@ __device__ inline float tri_k(BufRsrc r, const TapGeo& g) {
     const f32x2_t c10 = ld_pair(r, o + (int)RR4), c11 = ld_pair(r, o + (int)(RR4 + R4));
     return flerp(flerp(flerp(c00.x, c00.y, g.wz), flerp(c01.x, c01.y, g.wz), g.wy),
                  flerp(flerp(c10.x, c10.y, g.wz), flerp(c11.x, c11.y, g.wz), g.wy), g.wx);
+}
+// The same blend from (y, z) quads (16 bytes per padded texel: p(y,z), p(y,z+1), p(y+1,z),
+// p(y+1,z+1)) at byte offset BASE, RR16 = 16 R^2: two 16-byte loads (slices x and x + 1) instead of
+// four 8-byte pairs; same taps, blend and rounding.
+template <unsigned BASE, unsigned RR16>
+__device__ inline float tri_q(BufRsrc r, const TapGeo& g) {
+    const int o = (int)(g.o4 * 4u + BASE);
+    const f32x4_t q0 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
+    const f32x4_t q1 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o + (int)RR16, 0, 0));
+    return flerp(flerp(flerp(q0.x, q0.y, g.wz), flerp(q0.z, q0.w, g.wz), g.wy),
+                 flerp(flerp(q1.x, q1.y, g.wz), flerp(q1.z, q1.w, g.wz), g.wy), g.wx);
 }
 __device__ inline float tri_geo(BufRsrc r, unsigned base, unsigned R4, unsigned RR4, const TapGeo& g) {
     const int o = (int)(g.o4 + base);
@@ -298,8 +317,9 @@ template <int M>
 __device__ inline v3 gather_chunk(const FastView& fv, const FastStencil& s, const FastPixel& px) {
     static_assert(M >= 2 && M <= 7, "uniform chunks use density mips 2..7");
     constexpr unsigned res = 256u >> M, R = res + 3u;
-    constexpr unsigned DB = (unsigned)pmip_offset(M) * 4u, TB = (unsigned)ptr_offset(M - 1) * 4u;
-    constexpr unsigned R4 = 4u * R, RR4 = 4u * R * R;
+    // byte offsets of the level in the density and TR quads; 16 R^2 bytes per x slice
+    constexpr unsigned QDB = (unsigned)qmip_offset(M) * 16u, QTB = (unsigned)qtr_offset(M - 1) * 16u;
+    constexpr unsigned RR16 = 16u * R * R;
     static_assert(res == (128u >> (M - 1)), "TR level M - 1 has the density level's resolution");
     constexpr float fres = (float)res;
     v3 p;
@@ -318,7 +338,7 @@ __device__ inline v3 gather_chunk(const FastView& fv, const FastStencil& s, cons
     if (M >= 5 || (any_out && outside))
         c = mk3(clampf(c.x, 0.0f, fres + 1.0f), clampf(c.y, 0.0f, fres + 1.0f), clampf(c.z, 0.0f, fres + 1.0f));
     const TapGeo g = tap_geo_k<R>(c.x, c.y, c.z);
-    const float rho = tri_k<DB, R4, RR4>(fv.pmips, g);
+    const float rho = tri_q<QDB, RR16>(fv.qm, g);
     const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
     // ShadowTerm_TRTex: inside the box the density blend's corner and weights (replicated TR
     // edges); outside, the light ray's entry point, clamp addressing
@@ -329,7 +349,7 @@ __device__ inline v3 gather_chunk(const FastView& fv, const FastStencil& s, cons
         const v3 ct = padded_coord(q, fres);
         gt = tap_geo_k<R>(clampf(ct.x, 1.0f, fres), clampf(ct.y, 1.0f, fres), clampf(ct.z, 1.0f, fres));
     }
-    const float T = tri_k<TB, R4, RR4>(fv.ptr, gt);
+    const float T = tri_q<QTB, RR16>(fv.qt, gt);
     return mk3(xv, xh, miss ? 1.0f : T);
 }
 

@@ -169,14 +169,14 @@ static int tr_slot_claim(mrp_ctx* c) {
     if ((int)c->tr_slots.size() < c->tr_slot_cap) {
         TrSlot t;
         if (hipMalloc(&t.d_tr, kTrFloats * sizeof(float)) != hipSuccess ||
-            hipMalloc(&t.d_ptr, kPadTrFloats * sizeof(float)) != hipSuccess ||
+            hipMalloc(&t.d_ptr, kPadTrAlloc * sizeof(float)) != hipSuccess ||
             hipEventCreateWithFlags(&t.last_read, hipEventDisableTiming) != hipSuccess) {
             hipFree(t.d_tr);
             hipFree(t.d_ptr);
             return -1;
         }
         // the padded copy's border must read as zero; the bake writes the interior only
-        if (hipMemsetAsync(t.d_ptr, 0, kPadTrFloats * sizeof(float), c->fused->aux) != hipSuccess) return -1;
+        if (hipMemsetAsync(t.d_ptr, 0, kPadTrAlloc * sizeof(float), c->fused->aux) != hipSuccess) return -1;
         c->tr_slots.push_back(t);
         return (int)c->tr_slots.size() - 1;
     }

@@ -136,6 +136,17 @@ struct TrArgs {
     int bstart[9];       // k_tr_bake_lvl: first workgroup of each level
     v3 light;
 };
+// Padded TR texel value T at padded position (px, py, pz) of a level with Rp^3 padded texels, into
+// the (y, z) quads that hold it (qd: the level's quads, 4 floats each): element 0 of quad
+// (px, py, pz), 1 of (px, py, pz - 1), 2 of (px, py - 1, pz), 3 of (px, py - 1, pz - 1).  Every
+// padded position is written, so every quad a blend reads (y, z <= Rp - 2) is complete.
+__device__ inline void tr_quad_put(float* qd, int Rp, int px, int py, int pz, float T) {
+    const int i = (px * Rp + py) * Rp + pz;
+    qd[4 * i] = T;
+    if (pz > 0) qd[4 * (i - 1) + 1] = T;
+    if (py > 0) qd[4 * (i - Rp) + 2] = T;
+    if (py > 0 && pz > 0) qd[4 * (i - Rp - 1) + 3] = T;
+}
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_tr_bake(TrArgs a) {
     const long long total = a.start[8];
     for (long long g = (long long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long long)gridDim.x * 256) {
@@ -216,9 +227,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             const int hi[3] = {x == res - 1 ? res + 2 : x + 1, y == res - 1 ? res + 2 : y + 1,
                                z == res - 1 ? res + 2 : z + 1};
             float* dst = a.tr_pad + ptr_offset(m);
+            float* qd = m >= 1 ? a.tr_pad + kPadTrQuadBase + 4 * qtr_offset(m) : nullptr;
             for (int px = lo[0]; px <= hi[0]; ++px)
                 for (int py = lo[1]; py <= hi[1]; ++py)
-                    for (int pz = lo[2]; pz <= hi[2]; ++pz) dst[((long long)px * R + py) * R + pz] = T;
+                    for (int pz = lo[2]; pz <= hi[2]; ++pz) {
+                        dst[((long long)px * R + py) * R + pz] = T;
+                        if (qd) tr_quad_put(qd, R, px, py, pz, T);
+                    }
         }
     }
 }
@@ -302,9 +317,13 @@ __device__ inline void tr_quad_texel(const TrArgs& a, int m, unsigned i_lvl) {
         const int hi[3] = {x == res - 1 ? res + 2 : x + 1, y == res - 1 ? res + 2 : y + 1,
                            z == res - 1 ? res + 2 : z + 1};
         float* dst = a.tr_pad + ptr_offset(m);
+        float* qd = m >= 1 ? a.tr_pad + kPadTrQuadBase + 4 * qtr_offset(m) : nullptr;
         for (int px = lo[0]; px <= hi[0]; ++px)
             for (int py = lo[1]; py <= hi[1]; ++py)
-                for (int pz = lo[2]; pz <= hi[2]; ++pz) dst[(px * Rp + py) * Rp + pz] = T;
+                for (int pz = lo[2]; pz <= hi[2]; ++pz) {
+                    dst[(px * Rp + py) * Rp + pz] = T;
+                    if (qd) tr_quad_put(qd, Rp, px, py, pz, T);
+                }
     }
 }
 // Block blk of the levels a.bstart covers (256 texels per workgroup of 256 threads, or
