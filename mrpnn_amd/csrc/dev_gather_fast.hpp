@@ -38,15 +38,16 @@ MRP_HD constexpr long long ptr_offset(int m) {  // TR mips 0..6, base 128
 }
 constexpr long long kPadMipFloats = pmip_offset(8);
 constexpr long long kPadTrFloats = ptr_offset(7);
-// (y, z) quads of the padded density mips 1..7 (the TR bake's sources): texel (x, y, z) of a
-// padded level holds (p(x,y,z), p(x,y,z+1), p(x,y+1,z), p(x,y+1,z+1)), so a trilinear blend
-// is two 16-byte loads (slices x and x + 1).  Offsets in quads.
-MRP_HD constexpr long long qmip_offset(int m) { return pmip_offset(m) - pmip_offset(1); }
+// (y, z) quads of the padded density mips 0..7 (the fast gather's and the TR bake's sources):
+// texel (x, y, z) of a padded level holds (p(x,y,z), p(x,y,z+1), p(x,y+1,z), p(x,y+1,z+1)), so a
+// trilinear blend is two 16-byte loads (slices x and x + 1).  Offsets in quads; a level's quads
+// sit at the texel offsets of the padded mips.
+MRP_HD constexpr long long qmip_offset(int m) { return pmip_offset(m); }
 constexpr long long kQuadTexels = qmip_offset(8);
-// (y, z) quads of the padded TR mips 1..6 (the TR blends of stencil chunks 16..159), stored after
-// the padded TR mips in the same allocation (from float kPadTrQuadBase, 16-byte aligned); offsets
-// in quads.  The bake writes them together with the padded copy.
-MRP_HD constexpr long long qtr_offset(int m) { return ptr_offset(m) - ptr_offset(1); }
+// (y, z) quads of the padded TR mips 0..6 (the fast gather's TR blends), stored after the padded
+// TR mips in the same allocation (from float kPadTrQuadBase, 16-byte aligned); offsets in quads.
+// The bake writes them together with the padded copy.
+MRP_HD constexpr long long qtr_offset(int m) { return ptr_offset(m); }
 constexpr long long kQuadTrTexels = qtr_offset(7);
 constexpr long long kPadTrQuadBase = (kPadTrFloats + 3) & ~3LL;
 constexpr long long kPadTrAlloc = kPadTrQuadBase + 4 * kQuadTrTexels;  // floats of the allocation
@@ -90,7 +91,7 @@ __device__ inline f32x2_t ld_pair(BufRsrc r, int byte_off) {
 // Per-tile (uniform) inputs.
 struct FastView {
     BufRsrc pmips, ptr, rows;  // padded density mips, padded TR mips, LUT rows
-    BufRsrc qm, qt;            // (y, z) quads of the padded density mips 1..7 and TR mips 1..6
+    BufRsrc qm, qt;            // (y, z) quads of the padded density mips 0..7 and TR mips 0..6
     v3 L, LX, invLX;  // light direction as the gather uses it; LX = normalize(L); inv3(LX)
     v3 a2f;           // 0.4999 * invLX (the box half-width over the light direction)
     float alpha64;    // alpha / 64
@@ -157,6 +158,15 @@ __device__ inline float tri_k(BufRsrc r, const TapGeo& g) {
 template <unsigned BASE, unsigned RR16>
 __device__ inline float tri_q(BufRsrc r, const TapGeo& g) {
     const int o = (int)(g.o4 * 4u + BASE);
+    const f32x4_t q0 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
+    const f32x4_t q1 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o + (int)RR16, 0, 0));
+    return flerp(flerp(flerp(q0.x, q0.y, g.wz), flerp(q0.z, q0.w, g.wz), g.wy),
+                 flerp(flerp(q1.x, q1.y, g.wz), flerp(q1.z, q1.w, g.wz), g.wy), g.wx);
+}
+// The same with the level's geometry at run time: qbase = 16 x the level's texel offset, RR16 =
+// 16 R^2 (the byte offsets and strides of the pair layout, times four).
+__device__ inline float tri_q_rt(BufRsrc r, unsigned qbase, unsigned RR16, const TapGeo& g) {
+    const int o = (int)(g.o4 * 4u + qbase);
     const f32x4_t q0 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
     const f32x4_t q1 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o + (int)RR16, 0, 0));
     return flerp(flerp(flerp(q0.x, q0.y, g.wz), flerp(q0.z, q0.w, g.wz), g.wy),
@@ -257,7 +267,8 @@ __device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3
         const v3 c = padded_coord(q, s.tres);
         g = tap_geo(s.tRu, clampf(c.x, 1.0f, s.tres), clampf(c.y, 1.0f, s.tres), clampf(c.z, 1.0f, s.tres));
     }
-    const float T = tri_geo(fv.ptr, s.tbase, s.tR4, s.tRR4, g);
+    // quads: the pair layout's byte offsets and strides times four
+    const float T = tri_q_rt(fv.qt, s.tbase * 4u, s.tRR4 * 4u, g);
     return miss ? 1.0f : T;
 }
 
@@ -291,7 +302,7 @@ __device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const
         const float hi = s.dres + 1.0f;
         g = tap_geo(s.dRu, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi), clampf(c.z, 0.0f, hi));
     }
-    const float rho = tri_geo(fv.pmips, s.dbase, s.dR4, s.dRR4, g);
+    const float rho = tri_q_rt(fv.qm, s.dbase * 4u, s.dRR4 * 4u, g);
     const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
     const float xs = shadow_fast(fv, s, p, g, same, outside);
     return mk3(xv, xh, xs);

@@ -130,7 +130,7 @@ struct TrArgs {
     float* tr[8];
     float* tr_pad;       // zero-padded copy of levels 0..6 (fast gather), may be null
     const float* pmips;  // zero-padded density mips 0..7, may be null (exact border sampling)
-    const float4* qmips; // (y, z) quads of the padded mips 1..7 (k_tr_bake_lvl)
+    const float4* qmips; // (y, z) quads of the padded density mips 0..7
     float alpha_lvl[8];
     long long start[9];  // flattened texel offsets of each level
     int bstart[9];       // k_tr_bake_lvl: first workgroup of each level
@@ -227,12 +227,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             const int hi[3] = {x == res - 1 ? res + 2 : x + 1, y == res - 1 ? res + 2 : y + 1,
                                z == res - 1 ? res + 2 : z + 1};
             float* dst = a.tr_pad + ptr_offset(m);
-            float* qd = m >= 1 ? a.tr_pad + kPadTrQuadBase + 4 * qtr_offset(m) : nullptr;
+            float* qd = a.tr_pad + kPadTrQuadBase + 4 * qtr_offset(m);
             for (int px = lo[0]; px <= hi[0]; ++px)
                 for (int py = lo[1]; py <= hi[1]; ++py)
                     for (int pz = lo[2]; pz <= hi[2]; ++pz) {
                         dst[((long long)px * R + py) * R + pz] = T;
-                        if (qd) tr_quad_put(qd, R, px, py, pz, T);
+                        tr_quad_put(qd, R, px, py, pz, T);
                     }
         }
     }
@@ -317,12 +317,12 @@ __device__ inline void tr_quad_texel(const TrArgs& a, int m, unsigned i_lvl) {
         const int hi[3] = {x == res - 1 ? res + 2 : x + 1, y == res - 1 ? res + 2 : y + 1,
                            z == res - 1 ? res + 2 : z + 1};
         float* dst = a.tr_pad + ptr_offset(m);
-        float* qd = m >= 1 ? a.tr_pad + kPadTrQuadBase + 4 * qtr_offset(m) : nullptr;
+        float* qd = a.tr_pad + kPadTrQuadBase + 4 * qtr_offset(m);
         for (int px = lo[0]; px <= hi[0]; ++px)
             for (int py = lo[1]; py <= hi[1]; ++py)
                 for (int pz = lo[2]; pz <= hi[2]; ++pz) {
                     dst[(px * Rp + py) * Rp + pz] = T;
-                    if (qd) tr_quad_put(qd, Rp, px, py, pz, T);
+                    tr_quad_put(qd, Rp, px, py, pz, T);
                 }
     }
 }
@@ -561,14 +561,18 @@ __global__ __launch_bounds__(kL0Threads) void k_tr_bake_l0(TrArgs a, BakeL0 b) {
         }
         const float T = expf(-sum * a.alpha_lvl[0] * st) * 1.0f;
         a.tr[0][(x * 128 + y) * 128 + z] = T;
-        if (a.tr_pad) {  // padded copy with replicated edges (as k_tr_bake)
+        if (a.tr_pad) {  // padded copy with replicated edges and its quads (as k_tr_bake)
             const int Rp = pad_dim(128);
             const int lo[3] = {x == 0 ? 0 : x + 1, y == 0 ? 0 : y + 1, z == 0 ? 0 : z + 1};
             const int hi[3] = {x == 127 ? 130 : x + 1, y == 127 ? 130 : y + 1, z == 127 ? 130 : z + 1};
             float* dst = a.tr_pad;
+            float* qd = a.tr_pad + kPadTrQuadBase;
             for (int px = lo[0]; px <= hi[0]; ++px)
                 for (int py = lo[1]; py <= hi[1]; ++py)
-                    for (int pz = lo[2]; pz <= hi[2]; ++pz) dst[(px * Rp + py) * Rp + pz] = T;
+                    for (int pz = lo[2]; pz <= hi[2]; ++pz) {
+                        dst[(px * Rp + py) * Rp + pz] = T;
+                        tr_quad_put(qd, Rp, px, py, pz, T);
+                    }
         }
     }
 }
@@ -675,12 +679,12 @@ __global__ __launch_bounds__(256) void k_pad_mips(PadArgs a, float* __restrict__
     }
 }
 
-// (y, z) quads of the padded density mips 1..7 (every padded texel, the border included; taps
+// (y, z) quads of the padded density mips 0..7 (every padded texel, the border included; taps
 // beyond the padded level read 0).
 __global__ __launch_bounds__(256) void k_quad_mips(const float* __restrict__ pm, float4* __restrict__ q) {
     const long long total = kQuadTexels;
     for (long long g = (long long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long long)gridDim.x * 256) {
-        int m = 1;
+        int m = 0;
         while (m < 7 && g >= qmip_offset(m + 1)) ++m;
         const int R = pad_dim(256 >> m);
         const int l = (int)(g - qmip_offset(m));

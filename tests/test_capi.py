@@ -53,10 +53,10 @@ def test_argument_errors_are_status_codes(lib):
     # mips 0..7 and TR mips 0..6
     pad_d = sum(((256 >> m) + 3) ** 3 for m in range(8))
     pad_t = sum(((128 >> m) + 3) ** 3 for m in range(7))
-    # + (y, z) quads (4 floats per padded texel) of TR mips 1..6 after them, from a 16-byte boundary
-    pad_t = (pad_t + 3) // 4 * 4 + 4 * sum(((128 >> m) + 3) ** 3 for m in range(1, 7))
-    # + (y, z) quads (16 bytes per padded texel) of density mips 1..7 for the TR bake
-    quads = sum(((256 >> m) + 3) ** 3 for m in range(1, 8))
+    # + (y, z) quads (4 floats per padded texel) of TR mips 0..6 after them, from a 16-byte boundary
+    pad_t = (pad_t + 3) // 4 * 4 + 4 * sum(((128 >> m) + 3) ** 3 for m in range(7))
+    # + (y, z) quads (16 bytes per padded texel) of density mips 0..7 (fast gather, TR bake)
+    quads = sum(((256 >> m) + 3) ** 3 for m in range(8))
     # + one emptiness byte per 8^3 brick of the march ((N + 8) // 8 per axis)
     assert lib.mrp_volume_bytes(512) == (512 ** 3 + 19173961 + 2396745 + pad_d + pad_t) * 4 + quads * 16 + 65 ** 3
 
