@@ -35,7 +35,7 @@ struct TrSlot {
     float light[3]{};
     float alpha = 0.0f;
     float* d_tr = nullptr;   // TR mips 0..7 (kTrFloats)
-    float* d_ptr = nullptr;  // zero-padded TR mips 0..6 (kPadTrFloats)
+    float* d_ptr = nullptr;  // (y, z) quads of the zero-padded TR mips 0..6 (kPadTrAlloc floats)
     uint64_t stamp = 0;      // last prepare call that used it
     bool valid = false;
     hipEvent_t last_read = nullptr;  // recorded after the last batch that read this set
@@ -91,7 +91,7 @@ struct mrp_volume {
     float* d_tr = nullptr;
     float* d_pmips = nullptr;  // zero-padded density mips 0..7
     float4* d_qmips = nullptr; // (y, z) quads of padded density mips 1..7 (TR bake)
-    float* d_ptr = nullptr;    // zero-padded TR mips 0..6
+    float* d_ptr = nullptr;    // (y, z) quads of the zero-padded TR mips 0..6
     uint8_t* d_occ = nullptr;  // brick emptiness for the first-hit march, occ_n^3
     int occ_n = 0;
     float* mip[9]{};

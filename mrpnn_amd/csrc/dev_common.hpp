@@ -103,7 +103,7 @@ struct DevVolume {
     const float* tr[8];    // (128>>m)^3
     const float* lut;      // 512*512, [j*512+i]
     const float* pmips;    // zero-padded density mips 0..7 (fast gather)
-    const float* ptr;      // zero-padded TR mips 0..6, then (y, z) quads of levels 1..6 (fast gather)
+    const float* ptr;      // (y, z) quads of the zero-padded TR mips 0..6, 4 floats each (fast gather)
     const float4* qmips;   // (y, z) quads of the zero-padded density mips 1..7 (fast gather)
     const uint8_t* occ;    // brick emptiness of the grid, occ_n^3 (see brick_of)
     int occ_n;

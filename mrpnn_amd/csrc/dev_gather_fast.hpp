@@ -44,13 +44,11 @@ constexpr long long kPadTrFloats = ptr_offset(7);
 // sit at the texel offsets of the padded mips.
 MRP_HD constexpr long long qmip_offset(int m) { return pmip_offset(m); }
 constexpr long long kQuadTexels = qmip_offset(8);
-// (y, z) quads of the padded TR mips 0..6 (the fast gather's TR blends), stored after the padded
-// TR mips in the same allocation (from float kPadTrQuadBase, 16-byte aligned); offsets in quads.
-// The bake writes them together with the padded copy.
+// (y, z) quads of the zero-padded, edge-replicated TR mips 0..6 (the fast gather's TR blends),
+// written by the bake; offsets in quads, at the texel offsets of the padded levels.
 MRP_HD constexpr long long qtr_offset(int m) { return ptr_offset(m); }
 constexpr long long kQuadTrTexels = qtr_offset(7);
-constexpr long long kPadTrQuadBase = (kPadTrFloats + 3) & ~3LL;
-constexpr long long kPadTrAlloc = kPadTrQuadBase + 4 * kQuadTrTexels;  // floats of the allocation
+constexpr long long kPadTrAlloc = 4 * kQuadTrTexels;  // floats of a TR set's quad allocation
 
 // One stencil point of the fast gather (80 bytes).
 struct FastStencil {
@@ -90,8 +88,8 @@ __device__ inline f32x2_t ld_pair(BufRsrc r, int byte_off) {
 
 // Per-tile (uniform) inputs.
 struct FastView {
-    BufRsrc pmips, ptr, rows;  // padded density mips, padded TR mips, LUT rows
-    BufRsrc qm, qt;            // (y, z) quads of the padded density mips 0..7 and TR mips 0..6
+    BufRsrc rows;    // LUT rows
+    BufRsrc qm, qt;  // (y, z) quads of the padded density mips 0..7 and TR mips 0..6
     v3 L, LX, invLX;  // light direction as the gather uses it; LX = normalize(L); inv3(LX)
     v3 a2f;           // 0.4999 * invLX (the box half-width over the light direction)
     float alpha64;    // alpha / 64

@@ -647,10 +647,8 @@ __device__ inline FastGatherSrc fast_src(const DevVolume& vol, const ViewConst& 
                                          const FastStencil* sfs, const float* lut_rows) {
     FastGatherSrc src;
     src.st = sfs;
-    src.fv.pmips = make_rsrc(vol.pmips, kPadMipFloats);
-    src.fv.ptr = make_rsrc(vol.ptr, kPadTrFloats);
     src.fv.qm = make_rsrc((const float*)vol.qmips, 4 * kQuadTexels);
-    src.fv.qt = make_rsrc(vol.ptr + kPadTrQuadBase, 4 * kQuadTrTexels);
+    src.fv.qt = make_rsrc(vol.ptr, 4 * kQuadTrTexels);
     src.fv.rows = make_rsrc(lut_rows, (long long)kPC * kLutRow);
     src.fv.L = ps.L;
     src.fv.LX = ps.LX;

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_resample(const float* __restrict__ src,
 struct TrArgs {
     const float* mips[9];
     float* tr[8];
-    float* tr_pad;       // zero-padded copy of levels 0..6 (fast gather), may be null
+    float* tr_pad;       // (y, z) quads of the zero-padded, edge-replicated levels 0..6, may be null
     const float* pmips;  // zero-padded density mips 0..7, may be null (exact border sampling)
     const float4* qmips; // (y, z) quads of the padded density mips 0..7
     float alpha_lvl[8];
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         const float T = expf(-sum * a.alpha_lvl[m] * st) * 1.0f;
         a.tr[m][l] = T;
         if (a.tr_pad && m < 7) {
-            // padded copy with replicated edges: padded index i + 1 holds texel i, indices 0
+            // (y, z) quads of the padded copy with replicated edges: padded index i + 1 holds texel i, indices 0
             // and res + 1, res + 2 repeat the first and last texel of each axis, so a blend at
             // any coordinate in [0.5, res + 0.5] equals the clamp-addressed one (a blend of two
             // equal taps is exact)
@@ -226,14 +226,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             const int lo[3] = {x == 0 ? 0 : x + 1, y == 0 ? 0 : y + 1, z == 0 ? 0 : z + 1};
             const int hi[3] = {x == res - 1 ? res + 2 : x + 1, y == res - 1 ? res + 2 : y + 1,
                                z == res - 1 ? res + 2 : z + 1};
-            float* dst = a.tr_pad + ptr_offset(m);
-            float* qd = a.tr_pad + kPadTrQuadBase + 4 * qtr_offset(m);
+            float* qd = a.tr_pad + 4 * qtr_offset(m);
             for (int px = lo[0]; px <= hi[0]; ++px)
                 for (int py = lo[1]; py <= hi[1]; ++py)
-                    for (int pz = lo[2]; pz <= hi[2]; ++pz) {
-                        dst[((long long)px * R + py) * R + pz] = T;
-                        tr_quad_put(qd, R, px, py, pz, T);
-                    }
+                    for (int pz = lo[2]; pz <= hi[2]; ++pz) tr_quad_put(qd, R, px, py, pz, T);
         }
     }
 }
@@ -316,14 +312,10 @@ __device__ inline void tr_quad_texel(const TrArgs& a, int m, unsigned i_lvl) {
         const int lo[3] = {x == 0 ? 0 : x + 1, y == 0 ? 0 : y + 1, z == 0 ? 0 : z + 1};
         const int hi[3] = {x == res - 1 ? res + 2 : x + 1, y == res - 1 ? res + 2 : y + 1,
                            z == res - 1 ? res + 2 : z + 1};
-        float* dst = a.tr_pad + ptr_offset(m);
-        float* qd = a.tr_pad + kPadTrQuadBase + 4 * qtr_offset(m);
+        float* qd = a.tr_pad + 4 * qtr_offset(m);
         for (int px = lo[0]; px <= hi[0]; ++px)
             for (int py = lo[1]; py <= hi[1]; ++py)
-                for (int pz = lo[2]; pz <= hi[2]; ++pz) {
-                    dst[(px * Rp + py) * Rp + pz] = T;
-                    tr_quad_put(qd, Rp, px, py, pz, T);
-                }
+                for (int pz = lo[2]; pz <= hi[2]; ++pz) tr_quad_put(qd, Rp, px, py, pz, T);
     }
 }
 // Block blk of the levels a.bstart covers (256 texels per workgroup of 256 threads, or
@@ -561,18 +553,13 @@ __global__ __launch_bounds__(kL0Threads) void k_tr_bake_l0(TrArgs a, BakeL0 b) {
         }
         const float T = expf(-sum * a.alpha_lvl[0] * st) * 1.0f;
         a.tr[0][(x * 128 + y) * 128 + z] = T;
-        if (a.tr_pad) {  // padded copy with replicated edges and its quads (as k_tr_bake)
+        if (a.tr_pad) {  // quads of the padded level with replicated edges (as k_tr_bake)
             const int Rp = pad_dim(128);
             const int lo[3] = {x == 0 ? 0 : x + 1, y == 0 ? 0 : y + 1, z == 0 ? 0 : z + 1};
             const int hi[3] = {x == 127 ? 130 : x + 1, y == 127 ? 130 : y + 1, z == 127 ? 130 : z + 1};
-            float* dst = a.tr_pad;
-            float* qd = a.tr_pad + kPadTrQuadBase;
             for (int px = lo[0]; px <= hi[0]; ++px)
                 for (int py = lo[1]; py <= hi[1]; ++py)
-                    for (int pz = lo[2]; pz <= hi[2]; ++pz) {
-                        dst[(px * Rp + py) * Rp + pz] = T;
-                        tr_quad_put(qd, Rp, px, py, pz, T);
-                    }
+                    for (int pz = lo[2]; pz <= hi[2]; ++pz) tr_quad_put(a.tr_pad, Rp, px, py, pz, T);
         }
     }
 }

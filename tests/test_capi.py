@@ -50,11 +50,9 @@ def test_argument_errors_are_status_codes(lib):
     assert lib.mrp_pt_render(None, None, None, 0, 1, 1, 0, 1.0, None, None) == 1
     assert lib.mrp_predict_features(None, None, 1, None, 0.5, 0, None) == 1
     # grid + density mips 0..8 + TR mips 0..7 + zero-padded copies ((res+3)^3) of density
-    # mips 0..7 and TR mips 0..6
+    # mips 0..7 + (y, z) quads (4 floats per padded texel) of the padded TR mips 0..6
     pad_d = sum(((256 >> m) + 3) ** 3 for m in range(8))
-    pad_t = sum(((128 >> m) + 3) ** 3 for m in range(7))
-    # + (y, z) quads (4 floats per padded texel) of TR mips 0..6 after them, from a 16-byte boundary
-    pad_t = (pad_t + 3) // 4 * 4 + 4 * sum(((128 >> m) + 3) ** 3 for m in range(7))
+    pad_t = 4 * sum(((128 >> m) + 3) ** 3 for m in range(7))
     # + (y, z) quads (16 bytes per padded texel) of density mips 0..7 (fast gather, TR bake)
     quads = sum(((256 >> m) + 3) ** 3 for m in range(8))
     # + one emptiness byte per 8^3 brick of the march ((N + 8) // 8 per axis)
