@@ -7,8 +7,10 @@
 //     level, data at [1,res] per axis): the continuous coordinate is clamped into the padded
 //     range, so every tap is in bounds and no per-tap border test is needed; the clamp also
 //     reproduces the border (density) and clamp (TR) addressing exactly;
-//   * every tap pair (z, z+1) is one 64-bit buffer load at a 32-bit offset from a uniform
-//     buffer resource;
+//   * the padded levels are stored as (y, z) quads (texel (x, y, z) holds its own value and those
+//     at z+1, y+1 and y+1, z+1), so a trilinear blend is two 16-byte buffer loads (slices x and
+//     x+1) at 32-bit offsets from a uniform buffer resource, instead of four 8-byte (z, z+1)
+//     pairs: half the load instructions, whose issue the kernel is short of;
 //   * the 144 points of stencil chunks 16..159 share one density mip per 16-point chunk, so
 //     their level geometry is a compile-time constant (gather_chunk): no per-point table reads
 //     of it, the texel index is two 24-bit multiply-adds by constants;
