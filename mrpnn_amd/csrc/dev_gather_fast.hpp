@@ -40,10 +40,10 @@ MRP_HD constexpr long long ptr_offset(int m) {  // TR mips 0..6, base 128
 }
 constexpr long long kPadMipFloats = pmip_offset(8);
 constexpr long long kPadTrFloats = ptr_offset(7);
-// (y, z) quads of the padded density mips 0..7 (the fast gather's and the TR bake's sources):
-// texel (x, y, z) of a padded level holds (p(x,y,z), p(x,y,z+1), p(x,y+1,z), p(x,y+1,z+1)), so a
-// trilinear blend is two 16-byte loads (slices x and x + 1).  Offsets in quads; a level's quads
-// sit at the texel offsets of the padded mips.
+// (y, z) difference quads of the padded density mips 0..7 (the fast gather's and the TR bake's
+// sources): texel (x, y, z) of a padded level holds (p(x,y,z), p(x,y,z+1) - p(x,y,z), p(x,y+1,z),
+// p(x,y+1,z+1) - p(x,y+1,z)), so a trilinear blend is two 16-byte loads (slices x and x + 1).
+// Offsets in quads; a level's quads sit at the texel offsets of the padded mips.
 MRP_HD constexpr long long qmip_offset(int m) { return pmip_offset(m); }
 constexpr long long kQuadTexels = qmip_offset(8);
 // (y, z) quads of the zero-padded, edge-replicated TR mips 0..6 (the fast gather's TR blends),
@@ -152,25 +152,35 @@ __device__ inline float tri_k(BufRsrc r, const TapGeo& g) {
     return flerp(flerp(flerp(c00.x, c00.y, g.wz), flerp(c01.x, c01.y, g.wz), g.wy),
                  flerp(flerp(c10.x, c10.y, g.wz), flerp(c11.x, c11.y, g.wz), g.wy), g.wx);
 }
-// The same blend from (y, z) quads (16 bytes per padded texel: p(y,z), p(y,z+1), p(y+1,z),
-// p(y+1,z+1)) at byte offset BASE, RR16 = 16 R^2: two 16-byte loads (slices x and x + 1) instead of
-// four 8-byte pairs; same taps, blend and rounding.
-template <unsigned BASE, unsigned RR16>
+// The same blend from (y, z) quads (16 bytes per padded texel) at byte offset BASE, RR16 = 16 R^2:
+// two 16-byte loads (slices x and x + 1) instead of four 8-byte pairs; same taps, blend and
+// rounding.  Plain quads hold (p(y,z), p(y,z+1), p(y+1,z), p(y+1,z+1)); difference quads (DIFF,
+// the density quads) hold (p(y,z), p(y,z+1) - p(y,z), p(y+1,z), p(y+1,z+1) - p(y+1,z)), the
+// differences the z lerps a + w (b - a) form, computed once when the quads are built: those four
+// lerps are then one FMA each, with the same bits.
+template <bool DIFF>
+__device__ inline float quad_blend(const f32x4_t& q0, const f32x4_t& q1, const TapGeo& g) {
+    if (DIFF)
+        return flerp(flerp(fmaf(g.wz, q0.y, q0.x), fmaf(g.wz, q0.w, q0.z), g.wy),
+                     flerp(fmaf(g.wz, q1.y, q1.x), fmaf(g.wz, q1.w, q1.z), g.wy), g.wx);
+    return flerp(flerp(flerp(q0.x, q0.y, g.wz), flerp(q0.z, q0.w, g.wz), g.wy),
+                 flerp(flerp(q1.x, q1.y, g.wz), flerp(q1.z, q1.w, g.wz), g.wy), g.wx);
+}
+template <unsigned BASE, unsigned RR16, bool DIFF>
 __device__ inline float tri_q(BufRsrc r, const TapGeo& g) {
     const int o = (int)(g.o4 * 4u + BASE);
     const f32x4_t q0 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
     const f32x4_t q1 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o + (int)RR16, 0, 0));
-    return flerp(flerp(flerp(q0.x, q0.y, g.wz), flerp(q0.z, q0.w, g.wz), g.wy),
-                 flerp(flerp(q1.x, q1.y, g.wz), flerp(q1.z, q1.w, g.wz), g.wy), g.wx);
+    return quad_blend<DIFF>(q0, q1, g);
 }
 // The same with the level's geometry at run time: qbase = 16 x the level's texel offset, RR16 =
 // 16 R^2 (the byte offsets and strides of the pair layout, times four).
+template <bool DIFF>
 __device__ inline float tri_q_rt(BufRsrc r, unsigned qbase, unsigned RR16, const TapGeo& g) {
     const int o = (int)(g.o4 * 4u + qbase);
     const f32x4_t q0 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0));
     const f32x4_t q1 = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, o + (int)RR16, 0, 0));
-    return flerp(flerp(flerp(q0.x, q0.y, g.wz), flerp(q0.z, q0.w, g.wz), g.wy),
-                 flerp(flerp(q1.x, q1.y, g.wz), flerp(q1.z, q1.w, g.wz), g.wy), g.wx);
+    return quad_blend<DIFF>(q0, q1, g);
 }
 __device__ inline float tri_geo(BufRsrc r, unsigned base, unsigned R4, unsigned RR4, const TapGeo& g) {
     const int o = (int)(g.o4 + base);
@@ -268,7 +278,7 @@ __device__ inline float shadow_fast(const FastView& fv, const FastStencil& s, v3
         g = tap_geo(s.tRu, clampf(c.x, 1.0f, s.tres), clampf(c.y, 1.0f, s.tres), clampf(c.z, 1.0f, s.tres));
     }
     // quads: the pair layout's byte offsets and strides times four
-    const float T = tri_q_rt(fv.qt, s.tbase * 4u, s.tRR4 * 4u, g);
+    const float T = tri_q_rt<false>(fv.qt, s.tbase * 4u, s.tRR4 * 4u, g);
     return miss ? 1.0f : T;
 }
 
@@ -302,7 +312,7 @@ __device__ inline v3 gather_fast(const FastView& fv, const FastStencil& s, const
         const float hi = s.dres + 1.0f;
         g = tap_geo(s.dRu, clampf(c.x, 0.0f, hi), clampf(c.y, 0.0f, hi), clampf(c.z, 0.0f, hi));
     }
-    const float rho = tri_q_rt(fv.qm, s.dbase * 4u, s.dRR4 * 4u, g);
+    const float rho = tri_q_rt<true>(fv.qm, s.dbase * 4u, s.dRR4 * 4u, g);
     const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
     const float xs = shadow_fast(fv, s, p, g, same, outside);
     return mk3(xv, xh, xs);
@@ -349,7 +359,7 @@ __device__ inline v3 gather_chunk(const FastView& fv, const FastStencil& s, cons
     if (M >= 5 || (any_out && outside))
         c = mk3(clampf(c.x, 0.0f, fres + 1.0f), clampf(c.y, 0.0f, fres + 1.0f), clampf(c.z, 0.0f, fres + 1.0f));
     const TapGeo g = tap_geo_k<R>(c.x, c.y, c.z);
-    const float rho = tri_q<QDB, RR16>(fv.qm, g);
+    const float rho = tri_q<QDB, RR16, true>(fv.qm, g);
     const float xv = log_ge1(fmaf(fv.alpha64, rho, 1.0f));
     // ShadowTerm_TRTex: inside the box the density blend's corner and weights (replicated TR
     // edges); outside, the light ray's entry point, clamp addressing
@@ -360,7 +370,7 @@ __device__ inline v3 gather_chunk(const FastView& fv, const FastStencil& s, cons
         const v3 ct = padded_coord(q, fres);
         gt = tap_geo_k<R>(clampf(ct.x, 1.0f, fres), clampf(ct.y, 1.0f, fres), clampf(ct.z, 1.0f, fres));
     }
-    const float T = tri_q<QTB, RR16>(fv.qt, gt);
+    const float T = tri_q<QTB, RR16, false>(fv.qt, gt);
     return mk3(xv, xh, miss ? 1.0f : T);
 }
 

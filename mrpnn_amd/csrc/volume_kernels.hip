@@ -297,9 +297,10 @@ __device__ inline void tr_quad_texel(const TrArgs& a, int m, unsigned i_lvl) {
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            // quad (x, y) = (y, z), (y, z+1), (y+1, z), (y+1, z+1) of slice x
-            const float v = flerp(flerp(flerp(q0[k].x, q0[k].y, wz[k]), flerp(q0[k].z, q0[k].w, wz[k]), wy[k]),
-                                  flerp(flerp(q1[k].x, q1[k].y, wz[k]), flerp(q1[k].z, q1[k].w, wz[k]), wy[k]),
+            // difference quad (x, y) = (y, z), (y, z+1) - (y, z), (y+1, z), (y+1, z+1) - (y+1, z) of
+            // slice x: the z lerps a + w (b - a) are one FMA each
+            const float v = flerp(flerp(fmaf(wz[k], q0[k].y, q0[k].x), fmaf(wz[k], q0[k].w, q0[k].z), wy[k]),
+                                  flerp(fmaf(wz[k], q1[k].y, q1[k].x), fmaf(wz[k], q1[k].w, q1[k].z), wy[k]),
                                   wx[k]);
             sum = sum + v;
         }
@@ -666,8 +667,8 @@ __global__ __launch_bounds__(256) void k_pad_mips(PadArgs a, float* __restrict__
     }
 }
 
-// (y, z) quads of the padded density mips 0..7 (every padded texel, the border included; taps
-// beyond the padded level read 0).
+// (y, z) difference quads of the padded density mips 0..7 (every padded texel, the border
+// included; taps beyond the padded level read 0).
 __global__ __launch_bounds__(256) void k_quad_mips(const float* __restrict__ pm, float4* __restrict__ q) {
     const long long total = kQuadTexels;
     for (long long g = (long long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long long)gridDim.x * 256) {
@@ -678,11 +679,12 @@ __global__ __launch_bounds__(256) void k_quad_mips(const float* __restrict__ pm,
         const int z = l % R, y = (l / R) % R;
         const float* p = pm + pmip_offset(m);
         const bool zn = z + 1 < R, yn = y + 1 < R;
+        // difference quads: the z neighbours as differences, rounded as a lerp's b - a is
         float4 o;
         o.x = p[l];
-        o.y = zn ? p[l + 1] : 0.0f;
+        o.y = (zn ? p[l + 1] : 0.0f) - o.x;
         o.z = yn ? p[l + R] : 0.0f;
-        o.w = (yn && zn) ? p[l + R + 1] : 0.0f;
+        o.w = ((yn && zn) ? p[l + R + 1] : 0.0f) - o.z;
         q[g] = o;
     }
 }
