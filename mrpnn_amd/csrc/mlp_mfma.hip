@@ -316,6 +316,23 @@ __device__ inline u32x4_t lds_frag(const char* p) {
     }
 }
 
+// One bf16 MFMA step acc += W x: a K <= 16 layer (CMP: compact fragments, inputs in slots 0..3 of
+// each lane) runs as a 16x16x16 MFMA on those slots (the same products: the 16 x 32 form's slots
+// 4..7 are zero in both operands), a wider one as 16x16x32.
+typedef short s16x4_t __attribute__((ext_vector_type(4)));
+template <bool CMP>
+__device__ inline f32x4_t mfma_bf16(const u32x4_t& w, const u32x4_t& x, const f32x4_t& acc) {
+    if constexpr (CMP) {
+        typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+        const s16x4_t w4 = __builtin_bit_cast(s16x4_t, u32x2_t{w[0], w[1]});
+        const s16x4_t x4 = __builtin_bit_cast(s16x4_t, u32x2_t{x[0], x[1]});
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w4, x4, acc, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, w), __builtin_bit_cast(bf16x8_t, x),
+                                                       acc, 0, 0, 0);
+    }
+}
+
 // Calibration: the largest |x| over one layer call's inputs (the real rows of the 16-pixel tile),
 // atomically folded into net.cal[id] (non-negative floats order as their bit patterns).
 template <int N>
@@ -392,11 +409,8 @@ __device__ inline void dense(const NetRs& net, const float (&in)[mfma_nk(ID)][8]
             f32x4_t acl = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int kc = 0; kc < NK; ++kc) {
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, wa[cur][kc]),
-                                                              __builtin_bit_cast(bf16x8_t, b[kc]), acc, 0, 0, 0);
-                if constexpr (TWO)
-                    acl = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, wl[cur][kc]),
-                                                                  __builtin_bit_cast(bf16x8_t, b[kc]), acl, 0, 0, 0);
+                acc = mfma_bf16<CMP>(wa[cur][kc], b[kc], acc);
+                if constexpr (TWO) acl = mfma_bf16<CMP>(wl[cur][kc], b[kc], acl);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -805,12 +819,10 @@ __device__ inline void frag1_apply(const NetRs& net, const Frag1& f, const float
         const u32x4_t b = u32x4_t{pack_bf16x2(in[0], in[1]), pack_bf16x2(in[2], in[3]), 0u, 0u};
         f32x4_t acc = f32x4_t{__uint_as_float(f.b[0]), __uint_as_float(f.b[1]), __uint_as_float(f.b[2]),
                               __uint_as_float(f.b[3])};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, f.w), __builtin_bit_cast(bf16x8_t, b),
-                                                      acc, 0, 0, 0);
+        constexpr bool CMP = compact_frag(PREC, ID);
+        acc = mfma_bf16<CMP>(f.w, b, acc);
         f32x4_t acl = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};  // lo terms: an independent MFMA (as dense())
-        if constexpr (TWO)
-            acl = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, f.wl),
-                                                          __builtin_bit_cast(bf16x8_t, b), acl, 0, 0, 0);
+        if constexpr (TWO) acl = mfma_bf16<CMP>(f.wl, b, acl);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const float v = TWO ? acc[i] + acl[i] : acc[i];
