@@ -133,7 +133,7 @@ struct TrArgs {
     const float4* qmips; // (y, z) quads of the padded density mips 0..7
     float alpha_lvl[8];
     long long start[9];  // flattened texel offsets of each level
-    int bstart[9];       // k_tr_bake_lvl: first workgroup of each level
+    int bstart[9];       // levels 1..7 by tr_quad_block: first workgroup of each level
     v3 light;
 };
 // Padded TR texel value T at padded position (px, py, pz) of a level with Rp^3 padded texels, into
@@ -234,10 +234,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     }
 }
 
-// The bake over the (y, z) quads of the zero-padded density mips, one level per workgroup: the
-// level, its resolution and strides are wave-uniform (scalar registers), texel coordinates are
-// shifts of the thread index, a sample's eight taps are two 16-byte loads (slices x and x + 1),
-// and each thread keeps MRP_TR_BAKE_ROUND samples' loads in flight.  The wavefront's 64 lanes
+// The bake over the (y, z) difference quads of the zero-padded density mips, one level per
+// workgroup: the level, its resolution and strides are wave-uniform (scalar registers), texel
+// coordinates are shifts of the thread index, a sample's eight taps are two 16-byte loads (slices
+// x and x + 1), and each thread keeps MRP_TR_BAKE_ROUND samples' loads in flight.  The wavefront's 64 lanes
 // scatter over tens of cache lines per load (their rays leave the box at different distances, so
 // their samples drift apart), and the texture pipeline's time follows the cache lines it
 // touches: two quad loads per sample touch half the lines of four pair loads.  The arithmetic of
@@ -319,8 +319,7 @@ __device__ inline void tr_quad_texel(const TrArgs& a, int m, unsigned i_lvl) {
                 for (int pz = lo[2]; pz <= hi[2]; ++pz) tr_quad_put(qd, Rp, px, py, pz, T);
     }
 }
-// Block blk of the levels a.bstart covers (256 texels per workgroup of 256 threads, or
-// nthreads per workgroup), level from a.bstart.
+// Block blk of the levels a.bstart covers (nthreads texels per workgroup), level from a.bstart.
 __device__ inline void tr_quad_block(const TrArgs& a, int blk, int nthreads) {
     int m = 0;
     while (m < 7 && blk >= a.bstart[m + 1]) ++m;
@@ -329,21 +328,18 @@ __device__ inline void tr_quad_block(const TrArgs& a, int blk, int nthreads) {
     if (i_lvl >= (1u << (3 * lres))) return;
     tr_quad_texel(a, m, i_lvl);
 }
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_tr_bake_lvl(TrArgs a) {
-    tr_quad_block(a, (int)blockIdx.x, 256);
-}
 
 // ------------------------------------------------------------------------------------------
 // TR level 0 (87 % of the bake) from LDS.  Every sample of a texel lies on the ray from the
 // texel towards the light, so in sheared coordinates along the light's principal axis A
 // (u' = c_U - r_U c_A, v' = c_V - r_V c_A with r = L / L_A, |r| <= 1, padded texel coordinates
-// c) a texel's whole ray stays at its own (u', v').  A workgroup takes the texels of one 8 x 8
-// tile of (u', v'), stages for each of the 130 padded slabs c_A = s the 14 x 14 window of density
+// c) a texel's whole ray stays at its own (u', v').  A workgroup takes the texels of one 4 x 4
+// tile of (u', v'), stages for each of the 130 padded slabs c_A = s the 10 x 10 window of density
 // mip 1 that the tile's rays cross there (window origin floor(u'_lo + r_U s - |r_U| - 0.01),
 // wide enough for the tile, the ray's slope over the slab pair a blend touches, the second tap
 // and float slack of the running position sum), and blends every sample from LDS: the texture
 // pipeline then carries no tap traffic and the concurrent march has it to itself.  Each texel's
-// arithmetic is k_tr_bake_lvl's (same positions, taps, blend and summation order), only the
+// arithmetic is tr_quad_texel's (same positions, taps, blend and summation order), only the
 // tap loads come from LDS.
 // ------------------------------------------------------------------------------------------
 constexpr int kL0Tile = 4;              // tile side in (u', v'), texels
