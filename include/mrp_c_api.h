@@ -50,7 +50,7 @@ extern "C" {
                                        march, network) into a HIP graph once per launch shape and
                                        replays it; timing then reports the whole replay as ms_predict */
 #define MRP_OPT_TR_SLOTS 3          /* capacity of the batched path's TR cache: one set of transmittance
-                                       mips (about 20 MB) per distinct (volume, light_dir, alpha) of a
+                                       mips and their quads (about 51 MB) per distinct (volume, light_dir, alpha) of a
                                        batch (1..4096, default 64) */
 #define MRP_OPT_LDS_WEIGHTS 5       /* 1 (default): the fused MFMA kernel runs one 768-thread workgroup
                                        per CU holding all weight fragments and biases in LDS;
