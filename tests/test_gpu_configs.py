@@ -113,7 +113,9 @@ def c2vol(gctx):
     vol.close()
 
 
-@pytest.mark.parametrize("sun_index", [0, 5])
+# suns along -x (0, 5), +y (4) and -z (7): the TR bake's three kernel instantiations and both
+# walking directions, whose (y, z) TR quads the fast gather reads
+@pytest.mark.parametrize("sun_index", [0, 4, 5, 7])
 def test_c2_1080p_512_features_network_and_bf16(gctx, orc, c2vol, sun_index):
     import torch
     import bench
