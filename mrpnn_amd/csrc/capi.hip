@@ -582,7 +582,6 @@ DevVolume mrp_dev_volume(const mrp_volume* v) {
     for (int m = 0; m < 9; ++m) d.mips[m] = v->mip[m];
     for (int m = 0; m < 8; ++m) d.tr[m] = v->trm[m];
     d.lut = v->ctx->d_lut;
-    d.pmips = v->d_pmips;
     d.ptr = v->d_ptr;
     d.qmips = v->d_qmips;
     d.occ = v->d_occ;

@@ -102,9 +102,8 @@ struct DevVolume {
     const float* mips[9];  // (256>>m)^3
     const float* tr[8];    // (128>>m)^3
     const float* lut;      // 512*512, [j*512+i]
-    const float* pmips;    // zero-padded density mips 0..7 (fast gather)
     const float* ptr;      // (y, z) quads of the zero-padded TR mips 0..6, 4 floats each (fast gather)
-    const float4* qmips;   // (y, z) quads of the zero-padded density mips 1..7 (fast gather)
+    const float4* qmips;   // (y, z) difference quads of the zero-padded density mips 0..7 (fast gather)
     const uint8_t* occ;    // brick emptiness of the grid, occ_n^3 (see brick_of)
     int occ_n;
     int N;
