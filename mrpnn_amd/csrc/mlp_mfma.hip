@@ -1292,9 +1292,9 @@ hipError_t launch_fast_features(const DevVolume* vols, const ViewSlot* slots, co
 }
 
 template <int PREC>
-__global__ __launch_bounds__(256) void k_mlp_mfma_features(MfmaNet net, const float* __restrict__ W,
-                                                           const float* __restrict__ feats, int n, v3 sr, float g,
-                                                           float* __restrict__ rgb) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_mlp_mfma_features(
+    MfmaNet net, const float* __restrict__ W, const float* __restrict__ feats, int n, v3 sr, float g,
+    float* __restrict__ rgb) {
     __shared__ float sW1[1200];
     __shared__ float sG[kGates * kGateFloats];
     load_shared(sW1, sG, nullptr, W, nullptr);
